@@ -1,0 +1,217 @@
+"""Benchmark driver: prints ONE JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload traversal]
+
+Workload ``traversal``: the traversal microbenchmark on a 2^20-triangle
+Perlin-displaced sphere (CWBVH built by this library).  One step = 2^21
+primary closest-hit rays + 2^21 cosine bounce closest-hit rays + 2^21
+shadow any-hit rays, all resident in HBM before timing starts.
+
+The roofline object is for the dominant kernel (bounce closest-hit):
+algorithmic bytes per launch = sum over rays of 32 B ray + 16 B hit +
+80 B per node decoded + 48 B per triangle tested, with the per-ray counts
+taken from the counting variant of the same kernel on the same rays; its
+launch time is measured with HIP events on the stream it runs on.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s + equal-time relMSE, NRC vs plain PT, 1080p, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def bench_traversal(args, torch):
+    from tfg_amd import _lib, scenes, workloads
+
+    t0 = time.time()
+    sc = scenes.displaced_sphere()
+    host = _lib.CwbvhHost(sc.verts, 0)
+    dev = host.upload()
+    _log(f"scene {sc.n_tris} tris, {host.num_nodes} nodes, depth {host.max_depth}, built in {time.time() - t0:.1f}s")
+    W, H = 2048, 1024
+    prim = workloads.primary_rays(sc, W, H, seed=7)
+    n = prim.shape[0]
+    d_prim = torch.from_numpy(prim).cuda()
+    d_hp = torch.empty((n, 4), dtype=torch.int32, device="cuda")
+    dev.trace_closest(d_prim, d_hp, n)
+    torch.cuda.synchronize()
+    hp = d_hp.cpu().numpy().view(np.uint32)
+    hits = np.zeros(n, [("t", np.float32), ("u", np.float32), ("v", np.float32), ("prim", np.uint32)])
+    hits["t"] = hp[:, 0].view(np.float32)
+    hits["prim"] = hp[:, 3]
+    bounce = workloads.bounce_rays(sc, prim, hits, seed=8)
+    shadow = workloads.shadow_rays(sc, prim, hits, seed=9)
+    d_b = torch.from_numpy(bounce).cuda()
+    d_s = torch.from_numpy(shadow).cuda()
+    d_hb = torch.empty((n, 4), dtype=torch.int32, device="cuda")
+    d_occ = torch.empty(n, dtype=torch.int32, device="cuda")
+    cnt = torch.empty((n, 2), dtype=torch.int32, device="cuda")
+
+    # algorithmic bytes of each launch from the counting variant
+    def algo_bytes(fn, rays_dev):
+        fn(rays_dev, cnt)
+        torch.cuda.synchronize()
+        c = cnt.cpu().numpy().view(np.uint32).astype(np.int64)
+        nodes = c[:, 0] & 0x7FFFFFFF
+        assert not (c[:, 0] >> 31).any(), "traversal stack overflow"
+        return int((32 + 16) * n + 80 * nodes.sum() + 48 * c[:, 1].sum()), float(nodes.mean()), float(c[:, 1].mean())
+
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    launches = {
+        "primary_closest": lambda c=None: dev.trace_closest(d_prim, d_hp, n, counts=c, stream=sh),
+        "bounce_closest": lambda c=None: dev.trace_closest(d_b, d_hb, n, counts=c, stream=sh),
+        "shadow_any": lambda c=None: dev.trace_any(d_s, d_occ, n, counts=c, stream=sh),
+    }
+    bytes_per = {}
+    visits = {}
+    for k, f in launches.items():
+        b, nv, nt = algo_bytes(lambda r, c, f=f: f(c), None)
+        bytes_per[k] = b
+        visits[k] = (nv, nt)
+    _log(f"per-ray visits (nodes, tris): {visits}")
+
+    for _ in range(args.warmup):
+        for f in launches.values():
+            f()
+    torch.cuda.synchronize()
+    ev = {k: [] for k in launches}
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        for k, f in launches.items():
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            f()
+            b.record(stream)
+            ev[k].append((a, b))
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    total_ms = start.elapsed_time(stop)
+    per_kernel_ms = {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in ev.items()}
+    ms_step = total_ms / args.steps
+    rays_step = 3 * n
+    value = rays_step / (ms_step * 1e-3) / 1e6
+    dom = "bounce_closest"
+    achieved = bytes_per[dom] / (per_kernel_ms[dom] * 1e-3) / 1e9
+    res = {
+        "value": value,
+        "ms_per_step": ms_step,
+        "per_kernel_ms": per_kernel_ms,
+        "mrays_per_kernel": {k: n / (v * 1e-3) / 1e6 for k, v in per_kernel_ms.items()},
+        "roofline": {
+            "kernel": "k_trace<watertight, closest> (bounce rays)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes": bytes_per[dom],
+        },
+        "config": {
+            "workload": "traversal microbench: displaced sphere 1,048,576 tris (Perlin amp 0.05, seed 1), "
+                        "2^21 primary + 2^21 cosine bounce closest-hit + 2^21 shadow any-hit rays",
+            "tris": sc.n_tris,
+            "cwbvh_nodes": host.num_nodes,
+            "tree_depth": host.max_depth,
+            "rays_per_step": rays_step,
+            "visits_per_ray": visits,
+            "seeds": {"scene": 1, "primary": 7, "bounce": 8, "shadow": 9},
+        },
+        "wall_s": wall,
+        "dtype": "fp32",
+    }
+    res["_cpu_inputs"] = (host, prim, bounce, shadow)
+    return res
+
+
+def cpu_baseline_traversal(host, prim, bounce, shadow):
+    import oracle
+
+    cores = os.cpu_count() or 1
+    sample = 1 << 17
+    rng = np.random.default_rng(0)
+    total_rays = 0
+    t0 = time.perf_counter()
+    for rays, kind in ((prim, "closest"), (bounce, "closest"), (shadow, "any")):
+        sel = np.ascontiguousarray(rays[rng.choice(rays.shape[0], sample, replace=False)])
+        if kind == "closest":
+            oracle.cwbvh_trace(host.nodes, host.tris, 0, sel, nthreads=cores)
+        else:
+            oracle.cwbvh_occluded(host.nodes, host.tris, 0, sel, nthreads=cores)
+        total_rays += sample
+    dt = time.perf_counter() - t0
+    return {
+        "value": total_rays / dt / 1e6,
+        "unit": "Mrays/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"3 x 2^17 rays (primary, bounce closest-hit; shadow any-hit) drawn from the same sets, "
+                  f"oracle CWBVH traversal on {cores} threads, {dt:.2f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="traversal", choices=["traversal"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+    res = bench_traversal(args, torch)
+    cpu_in = res.pop("_cpu_inputs")
+    cpu = None if args.no_cpu else cpu_baseline_traversal(*cpu_in)
+    out = {
+        "metric": METRIC,
+        "value": round(res["value"], 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": res["dtype"],
+        "data": "synthetic (seeded procedural scene and rays)",
+        "config": res["config"],
+        "per_kernel_ms": res["per_kernel_ms"],
+        "mrays_per_kernel": res["mrays_per_kernel"],
+        "roofline": res["roofline"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,152 @@
+"""CPU oracle for the tfg_amd hot path (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py may import this package; the product path never does.
+
+``liboracle.so`` (oracle.cpp, built by ``make -C oracle``) restates the
+reference's RNG, camera, CWBVH decode, BVH2 build and Moller-Trumbore test;
+``mlp_oracle.py`` is the FP64 NumPy MLP.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+class TfgRay(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")]
+
+
+_lib = None
+
+_SIGS = {
+    "orc_pcg32": (C.c_uint32, [C.POINTER(C.c_uint32)]),
+    "orc_pcg32_float": (C.c_float, [C.POINTER(C.c_uint32)]),
+    "orc_seed": (C.c_uint32, [C.c_int, C.c_int, C.c_int]),
+    "orc_stream": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "orc_camera_default": (None, [C.c_int, C.c_int, C.c_void_p]),
+    "orc_camera_make": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_void_p]),
+    "orc_camera_ray": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_void_p,
+                              C.c_void_p]),
+    "orc_cwbvh_trace": (None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                               C.c_int]),
+    "orc_cwbvh_occluded": (None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                  C.c_void_p, C.c_int]),
+    "orc_brute_trace": (None, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "orc_tri_watertight": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]),
+    "orc_bvh2_build": (C.c_void_p, [C.c_void_p, C.c_uint32]),
+    "orc_bvh2_free": (None, [C.c_void_p]),
+    "orc_bvh2_num_nodes": (C.c_uint32, [C.c_void_p]),
+    "orc_bvh2_trace": (None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int]),
+    "orc_cwbvh_validate": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                     C.c_void_p]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+HIT_DTYPE = np.dtype([("t", np.float32), ("u", np.float32), ("v", np.float32), ("prim", np.uint32)])
+
+
+def rays_array(o: np.ndarray, d: np.ndarray, tmin=0.0, tmax=np.inf) -> np.ndarray:
+    """(n, 8) float32 rays in tfg_ray layout."""
+    n = o.shape[0]
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3] = o
+    r[:, 3] = tmin
+    r[:, 4:7] = d
+    r[:, 7] = tmax
+    return r
+
+
+def cwbvh_trace(nodes, tris, fmt, rays, nthreads=os.cpu_count() or 1, counts=False):
+    n = rays.shape[0]
+    hits = np.zeros(n, HIT_DTYPE)
+    cnt = np.zeros((n, 2), np.uint32) if counts else None
+    lib().orc_cwbvh_trace(_p(nodes), _p(tris), fmt, _p(rays), n, _p(hits), _p(cnt) if counts else None, nthreads)
+    return (hits, cnt) if counts else hits
+
+
+def cwbvh_occluded(nodes, tris, fmt, rays, nthreads=os.cpu_count() or 1, counts=False):
+    n = rays.shape[0]
+    occ = np.zeros(n, np.uint32)
+    cnt = np.zeros((n, 2), np.uint32) if counts else None
+    lib().orc_cwbvh_occluded(_p(nodes), _p(tris), fmt, _p(rays), n, _p(occ), _p(cnt) if counts else None, nthreads)
+    return (occ, cnt) if counts else occ
+
+
+def brute_trace(verts, rays):
+    v = np.ascontiguousarray(verts, np.float32)
+    n = rays.shape[0]
+    hits = np.zeros(n, HIT_DTYPE)
+    lib().orc_brute_trace(_p(v), v.shape[0], _p(rays), n, _p(hits))
+    return hits
+
+
+def validate(nodes, tris, fmt, verts):
+    d = C.c_uint32()
+    v = np.ascontiguousarray(verts, np.float32)
+    rc = lib().orc_cwbvh_validate(_p(nodes), nodes.shape[0], _p(tris), tris.shape[0], fmt, _p(v), C.byref(d))
+    return rc, d.value
+
+
+PRIM_DTYPE = np.dtype([
+    ("tipo", np.int32), ("emision", np.float32, 3), ("difuso", np.float32, 3), ("especular", np.float32, 3),
+    ("transmision", np.float32, 3), ("ior", np.float32), ("g", np.float64, 18)])
+assert PRIM_DTYPE.itemsize == 200
+
+
+def primitivas_from_verts(verts, normals=None):
+    """Primitiva records (primitiva.h:17-39) for a triangle list."""
+    n = verts.shape[0]
+    p = np.zeros(n, PRIM_DTYPE)
+    p["tipo"] = 2
+    p["ior"] = 1.0
+    p["g"][:, 0:9] = verts.astype(np.float64)
+    if normals is not None:
+        p["g"][:, 9:18] = normals.astype(np.float64)
+    return p
+
+
+class Bvh2:
+    """bvh.cpp median-split BVH2 traversed with primitiva.h Moller-Trumbore."""
+
+    def __init__(self, verts):
+        self.prims = primitivas_from_verts(np.asarray(verts, np.float32))
+        self._h = lib().orc_bvh2_build(self.prims.ctypes.data, self.prims.shape[0])
+        self.num_nodes = lib().orc_bvh2_num_nodes(self._h)
+
+    def trace(self, rays, nthreads=os.cpu_count() or 1):
+        n = rays.shape[0]
+        hits = np.zeros(n, HIT_DTYPE)
+        lib().orc_bvh2_trace(self._h, _p(rays), n, _p(hits), nthreads)
+        return hits
+
+    def __del__(self):
+        try:
+            lib().orc_bvh2_free(self._h)
+        except Exception:
+            pass

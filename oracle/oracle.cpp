@@ -1,0 +1,668 @@
+// oracle.cpp - CPU restatement of the reference hot path (test
+// infrastructure; see oracle.h).  Compiled with -ffp-contract=off so every
+// FP32 operation rounds exactly where it is written, like the HIP kernels
+// built with the same flag.
+//
+// Pinning of each part (tests/ hold the checks):
+//   RNG               closed-form uint32 recurrences cross-checked by an
+//                     independent NumPy restatement and by hand-computed
+//                     known answers (tests/test_oracle_rng_camera.py).
+//   camera            invariants: unit directions, centre pixel -> front,
+//                     corner rays span the 45 deg field of view.
+//   watertight test   closed forms: axis-aligned hits at known t/u/v,
+//                     parallel rays miss, shared-edge rays hit exactly one
+//                     side's set, agreement with FP64 Moller-Trumbore.
+//   CWBVH traversal   equals brute force over all triangles (bit-exact ids,
+//                     t, u, v) on random meshes; any-hit equals
+//                     "closest-hit t < tmax"; orc_cwbvh_validate proves
+//                     decoded boxes contain their triangles.
+//   BVH2 + MT         equals FP64-vector brute force MT; agrees with the
+//                     watertight path except on edge/tie rays.
+#include "oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+inline uint32_t f2u(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+inline float u2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+// ---------------------------------------------------------------- rng.h
+// rng.h:12-17
+inline uint32_t pcg32(uint32_t& s) {
+    s = s * 747796405u + 2891336453u;
+    const uint32_t word = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+// rng.h:20-22 (u32 -> f32 conversion rounds to nearest, so 1.0f is reachable)
+inline float pcg32_float(uint32_t& s) { return (float)pcg32(s) * (1.0f / 4294967296.0f); }
+// rng.h:25-35
+inline uint32_t seed_of(int px, int py, int frame) {
+    uint32_t s = (uint32_t)(py << 16) | (uint32_t)(px & 0xFFFF);
+    s ^= (uint32_t)frame * 719393u;
+    s = (s ^ 61u) ^ (s >> 16u);
+    s *= 0x27d4eb2du;
+    s = s ^ (s >> 15u);
+    return s == 0u ? 1u : s;
+}
+
+// --------------------------------------------------- ray precomputation
+struct RayPre {
+    float O[3], D[3], rD[3];
+    float tmin, tmax;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+    uint32_t octinv;
+};
+
+inline float safe_rcp(float d) {
+    if (std::fabs(d) < 1e-30f) d = std::copysign(1e-30f, d);
+    return 1.0f / d;
+}
+
+RayPre make_ray(const tfg_ray& r) {
+    RayPre p;
+    p.O[0] = r.ox; p.O[1] = r.oy; p.O[2] = r.oz;
+    p.D[0] = r.dx; p.D[1] = r.dy; p.D[2] = r.dz;
+    for (int a = 0; a < 3; ++a) p.rD[a] = safe_rcp(p.D[a]);
+    p.tmin = r.tmin;
+    p.tmax = r.tmax;
+    const float ax = std::fabs(p.D[0]), ay = std::fabs(p.D[1]), az = std::fabs(p.D[2]);
+    p.kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
+    p.kx = (p.kz + 1) % 3;
+    p.ky = (p.kx + 1) % 3;
+    if (p.D[p.kz] < 0.0f) std::swap(p.kx, p.ky);
+    p.Sx = p.D[p.kx] / p.D[p.kz];
+    p.Sy = p.D[p.ky] / p.D[p.kz];
+    p.Sz = 1.0f / p.D[p.kz];
+    p.octinv = 7u - ((p.rD[0] < 0.0f ? 4u : 0u) | (p.rD[1] < 0.0f ? 2u : 0u) | (p.rD[2] < 0.0f ? 1u : 0u));
+    return p;
+}
+
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013), FP32 with
+// the paper's FP64 recomputation of zero edge functions.  Returns t, and the
+// barycentric weights of v1 (u) and v2 (v) as the reference's (u, v).
+bool watertight(const float* v0, const float* v1, const float* v2, const RayPre& r, float& t, float& u,
+                float& v) {
+    float A[3], B[3], C[3];
+    for (int a = 0; a < 3; ++a) {
+        A[a] = v0[a] - r.O[a];
+        B[a] = v1[a] - r.O[a];
+        C[a] = v2[a] - r.O[a];
+    }
+    const float Ax = A[r.kx] - r.Sx * A[r.kz];
+    const float Ay = A[r.ky] - r.Sy * A[r.kz];
+    const float Bx = B[r.kx] - r.Sx * B[r.kz];
+    const float By = B[r.ky] - r.Sy * B[r.kz];
+    const float Cx = C[r.kx] - r.Sx * C[r.kz];
+    const float Cy = C[r.ky] - r.Sy * C[r.kz];
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = r.Sz * A[r.kz];
+    const float Bz = r.Sz * B[r.kz];
+    const float Cz = r.Sz * C[r.kz];
+    const float T = U * Az + V * Bz + W * Cz;
+    t = T / det;
+    u = V / det;
+    v = W / det;
+    return true;
+}
+
+// Affine unit-triangle test, traverse_cwbvh.cuh:189-209.
+bool affine(const float* T, const RayPre& r, float tcur, float& t, float& u, float& v) {
+    const float* T0 = T;
+    const float* T1 = T + 4;
+    const float* T2 = T + 8;
+    const float transS = T2[0] * r.O[0] + T2[1] * r.O[1] + T2[2] * r.O[2] + T2[3];
+    const float transD = T2[0] * r.D[0] + T2[1] * r.D[1] + T2[2] * r.D[2];
+    const float d = -transS / transD;
+    if (!(d > r.tmin) || d > tcur) return false;
+    const float Ix = r.O[0] + d * r.D[0];
+    const float Iy = r.O[1] + d * r.D[1];
+    const float Iz = r.O[2] + d * r.D[2];
+    const float uu = T0[0] * Ix + T0[1] * Iy + T0[2] * Iz + T0[3];
+    const float vv = T1[0] * Ix + T1[1] * Iy + T1[2] * Iz + T1[3];
+    if (uu >= 0.0f && vv >= 0.0f && uu + vv < 1.0f) {
+        t = d;
+        u = uu;
+        v = vv;
+        return true;
+    }
+    return false;
+}
+
+constexpr float kBoxSlack = 1.00000095367431640625f;   // 1 + 2^-20
+
+inline uint32_t byte_of(uint32_t w, int j) { return (w >> (8 * j)) & 0xFFu; }
+
+// Decodes one 80 B node (traverse_cwbvh.cuh:75-178) and returns the 32-bit
+// hit mask: bits 24..31 = internal children (slot ^ octinv), bits 0..23 =
+// triangles of hit leaf children.
+uint32_t node_hitmask(const float* n, const RayPre& r, float tcur) {
+    const uint32_t w = f2u(n[3]);
+    const int ex = (int8_t)(w & 0xFF), ey = (int8_t)((w >> 8) & 0xFF), ez = (int8_t)((w >> 16) & 0xFF);
+    const float idirx = u2f((uint32_t)(ex + 127) << 23) * r.rD[0];
+    const float idiry = u2f((uint32_t)(ey + 127) << 23) * r.rD[1];
+    const float idirz = u2f((uint32_t)(ez + 127) << 23) * r.rD[2];
+    const float origx = (n[0] - r.O[0]) * r.rD[0];
+    const float origy = (n[1] - r.O[1]) * r.rD[1];
+    const float origz = (n[2] - r.O[2]) * r.rD[2];
+    uint32_t hitmask = 0;
+    for (int i = 0; i < 8; ++i) {
+        const int h = i >> 2, j = i & 3;
+        const uint32_t meta = byte_of(f2u(n[6 + h]), j);
+        const float qlx = (float)byte_of(f2u(n[8 + h]), j);
+        const float qly = (float)byte_of(f2u(n[10 + h]), j);
+        const float qlz = (float)byte_of(f2u(n[12 + h]), j);
+        const float qhx = (float)byte_of(f2u(n[14 + h]), j);
+        const float qhy = (float)byte_of(f2u(n[16 + h]), j);
+        const float qhz = (float)byte_of(f2u(n[18 + h]), j);
+        const float lox = r.rD[0] < 0.0f ? qhx : qlx, hix = r.rD[0] < 0.0f ? qlx : qhx;
+        const float loy = r.rD[1] < 0.0f ? qhy : qly, hiy = r.rD[1] < 0.0f ? qly : qhy;
+        const float loz = r.rD[2] < 0.0f ? qhz : qlz, hiz = r.rD[2] < 0.0f ? qlz : qhz;
+        const float tminx = std::fma(lox, idirx, origx), tmaxx = std::fma(hix, idirx, origx);
+        const float tminy = std::fma(loy, idiry, origy), tmaxy = std::fma(hiy, idiry, origy);
+        const float tminz = std::fma(loz, idirz, origz), tmaxz = std::fma(hiz, idirz, origz);
+        const float tnear = std::fmax(std::fmax(std::fmax(tminx, tminy), tminz), r.tmin);
+        const float tfar = std::fmin(std::fmin(std::fmin(tmaxx, tmaxy), tmaxz), tcur);
+        if (!(tnear <= tfar * kBoxSlack)) continue;
+        const uint32_t is_inner = (meta & (meta << 1)) & 0x10u;
+        const uint32_t bit_index = (is_inner ? (meta ^ r.octinv) : meta) & 0x1Fu;
+        const uint32_t child_bits = (meta >> 5) & 7u;
+        hitmask |= child_bits << bit_index;
+    }
+    return hitmask;
+}
+
+inline int highest_bit(uint32_t x) { return 31 - __builtin_clz(x); }
+
+// traverse_cwbvh.cuh:51-257 control flow: node groups popped highest bit
+// first, leaf triangles of the decoded node tested before descending.
+// any_hit: return at the first accepted triangle.
+template <bool kAnyHit>
+bool traverse(const float* nodes, const float* tris, uint32_t format, const RayPre& r, tfg_hit& hit,
+              uint32_t* counts) {
+    uint32_t stack_x[64], stack_y[64];
+    int sp = 0;
+    uint32_t ng_x = 0, ng_y = 0x80000000u;
+    float tcur = r.tmax;
+    uint32_t best_addr = 0xFFFFFFFFu, best_prim = 0xFFFFFFFFu;
+    float bu = 0.f, bv = 0.f;
+    uint32_t n_nodes = 0, n_tris = 0;
+    bool found = false;
+    for (;;) {
+        uint32_t tg_x = 0, tg_y = 0;
+        if (ng_y > 0x00FFFFFFu) {
+            const uint32_t imask = ng_y;
+            const int bit = highest_bit(ng_y & 0xFF000000u);
+            ng_y &= ~(1u << bit);
+            if (ng_y > 0x00FFFFFFu) {
+                stack_x[sp] = ng_x;
+                stack_y[sp] = ng_y;
+                ++sp;
+            }
+            const uint32_t slot = ((uint32_t)bit - 24u) ^ r.octinv;
+            const uint32_t rel = (uint32_t)__builtin_popcount(imask & ~(0xFFFFFFFFu << slot));
+            const uint32_t ni = ng_x + rel;
+            const float* n = nodes + 20 * (size_t)ni;
+            ++n_nodes;
+            const uint32_t hm = node_hitmask(n, r, tcur);
+            ng_x = f2u(n[4]);
+            ng_y = (hm & 0xFF000000u) | (f2u(n[3]) >> 24);
+            tg_x = f2u(n[5]);
+            tg_y = hm & 0x00FFFFFFu;
+        }
+        while (tg_y != 0) {
+            const int b = highest_bit(tg_y);
+            tg_y &= ~(1u << b);
+            const uint32_t addr = tg_x + 4u * (uint32_t)b;
+            const float* T = tris + 4 * (size_t)addr;
+            ++n_tris;
+            float t, u, v;
+            bool ok;
+            if (format == 0) {
+                ok = watertight(T, T + 4, T + 8, r, t, u, v) && t > r.tmin && t <= tcur;
+            } else {
+                ok = affine(T, r, tcur, t, u, v);
+            }
+            if (!ok) continue;
+            const uint32_t prim = f2u(T[15]);
+            if (kAnyHit) {
+                if (t < r.tmax) {
+                    if (counts) { counts[0] = n_nodes; counts[1] = n_tris; }
+                    return true;
+                }
+                continue;
+            }
+            if (t < tcur || prim < best_prim) {
+                tcur = t;
+                bu = u;
+                bv = v;
+                best_addr = addr;
+                best_prim = prim;
+                found = true;
+            }
+        }
+        if (ng_y <= 0x00FFFFFFu) {
+            if (sp > 0) {
+                --sp;
+                ng_x = stack_x[sp];
+                ng_y = stack_y[sp];
+            } else {
+                break;
+            }
+        }
+    }
+    (void)best_addr;
+    if (counts) { counts[0] = n_nodes; counts[1] = n_tris; }
+    if (kAnyHit) return false;
+    hit.t = tcur;
+    hit.u = found ? bu : 0.f;
+    hit.v = found ? bv : 0.f;
+    hit.prim = found ? best_prim : 0xFFFFFFFFu;
+    return found;
+}
+
+template <typename F>
+void parallel_for(uint32_t n, int nthreads, F f) {
+    if (nthreads <= 1 || n < 1024) {
+        for (uint32_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    const uint32_t chunk = 256;
+    std::atomic<uint32_t> next{0};
+    for (int k = 0; k < nthreads; ++k) {
+        th.emplace_back([&] {
+            for (;;) {
+                const uint32_t b = next.fetch_add(chunk);
+                if (b >= n) break;
+                const uint32_t e = std::min(n, b + chunk);
+                for (uint32_t i = b; i < e; ++i) f(i);
+            }
+        });
+    }
+    for (auto& t : th) t.join();
+}
+
+// ------------------------------------------------ reference BVH2 + MT
+struct V3 { double x, y, z; };
+inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline double comp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+struct Bvh2 {
+    struct Node { double mn[3], mx[3]; int l = -1, r = -1, first = 0, end = 0; bool leaf = false; };
+    std::vector<Node> nodes;
+    std::vector<V3> v0, v1, v2;     // in BVH order
+    std::vector<uint32_t> prim;     // original index per BVH slot
+};
+
+// primitiva.h:164-183 (vectors FP64, det/u/v/t stored in FP32).
+bool moller_trumbore(V3 v0, V3 v1, V3 v2, V3 o, V3 d, float& t) {
+    const V3 v0v1 = sub(v1, v0), v0v2 = sub(v2, v0);
+    const V3 pvec = cross(d, v0v2);
+    const float det = (float)dot(v0v1, pvec);
+    if (std::fabs(det) < 1e-6f) return false;
+    const float inv = 1.0f / det;
+    const V3 tvec = sub(o, v0);
+    const float u = (float)(dot(tvec, pvec) * inv);
+    if (u < 0.0f || u > 1.0f) return false;
+    const V3 qvec = cross(tvec, v0v1);
+    const float v = (float)(dot(d, qvec) * inv);
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = (float)(dot(v0v2, qvec) * inv);
+    return t > 1e-6f;
+}
+
+// bvh.cpp:56-100
+int bvh2_rec(Bvh2& b, const std::vector<V3>& a0, const std::vector<V3>& a1, const std::vector<V3>& a2,
+             std::vector<int>& idx, int begin, int end) {
+    const int me = (int)b.nodes.size();
+    b.nodes.push_back({});
+    Bvh2::Node nd;
+    for (int k = 0; k < 3; ++k) { nd.mn[k] = 1e30; nd.mx[k] = -1e30; }
+    for (int i = begin; i < end; ++i) {
+        const int t = idx[i];
+        const V3 vs[3] = {a0[t], a1[t], a2[t]};
+        for (const V3& p : vs)
+            for (int k = 0; k < 3; ++k) {
+                nd.mn[k] = std::fmin(nd.mn[k], comp(p, k));
+                nd.mx[k] = std::fmax(nd.mx[k], comp(p, k));
+            }
+    }
+    const int cnt = end - begin;
+    if (cnt <= 4) {
+        nd.leaf = true;
+        nd.first = begin;
+        nd.end = end;
+        b.nodes[me] = nd;
+        return me;
+    }
+    // boundingBox.h:31-36
+    const double tx = nd.mx[0] - nd.mn[0], ty = nd.mx[1] - nd.mn[1], tz = nd.mx[2] - nd.mn[2];
+    const int axis = (tx > ty && tx > tz) ? 0 : (ty > tz ? 1 : 2);
+    auto centroid = [&](int t) {
+        const double lo = std::fmin(std::fmin(comp(a0[t], axis), comp(a1[t], axis)), comp(a2[t], axis));
+        const double hi = std::fmax(std::fmax(comp(a0[t], axis), comp(a1[t], axis)), comp(a2[t], axis));
+        return (lo + hi) * 0.5;
+    };
+    // bvh.cpp:83-88 sorts with std::sort; a stable sort fixes the order of
+    // equal centroids so the oracle is deterministic.
+    std::stable_sort(idx.begin() + begin, idx.begin() + end, [&](int x, int y) { return centroid(x) < centroid(y); });
+    const int mid = begin + cnt / 2;
+    const int l = bvh2_rec(b, a0, a1, a2, idx, begin, mid);
+    const int r = bvh2_rec(b, a0, a1, a2, idx, mid, end);
+    nd.l = l;
+    nd.r = r;
+    b.nodes[me] = nd;
+    return me;
+}
+
+// boundingBox.h:50-82, except that a flat box (tMax == tMin on an axis)
+// is kept: the reference's `tMax <= tMin` rejects axis-aligned leaves.
+bool bvh2_box(const Bvh2::Node& n, V3 o, V3 d) {
+    float tMin = -1e30f, tMax = 1e30f;
+    for (int i = 0; i < 3; ++i) {
+        const float invD = 1.0f / (float)comp(d, i);
+        float t0 = (float)((n.mn[i] - comp(o, i)) * invD);
+        float t1 = (float)((n.mx[i] - comp(o, i)) * invD);
+        if (invD < 0.0f) std::swap(t0, t1);
+        if (t0 > tMin) tMin = t0;
+        if (t1 < tMax) tMax = t1;
+        if (tMax < tMin) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+// ====================================================== exported API
+extern "C" {
+
+uint32_t orc_pcg32(uint32_t* state) { return pcg32(*state); }
+float orc_pcg32_float(uint32_t* state) { return pcg32_float(*state); }
+uint32_t orc_seed(int px, int py, int frame) { return seed_of(px, py, frame); }
+
+void orc_stream(int px, int py, int frame, int n, uint32_t* out_u32, float* out_f32) {
+    uint32_t s = seed_of(px, py, frame);
+    uint32_t s2 = s;
+    for (int i = 0; i < n; ++i) {
+        if (out_u32) out_u32[i] = pcg32(s);
+        if (out_f32) out_f32[i] = pcg32_float(s2);
+    }
+}
+
+static void norm3(double v[3]) {
+    const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (l > 0.0) { v[0] /= l; v[1] /= l; v[2] /= l; }
+}
+
+// camara.h:29-38: front, up normalised; left = front x up; up = left x front.
+void orc_camera_make(const double pos[3], const double front[3], const double up[3], double fov_deg, int width,
+                     int height, tfg_camara* out) {
+    double f[3] = {front[0], front[1], front[2]}, u[3] = {up[0], up[1], up[2]};
+    norm3(f);
+    norm3(u);
+    double l[3] = {f[1] * u[2] - f[2] * u[1], f[2] * u[0] - f[0] * u[2], f[0] * u[1] - f[1] * u[0]};
+    norm3(l);
+    double u2[3] = {l[1] * f[2] - l[2] * f[1], l[2] * f[0] - l[0] * f[2], l[0] * f[1] - l[1] * f[0]};
+    norm3(u2);
+    // RenderKernel.cu:70-74 (float arithmetic on a double M_PI product)
+    const float focal = 1.0f;
+    const float fov_rad = (float)(fov_deg * M_PI / 180.0);
+    const float alto = (float)(2.0f * focal * std::tan(fov_rad / 2.0f));
+    const float ancho = alto * ((float)width / (float)height);
+    out->posicion = {pos[0], pos[1], pos[2]};
+    out->frente = {f[0], f[1], f[2]};
+    out->arriba = {u2[0], u2[1], u2[2]};
+    out->izquierda = {l[0], l[1], l[2]};
+    out->ancho_plano = ancho;
+    out->alto_plano = alto;
+    out->distancia_focal = focal;
+}
+
+// RenderKernel.cu:68-90: FOV 45, focal 1, pos (0,-0.75,4.25), front (0,0,-1).
+void orc_camera_default(int width, int height, tfg_camara* out) {
+    const double pos[3] = {0.0, -0.75, 4.25}, front[3] = {0.0, 0.0, -1.0}, up[3] = {0.0, 1.0, 0.0};
+    orc_camera_make(pos, front, up, 45.0, width, height, out);
+}
+
+// camara.h:42-59
+void orc_camera_ray(const tfg_camara* c, int x, int y, int w, int h, float ox, float oy, double origin[3],
+                    double dir[3]) {
+    const float u = ((float)x + ox) / (float)w;
+    const float v = ((float)y + oy) / (float)h;
+    const float px = (2.0f * u - 1.0f) * (c->ancho_plano * 0.5f);
+    const float py = (1.0f - 2.0f * v) * (c->alto_plano * 0.5f);
+    const double f = c->distancia_focal;
+    double p[3] = {c->posicion.x + c->frente.x * f + c->izquierda.x * (double)px + c->arriba.x * (double)py,
+                   c->posicion.y + c->frente.y * f + c->izquierda.y * (double)px + c->arriba.y * (double)py,
+                   c->posicion.z + c->frente.z * f + c->izquierda.z * (double)px + c->arriba.z * (double)py};
+    double d[3] = {p[0] - c->posicion.x, p[1] - c->posicion.y, p[2] - c->posicion.z};
+    norm3(d);
+    origin[0] = c->posicion.x;
+    origin[1] = c->posicion.y;
+    origin[2] = c->posicion.z;
+    dir[0] = d[0];
+    dir[1] = d[1];
+    dir[2] = d[2];
+}
+
+void orc_cwbvh_trace(const float* nodes, const float* tris, uint32_t format, const tfg_ray* rays, uint32_t n,
+                     tfg_hit* hits, uint32_t* counts, int nthreads) {
+    parallel_for(n, nthreads, [&](uint32_t i) {
+        const RayPre r = make_ray(rays[i]);
+        traverse<false>(nodes, tris, format, r, hits[i], counts ? counts + 2 * (size_t)i : nullptr);
+    });
+}
+
+void orc_cwbvh_occluded(const float* nodes, const float* tris, uint32_t format, const tfg_ray* rays, uint32_t n,
+                        uint32_t* occluded, uint32_t* counts, int nthreads) {
+    parallel_for(n, nthreads, [&](uint32_t i) {
+        const RayPre r = make_ray(rays[i]);
+        tfg_hit h;
+        occluded[i] = traverse<true>(nodes, tris, format, r, h, counts ? counts + 2 * (size_t)i : nullptr) ? 1u : 0u;
+    });
+}
+
+void orc_brute_trace(const float* verts, uint32_t n_tris, const tfg_ray* rays, uint32_t n, tfg_hit* hits) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const RayPre r = make_ray(rays[i]);
+        float best = r.tmax, bu = 0.f, bv = 0.f;
+        uint32_t bp = 0xFFFFFFFFu;
+        for (uint32_t k = 0; k < n_tris; ++k) {
+            const float* v = verts + 9 * (size_t)k;
+            float t, u, vv;
+            if (!watertight(v, v + 3, v + 6, r, t, u, vv)) continue;
+            if (!(t > r.tmin) || t > best) continue;
+            if (t < best || k < bp) { best = t; bu = u; bv = vv; bp = k; }
+        }
+        hits[i].t = best;
+        hits[i].u = bp == 0xFFFFFFFFu ? 0.f : bu;
+        hits[i].v = bp == 0xFFFFFFFFu ? 0.f : bv;
+        hits[i].prim = bp;
+    }
+}
+
+int orc_tri_watertight(const float v[9], const float o[3], const float d[3], float tmin, float tmax, float* t,
+                       float* u, float* v_out) {
+    tfg_ray ray = {o[0], o[1], o[2], tmin, d[0], d[1], d[2], tmax};
+    const RayPre r = make_ray(ray);
+    float tt, uu, vv;
+    if (!watertight(v, v + 3, v + 6, r, tt, uu, vv)) return 0;
+    if (!(tt > tmin) || tt > tmax) return 0;
+    *t = tt;
+    *u = uu;
+    *v_out = vv;
+    return 1;
+}
+
+void* orc_bvh2_build(const tfg_primitiva* prims, uint32_t n) {
+    auto* b = new Bvh2();
+    std::vector<V3> a0, a1, a2;
+    std::vector<uint32_t> orig;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (prims[i].tipo != TFG_TRIANGULO) continue;
+        const auto& t = prims[i].g.triangulo;
+        a0.push_back({t.v0.x, t.v0.y, t.v0.z});
+        a1.push_back({t.v1.x, t.v1.y, t.v1.z});
+        a2.push_back({t.v2.x, t.v2.y, t.v2.z});
+        orig.push_back(i);
+    }
+    std::vector<int> idx(a0.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+    if (!idx.empty()) bvh2_rec(*b, a0, a1, a2, idx, 0, (int)idx.size());
+    for (int t : idx) {
+        b->v0.push_back(a0[t]);
+        b->v1.push_back(a1[t]);
+        b->v2.push_back(a2[t]);
+        b->prim.push_back(orig[t]);
+    }
+    return b;
+}
+
+void orc_bvh2_free(void* bvh) { delete static_cast<Bvh2*>(bvh); }
+uint32_t orc_bvh2_num_nodes(const void* bvh) { return (uint32_t) static_cast<const Bvh2*>(bvh)->nodes.size(); }
+
+// escenario.h:187-237 stack traversal; closest hit with the smaller
+// original index winning exact ties (the reference keeps the first found).
+void orc_bvh2_trace(const void* bvh, const tfg_ray* rays, uint32_t n, tfg_hit* hits, int nthreads) {
+    const Bvh2& b = *static_cast<const Bvh2*>(bvh);
+    parallel_for(n, nthreads, [&](uint32_t i) {
+        const tfg_ray& ry = rays[i];
+        V3 o = {ry.ox, ry.oy, ry.oz};
+        V3 d = {ry.dx, ry.dy, ry.dz};
+        const double l = std::sqrt(dot(d, d));
+        if (l > 0.0) d = {d.x / l, d.y / l, d.z / l};
+        float best = ry.tmax;
+        uint32_t bp = 0xFFFFFFFFu;
+        const float tlo = std::max(1e-6f, ry.tmin);
+        if (!b.nodes.empty()) {
+            int stack[128];
+            int sp = 0;
+            stack[sp++] = 0;
+            while (sp > 0) {
+                const Bvh2::Node& nd = b.nodes[stack[--sp]];
+                if (!bvh2_box(nd, o, d)) continue;
+                if (nd.leaf) {
+                    for (int k = nd.first; k < nd.end; ++k) {
+                        float t;
+                        if (!moller_trumbore(b.v0[k], b.v1[k], b.v2[k], o, d, t)) continue;
+                        if (!(t > tlo) || t > best) continue;
+                        if (t < best || b.prim[k] < bp) { best = t; bp = b.prim[k]; }
+                    }
+                } else {
+                    stack[sp++] = nd.l;
+                    stack[sp++] = nd.r;
+                }
+            }
+        }
+        hits[i].t = best;
+        hits[i].u = 0.f;
+        hits[i].v = 0.f;
+        hits[i].prim = bp;
+    });
+}
+
+namespace {
+struct Validator {
+    const float* nodes;
+    uint32_t num_nodes;
+    const float* tris;
+    uint32_t num_tris;
+    uint32_t format;
+    const float* verts;
+    std::vector<uint8_t> seen;
+    uint32_t depth = 0;
+    int err = 0;
+
+    // Returns the prim ids below node ni (appended to out).
+    void visit(uint32_t ni, uint32_t d, std::vector<uint32_t>& out) {
+        if (err) return;
+        if (ni >= num_nodes) { err = 1; return; }
+        if (d > 64) { err = 2; return; }
+        depth = std::max(depth, d);
+        const float* n = nodes + 20 * (size_t)ni;
+        const uint32_t w = f2u(n[3]);
+        const int e[3] = {(int8_t)(w & 0xFF), (int8_t)((w >> 8) & 0xFF), (int8_t)((w >> 16) & 0xFF)};
+        const uint32_t imask = w >> 24;
+        const uint32_t child_base = f2u(n[4]);
+        const uint32_t tri_base = f2u(n[5]);
+        if (tri_base % 4 != 0) { err = 3; return; }
+        uint32_t rank = 0;
+        for (int s = 0; s < 8; ++s) {
+            const int h = s >> 2, j = s & 3;
+            const uint32_t meta = byte_of(f2u(n[6 + h]), j);
+            double lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) {
+                const double sc = std::ldexp(1.0, e[a]);
+                lo[a] = (double)n[a] + sc * byte_of(f2u(n[8 + 2 * a + h]), j);
+                hi[a] = (double)n[a] + sc * byte_of(f2u(n[14 + 2 * a + h]), j);
+            }
+            std::vector<uint32_t> below;
+            if (imask & (1u << s)) {
+                if (meta != ((1u << 5) | (24u + s))) { err = 4; return; }
+                visit(child_base + rank, d + 1, below);
+                ++rank;
+            } else if (meta != 0) {
+                const uint32_t off = meta & 0x1F, bits = meta >> 5;
+                if (bits != 1 && bits != 3 && bits != 7) { err = 5; return; }
+                const uint32_t cnt = bits == 1 ? 1 : (bits == 3 ? 2 : 3);
+                if (off + cnt > 24) { err = 6; return; }
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    const uint32_t rec = tri_base / 4 + off + k;
+                    if (rec >= num_tris) { err = 7; return; }
+                    if (seen[rec]) { err = 8; return; }
+                    seen[rec] = 1;
+                    below.push_back(f2u(tris[16 * (size_t)rec + 15]));
+                }
+            }
+            if (err) return;
+            for (uint32_t p : below) {
+                const float* v = verts + 9 * (size_t)p;
+                for (int k = 0; k < 3; ++k)
+                    for (int a = 0; a < 3; ++a)
+                        if ((double)v[3 * k + a] < lo[a] || (double)v[3 * k + a] > hi[a]) { err = 9; return; }
+                out.push_back(p);
+            }
+        }
+        if (rank != (uint32_t)__builtin_popcount(imask)) err = 10;
+    }
+};
+}  // namespace
+
+int orc_cwbvh_validate(const float* nodes, uint32_t num_nodes, const float* tris, uint32_t num_tris, uint32_t format,
+                       const float* verts, uint32_t* depth) {
+    Validator v{nodes, num_nodes, tris, num_tris, format, verts, std::vector<uint8_t>(num_tris, 0)};
+    std::vector<uint32_t> all;
+    v.visit(0, 1, all);
+    if (depth) *depth = v.depth;
+    if (v.err) return v.err;
+    for (uint8_t s : v.seen)
+        if (!s) return 11;
+    std::vector<uint8_t> prim_seen(num_tris, 0);
+    for (uint32_t p : all) {
+        if (p >= num_tris || prim_seen[p]) return 12;
+        prim_seen[p] = 1;
+    }
+    (void)format;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+/*
+ * oracle.h - CPU restatement of the reference path tracer's hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * library in tfg_amd/: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load it.  Product code never links it.
+ *
+ * Each function names the reference file:line it restates.  Pinning (what
+ * checks each function against something other than itself) is listed in
+ * the header of oracle.cpp.
+ */
+#ifndef TFG_ORACLE_H
+#define TFG_ORACLE_H
+
+#include <stdint.h>
+#include "../include/tfg_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* rng.h:12-35 */
+uint32_t orc_pcg32(uint32_t* state);
+float orc_pcg32_float(uint32_t* state);
+uint32_t orc_seed(int px, int py, int frame);
+/* Writes n draws of the stream seeded by orc_seed(px, py, frame). */
+void orc_stream(int px, int py, int frame, int n, uint32_t* out_u32, float* out_f32);
+
+/* RenderKernel.cu:68-90 + camara.h:29-38 (FP64 Vector3d, FP32 plane sizes) */
+void orc_camera_default(int width, int height, tfg_camara* out);
+void orc_camera_make(const double pos[3], const double front[3], const double up[3], double fov_deg,
+                     int width, int height, tfg_camara* out);
+/* camara.h:42-59: primary ray for pixel (x, y) with jitter (ox, oy). */
+void orc_camera_ray(const tfg_camara* cam, int x, int y, int w, int h, float ox, float oy,
+                    double origin[3], double dir[3]);
+
+/* traverse_cwbvh.cuh:51-257 decode + this library's triangle tests.
+ * format 0 = watertight vertex records, 1 = affine (reference) records.
+ * counts (nullable): per ray {nodes decoded, triangles tested}. */
+void orc_cwbvh_trace(const float* nodes, const float* tris, uint32_t format, const tfg_ray* rays,
+                     uint32_t n, tfg_hit* hits, uint32_t* counts, int nthreads);
+void orc_cwbvh_occluded(const float* nodes, const float* tris, uint32_t format, const tfg_ray* rays,
+                        uint32_t n, uint32_t* occluded, uint32_t* counts, int nthreads);
+/* Brute force over all triangles with the same watertight test. verts: 9
+ * floats per triangle, prim id = triangle index. */
+void orc_brute_trace(const float* verts, uint32_t n_tris, const tfg_ray* rays, uint32_t n, tfg_hit* hits);
+/* Single watertight test (exposed for the closed-form unit tests). */
+int orc_tri_watertight(const float v[9], const float o[3], const float d[3], float tmin, float tmax,
+                       float* t, float* u, float* v_out);
+
+/* bvh.cpp:14-115 (median split BVH2, leaf <= 4) over Primitiva triangles,
+ * traversed as escenario.h:187-237 with primitiva.h:164-183 Moller-Trumbore
+ * (FP64 vectors, FP32 scalars).  Ray directions are renormalised in FP64
+ * as Rayo does (rayo.h:22-23).  Returns hit t in FP32, prim = original
+ * index.  Hits must satisfy t > tmin (escenario.h:349 uses EPSILON). */
+void* orc_bvh2_build(const tfg_primitiva* prims, uint32_t n);
+void orc_bvh2_free(void* bvh);
+uint32_t orc_bvh2_num_nodes(const void* bvh);
+void orc_bvh2_trace(const void* bvh, const tfg_ray* rays, uint32_t n, tfg_hit* hits, int nthreads);
+
+/* CWBVH structural check: every child box decoded in FP64 contains every
+ * triangle below it; every triangle record appears exactly once.
+ * Returns 0 when valid, otherwise a nonzero code; *depth gets the max
+ * node depth. */
+int orc_cwbvh_validate(const float* nodes, uint32_t num_nodes, const float* tris, uint32_t num_tris,
+                       uint32_t format, const float* verts, uint32_t* depth);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

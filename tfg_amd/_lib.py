@@ -1,0 +1,159 @@
+"""ctypes binding of libtfg_hip.so (the C ABI declared in include/tfg.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C tfg_amd/csrc``).  There is no fallback: if the library is missing,
+importing anything that needs it raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtfg_hip.so")
+
+TFG_TRI_WATERTIGHT = 0
+TFG_TRI_AFFINE = 1
+TFG_MAX_TREE_DEPTH = 25
+
+
+class TfgError(RuntimeError):
+    pass
+
+
+class CwbvhGpu(C.Structure):
+    """tfg_cwbvh_gpu == TinyBVHD_GPU (tinybvh_wrapper.h:15-21)."""
+
+    _fields_ = [("d_nodos", C.c_void_p), ("d_tris", C.c_void_p), ("num_nodos", C.c_uint32), ("num_tris", C.c_uint32)]
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "tfg_last_error": (C.c_char_p, []),
+    "tfg_version": (C.c_int, []),
+    "tfg_cwbvh_build": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "tfg_cwbvh_host_info": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_double)],
+    ),
+    "tfg_cwbvh_host_nodes": (C.c_void_p, [C.c_void_p]),
+    "tfg_cwbvh_host_tris": (C.c_void_p, [C.c_void_p]),
+    "tfg_cwbvh_host_free": (None, [C.c_void_p]),
+    "tfg_cwbvh_upload": (C.c_int, [C.c_void_p, C.POINTER(CwbvhGpu)]),
+    "tfg_cwbvh_gpu_free": (C.c_int, [C.POINTER(CwbvhGpu)]),
+    "tfg_trace_closest": (
+        C.c_int,
+        [C.POINTER(CwbvhGpu), C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
+    ),
+    "tfg_trace_any": (
+        C.c_int,
+        [C.POINTER(CwbvhGpu), C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
+    ),
+}
+
+
+def lib():
+    """Loads libtfg_hip.so (raises TfgError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TfgError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() (make -C tfg_amd/csrc)")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def declared_symbols():
+    return list(_SIGS)
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = lib().tfg_last_error().decode(errors="replace")
+        raise TfgError(f"{what or 'tfg call'} failed ({rc}): {msg}")
+
+
+def ptr(x) -> int:
+    """Address of a numpy array or a torch tensor."""
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data
+    return x.data_ptr()
+
+
+class CwbvhHost:
+    """Host CWBVH built by tfg_cwbvh_build; nodes/tris exposed as numpy views."""
+
+    def __init__(self, verts: np.ndarray, fmt: int = TFG_TRI_WATERTIGHT):
+        v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 9)
+        self._h = C.c_void_p()
+        check(lib().tfg_cwbvh_build(v.ctypes.data, v.shape[0], fmt, C.byref(self._h)), "tfg_cwbvh_build")
+        self.format = fmt
+        nn, nt, dp = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        sah = C.c_double()
+        check(lib().tfg_cwbvh_host_info(self._h, C.byref(nn), C.byref(nt), C.byref(dp), C.byref(sah)))
+        self.num_nodes, self.num_tris, self.max_depth, self.sah_cost = nn.value, nt.value, dp.value, sah.value
+        n_addr = lib().tfg_cwbvh_host_nodes(self._h)
+        t_addr = lib().tfg_cwbvh_host_tris(self._h)
+        self.nodes = np.ctypeslib.as_array((C.c_float * (20 * self.num_nodes)).from_address(n_addr)).reshape(-1, 20)
+        self.tris = np.ctypeslib.as_array((C.c_float * (16 * self.num_tris)).from_address(t_addr)).reshape(-1, 16)
+
+    def upload(self) -> "DeviceBvh":
+        return DeviceBvh(self)
+
+    def close(self):
+        if self._h:
+            lib().tfg_cwbvh_host_free(self._h)
+            self._h = C.c_void_p()
+            self.nodes = self.tris = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBvh:
+    """Device copy of a CwbvhHost (tfg_cwbvh_upload)."""
+
+    def __init__(self, host: CwbvhHost):
+        self.desc = CwbvhGpu()
+        self.format = host.format
+        check(lib().tfg_cwbvh_upload(host._h, C.byref(self.desc)), "tfg_cwbvh_upload")
+
+    def trace_closest(self, rays, hits, n: int, counts=None, stream=None):
+        check(
+            lib().tfg_trace_closest(
+                C.byref(self.desc), self.format, ptr(rays), ptr(hits), ptr(counts) if counts is not None else None, n,
+                stream,
+            ),
+            "tfg_trace_closest",
+        )
+
+    def trace_any(self, rays, occluded, n: int, counts=None, stream=None):
+        check(
+            lib().tfg_trace_any(
+                C.byref(self.desc), self.format, ptr(rays), ptr(occluded), ptr(counts) if counts is not None else None,
+                n, stream,
+            ),
+            "tfg_trace_any",
+        )
+
+    def close(self):
+        if self.desc.d_nodos:
+            check(lib().tfg_cwbvh_gpu_free(C.byref(self.desc)), "tfg_cwbvh_gpu_free")
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,129 @@
+// api_bvh.cpp - C ABI for CWBVH construction and upload, and the
+// thread-local error message.
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <unordered_map>
+
+#include "cwbvh_build.h"
+#include "host_common.h"
+
+namespace tfg {
+
+static thread_local std::string g_error;
+
+void set_error(const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+}
+
+// Depth of every tree uploaded by this library, keyed by its node buffer,
+// so the traversal entry points can refuse trees deeper than their stack.
+static std::mutex g_depth_mu;
+static std::unordered_map<const void*, uint32_t> g_depth;
+
+uint32_t known_tree_depth(const void* d_nodes) {
+    std::lock_guard<std::mutex> lk(g_depth_mu);
+    auto it = g_depth.find(d_nodes);
+    return it == g_depth.end() ? 0u : it->second;
+}
+
+void register_tree_depth(const void* d_nodes, uint32_t depth) {
+    std::lock_guard<std::mutex> lk(g_depth_mu);
+    if (depth == 0) g_depth.erase(d_nodes);
+    else g_depth[d_nodes] = depth;
+}
+
+}  // namespace tfg
+
+struct tfg_cwbvh_host {
+    tfg::CwbvhHost b;
+};
+
+extern "C" {
+
+const char* tfg_last_error(void) { return tfg::g_error.c_str(); }
+int tfg_version(void) { return 1; }
+
+int tfg_cwbvh_build(const float* verts, uint32_t n_tris, uint32_t format, tfg_cwbvh_host** out) {
+    if (out == nullptr) { tfg::set_error("tfg_cwbvh_build: out is NULL"); return TFG_E_INVALID; }
+    *out = nullptr;
+    if (verts == nullptr || n_tris == 0) { tfg::set_error("tfg_cwbvh_build: no triangles"); return TFG_E_INVALID; }
+    if (format != TFG_TRI_WATERTIGHT && format != TFG_TRI_AFFINE) {
+        tfg::set_error("tfg_cwbvh_build: unknown triangle format %u", format);
+        return TFG_E_INVALID;
+    }
+    auto* h = new (std::nothrow) tfg_cwbvh_host();
+    if (!h) { tfg::set_error("tfg_cwbvh_build: out of host memory"); return TFG_E_NOMEM; }
+    if (!tfg::build_cwbvh(verts, n_tris, format, h->b)) {
+        delete h;
+        tfg::set_error("tfg_cwbvh_build: invalid input (non-finite vertex or too many triangles)");
+        return TFG_E_INVALID;
+    }
+    if (h->b.max_depth > TFG_MAX_TREE_DEPTH) {
+        const uint32_t d = h->b.max_depth;
+        delete h;
+        tfg::set_error("tfg_cwbvh_build: tree depth %u exceeds %d", d, TFG_MAX_TREE_DEPTH);
+        return TFG_E_LIMIT;
+    }
+    *out = h;
+    return TFG_OK;
+}
+
+int tfg_cwbvh_host_info(const tfg_cwbvh_host* h, uint32_t* num_nodes, uint32_t* num_tris, uint32_t* max_depth,
+                        double* sah_cost) {
+    if (!h) { tfg::set_error("tfg_cwbvh_host_info: NULL handle"); return TFG_E_INVALID; }
+    if (num_nodes) *num_nodes = h->b.num_nodes;
+    if (num_tris) *num_tris = h->b.num_tris;
+    if (max_depth) *max_depth = h->b.max_depth;
+    if (sah_cost) *sah_cost = h->b.sah_cost;
+    return TFG_OK;
+}
+
+const float* tfg_cwbvh_host_nodes(const tfg_cwbvh_host* h) { return h ? h->b.nodes.data() : nullptr; }
+const float* tfg_cwbvh_host_tris(const tfg_cwbvh_host* h) { return h ? h->b.tris.data() : nullptr; }
+void tfg_cwbvh_host_free(tfg_cwbvh_host* h) { delete h; }
+
+int tfg_cwbvh_upload(const tfg_cwbvh_host* h, tfg_cwbvh_gpu* out) {
+    if (!h || !out) { tfg::set_error("tfg_cwbvh_upload: NULL argument"); return TFG_E_INVALID; }
+    std::memset(out, 0, sizeof(*out));
+    const size_t nb = h->b.nodes.size() * sizeof(float);
+    const size_t tb = h->b.tris.size() * sizeof(float);
+    void* dn = nullptr;
+    void* dt = nullptr;
+    hipError_t e = hipMalloc(&dn, nb);
+    if (e != hipSuccess) return tfg::hip_fail(e, "hipMalloc(nodes)");
+    e = hipMalloc(&dt, tb);
+    if (e != hipSuccess) { (void)hipFree(dn); return tfg::hip_fail(e, "hipMalloc(tris)"); }
+    e = hipMemcpy(dn, h->b.nodes.data(), nb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dt, h->b.tris.data(), tb, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(dn);
+        (void)hipFree(dt);
+        return tfg::hip_fail(e, "hipMemcpy(cwbvh)");
+    }
+    out->d_nodos = dn;
+    out->d_tris = dt;
+    out->num_nodos = h->b.num_nodes;
+    out->num_tris = h->b.num_tris;
+    tfg::register_tree_depth(dn, h->b.max_depth);
+    return TFG_OK;
+}
+
+int tfg_cwbvh_gpu_free(tfg_cwbvh_gpu* g) {
+    if (!g) { tfg::set_error("tfg_cwbvh_gpu_free: NULL"); return TFG_E_INVALID; }
+    tfg::register_tree_depth(g->d_nodos, 0);
+    hipError_t e1 = g->d_nodos ? hipFree(g->d_nodos) : hipSuccess;
+    hipError_t e2 = g->d_tris ? hipFree(g->d_tris) : hipSuccess;
+    g->d_nodos = g->d_tris = nullptr;
+    g->num_nodos = g->num_tris = 0;
+    if (e1 != hipSuccess) return tfg::hip_fail(e1, "hipFree(nodes)");
+    if (e2 != hipSuccess) return tfg::hip_fail(e2, "hipFree(tris)");
+    return TFG_OK;
+}
+
+}  // extern "C"

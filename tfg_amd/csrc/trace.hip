@@ -1,0 +1,89 @@
+// trace.hip - batched CWBVH closest-hit and any-hit kernels and their C ABI.
+#include <hip/hip_runtime.h>
+
+#include "host_common.h"
+#include "trace_device.h"
+
+namespace tfg {
+uint32_t known_tree_depth(const void* d_nodes);
+
+namespace {
+
+constexpr int kBlock = 128;
+constexpr int kStack = TFG_MAX_TREE_DEPTH - 1;
+
+template <int kFormat, bool kAnyHit, bool kStats>
+__global__ __launch_bounds__(kBlock) void k_trace(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                                  const float4* __restrict__ rays, float4* __restrict__ hits,
+                                                  uint32_t* __restrict__ occluded, uint32_t* __restrict__ counts,
+                                                  uint32_t n) {
+    __shared__ uint2 stack[kStack * kBlock];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = rays[2 * (size_t)i];
+    const float4 b = rays[2 * (size_t)i + 1];
+    const RayPre r = make_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
+    TraceStats st;
+    float t, u, v;
+    uint32_t prim;
+    const bool hit = traverse<kFormat, kAnyHit, kStack, 0>(nodes, tris, nullptr, r, stack + threadIdx.x, kBlock, t, u,
+                                                           v, prim, kStats ? &st : nullptr);
+    if (kAnyHit) {
+        occluded[i] = hit ? 1u : 0u;
+    } else {
+        hits[i] = make_float4(t, u, v, __uint_as_float(prim));
+    }
+    if (kStats) {
+        counts[2 * (size_t)i] = st.nodes | (st.overflow << 31);
+        counts[2 * (size_t)i + 1] = st.tris;
+    }
+}
+
+template <bool kAnyHit>
+int launch(const tfg_cwbvh_gpu* bvh, uint32_t format, const tfg_ray* rays, void* out, uint32_t* counts, uint32_t n,
+           tfg_stream stream, const char* name) {
+    if (!bvh || !rays || !out) { set_error("%s: NULL argument", name); return TFG_E_INVALID; }
+    if (format > 1) { set_error("%s: unknown triangle format %u", name, format); return TFG_E_INVALID; }
+    if (n == 0) return TFG_OK;
+    if (!bvh->d_nodos || !bvh->d_tris || bvh->num_nodos == 0) {
+        set_error("%s: empty CWBVH", name);
+        return TFG_E_INVALID;
+    }
+    const uint32_t depth = known_tree_depth(bvh->d_nodos);
+    if (depth > TFG_MAX_TREE_DEPTH) {
+        set_error("%s: tree depth %u exceeds %d", name, depth, TFG_MAX_TREE_DEPTH);
+        return TFG_E_LIMIT;
+    }
+    const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
+    const float4* nd = static_cast<const float4*>(bvh->d_nodos);
+    const float4* tr = static_cast<const float4*>(bvh->d_tris);
+    const float4* ry = reinterpret_cast<const float4*>(rays);
+    float4* h = kAnyHit ? nullptr : static_cast<float4*>(out);
+    uint32_t* o = kAnyHit ? static_cast<uint32_t*>(out) : nullptr;
+#define TFG_LAUNCH(F, S) hipLaunchKernelGGL((k_trace<F, kAnyHit, S>), grid, block, 0, stream, nd, tr, ry, h, o, counts, n)
+    if (format == 0) {
+        if (counts) TFG_LAUNCH(0, true); else TFG_LAUNCH(0, false);
+    } else {
+        if (counts) TFG_LAUNCH(1, true); else TFG_LAUNCH(1, false);
+    }
+#undef TFG_LAUNCH
+    TFG_HIP_CHECK(hipGetLastError());
+    return TFG_OK;
+}
+
+}  // namespace
+}  // namespace tfg
+
+extern "C" {
+
+int tfg_trace_closest(const tfg_cwbvh_gpu* bvh, uint32_t format, const tfg_ray* d_rays, tfg_hit* d_hits,
+                      uint32_t* d_counts, uint32_t n, tfg_stream stream) {
+    return tfg::launch<false>(bvh, format, d_rays, d_hits, d_counts, n, stream, "tfg_trace_closest");
+}
+
+int tfg_trace_any(const tfg_cwbvh_gpu* bvh, uint32_t format, const tfg_ray* d_rays, uint32_t* d_occluded,
+                  uint32_t* d_counts, uint32_t n, tfg_stream stream) {
+    return tfg::launch<true>(bvh, format, d_rays, d_occluded, d_counts, n, stream, "tfg_trace_any");
+}
+
+}  // extern "C"

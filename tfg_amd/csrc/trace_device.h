@@ -1,0 +1,270 @@
+// trace_device.h - CWBVH closest-hit / any-hit traversal for gfx950.
+//
+// Decodes the 80 B node format of traverse_cwbvh.cuh:75-185 with CDNA4
+// integer ops (v_ffbh_u32 for the highest set bit, v_bcnt_u32_b32 for the
+// child rank, v_cvt_f32_ubyteN for the quantised bounds) and intersects the
+// 64 B triangle records with the FP32 watertight test of Woop, Benthin and
+// Wald (JCGT 2013) or, for the affine record format, the unit-triangle test
+// of traverse_cwbvh.cuh:189-209.
+//
+// Every FP32 operation is written out in the order the CPU oracle uses and
+// the translation unit is compiled with -ffp-contract=off, so hit ids, t, u
+// and v are bit-identical to the oracle on the same rays.
+//
+// Result semantics: closest hit with tmin < t <= tmax; equal t resolved to
+// the smaller primitive index (so the result does not depend on traversal
+// order); any-hit reports a hit with tmin < t < tmax.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tfg {
+
+constexpr float kBoxSlack = 1.00000095367431640625f;   // 1 + 2^-20
+
+struct RayPre {
+    float O[3], D[3], rD[3];
+    float tmin, tmax;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+    uint32_t octinv;
+};
+
+__device__ __forceinline__ float safe_rcp(float d) {
+    if (fabsf(d) < 1e-30f) d = copysignf(1e-30f, d);
+    return 1.0f / d;
+}
+
+__device__ __forceinline__ float sel3(const float* v, int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
+
+__device__ __forceinline__ RayPre make_ray(float ox, float oy, float oz, float dx, float dy, float dz, float tmin,
+                                           float tmax) {
+    RayPre p;
+    p.O[0] = ox; p.O[1] = oy; p.O[2] = oz;
+    p.D[0] = dx; p.D[1] = dy; p.D[2] = dz;
+    p.rD[0] = safe_rcp(dx);
+    p.rD[1] = safe_rcp(dy);
+    p.rD[2] = safe_rcp(dz);
+    p.tmin = tmin;
+    p.tmax = tmax;
+    const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    int kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    const float dkz = sel3(p.D, kz);
+    if (dkz < 0.0f) { const int t = kx; kx = ky; ky = t; }
+    p.kx = kx; p.ky = ky; p.kz = kz;
+    p.Sx = sel3(p.D, kx) / dkz;
+    p.Sy = sel3(p.D, ky) / dkz;
+    p.Sz = 1.0f / dkz;
+    p.octinv = 7u - ((p.rD[0] < 0.0f ? 4u : 0u) | (p.rD[1] < 0.0f ? 2u : 0u) | (p.rD[2] < 0.0f ? 1u : 0u));
+    return p;
+}
+
+// Watertight test.  Vertices arrive as float4 (w ignored).
+__device__ __forceinline__ bool watertight(const float4 a, const float4 b, const float4 c, const RayPre& r,
+                                           float& t, float& u, float& v) {
+    const float A[3] = {a.x - r.O[0], a.y - r.O[1], a.z - r.O[2]};
+    const float B[3] = {b.x - r.O[0], b.y - r.O[1], b.z - r.O[2]};
+    const float C[3] = {c.x - r.O[0], c.y - r.O[1], c.z - r.O[2]};
+    const float Akz = sel3(A, r.kz), Bkz = sel3(B, r.kz), Ckz = sel3(C, r.kz);
+    const float Ax = sel3(A, r.kx) - r.Sx * Akz;
+    const float Ay = sel3(A, r.ky) - r.Sy * Akz;
+    const float Bx = sel3(B, r.kx) - r.Sx * Bkz;
+    const float By = sel3(B, r.ky) - r.Sy * Bkz;
+    const float Cx = sel3(C, r.kx) - r.Sx * Ckz;
+    const float Cy = sel3(C, r.ky) - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = r.Sz * Akz;
+    const float Bz = r.Sz * Bkz;
+    const float Cz = r.Sz * Ckz;
+    const float T = U * Az + V * Bz + W * Cz;
+    t = T / det;
+    u = V / det;
+    v = W / det;
+    return true;
+}
+
+// Affine unit-triangle test (traverse_cwbvh.cuh:189-209), T2 first so a
+// plane miss costs one 16 B load.
+__device__ __forceinline__ bool affine(const float4* __restrict__ tri, const RayPre& r, float tcur, float& t,
+                                       float& u, float& v) {
+    const float4 T2 = tri[2];
+    const float transS = T2.x * r.O[0] + T2.y * r.O[1] + T2.z * r.O[2] + T2.w;
+    const float transD = T2.x * r.D[0] + T2.y * r.D[1] + T2.z * r.D[2];
+    const float d = -transS / transD;
+    if (!(d > r.tmin) || d > tcur) return false;
+    const float4 T0 = tri[0];
+    const float4 T1 = tri[1];
+    const float Ix = r.O[0] + d * r.D[0];
+    const float Iy = r.O[1] + d * r.D[1];
+    const float Iz = r.O[2] + d * r.D[2];
+    const float uu = T0.x * Ix + T0.y * Iy + T0.z * Iz + T0.w;
+    const float vv = T1.x * Ix + T1.y * Iy + T1.z * Iz + T1.w;
+    if (uu >= 0.0f && vv >= 0.0f && uu + vv < 1.0f) {
+        t = d;
+        u = uu;
+        v = vv;
+        return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ float ubyte_f(uint32_t w, int j) { return (float)((w >> (8 * j)) & 0xFFu); }
+
+// One child box test, written like the oracle's node_hitmask loop body.
+__device__ __forceinline__ uint32_t child_bits(uint32_t meta, float qlx, float qly, float qlz, float qhx, float qhy,
+                                               float qhz, float idirx, float idiry, float idirz, float origx,
+                                               float origy, float origz, const RayPre& r, float tcur) {
+    const float lox = r.rD[0] < 0.0f ? qhx : qlx, hix = r.rD[0] < 0.0f ? qlx : qhx;
+    const float loy = r.rD[1] < 0.0f ? qhy : qly, hiy = r.rD[1] < 0.0f ? qly : qhy;
+    const float loz = r.rD[2] < 0.0f ? qhz : qlz, hiz = r.rD[2] < 0.0f ? qlz : qhz;
+    const float tminx = fmaf(lox, idirx, origx), tmaxx = fmaf(hix, idirx, origx);
+    const float tminy = fmaf(loy, idiry, origy), tmaxy = fmaf(hiy, idiry, origy);
+    const float tminz = fmaf(loz, idirz, origz), tmaxz = fmaf(hiz, idirz, origz);
+    const float tnear = fmaxf(fmaxf(fmaxf(tminx, tminy), tminz), r.tmin);
+    const float tfar = fminf(fminf(fminf(tmaxx, tmaxy), tmaxz), tcur);
+    if (!(tnear <= tfar * kBoxSlack)) return 0u;
+    const uint32_t is_inner = (meta & (meta << 1)) & 0x10u;
+    const uint32_t bit_index = (is_inner ? (meta ^ r.octinv) : meta) & 0x1Fu;
+    return ((meta >> 5) & 7u) << bit_index;
+}
+
+// Decodes node (n0..n4) and returns the 32-bit hit mask.
+__device__ __forceinline__ uint32_t node_hitmask(const float4 n0, const float4 n1, const float4 n2, const float4 n3,
+                                                 const float4 n4, const RayPre& r, float tcur) {
+    const uint32_t w = __float_as_uint(n0.w);
+    const int ex = (int)(int8_t)(w & 0xFFu), ey = (int)(int8_t)((w >> 8) & 0xFFu), ez = (int)(int8_t)((w >> 16) & 0xFFu);
+    const float idirx = __uint_as_float((uint32_t)(ex + 127) << 23) * r.rD[0];
+    const float idiry = __uint_as_float((uint32_t)(ey + 127) << 23) * r.rD[1];
+    const float idirz = __uint_as_float((uint32_t)(ez + 127) << 23) * r.rD[2];
+    const float origx = (n0.x - r.O[0]) * r.rD[0];
+    const float origy = (n0.y - r.O[1]) * r.rD[1];
+    const float origz = (n0.z - r.O[2]) * r.rD[2];
+    const uint32_t m[2] = {__float_as_uint(n1.z), __float_as_uint(n1.w)};
+    const uint32_t lx[2] = {__float_as_uint(n2.x), __float_as_uint(n2.y)};
+    const uint32_t ly[2] = {__float_as_uint(n2.z), __float_as_uint(n2.w)};
+    const uint32_t lz[2] = {__float_as_uint(n3.x), __float_as_uint(n3.y)};
+    const uint32_t hx[2] = {__float_as_uint(n3.z), __float_as_uint(n3.w)};
+    const uint32_t hy[2] = {__float_as_uint(n4.x), __float_as_uint(n4.y)};
+    const uint32_t hz[2] = {__float_as_uint(n4.z), __float_as_uint(n4.w)};
+    uint32_t hitmask = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t meta = (m[h] >> (8 * j)) & 0xFFu;
+            hitmask |= child_bits(meta, ubyte_f(lx[h], j), ubyte_f(ly[h], j), ubyte_f(lz[h], j), ubyte_f(hx[h], j),
+                                  ubyte_f(hy[h], j), ubyte_f(hz[h], j), idirx, idiry, idirz, origx, origy, origz, r,
+                                  tcur);
+        }
+    }
+    return hitmask;
+}
+
+__device__ __forceinline__ int highest_bit(uint32_t x) { return 31 - __clz((int)x); }
+
+struct TraceStats {
+    uint32_t nodes = 0, tris = 0, overflow = 0;
+};
+
+// Per-lane LDS stack: entry k of this lane lives at base[k * stride].
+template <int kFormat, bool kAnyHit, int kStackDepth, int kTopNodes>
+__device__ __forceinline__ bool traverse(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                         const float4* __restrict__ top_nodes, const RayPre& r, uint2* stack,
+                                         int stride, float& t_out, float& u_out, float& v_out, uint32_t& prim_out,
+                                         TraceStats* stats) {
+    int sp = 0;
+    uint32_t ng_x = 0, ng_y = 0x80000000u;
+    float tcur = r.tmax;
+    uint32_t best_prim = 0xFFFFFFFFu;
+    float bu = 0.f, bv = 0.f;
+    bool found = false;
+    for (;;) {
+        uint32_t tg_x = 0, tg_y = 0;
+        if (ng_y > 0x00FFFFFFu) {
+            const uint32_t imask = ng_y;
+            const int bit = highest_bit(ng_y & 0xFF000000u);
+            ng_y &= ~(1u << bit);
+            if (ng_y > 0x00FFFFFFu) {
+                if (sp < kStackDepth) {
+                    stack[sp * stride] = make_uint2(ng_x, ng_y);
+                    ++sp;
+                } else if (stats) {
+                    stats->overflow = 1;   // host refuses trees this deep
+                }
+            }
+            const uint32_t slot = ((uint32_t)bit - 24u) ^ r.octinv;
+            const uint32_t rel = (uint32_t)__popc(imask & ~(0xFFFFFFFFu << slot));
+            const uint32_t ni = ng_x + rel;
+            float4 n0, n1, n2, n3, n4;
+            if (kTopNodes > 0 && ni < (uint32_t)kTopNodes) {
+                const float4* n = top_nodes + 5 * ni;
+                n0 = n[0]; n1 = n[1]; n2 = n[2]; n3 = n[3]; n4 = n[4];
+            } else {
+                const float4* n = nodes + 5 * (size_t)ni;
+                n0 = n[0]; n1 = n[1]; n2 = n[2]; n3 = n[3]; n4 = n[4];
+            }
+            if (stats) stats->nodes++;
+            const uint32_t hm = node_hitmask(n0, n1, n2, n3, n4, r, tcur);
+            ng_x = __float_as_uint(n1.x);
+            ng_y = (hm & 0xFF000000u) | (__float_as_uint(n0.w) >> 24);
+            tg_x = __float_as_uint(n1.y);
+            tg_y = hm & 0x00FFFFFFu;
+        }
+        while (tg_y != 0) {
+            const int b = highest_bit(tg_y);
+            tg_y &= ~(1u << b);
+            const float4* T = tris + (size_t)(tg_x + 4u * (uint32_t)b);
+            if (stats) stats->tris++;
+            float t, u, v;
+            bool ok;
+            if (kFormat == 0) {
+                ok = watertight(T[0], T[1], T[2], r, t, u, v) && t > r.tmin && t <= tcur;
+            } else {
+                ok = affine(T, r, tcur, t, u, v);
+            }
+            if (!ok) continue;
+            const uint32_t prim = __float_as_uint(T[3].w);
+            if (kAnyHit) {
+                if (t < r.tmax) return true;
+                continue;
+            }
+            if (t < tcur || prim < best_prim) {
+                tcur = t;
+                bu = u;
+                bv = v;
+                best_prim = prim;
+                found = true;
+            }
+        }
+        if (ng_y <= 0x00FFFFFFu) {
+            if (sp > 0) {
+                --sp;
+                const uint2 e = stack[sp * stride];
+                ng_x = e.x;
+                ng_y = e.y;
+            } else {
+                break;
+            }
+        }
+    }
+    if (kAnyHit) return false;
+    t_out = tcur;
+    u_out = found ? bu : 0.f;
+    v_out = found ? bv : 0.f;
+    prim_out = found ? best_prim : 0xFFFFFFFFu;
+    return found;
+}
+
+}  // namespace tfg
