@@ -145,6 +145,106 @@ def bench_traversal(args, torch):
     return res
 
 
+def bench_pt(args, torch):
+    """Plain FP32 wavefront path tracer, 64-instance scene, 1920x1080, 1 spp per
+    frame, max depth 8, Russian roulette after depth 3; hipGraph-captured
+    frames with autotuned traversal launch shapes.  One step = one frame."""
+    from tfg_amd import pt, scenes
+
+    t0 = time.time()
+    sc = scenes.instances()
+    ds = pt.DeviceScene(sc, torch)
+    _log(f"scene {sc.n_tris} tris, {ds.host.num_nodes} nodes, depth {ds.host.max_depth} ({time.time() - t0:.1f}s)")
+    W, H, MD, RR = 1920, 1080, 8, 3
+    tracer = pt.PathTracer(W, H, MD, RR, mode=0, use_graph=True)
+    tracer.set_scene(ds)
+    tracer.set_camera(sc.camera.frame(W, H))
+    shape = tracer.autotune()
+    _log(f"autotuned launch shape {shape}")
+    img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
+    accum = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
+    for f in range(args.warmup):
+        tracer.render(f, img, accum)
+    torch.cuda.synchronize()
+    tracer.reset_totals()
+    stream = torch.cuda.current_stream()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    a.record(stream)
+    for k in range(args.steps):
+        tracer.render(1000 + k, img, accum, stream=stream.cuda_stream)
+    b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    ms_frame = a.elapsed_time(b) / args.steps
+    ext, shd = tracer.ray_totals()
+    rays_frame = (ext + shd) / args.steps
+    value = rays_frame / (ms_frame * 1e-3) / 1e6
+    prof = tracer.profile_frame(1000, img)
+    ms_ext = sum(prof["ms_extend"])
+    n_ext = sum(prof["ext_rays"])
+    ext_bytes = 48 * n_ext + 80 * prof["ext_nodes"] + 48 * prof["ext_tris"]
+    achieved = ext_bytes / (ms_ext * 1e-3) / 1e9
+    ms_con = sum(prof["ms_connect"])
+    n_shd = sum(prof["shadow_rays"])
+    con_bytes = 48 * n_shd + 80 * prof["shadow_nodes"] + 48 * prof["shadow_tris"]
+    res = {
+        "value": value,
+        "ms_per_step": ms_frame,
+        "per_kernel_ms": {"generate": prof["ms_generate"], "extend": ms_ext, "shade": sum(prof["ms_shade"]),
+                          "connect": ms_con, "finalize": prof["ms_finalize"], "profiled_frame": prof["ms_total"]},
+        "mrays_per_kernel": {"extend": n_ext / (ms_ext * 1e-3) / 1e6, "connect": n_shd / (ms_con * 1e-3) / 1e6},
+        "roofline": {
+            "kernel": "k_extend (closest-hit stage, all depths of one frame)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes": int(ext_bytes),
+            "connect_achieved_gbs": con_bytes / (ms_con * 1e-3) / 1e9,
+        },
+        "config": {
+            "workload": "plain FP32 wavefront PT: 64 seeded instances (32 displaced spheres + 32 boxes, "
+                        f"{sc.n_tris} tris) + ground, 4 point lights, Lambertian+GGX; 1920x1080, 1 spp/frame, "
+                        "max depth 8, RR after depth 3, hipGraph frame",
+            "tris": sc.n_tris,
+            "cwbvh_nodes": ds.host.num_nodes,
+            "rays_per_frame": rays_frame,
+            "ext_rays_per_depth": prof["ext_rays"],
+            "shadow_rays_per_depth": prof["shadow_rays"],
+            "launch": shape,
+            "seeds": {"scene": sc.seed, "frames": [1000, 1000 + args.steps - 1]},
+        },
+        "wall_s": wall,
+        "dtype": "fp32",
+        "_cpu_inputs": (sc, ds.host, W, H, MD, RR),
+    }
+    return res
+
+
+def cpu_baseline_pt(sc, host, W, H, MD, RR):
+    import oracle
+
+    cores = os.cpu_count() or 1
+    R = oracle.Renderer(sc, host.nodes, host.tris, W, H)
+    y0, y1 = 520, 560
+    t0 = time.perf_counter()
+    _, rays = R.render_rows(y0, y1, 1000, 1, MD, RR, nthreads=cores)
+    dt = time.perf_counter() - t0
+    return {
+        "value": float(rays.sum()) / dt / 1e6,
+        "unit": "Mrays/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"rows {y0}-{y1} of the same 1080p frame (1 spp, {int(rays[0])} extension + {int(rays[1])} "
+                  f"shadow rays) through the oracle integrator on {cores} threads, {dt:.2f} s",
+    }
+
+
 def cpu_baseline_traversal(host, prim, bounce, shadow):
     import oracle
 
@@ -176,7 +276,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="traversal", choices=["traversal"])
+    ap.add_argument("--workload", default="traversal", choices=["traversal", "pt"])
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -188,9 +288,14 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
-    res = bench_traversal(args, torch)
-    cpu_in = res.pop("_cpu_inputs")
-    cpu = None if args.no_cpu else cpu_baseline_traversal(*cpu_in)
+    if args.workload == "pt":
+        res = bench_pt(args, torch)
+        cpu_in = res.pop("_cpu_inputs")
+        cpu = None if args.no_cpu else cpu_baseline_pt(*cpu_in)
+    else:
+        res = bench_traversal(args, torch)
+        cpu_in = res.pop("_cpu_inputs")
+        cpu = None if args.no_cpu else cpu_baseline_traversal(*cpu_in)
     out = {
         "metric": METRIC,
         "value": round(res["value"], 3),

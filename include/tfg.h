@@ -71,6 +71,67 @@ int tfg_trace_closest(const tfg_cwbvh_gpu* bvh, uint32_t format, const tfg_ray* 
 int tfg_trace_any(const tfg_cwbvh_gpu* bvh, uint32_t format, const tfg_ray* d_rays, uint32_t* d_occluded,
                   uint32_t* d_counts, uint32_t n, tfg_stream stream);
 
+
+/* ---- wavefront path tracer ------------------------------------------------
+ * Replaces kernelRender_tiny (RenderKernel.cu:28-47, render.h:416-466): one
+ * path per pixel per frame as max_depth rounds of extend / shade / connect
+ * over compacted device queues, optionally captured in a hipGraph.
+ * Pixel seeds follow inicializarSemilla(x, y, frame) (rng.h:25-35). */
+typedef struct tfg_pt tfg_pt;
+
+enum {
+    TFG_PT_INFER = 1,   /* usar_red_inferencia: capture cache queries */
+    TFG_PT_TRAIN = 2,   /* entrenar_red: training paths and records  */
+};
+
+typedef struct tfg_pt_config {
+    uint32_t width, height;
+    uint32_t max_depth;      /* bounces, <= 32 (render.h:63 uses 20)        */
+    uint32_t rr_depth;       /* Russian roulette when depth > rr_depth (render.h:210 uses 10) */
+    uint32_t mode;           /* TFG_PT_INFER | TFG_PT_TRAIN; 0 = plain path tracing */
+    float train_fraction;    /* training-pixel probability (render.h:443 uses 0.05) */
+    uint32_t use_graph;      /* 1: capture the frame into a hipGraph and replay it */
+} tfg_pt_config;
+
+typedef struct tfg_pt_scene {
+    tfg_cwbvh_gpu bvh;                  /* watertight triangle records */
+    const tfg_tri_shading* d_shading;   /* per primIdx */
+    const tfg_material* d_materials;
+    const tfg_point_light* d_lights;
+    uint32_t n_lights;
+} tfg_pt_scene;
+
+#define TFG_PT_NCOUNTERS 72
+int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out);
+int tfg_pt_set_scene(tfg_pt* pt, const tfg_pt_scene* scene);
+int tfg_pt_set_camera(tfg_pt* pt, const tfg_camera* cam);
+/* Renders frame `frame` (1 sample per pixel): writes d_image[W*H] and, if
+ * d_accum is not NULL, adds the frame to d_accum[3*W*H] (FP64). */
+int tfg_pt_render(tfg_pt* pt, int32_t frame, tfg_color* d_image, double* d_accum, tfg_stream stream);
+/* Copies the device counters of the last frame to host (synchronises):
+ * [8 + d] extension rays at depth d, [40 + d] shadow rays at depth d,
+ * [3] training records, [4] cache queries. */
+int tfg_pt_counters(tfg_pt* pt, uint32_t* out);
+/* Extension and shadow rays traced since creation / the last reset,
+ * including the last frame (synchronises). */
+int tfg_pt_ray_totals(tfg_pt* pt, uint64_t* out2);
+int tfg_pt_reset_totals(tfg_pt* pt);
+/* One frame launched stage by stage (no graph) with HIP events around every
+ * launch and node/triangle visit counters in the traversal stages: for the
+ * roofline of the path tracer's kernels.  Synchronises. */
+typedef struct tfg_pt_profile {
+    float ms_generate, ms_finalize, ms_total;
+    float ms_extend[32], ms_shade[32], ms_connect[32];
+    uint32_t ext_rays[32], shadow_rays[32];
+    uint64_t ext_nodes, ext_tris, shadow_nodes, shadow_tris;
+} tfg_pt_profile;
+int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_profile* out);
+/* Times candidate launch shapes of the traversal stages on the current scene
+ * and keeps the fastest (drops any captured graph). */
+int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream);
+int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_grid, uint32_t* shade_grid);
+int tfg_pt_destroy(tfg_pt* pt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,7 +4,8 @@
  * structs so that buffers filled by a reference caller can be handed over
  * unchanged; the second group is this library's own compact device layout.
  *
- * Reference layouts (sizes confirmed by static_asserts in tfg_layout_check.cpp):
+ * Reference layouts (sizes and offsets fixed by static_asserts in
+ * tfg_amd/csrc/layout_check.cpp):
  *   Vector3d              vector3d.h:14-19        24 B (double x,y,z)
  *   Color                 color.h:13-15           12 B (float r,g,b)
  *   Primitiva             primitiva.h:17-39      200 B (tagged union)
@@ -102,7 +103,7 @@ typedef struct tfg_hit {
     uint32_t prim;
 } tfg_hit;
 
-/* Material table entry (48 B).  kd/ks/kt follow Primitiva's difuso /
+/* Material table entry (64 B).  kd/ks/kt follow Primitiva's difuso /
  * especular / transmision; roughness > 0 turns the specular lobe into GGX,
  * roughness == 0 keeps the reference's ideal mirror. */
 typedef struct tfg_material {
@@ -112,12 +113,15 @@ typedef struct tfg_material {
     float le[3]; float pad1;
 } tfg_material;
 
-/* Per-triangle shading record (48 B): vertex normals (0 vectors = use the
- * geometric normal) and material index.  Indexed by primIdx. */
+/* Per-triangle shading record (64 B), indexed by primIdx: vertex normals
+ * (used when flags bit 0 is set, as escenario.h:126-134 does for triangles
+ * whose vertex normals differ), material index and the unnormalised
+ * geometric normal (v1 - v0) x (v2 - v0). */
 typedef struct tfg_tri_shading {
     float n0[3]; uint32_t material;
     float n1[3]; uint32_t flags;
-    float n2[3]; uint32_t pad;
+    float n2[3]; uint32_t pad0;
+    float ng[3]; uint32_t pad1;
 } tfg_tri_shading;
 
 typedef struct tfg_point_light {

@@ -46,6 +46,12 @@ _SIGS = {
     "orc_bvh2_trace": (None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int]),
     "orc_cwbvh_validate": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
                                      C.c_void_p]),
+    "orc_render": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                          C.c_void_p, C.c_int]),
+    "orc_render_rows": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_void_p, C.c_void_p, C.c_int]),
+    "orc_render_pixel": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                C.c_void_p]),
 }
 
 
@@ -150,3 +156,42 @@ class Bvh2:
             lib().orc_bvh2_free(self._h)
         except Exception:
             pass
+
+
+class OrcScene(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("tris", C.c_void_p), ("shading", C.c_void_p), ("materials", C.c_void_p),
+                ("lights", C.c_void_p), ("n_lights", C.c_uint32), ("cam", C.c_void_p)]
+
+
+class Renderer:
+    """CPU restatement of the integrator (render.h) over a host CWBVH."""
+
+    def __init__(self, scene, nodes, tris, width, height):
+        self._keep = [np.ascontiguousarray(nodes, np.float32), np.ascontiguousarray(tris, np.float32),
+                      np.ascontiguousarray(scene.shading_records()),
+                      np.ascontiguousarray(scene.materials, np.float32),
+                      np.ascontiguousarray(scene.lights, np.float32),
+                      np.ascontiguousarray(scene.camera.frame(width, height), np.float32)]
+        k = self._keep
+        self.desc = OrcScene(_p(k[0]), _p(k[1]), _p(k[2]), _p(k[3]), _p(k[4]), scene.lights.shape[0], _p(k[5]))
+        self.width, self.height = width, height
+
+    def render(self, frame0, spp, max_depth, rr_depth, nthreads=os.cpu_count() or 1):
+        acc = np.zeros((self.height * self.width, 3), np.float64)
+        rays = np.zeros(2, np.uint64)
+        lib().orc_render(C.byref(self.desc), self.width, self.height, frame0, spp, max_depth, rr_depth, _p(acc),
+                         _p(rays), nthreads)
+        return acc, rays
+
+    def render_rows(self, y0, y1, frame0, spp, max_depth, rr_depth, nthreads=os.cpu_count() or 1):
+        acc = np.zeros(((y1 - y0) * self.width, 3), np.float64)
+        rays = np.zeros(2, np.uint64)
+        lib().orc_render_rows(C.byref(self.desc), self.width, self.height, y0, y1, frame0, spp, max_depth, rr_depth,
+                              _p(acc), _p(rays), nthreads)
+        return acc, rays
+
+    def pixel(self, x, y, frame, max_depth, rr_depth):
+        out = np.zeros(3, np.float32)
+        lib().orc_render_pixel(C.byref(self.desc), self.width, self.height, x, y, frame, max_depth, rr_depth,
+                               _p(out))
+        return out

@@ -24,6 +24,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <thread>
 #include <vector>
 
@@ -663,6 +664,251 @@ int orc_cwbvh_validate(const float* nodes, uint32_t num_nodes, const float* tris
     }
     (void)format;
     return 0;
+}
+
+}  // extern "C"
+
+// =================================================== integrator (render.h)
+namespace {
+
+struct D3 { double x, y, z; };
+inline D3 d3(double x, double y, double z) { return {x, y, z}; }
+inline D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline D3 operator*(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline D3 operator-(D3 a) { return {-a.x, -a.y, -a.z}; }
+inline double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline D3 dcross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline D3 dnorm(D3 a) {   // vector3d.h:65-71
+    const double l = std::sqrt(ddot(a, a));
+    return l > 0.0 ? a * (1.0 / l) : a;
+}
+inline D3 dld(const float* p) { return {p[0], p[1], p[2]}; }
+
+struct Col { float r, g, b; };   // color.h
+inline Col operator+(Col a, Col b) { return {a.r + b.r, a.g + b.g, a.b + b.b}; }
+inline Col operator*(Col a, Col b) { return {a.r * b.r, a.g * b.g, a.b * b.b}; }
+inline Col operator*(Col a, float s) { return {a.r * s, a.g * s, a.b * s}; }
+inline Col cdiv(Col a, float s) { return s == 0.0f ? Col{0, 0, 0} : Col{a.r / s, a.g / s, a.b / s}; }   // color.h:42-48
+inline float cmax(Col c) { float m = c.r >= c.g ? c.r : c.g; return c.b > m ? c.b : m; }   // color.h:58-70
+inline Col cld(const float* p) { return {p[0], p[1], p[2]}; }
+
+const double kPiD = 3.14159265358979323846;
+const float kEpsF = 1e-3f;
+
+struct OMat {
+    Col kd, ks, kt, le;
+    float rough, ior;
+};
+OMat omat(const tfg_material& m) { return {cld(m.kd), cld(m.ks), cld(m.kt), cld(m.le), m.roughness, m.ior}; }
+bool o_diffuse(const OMat& m) { return cmax(m.kd) > kEpsF; }
+bool o_glossy(const OMat& m) { return cmax(m.ks) > kEpsF && m.rough > 0.0f; }
+
+struct OFrame { D3 t, b, n; };
+OFrame oframe(D3 n) {   // render.h:396-413
+    D3 t = std::fabs(n.x) > std::fabs(n.y) ? d3(n.z, 0, -n.x) : d3(0, -n.z, n.y);
+    t = dnorm(t);
+    D3 b = dnorm(dcross(n, t));
+    return {t, b, n};
+}
+D3 o_world(const OFrame& f, D3 l) { return f.t * l.x + f.b * l.y + f.n * l.z; }
+D3 o_local(const OFrame& f, D3 w) { return {ddot(w, f.t), ddot(w, f.b), ddot(w, f.n)}; }
+
+double ggx_alpha_d(float r) { return std::max((double)r * r, 1e-3); }
+double ggx_D_d(double a2, double nh) { const double d = nh * nh * (a2 - 1.0) + 1.0; return a2 / (kPiD * d * d); }
+double ggx_G1_d(double a2, double nv) { nv = std::max(nv, 0.0); return 2.0 * nv / (nv + std::sqrt(a2 + (1.0 - a2) * nv * nv)); }
+Col schlick_d(Col f0, double c) {
+    const double m = 1.0 - std::min(std::max(c, 0.0), 1.0);
+    const float m5 = (float)(m * m * m * m * m);
+    return {f0.r + (1.0f - f0.r) * m5, f0.g + (1.0f - f0.g) * m5, f0.b + (1.0f - f0.b) * m5};
+}
+Col ggx_eval_d(Col ks, double alpha, D3 wo, D3 wi) {
+    if (wo.z <= 0.0 || wi.z <= 0.0) return {0, 0, 0};
+    const D3 h = dnorm(wo + wi);
+    const double a2 = alpha * alpha;
+    const double v = ggx_D_d(a2, h.z) * ggx_G1_d(a2, wo.z) * ggx_G1_d(a2, wi.z) / (4.0 * wo.z * wi.z);
+    return schlick_d(ks, ddot(wi, h)) * (float)v;
+}
+D3 ggx_vndf_d(D3 wo, double alpha, double u1, double u2) {   // Heitz 2018
+    const D3 vh = dnorm(d3(alpha * wo.x, alpha * wo.y, wo.z));
+    const double lensq = vh.x * vh.x + vh.y * vh.y;
+    const D3 t1 = lensq > 0.0 ? d3(-vh.y, vh.x, 0.0) * (1.0 / std::sqrt(lensq)) : d3(1, 0, 0);
+    const D3 t2 = dcross(vh, t1);
+    const double r = std::sqrt(u1), phi = 2.0 * kPiD * u2;
+    const double p1 = r * std::cos(phi);
+    const double s = 0.5 * (1.0 + vh.z);
+    const double p2 = (1.0 - s) * std::sqrt(std::max(0.0, 1.0 - p1 * p1)) + s * r * std::sin(phi);
+    const D3 nh = t1 * p1 + t2 * p2 + vh * std::sqrt(std::max(0.0, 1.0 - p1 * p1 - p2 * p2));
+    return dnorm(d3(alpha * nh.x, alpha * nh.y, std::max(0.0, nh.z)));
+}
+
+bool o_closest(const orc_scene* sc, D3 o, D3 d, tfg_hit& h) {
+    tfg_ray r = {(float)o.x, (float)o.y, (float)o.z, 0.0f, (float)d.x, (float)d.y, (float)d.z,
+                 std::numeric_limits<float>::infinity()};
+    const RayPre rp = make_ray(r);
+    return traverse<false>(sc->nodes, sc->tris, 0, rp, h, nullptr);
+}
+bool o_occluded(const orc_scene* sc, D3 o, D3 d, float tmax) {
+    tfg_ray r = {(float)o.x, (float)o.y, (float)o.z, 0.0f, (float)d.x, (float)d.y, (float)d.z, tmax};
+    const RayPre rp = make_ray(r);
+    tfg_hit h;
+    return traverse<true>(sc->nodes, sc->tris, 0, rp, h, nullptr);
+}
+
+// escenario.h:144-184 with the mesh occluding (the reference's CWBVH mode
+// never tests shadow rays against the mesh, escenario.h:168,241-244).
+Col o_direct(const orc_scene* sc, D3 ps, D3 n, D3 wo, const OMat& m, uint64_t& nshadow) {
+    Col L = {0, 0, 0};
+    for (uint32_t i = 0; i < sc->n_lights; ++i) {
+        const tfg_point_light& lt = sc->lights[i];
+        D3 dl = dld(lt.pos) - ps;
+        float dist = (float)std::sqrt(ddot(dl, dl));
+        dl = dnorm(dl);
+        const float c = (float)ddot(n, dl);
+        if (c <= 0.0f) continue;
+        ++nshadow;
+        if (o_occluded(sc, ps + dl * kEpsF, dl, dist)) continue;
+        Col f = cdiv(m.kd, (float)kPiD);
+        if (o_glossy(m)) {
+            const OFrame fr = oframe(n);
+            f = f + ggx_eval_d(m.ks, ggx_alpha_d(m.rough), o_local(fr, wo), o_local(fr, dl));
+        }
+        dist = std::max(dist, 1.0f);
+        const Col Li = cdiv(cld(lt.intensity), dist * dist);
+        L = L + Li * f * c;
+    }
+    return L;
+}
+
+Col o_path(const orc_scene* sc, int W, int H, int x, int y, int frame, int max_depth, int rr_depth,
+           uint64_t& next, uint64_t& nshadow) {
+    uint32_t s = seed_of(x, y, frame);
+    const float ox = pcg32_float(s);
+    const float oy = pcg32_float(s);
+    const tfg_camera& c = *sc->cam;
+    // camara.h:42-59: plane point minus eye, normalised (in FP64)
+    const float u = ((float)x + ox) / (float)W;
+    const float v = ((float)y + oy) / (float)H;
+    const float px = (2.0f * u - 1.0f) * (c.plane_w * 0.5f);
+    const float py = (1.0f - 2.0f * v) * (c.plane_h * 0.5f);
+    D3 o = dld(c.pos);
+    D3 d = dnorm(dld(c.front) * (double)c.focal + dld(c.left) * (double)px + dld(c.up) * (double)py);
+    Col color = {0, 0, 0}, camino = {1, 1, 1};
+    double ior = 1.0;
+    for (int depth = 0; depth < max_depth; ++depth) {
+        tfg_hit h;
+        ++next;
+        if (!o_closest(sc, o, d, h)) break;   // black background, render.h:227-239
+        const D3 p = o + d * (double)h.t;
+        const tfg_tri_shading& sh = sc->shading[h.prim];
+        const OMat m = omat(sc->materials[sh.material]);
+        D3 n;
+        bool have = false;
+        if (sh.flags & 1u) {   // primitiva.h:84-101
+            const float b0 = 1.0f - h.u - h.v;
+            const D3 ns = dld(sh.n0) * (double)b0 + dld(sh.n1) * (double)h.u + dld(sh.n2) * (double)h.v;
+            const float len = (float)std::sqrt(ddot(ns, ns));
+            if (len > 1e-6f) { n = ns * (double)(1.0f / len); have = true; }
+        }
+        if (!have) {
+            n = dnorm(dld(sh.ng));
+            if (ddot(n, d) > 0.0) n = -n;
+        }
+        if (cmax(m.le) > 0.0f) { color = color + camino * m.le; break; }
+        const D3 wo = -d;
+        if (o_diffuse(m) || o_glossy(m)) color = color + camino * o_direct(sc, p + n * kEpsF, n, wo, m, nshadow);
+        const double mkd = cmax(m.kd), mks = cmax(m.ks), mkt = cmax(m.kt);
+        const double sum = mkd + mks + mkt;
+        if (!(sum > 0.0)) break;
+        const double pd = mkd / sum, ps = mks / sum, pt = 1.0 - pd - ps;
+        const double r = pcg32_float(s);
+        D3 wi;
+        if (r < pd) {
+            camino = cdiv(camino * m.kd, (float)pd);
+            const double r1 = pcg32_float(s), r2 = pcg32_float(s);
+            const double phi = 2.0 * M_PI * r1, th = std::acos(std::sqrt(1.0 - r2));
+            const D3 wl = {std::sin(th) * std::cos(phi), std::sin(th) * std::sin(phi), std::cos(th)};
+            if (wl.z <= kEpsF) break;
+            wi = o_world(oframe(n), wl);
+        } else if (r < pd + ps) {
+            if (o_glossy(m)) {
+                const double r1 = pcg32_float(s), r2 = pcg32_float(s);
+                const OFrame fr = oframe(n);
+                const D3 wol = o_local(fr, wo);
+                if (wol.z <= 0.0) break;
+                const double alpha = ggx_alpha_d(m.rough);
+                const D3 hm = ggx_vndf_d(wol, alpha, r1, r2);
+                const double oh = ddot(wol, hm);
+                const D3 wil = hm * (2.0 * oh) - wol;
+                if (wil.z <= 0.0) break;
+                camino = camino * schlick_d(m.ks, oh) * (float)(ggx_G1_d(alpha * alpha, wil.z) / ps);
+                wi = dnorm(o_world(fr, wil));
+            } else {
+                wi = dnorm(d - n * (2.0 * ddot(d, n)));
+                camino = cdiv(camino * m.ks, (float)ps);
+            }
+        } else {   // render.h:342-376
+            D3 N = n;
+            double cosi = std::max(-1.0, std::min(1.0, ddot(d, N)));
+            double etat;
+            if (cosi > 0.0) { N = -N; etat = 1.0; } else { cosi = -cosi; etat = m.ior; }
+            const double eta = ior / etat;
+            const double k = 1.0 - eta * eta * (1.0 - cosi * cosi);
+            if (k < 0.0) wi = dnorm(d - N * (2.0 * ddot(d, N)));
+            else { ior = etat; wi = dnorm(d * eta + N * (eta * cosi - std::sqrt(k))); }
+            camino = cdiv(camino * m.kt, (float)pt);
+        }
+        if (depth > rr_depth) {   // render.h:209-224
+            double prob = std::max((double)camino.r, std::max((double)camino.g, (double)camino.b));
+            if (prob > 0.95) prob = 0.95;
+            if (pcg32_float(s) > prob) break;
+            camino = cdiv(camino, (float)prob);
+        }
+        o = p + wi * kEpsF;
+        d = wi;
+    }
+    return color;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_render_rows(const orc_scene* sc, int width, int height, int y0, int y1, int frame0, int spp, int max_depth,
+                     int rr_depth, double* accum, uint64_t* rays_out, int nthreads) {
+    const uint32_t n = (uint32_t)width * (uint32_t)(y1 - y0);
+    std::atomic<uint64_t> ext{0}, shd{0};
+    parallel_for(n, nthreads, [&](uint32_t i) {
+        const int x = (int)(i % (uint32_t)width), y = y0 + (int)(i / (uint32_t)width);
+        uint64_t e = 0, sh = 0;
+        double acc[3] = {0, 0, 0};
+        for (int k = 0; k < spp; ++k) {
+            const Col c = o_path(sc, width, height, x, y, frame0 + k, max_depth, rr_depth, e, sh);
+            acc[0] += c.r;
+            acc[1] += c.g;
+            acc[2] += c.b;
+        }
+        accum[3 * (size_t)i] += acc[0];
+        accum[3 * (size_t)i + 1] += acc[1];
+        accum[3 * (size_t)i + 2] += acc[2];
+        ext += e;
+        shd += sh;
+    });
+    if (rays_out) { rays_out[0] = ext.load(); rays_out[1] = shd.load(); }
+}
+
+void orc_render(const orc_scene* sc, int width, int height, int frame0, int spp, int max_depth, int rr_depth,
+                double* accum, uint64_t* rays_out, int nthreads) {
+    orc_render_rows(sc, width, height, 0, height, frame0, spp, max_depth, rr_depth, accum, rays_out, nthreads);
+}
+
+void orc_render_pixel(const orc_scene* sc, int width, int height, int x, int y, int frame, int max_depth,
+                      int rr_depth, float* out_rgb) {
+    uint64_t e = 0, sh = 0;
+    const Col c = o_path(sc, width, height, x, y, frame, max_depth, rr_depth, e, sh);
+    out_rgb[0] = c.r;
+    out_rgb[1] = c.g;
+    out_rgb[2] = c.b;
 }
 
 }  // extern "C"

@@ -65,6 +65,34 @@ void orc_bvh2_trace(const void* bvh, const tfg_ray* rays, uint32_t n, tfg_hit* h
 int orc_cwbvh_validate(const float* nodes, uint32_t num_nodes, const float* tris, uint32_t num_tris,
                        uint32_t format, const float* verts, uint32_t* depth);
 
+
+/* render.h:33-466 + escenario.h:123-184 restated on the CPU: geometry and
+ * directions in FP64 (Vector3d), colours in FP32 (Color), intersections
+ * by the CWBVH decode above on FP32-cast rays.  Same extensions as the HIP
+ * integrator: GGX lobe for roughness > 0, shadow rays against the mesh,
+ * configurable max depth / Russian-roulette depth.
+ * cam: tfg_camera (16 floats); shading: tfg_tri_shading per primIdx;
+ * accum (W*H*3 doubles) += sum over frames frame0 .. frame0+spp-1;
+ * nthreads host threads.  Returns total extension and shadow rays traced
+ * through *rays_out[2] (nullable). */
+typedef struct orc_scene {
+    const float* nodes;
+    const float* tris;
+    const tfg_tri_shading* shading;
+    const tfg_material* materials;
+    const tfg_point_light* lights;
+    uint32_t n_lights;
+    const tfg_camera* cam;
+} orc_scene;
+void orc_render(const orc_scene* sc, int width, int height, int frame0, int spp, int max_depth, int rr_depth,
+                double* accum, uint64_t* rays_out, int nthreads);
+/* Rows [y0, y1) only; accum holds (y1 - y0) * width pixels. */
+void orc_render_rows(const orc_scene* sc, int width, int height, int y0, int y1, int frame0, int spp, int max_depth,
+                     int rr_depth, double* accum, uint64_t* rays_out, int nthreads);
+/* One path of pixel (x, y) at `frame`; out_rgb[3]. */
+void orc_render_pixel(const orc_scene* sc, int width, int height, int x, int y, int frame, int max_depth,
+                      int rr_depth, float* out_rgb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,7 +100,7 @@ def test_traversal_microbench_full_size(torch_cuda):
     hits = np.zeros(prim.shape[0], oracle.HIT_DTYPE)
     hits["t"] = hp[:, 0].view(np.float32)
     hits["prim"] = hp[:, 3]
-    assert (hp[:, 3] != 0xFFFFFFFF).mean() > 0.3
+    assert (hp[:, 3] != 0xFFFFFFFF).mean() > 0.4
     bounce = workloads.bounce_rays(sc, prim, hits, seed=8)
     shadow = workloads.shadow_rays(sc, prim, hits, seed=9)
     hb = _trace_gpu(torch, dbvh, bounce)

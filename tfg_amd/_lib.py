@@ -29,6 +29,28 @@ class CwbvhGpu(C.Structure):
     _fields_ = [("d_nodos", C.c_void_p), ("d_tris", C.c_void_p), ("num_nodos", C.c_uint32), ("num_tris", C.c_uint32)]
 
 
+class PtConfig(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
+                ("mode", C.c_uint32), ("train_fraction", C.c_float), ("use_graph", C.c_uint32)]
+
+
+class PtScene(C.Structure):
+    _fields_ = [("bvh", CwbvhGpu), ("d_shading", C.c_void_p), ("d_materials", C.c_void_p), ("d_lights", C.c_void_p),
+                ("n_lights", C.c_uint32)]
+
+
+class PtProfile(C.Structure):
+    _fields_ = [("ms_generate", C.c_float), ("ms_finalize", C.c_float), ("ms_total", C.c_float),
+                ("ms_extend", C.c_float * 32), ("ms_shade", C.c_float * 32), ("ms_connect", C.c_float * 32),
+                ("ext_rays", C.c_uint32 * 32), ("shadow_rays", C.c_uint32 * 32),
+                ("ext_nodes", C.c_uint64), ("ext_tris", C.c_uint64), ("shadow_nodes", C.c_uint64),
+                ("shadow_tris", C.c_uint64)]
+
+
+TFG_PT_INFER = 1
+TFG_PT_TRAIN = 2
+TFG_PT_NCOUNTERS = 72
+
 _lib = None
 
 # name -> (restype, argtypes)
@@ -53,6 +75,18 @@ _SIGS = {
         C.c_int,
         [C.POINTER(CwbvhGpu), C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
     ),
+    "tfg_pt_create": (C.c_int, [C.POINTER(PtConfig), C.POINTER(C.c_void_p)]),
+    "tfg_pt_set_scene": (C.c_int, [C.c_void_p, C.POINTER(PtScene)]),
+    "tfg_pt_set_camera": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tfg_pt_render": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tfg_pt_counters": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tfg_pt_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tfg_pt_ray_totals": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tfg_pt_profile_frame": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(PtProfile)]),
+    "tfg_pt_reset_totals": (C.c_int, [C.c_void_p]),
+    "tfg_pt_get_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_uint32)]),
+    "tfg_pt_destroy": (C.c_int, [C.c_void_p]),
 }
 
 

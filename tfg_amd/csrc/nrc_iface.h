@@ -1,0 +1,23 @@
+// nrc_iface.h - what the path tracer needs from the radiance cache.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/tfg_types.h"
+
+struct tfg_nrc;   // defined in nrc.hip
+
+// Device buffers of a path tracer frame that the cache reads.
+struct tfg_nrc_frame {
+    const float4* records = nullptr;   // kRecF4 float4 per training record
+    const float4* queries = nullptr;   // kQueryF4 float4 per query
+    const uint32_t* counters = nullptr;   // CNT_RECORDS, CNT_QUERIES
+    uint32_t n_paths = 0;
+    uint32_t mode = 0;
+};
+
+namespace tfg {
+// Enqueues, on stream s, the cache work of one frame: inference of every
+// query composited into image, then (when training) tail inference,
+// targets and the optimiser steps.  Graph-capturable.
+int nrc_frame_enqueue(tfg_nrc* nrc, const tfg_nrc_frame& f, tfg_color* image, hipStream_t s);
+}  // namespace tfg

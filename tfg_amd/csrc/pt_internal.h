@@ -1,0 +1,93 @@
+// pt_internal.h - device-side layout of the wavefront path tracer's state,
+// shared by pt.hip (generate / extend / shade / connect / finalize) and
+// nrc.hip (radiance-cache inference, targets and training).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tfg_types.h"
+
+namespace tfg {
+
+constexpr uint32_t kMaxDepth = 32;
+constexpr uint32_t kMiss = 0xFFFFFFFFu;
+
+// counters[] layout (uint32, device)
+enum : uint32_t {
+    CNT_Q0 = 0,           // ray queue 0 length
+    CNT_Q1 = 1,           // ray queue 1 length
+    CNT_SHADOW = 2,       // shadow queue length
+    CNT_RECORDS = 3,      // training records written this frame
+    CNT_QUERIES = 4,      // cache queries written this frame
+    CNT_EXT = 8,          // + depth: extension rays traced at that depth
+    CNT_SHD = 8 + kMaxDepth,   // + depth: shadow rays traced at that depth
+    CNT_TOTAL = 8 + 2 * kMaxDepth,
+    CNT_CUM = 80,         // 2 x uint64 at counters + CNT_CUM: rays of all previous frames
+    CNT_STAT = 84,        // 4 x uint64: extend nodes, extend tris, shadow nodes, shadow tris
+    CNT_ALLOC = 92,
+};
+
+// path flags
+enum : uint32_t {
+    PF_TRAIN = 1u << 0,      // training path (render.h:443)
+    PF_UNBIASED = 1u << 1,   // unbiased suffix (render.h:65)
+    PF_HEAD = 1u << 2,       // head vertex captured (render.h:191-201)
+    PF_QUERY = 1u << 3,      // cache query captured (render.h:182-190)
+    PF_TAIL = 1u << 4,       // suffix ended at a surface: tail data set
+};
+
+// shadow entry flag (top byte of the material word)
+constexpr uint32_t SF_HEAD_NEE = 1u << 24;   // also add this NEE to the head's prefix colour
+
+// frame mode bits
+enum : uint32_t {
+    MODE_INFER = 1u << 0,   // usar_red_inferencia
+    MODE_TRAIN = 1u << 1,   // entrenar_red
+};
+
+struct PtParams {
+    // scene
+    const float4* nodes;
+    const float4* tris;
+    const tfg_tri_shading* shading;
+    const tfg_material* mats;
+    const tfg_point_light* lights;
+    uint32_t n_lights;
+    const tfg_camera* cam;
+    const int32_t* frame;   // device frame index (seed), read by generate
+    uint32_t width, height, n_paths;
+    uint32_t max_depth, rr_depth, mode;
+    float train_frac;
+    // queues
+    float4* rays[2];        // 2 float4 per ray (tfg_ray)
+    uint32_t* qpath[2];
+    float4* hits;           // 1 float4 per queue entry
+    float4* shadow;         // 4 float4 per entry
+    uint32_t* counters;
+    // path state (indexed by path id = pixel)
+    float4* thr;            // throughput rgb, pdf of the last bounce
+    float4* rad;            // radiance rgb, current ior
+    uint32_t* rng;
+    uint32_t* flags;
+    float4* foot;           // a0, a_current, a_suffix, a_prefix (render.h:50-54)
+    float4* pre;            // colour before the head vertex's suffix (rgb)
+    float4* tthr;           // throughput x reflectance at the head / query vertex
+    float4* head;           // 4 float4: head (training) or query vertex data
+    float4* tail;           // 4 float4: tail vertex data
+    float4* tsuf;           // suffix throughput x tail reflectance
+    // cache I/O
+    float4* records;        // kRecF4 float4 per record
+    float4* queries;        // kQueryF4 float4 per query
+};
+
+// Vertex data packing (4 float4): (pos, roughness), (dir mapped to [0,1], kd.r),
+// (normal mapped to [0,1], kd.g), (kd.b, ks.rgb).  Mirrors DatosGeometricos
+// (mlp_types.h:32-38) plus the roughness input the GGX scenes need.
+constexpr int kVertF4 = 4;
+// Training record: head (4), tail (4), L_suffix (1, w = 1 valid), T_suffix (1),
+// normaliser (1).  Mirrors RegistroEntrenamiento (mlp_types.h:41-53).
+constexpr int kRecF4 = 11;
+// Query: vertex (4) + (throughput rgb, path id as bits).
+constexpr int kQueryF4 = 5;
+
+}  // namespace tfg

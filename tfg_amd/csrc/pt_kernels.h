@@ -1,0 +1,465 @@
+// pt_kernels.h - the wavefront stages of one frame.
+//
+//   generate : per pixel seed, jitter and primary ray     (render.h:416-443)
+//   extend   : closest hit of every queued ray             (escenario.h:316-366)
+//   shade    : emission, NEE request, cache capture, lobe
+//              sampling and Russian roulette               (render.h:67-241)
+//   connect  : shadow rays of the NEE requests             (escenario.h:144-184)
+//   finalize : pixel colour, training records, queries     (render.h:243-260,449-465)
+//
+// The path loop of lanzarRayoIterativo becomes max_depth rounds of
+// extend/shade/connect over compacted queues.  Queue appends are
+// wave-aggregated: one ballot, one atomic per wave, lane offsets by popcount.
+// Every kernel is a grid-stride (persistent) loop whose trip count comes from
+// device counters, so the whole frame can be captured in one hipGraph.
+#pragma once
+#include "pt_internal.h"
+#include "shade_device.h"
+#include "trace_device.h"
+
+namespace tfg {
+
+// One append slot per requesting lane; all lanes of the wave must call it.
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
+    const uint64_t m = __ballot(want);
+    if (m == 0) return 0;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)leader);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+__device__ __forceinline__ float4 f4(V3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+__device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
+
+// ------------------------------------------------------------------ reset
+// One block of 128 threads: folds the previous frame's ray counts into the
+// 64-bit running totals, then clears the per-frame counters.
+__global__ void k_frame_reset(PtParams P) {
+    const uint32_t i = threadIdx.x;
+    if (i < 64) {
+        const uint32_t e = wave_sum(i < kMaxDepth ? P.counters[CNT_EXT + i] : 0u);
+        const uint32_t s = wave_sum(i < kMaxDepth ? P.counters[CNT_SHD + i] : 0u);
+        if (i == 0) {
+            unsigned long long* cum = reinterpret_cast<unsigned long long*>(P.counters + CNT_CUM);
+            cum[0] += e;
+            cum[1] += s;
+        }
+    }
+    __syncthreads();
+    if (i < CNT_TOTAL) P.counters[i] = (i == CNT_Q0) ? P.n_paths : 0u;
+}
+
+__global__ void k_set_frame(int32_t* frame, int32_t value) { *frame = value; }
+
+// --------------------------------------------------------------- generate
+__global__ __launch_bounds__(256) void k_generate(PtParams P) {
+    const tfg_camera cam = *P.cam;
+    const int32_t frame = *P.frame;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n_paths; i += gridDim.x * blockDim.x) {
+        const int x = (int)(i % P.width), y = (int)(i / P.width);
+        uint32_t s = pixel_seed(x, y, frame);
+        const float ox = pcg32_float(s);
+        const float oy = pcg32_float(s);
+        const V3 d = camera_dir(cam, x, y, (int)P.width, (int)P.height, ox, oy);
+        uint32_t fl = 0;
+        // render.h:443: training pixel draw only when both flags are set
+        if ((P.mode & MODE_TRAIN) && (P.mode & MODE_INFER)) {
+            if (pcg32_float(s) < P.train_frac) fl |= PF_TRAIN;
+        } else if (P.mode & MODE_TRAIN) {
+            fl |= PF_TRAIN;
+        }
+        // render.h:65: unbiased suffix draw only for training paths
+        if ((fl & PF_TRAIN) && pcg32_float(s) < 0.0625f) fl |= PF_UNBIASED;
+        P.rays[0][2 * (size_t)i] = make_float4(cam.pos[0], cam.pos[1], cam.pos[2], 0.0f);
+        P.rays[0][2 * (size_t)i + 1] = f4(d, __int_as_float(0x7f800000));
+        P.qpath[0][i] = i;
+        P.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        P.rng[i] = s;
+        P.flags[i] = fl;
+        if (P.mode) P.foot[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// ----------------------------------------------------------------- extend
+__device__ __forceinline__ void add_stats(PtParams& P, int slot, const TraceStats& st) {
+    const uint32_t n = wave_sum(st.nodes), t = wave_sum(st.tris);
+    if (__lane_id() == 0 && (n | t)) {
+        unsigned long long* c = reinterpret_cast<unsigned long long*>(P.counters + CNT_STAT);
+        atomicAdd(&c[slot], (unsigned long long)n);
+        atomicAdd(&c[slot + 1], (unsigned long long)t);
+    }
+}
+
+// kStats: also count decoded nodes and tested triangles (profiling frames).
+template <int kBlock, bool kStats = false>
+__global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint32_t depth) {
+    constexpr int kStack = TFG_MAX_TREE_DEPTH - 1;
+    __shared__ uint2 stack[kStack * kBlock];
+    const uint32_t count = P.counters[q];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.counters[CNT_EXT + depth] = count;
+        P.counters[q ^ 1u] = 0;
+        P.counters[CNT_SHADOW] = 0;
+    }
+    const float4* rays = P.rays[q];
+    for (uint32_t b0 = blockIdx.x * kBlock; b0 < count; b0 += gridDim.x * kBlock) {
+        const uint32_t i = b0 + threadIdx.x;
+        TraceStats st;
+        if (i < count) {
+            const float4 a = rays[2 * (size_t)i];
+            const float4 b = rays[2 * (size_t)i + 1];
+            const RayPre r = make_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
+            float t, u, v;
+            uint32_t prim;
+            traverse<0, false, kStack, 0>(P.nodes, P.tris, nullptr, r, stack + threadIdx.x, kBlock, t, u, v, prim,
+                                         kStats ? &st : nullptr);
+            P.hits[i] = make_float4(t, u, v, __uint_as_float(prim));
+        }
+        if (kStats) add_stats(P, 0, st);
+    }
+}
+
+// ------------------------------------------------------------------ shade
+__device__ __forceinline__ void pack_vertex(float4* dst, V3 p, V3 dir_mapped, V3 n_mapped, const Mat& m) {
+    dst[0] = f4(p, m.rough);
+    dst[1] = f4(dir_mapped, m.kd.x);
+    dst[2] = f4(n_mapped, m.kd.y);
+    dst[3] = make_float4(m.kd.z, m.ks.x, m.ks.y, m.ks.z);
+}
+
+template <bool kNrc>
+__global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t depth) {
+    const uint32_t count = P.counters[q];
+    const uint32_t qo = q ^ 1u;
+    const bool last = depth + 1 >= P.max_depth;
+    for (uint32_t b = blockIdx.x * blockDim.x; b < count; b += gridDim.x * blockDim.x) {
+        const uint32_t i = b + threadIdx.x;
+        bool want_shadow = false, want_ray = false;
+        uint32_t pid = 0;
+        float4 e0, e1, e2, e3;           // shadow entry
+        float4 n0, n1;                   // next ray
+        if (i < count) {
+            pid = P.qpath[q][i];
+            const float4 ro = P.rays[q][2 * (size_t)i];
+            const float4 rd = P.rays[q][2 * (size_t)i + 1];
+            const float4 h = P.hits[i];
+            const uint32_t prim = __float_as_uint(h.w);
+            const float4 thr4 = P.thr[pid];
+            V3 thr = xyz(thr4);
+            float pdf = thr4.w;
+            const float4 rad4 = P.rad[pid];
+            V3 rad = xyz(rad4);
+            float ior = rad4.w;
+            uint32_t s = P.rng[pid];
+            uint32_t fl = P.flags[pid];
+            const uint32_t fl_in = fl;
+            float4 foot = kNrc ? P.foot[pid] : make_float4(0.f, 0.f, 0.f, 0.f);
+            bool rad_dirty = false;
+            const V3 d = xyz(rd);
+            if (prim != kMiss) {
+                const float t = h.x;
+                const V3 p = xyz(ro) + d * t;
+                const tfg_tri_shading sh = P.shading[prim];
+                const Mat m = load_mat(&P.mats[sh.material]);
+                const V3 n = shading_normal(sh, ld3(sh.ng), h.y, h.z, d);
+                const float cos_theta = fabsf(dot(n, -d));
+                const bool diffuse = mat_diffuse(m), glossy = mat_glossy(m), delta = mat_delta(m);
+                bool foot_ok = false;
+                if (kNrc) {   // render.h:86-102 path-footprint spread
+                    const float dist_sq = t * t;
+                    if (depth == 0) {
+                        foot.x = dist_sq / (4.0f * kPi * fmaxf(cos_theta, 1e-4f));
+                        foot.y = 0.0f;
+                    } else {
+                        const float spread = dist_sq / (fmaxf(pdf, 1e-4f) * fmaxf(cos_theta, 1e-4f));
+                        if (!(fl & PF_HEAD)) foot.y += spread;
+                        else foot.z += spread;
+                    }
+                    foot_ok = foot.y >= foot.x * 0.01f;
+                }
+                bool alive = true;
+                if (vmax(m.le) > 0.0f) {   // render.h:104-123
+                    rad = rad + thr * m.le;
+                    rad_dirty = true;
+                    if (kNrc && (fl & PF_HEAD)) {
+                        float4* tl = P.tail + (size_t)kVertF4 * pid;
+                        Mat z = m;
+                        z.kd = v3(0.f, 0.f, 0.f);
+                        z.ks = v3(0.f, 0.f, 0.f);
+                        pack_vertex(tl, p, -d, n, z);
+                        P.tsuf[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        fl |= PF_TAIL;
+                    }
+                    alive = false;
+                } else {
+                    const V3 wo = -d;
+                    if (diffuse || glossy) {   // NEE request (render.h:126-131)
+                        want_shadow = true;
+                        e0 = f4(p + n * kEps, __uint_as_float(pid));
+                        e1 = f4(n, __uint_as_float(sh.material));
+                        e2 = f4(wo, 0.0f);
+                        e3 = f4(thr, 0.0f);
+                    }
+                    bool done = false;
+                    const V3 dm = v3(wo.x * 0.5f + 0.5f, wo.y * 0.5f + 0.5f, wo.z * 0.5f + 0.5f);
+                    const V3 nm = v3(n.x * 0.5f + 0.5f, n.y * 0.5f + 0.5f, n.z * 0.5f + 0.5f);
+                    if (kNrc && (fl & PF_HEAD)) {   // render.h:134-161 suffix end
+                        const bool stop = !(fl & PF_UNBIASED) && foot.z >= foot.w;
+                        if (stop) {
+                            pack_vertex(P.tail + (size_t)kVertF4 * pid, p, dm, nm, m);
+                            P.tsuf[pid] = f4(thr * (m.kd + m.ks + m.kt), 0.0f);
+                            fl |= PF_TAIL;
+                            alive = false;
+                            done = true;
+                        }
+                    }
+                    if (kNrc && !done && diffuse && !delta && foot_ok) {   // render.h:166-202
+                        V3 refl = m.kd + m.ks + m.kt;
+                        refl = v3(fmaxf(refl.x, 1e-3f), fmaxf(refl.y, 1e-3f), fmaxf(refl.z, 1e-3f));
+                        if (!(fl & PF_TRAIN) && (P.mode & MODE_INFER)) {
+                            pack_vertex(P.head + (size_t)kVertF4 * pid, p, dm, nm, m);
+                            P.tthr[pid] = f4(thr * refl, 0.0f);
+                            fl |= PF_QUERY;
+                            alive = false;
+                            done = true;
+                        } else if ((fl & PF_TRAIN) && !(fl & PF_HEAD)) {
+                            pack_vertex(P.head + (size_t)kVertF4 * pid, p, dm, nm, m);
+                            P.tthr[pid] = f4(thr * refl, 0.0f);
+                            P.pre[pid] = f4(rad, 0.0f);
+                            foot.w = foot.y;
+                            fl |= PF_HEAD;
+                            if (want_shadow) e1.w = __uint_as_float(sh.material | SF_HEAD_NEE);
+                        }
+                    }
+                    if (!done) {   // calcularLuzIndirecta, render.h:264-335
+                        const float mkd = vmax(m.kd), mks = vmax(m.ks), mkt = vmax(m.kt);
+                        const float sum = mkd + mks + mkt;
+                        V3 wi = v3(0.f, 0.f, 1.f);
+                        if (!(sum > 0.0f)) {
+                            alive = false;
+                        } else {
+                            const float pd = mkd / sum, ps = mks / sum, pt = 1.0f - pd - ps;
+                            const float r = pcg32_float(s);
+                            if (r < pd) {
+                                thr = thr * m.kd * (1.0f / pd);
+                                const float r1 = pcg32_float(s);
+                                const float r2 = pcg32_float(s);
+                                const V3 wl = cosine_sample(r1, r2);
+                                if (wl.z <= kEps) {
+                                    alive = false;
+                                } else {
+                                    wi = to_world(make_frame(n), wl);
+                                    pdf = fmaxf(fabsf(dot(wi, n)), 1e-4f) / kPi;
+                                }
+                            } else if (r < pd + ps) {
+                                if (glossy) {
+                                    const float r1 = pcg32_float(s);
+                                    const float r2 = pcg32_float(s);
+                                    const Frame fr = make_frame(n);
+                                    const V3 wol = to_local(fr, wo);
+                                    const float alpha = ggx_alpha(m.rough);
+                                    if (wol.z <= 0.0f) {
+                                        alive = false;
+                                    } else {
+                                        const V3 hm = ggx_sample_vndf(wol, alpha, r1, r2);
+                                        const float oh = dot(wol, hm);
+                                        const V3 wil = hm * (2.0f * oh) - wol;
+                                        if (wil.z <= 0.0f) {
+                                            alive = false;
+                                        } else {
+                                            const float a2 = alpha * alpha;
+                                            thr = thr * schlick(m.ks, oh) * (ggx_G1(a2, wil.z) / ps);
+                                            wi = normalize(to_world(fr, wil));
+                                            pdf = ggx_D(a2, hm.z) * ggx_G1(a2, wol.z) / (4.0f * wol.z);
+                                        }
+                                    }
+                                } else {   // render.h:337-340
+                                    wi = normalize(d - n * (2.0f * dot(d, n)));
+                                    thr = thr * m.ks * (1.0f / ps);
+                                    pdf = 1e6f;
+                                }
+                            } else {   // render.h:342-376
+                                V3 N = n;
+                                float cosi = fminf(1.0f, fmaxf(-1.0f, dot(d, N)));
+                                float etat;
+                                if (cosi > 0.0f) {
+                                    N = -N;
+                                    etat = 1.0f;
+                                } else {
+                                    cosi = -cosi;
+                                    etat = m.ior;
+                                }
+                                const float eta = ior / etat;
+                                const float k = 1.0f - eta * eta * (1.0f - cosi * cosi);
+                                if (k < 0.0f) {
+                                    wi = normalize(d - N * (2.0f * dot(d, N)));
+                                } else {
+                                    ior = etat;
+                                    wi = normalize(d * eta + N * (eta * cosi - sqrtf(k)));
+                                }
+                                thr = thr * m.kt * (1.0f / pt);
+                                pdf = 1e6f;
+                            }
+                        }
+                        // Russian roulette, render.h:209-224 (after depth rr_depth)
+                        if (alive && depth > P.rr_depth) {
+                            const float prob = fminf(vmax(thr), 0.95f);
+                            if (pcg32_float(s) > prob) alive = false;
+                            else thr = thr * (1.0f / prob);
+                        }
+                        if (alive && !last) {
+                            want_ray = true;
+                            n0 = f4(p + wi * kEps, 0.0f);
+                            n1 = f4(wi, __int_as_float(0x7f800000));
+                        }
+                        P.thr[pid] = f4(thr, pdf);
+                        P.rng[pid] = s;
+                        rad_dirty = rad_dirty || ior != rad4.w;
+                    }
+                }
+            }
+            if (rad_dirty) P.rad[pid] = f4(rad, ior);
+            if (kNrc) {
+                P.foot[pid] = foot;
+                if (fl != fl_in) P.flags[pid] = fl;
+            }
+        }
+        const uint32_t ss = wave_append(want_shadow, &P.counters[CNT_SHADOW]);
+        if (want_shadow) {
+            float4* e = P.shadow + 4 * (size_t)ss;
+            e[0] = e0;
+            e[1] = e1;
+            e[2] = e2;
+            e[3] = e3;
+        }
+        const uint32_t rs = wave_append(want_ray, &P.counters[qo]);
+        if (want_ray) {
+            P.rays[qo][2 * (size_t)rs] = n0;
+            P.rays[qo][2 * (size_t)rs + 1] = n1;
+            P.qpath[qo][rs] = pid;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- connect
+template <int kBlock, bool kStats = false>
+__global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) {
+    constexpr int kStack = TFG_MAX_TREE_DEPTH - 1;
+    __shared__ uint2 stack[kStack * kBlock];
+    const uint32_t count = P.counters[CNT_SHADOW];
+    for (uint32_t b = blockIdx.x * kBlock; b < count; b += gridDim.x * kBlock) {
+        const uint32_t i = b + threadIdx.x;
+        uint32_t traced = 0;
+        TraceStats st;
+        if (i < count) {
+            const float4* e = P.shadow + 4 * (size_t)i;
+            const float4 e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3];
+            const V3 ps = xyz(e0);
+            const uint32_t pid = __float_as_uint(e0.w);
+            const V3 n = xyz(e1);
+            const uint32_t mw = __float_as_uint(e1.w);
+            const Mat m = load_mat(&P.mats[mw & 0x00FFFFFFu]);
+            const V3 wo = xyz(e2);
+            V3 sum = v3(0.f, 0.f, 0.f);
+            for (uint32_t l = 0; l < P.n_lights; ++l) {
+                V3 contrib, dir;
+                float dist;
+                if (!light_term(P.lights[l], ps, n, wo, m, contrib, dir, dist)) continue;
+                const V3 o = ps + dir * kEps;
+                const RayPre r = make_ray(o.x, o.y, o.z, dir.x, dir.y, dir.z, 0.0f, dist);
+                float t, u, v;
+                uint32_t prim;
+                ++traced;
+                const bool occ = traverse<0, true, kStack, 0>(P.nodes, P.tris, nullptr, r, stack + threadIdx.x, kBlock,
+                                                             t, u, v, prim, kStats ? &st : nullptr);
+                if (!occ) sum = sum + contrib;
+            }
+            const V3 c = xyz(e3) * sum;
+            float4 rd = P.rad[pid];
+            P.rad[pid] = make_float4(rd.x + c.x, rd.y + c.y, rd.z + c.z, rd.w);
+            if (mw & SF_HEAD_NEE) {
+                const float4 pr = P.pre[pid];
+                P.pre[pid] = make_float4(pr.x + c.x, pr.y + c.y, pr.z + c.z, pr.w);
+            }
+        }
+        const uint32_t tot = wave_sum(traced);
+        if (__lane_id() == 0 && tot) atomicAdd(&P.counters[CNT_SHD + depth], tot);
+        if (kStats) add_stats(P, 2, st);
+    }
+}
+
+// --------------------------------------------------------------- finalize
+// Plain mode: pixel = path radiance.  Cache modes: training pixels keep the
+// colour before their head (render.h:251-260); every captured head/query
+// vertex becomes a cache query; every training path with a head becomes a
+// record with L_suffix = radiance - prefix colour.
+template <bool kNrc>
+__global__ __launch_bounds__(256) void k_finalize(PtParams P, tfg_color* image, double* accum) {
+    for (uint32_t b = blockIdx.x * blockDim.x; b < P.n_paths; b += gridDim.x * blockDim.x) {
+        const uint32_t pid = b + threadIdx.x;
+        bool want_rec = false, want_q = false;
+        uint32_t fl = 0;
+        V3 pix = v3(0.f, 0.f, 0.f);
+        if (pid < P.n_paths) {
+            const float4 r = P.rad[pid];
+            pix = xyz(r);
+            if (kNrc) {
+                fl = P.flags[pid];
+                const bool head = (fl & PF_TRAIN) && (fl & PF_HEAD);
+                if (head) pix = xyz(P.pre[pid]);
+                want_rec = head;
+                want_q = head || (fl & PF_QUERY);
+            }
+            image[pid] = tfg_color{pix.x, pix.y, pix.z};
+            if (!kNrc && accum) {
+                accum[3 * (size_t)pid] += (double)pix.x;
+                accum[3 * (size_t)pid + 1] += (double)pix.y;
+                accum[3 * (size_t)pid + 2] += (double)pix.z;
+            }
+        }
+        if (!kNrc) continue;
+        const uint32_t rs = wave_append(want_rec, &P.counters[CNT_RECORDS]);
+        if (want_rec) {
+            float4* R = P.records + (size_t)kRecF4 * rs;
+            const float4* hd = P.head + (size_t)kVertF4 * pid;
+            for (int k = 0; k < kVertF4; ++k) R[k] = hd[k];
+            const bool tl = fl & PF_TAIL;
+            const float4* tv = P.tail + (size_t)kVertF4 * pid;
+            for (int k = 0; k < kVertF4; ++k) R[kVertF4 + k] = tl ? tv[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 pr = P.pre[pid];
+            const float4 r = P.rad[pid];
+            R[8] = make_float4(r.x - pr.x, r.y - pr.y, r.z - pr.z, 1.0f);
+            R[9] = tl ? P.tsuf[pid] : make_float4(0.f, 0.f, 0.f, 0.f);
+            R[10] = P.tthr[pid];
+        }
+        const uint32_t qs = wave_append(want_q, &P.counters[CNT_QUERIES]);
+        if (want_q) {
+            float4* Q = P.queries + (size_t)kQueryF4 * qs;
+            const float4* hd = P.head + (size_t)kVertF4 * pid;
+            for (int k = 0; k < kVertF4; ++k) Q[k] = hd[k];
+            const float4 tt = P.tthr[pid];
+            Q[4] = make_float4(tt.x, tt.y, tt.z, __uint_as_float(pid));
+        }
+    }
+}
+
+// image -> accum (after the cache composite)
+__global__ __launch_bounds__(256) void k_accumulate(const tfg_color* image, double* accum, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const tfg_color c = image[i];
+        accum[3 * (size_t)i] += (double)c.r;
+        accum[3 * (size_t)i + 1] += (double)c.g;
+        accum[3 * (size_t)i + 2] += (double)c.b;
+    }
+}
+
+}  // namespace tfg

@@ -1,0 +1,100 @@
+"""Host driver of the wavefront path tracer (tfg_pt_* in include/tfg.h).
+
+``DeviceScene`` uploads a :class:`tfg_amd.scenes.Scene` (CWBVH, shading
+records, materials, lights) to device memory; ``PathTracer`` renders frames
+into caller-owned device images.  Device memory is held in torch tensors.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+class DeviceScene:
+    def __init__(self, scene, torch):
+        self.scene = scene
+        self.host = _lib.CwbvhHost(scene.verts, _lib.TFG_TRI_WATERTIGHT)
+        self.bvh = self.host.upload()
+        self.shading = torch.from_numpy(np.ascontiguousarray(scene.shading_records()).view(np.int32)).cuda()
+        self.materials = torch.from_numpy(np.ascontiguousarray(scene.materials, np.float32)).cuda()
+        self.lights = torch.from_numpy(np.ascontiguousarray(scene.lights, np.float32)).cuda()
+        self.desc = _lib.PtScene()
+        self.desc.bvh = self.bvh.desc
+        self.desc.d_shading = self.shading.data_ptr()
+        self.desc.d_materials = self.materials.data_ptr()
+        self.desc.d_lights = self.lights.data_ptr()
+        self.desc.n_lights = scene.lights.shape[0]
+
+
+class PathTracer:
+    def __init__(self, width, height, max_depth, rr_depth, mode=0, train_fraction=0.05, use_graph=True):
+        self.cfg = _lib.PtConfig(width, height, max_depth, rr_depth, mode, train_fraction, 1 if use_graph else 0)
+        self._h = C.c_void_p()
+        check(lib().tfg_pt_create(C.byref(self.cfg), C.byref(self._h)), "tfg_pt_create")
+        self.width, self.height = width, height
+
+    def set_scene(self, dscene: DeviceScene):
+        self._scene = dscene
+        check(lib().tfg_pt_set_scene(self._h, C.byref(dscene.desc)), "tfg_pt_set_scene")
+
+    def set_camera(self, cam_frame: np.ndarray):
+        c = np.ascontiguousarray(cam_frame, np.float32)
+        assert c.size == 16
+        check(lib().tfg_pt_set_camera(self._h, c.ctypes.data), "tfg_pt_set_camera")
+
+    def render(self, frame: int, image, accum=None, stream=None):
+        check(lib().tfg_pt_render(self._h, int(frame), image.data_ptr(),
+                                  accum.data_ptr() if accum is not None else None, stream), "tfg_pt_render")
+
+    def counters(self) -> np.ndarray:
+        out = np.zeros(_lib.TFG_PT_NCOUNTERS, np.uint32)
+        check(lib().tfg_pt_counters(self._h, out.ctypes.data), "tfg_pt_counters")
+        return out
+
+    def rays_last_frame(self):
+        c = self.counters()
+        return int(c[8:40].sum()), int(c[40:72].sum())
+
+    def ray_totals(self):
+        out = np.zeros(2, np.uint64)
+        check(lib().tfg_pt_ray_totals(self._h, out.ctypes.data), "tfg_pt_ray_totals")
+        return int(out[0]), int(out[1])
+
+    def profile_frame(self, frame: int, image) -> dict:
+        pr = _lib.PtProfile()
+        check(lib().tfg_pt_profile_frame(self._h, int(frame), image.data_ptr(), C.byref(pr)), "tfg_pt_profile_frame")
+        d = self.cfg.max_depth
+        return {
+            "ms_generate": pr.ms_generate, "ms_finalize": pr.ms_finalize, "ms_total": pr.ms_total,
+            "ms_extend": list(pr.ms_extend[:d]), "ms_shade": list(pr.ms_shade[:d]),
+            "ms_connect": list(pr.ms_connect[:d]), "ext_rays": list(pr.ext_rays[:d]),
+            "shadow_rays": list(pr.shadow_rays[:d]), "ext_nodes": pr.ext_nodes, "ext_tris": pr.ext_tris,
+            "shadow_nodes": pr.shadow_nodes, "shadow_tris": pr.shadow_tris,
+        }
+
+    def reset_totals(self):
+        check(lib().tfg_pt_reset_totals(self._h), "tfg_pt_reset_totals")
+
+    def autotune(self, stream=None):
+        check(lib().tfg_pt_autotune(self._h, stream), "tfg_pt_autotune")
+        return self.launch_shape()
+
+    def launch_shape(self):
+        b, g, s = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().tfg_pt_get_launch(self._h, C.byref(b), C.byref(g), C.byref(s)))
+        return {"trace_block": b.value, "trace_grid": g.value, "shade_grid": s.value}
+
+    def close(self):
+        if self._h:
+            lib().tfg_pt_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

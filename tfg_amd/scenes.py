@@ -94,13 +94,18 @@ class Scene:
         return int(self.verts.shape[0])
 
     def shading_records(self) -> np.ndarray:
-        """(N, 12) uint32 view of tfg_tri_shading records."""
-        rec = np.zeros((self.n_tris, 12), np.float32)
+        """(N, 16) uint32 view of tfg_tri_shading records (64 B each)."""
+        rec = np.zeros((self.n_tris, 16), np.float32)
         rec[:, 0:3] = self.normals[:, 0:3]
         rec[:, 4:7] = self.normals[:, 3:6]
         rec[:, 8:11] = self.normals[:, 6:9]
+        v = self.verts.reshape(-1, 3, 3)
+        rec[:, 12:15] = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
         u = rec.view(np.uint32)
         u[:, 3] = self.mat_id
+        smooth = np.any(self.normals[:, 0:3] != self.normals[:, 3:6], axis=1) | np.any(
+            self.normals[:, 0:3] != self.normals[:, 6:9], axis=1)
+        u[:, 7] = smooth.astype(np.uint32)
         return u
 
 
@@ -199,7 +204,7 @@ def perlin3(p: np.ndarray, seed: int) -> np.ndarray:
 
 def displaced_uv_sphere(nu: int, nv: int, radius: float, amplitude: float, freq: float, seed: int,
                         center=(0.0, 0.0, 0.0), scale=(1.0, 1.0, 1.0)):
-    """UV sphere with nu segments and nv rings (2*nu*(nv-1) triangles), displaced
+    """UV sphere with nu segments and nv rings (2*nu*nv triangles), displaced
     along the normal by amplitude * perlin(freq * n).  Returns (verts, normals)."""
     th = np.linspace(0.0, math.pi, nv + 1)
     ph = np.linspace(0.0, 2.0 * math.pi, nu + 1)
@@ -305,14 +310,14 @@ def cornell(level: int = 4, seed: int = 0) -> Scene:
 
 
 def displaced_sphere(nu: int = 1024, nv: int = 512, seed: int = 1) -> Scene:
-    """Radius-1 UV sphere (2*nu*(nv-1) triangles; 1,046,528 at the defaults)
-    displaced by 0.05 * Perlin noise, one point light, camera at distance 3."""
+    """Radius-1 UV sphere (2*nu*nv triangles; 1,048,576 at the defaults)
+    displaced by 0.05 * Perlin noise, one point light, camera at distance 2.4."""
     v, n = displaced_uv_sphere(nu, nv, 1.0, 0.05, 4.0, seed)
     mats = np.stack([material(kd=(0.7, 0.7, 0.7))])
     lights = np.zeros((1, 8), np.float32)
     lights[0, 0:3] = (2.0, 3.0, 2.5)
     lights[0, 4:7] = (20.0, 20.0, 20.0)
-    cam = Camera(pos=(0.0, 0.0, 3.0), front=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fov_deg=45.0)
+    cam = Camera(pos=(0.0, 0.0, 2.4), front=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fov_deg=45.0)
     return Scene("displaced_sphere", v, n, np.zeros(v.shape[0], np.uint32), mats, lights, cam, seed)
 
 
