@@ -132,6 +132,90 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream);
 int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_grid, uint32_t* shade_grid);
 int tfg_pt_destroy(tfg_pt* pt);
 
+
+/* ---- radiance cache (NRC) -------------------------------------------------
+ * Replaces ColorMLP (mlp.h:19-70, mlp.cpp:16-170) and the NRC data kernels
+ * (NrcKernel.cu, RenderKernel.cu:92-166).  64 encoded inputs -> 5 x 64 ReLU
+ * -> 3 outputs, FP32 master weights, BF16 (or FP16) matrix-core operands
+ * with FP32 accumulation, RelativeL2Luminance loss with static loss
+ * scaling, Adam + EMA.  Attached to a tfg_pt, its inference and training
+ * run inside the path tracer's frame (and its hipGraph). */
+typedef struct tfg_nrc tfg_nrc;
+
+enum { TFG_NRC_BF16 = 0, TFG_NRC_FP16 = 1 };
+#define TFG_NRC_PARAMS 21504   /* 5 x 64 x 64 + 16 x 64 (output rows 3..15 are padding) */
+
+typedef struct tfg_nrc_config {
+    uint32_t batch;            /* samples per optimiser step, multiple of 64 (main.cpp:939: 16384) */
+    uint32_t steps_per_frame;  /* optimiser steps per frame */
+    uint32_t min_records;      /* no training with <= this many records (main.cpp:1043: 1024) */
+    float lr, beta1, beta2, eps, ema_decay, loss_scale;
+    float bounds_min[3], bounds_max[3];   /* SceneBounds (main.cpp:870-872) */
+    uint32_t seed;             /* weight initialisation */
+    uint32_t precision;        /* TFG_NRC_BF16 / TFG_NRC_FP16 */
+} tfg_nrc_config;
+
+typedef struct tfg_nrc_stats {
+    uint32_t steps;              /* optimiser steps applied */
+    uint32_t skipped_nonfinite;  /* steps dropped for a non-finite gradient */
+    uint32_t skipped_no_data;    /* steps skipped for too few records */
+    uint32_t seed_counter;       /* batch-index seed (mlp.cpp:102 step_counter) */
+    float last_loss;             /* loss of the last applied step */
+} tfg_nrc_stats;
+
+int tfg_nrc_default_config(tfg_nrc_config* cfg);
+int tfg_nrc_create(const tfg_nrc_config* cfg, tfg_nrc** out);
+int tfg_nrc_destroy(tfg_nrc* nrc);
+/* Checkpoint / restore (replaces mlp.cpp:173-276): TFG_NRC_PARAMS floats
+ * each, row-major per layer [out][in]; ema may be NULL (= master).
+ * set_weights resets the optimiser state. */
+int tfg_nrc_set_weights(tfg_nrc* nrc, const float* master, const float* ema);
+int tfg_nrc_get_weights(tfg_nrc* nrc, float* master, float* ema);
+int tfg_nrc_get_stats(tfg_nrc* nrc, tfg_nrc_stats* out);
+/* Evaluates the network on n vertex records (4 float4 each, the layout of
+ * the cache queries): raw outputs d_out[3n]; d_dump (nullable) receives the
+ * pre-activations of all 6 layers (n x 6 x 64 floats). */
+int tfg_nrc_eval(tfg_nrc* nrc, const float* d_verts, uint32_t n, float* d_out, float* d_dump, int use_train_weights,
+                 tfg_stream stream);
+/* `steps` optimiser steps on training records (11 float4 each) whose count
+ * is read from device memory at *d_n_records. */
+int tfg_nrc_train(tfg_nrc* nrc, const float* d_records, const uint32_t* d_n_records, uint32_t steps,
+                  tfg_stream stream);
+/* Gradient (unscaled, FP32) of the last optimiser step, to host. */
+int tfg_nrc_last_gradient(tfg_nrc* nrc, float* host_grad);
+/* Copies the last frame's training records (11 x 4 floats each; the path
+ * id is the bit pattern of float 39) and cache queries (5 x 4 floats; path
+ * id in float 19) to host memory, at most max_* of each; synchronises. */
+int tfg_pt_read_cache_data(tfg_pt* pt, float* records, uint32_t max_records, float* queries, uint32_t max_queries,
+                           uint32_t* n_records, uint32_t* n_queries);
+/* Attaches the cache to a path tracer created with a nonzero mode. */
+int tfg_pt_set_nrc(tfg_pt* pt, tfg_nrc* nrc);
+/* Switches the frame mode (0 = plain, TFG_PT_INFER | TFG_PT_TRAIN); a
+ * tracer created in plain mode cannot switch to a cache mode. */
+int tfg_pt_set_mode(tfg_pt* pt, uint32_t mode);
+/* Device buffers of the last frame's cache data: records (11 float4 each),
+ * queries (5 float4 each) and the counter array (records at [3], queries at [4]). */
+int tfg_pt_nrc_buffers(tfg_pt* pt, float** d_records, float** d_queries, uint32_t** d_counters);
+
+
+/* ---- reference launchers on the reference's struct layouts ------------------
+ * Same argument meaning as the host wrappers of RenderKernel.h:63-78 and
+ * NrcKernel.h:24-41; the CUDA grid/block arguments are dropped (the library
+ * picks its own launch), cudaStream_t becomes tfg_stream. */
+int tfg_launch_inicializar_camara(tfg_camara* d_cam, int width, int height, tfg_stream stream);   /* RenderKernel.h:67 */
+int tfg_launch_composite(tfg_color* d_img, const tfg_color* d_pred, const tfg_color* d_thr, int width, int height,
+                         tfg_stream stream);                                                      /* RenderKernel.h:63-65 */
+int tfg_launch_calcular_targets(const tfg_registro_entrenamiento* d_recs, const tfg_color* d_pred_tail,
+                                tfg_datos_mlp* d_out, int n, tfg_stream stream);                  /* RenderKernel.h:69-73 */
+int tfg_launch_preparar_inferencia_tail(const tfg_registro_entrenamiento* d_recs, tfg_datos_mlp* d_out, int n,
+                                        tfg_stream stream);                                       /* RenderKernel.h:75-78 */
+int tfg_launch_generar_indices(int* d_idx, int max_range, int count, uint32_t seed, tfg_stream stream);   /* NrcKernel.h:24-25 */
+int tfg_launch_preparar_entrenamiento(const tfg_datos_mlp* d_buf, const int* d_idx, int batch, int n_in, int n_out,
+                                      tfg_scene_bounds bounds, float* d_in, float* d_tgt, tfg_stream stream);   /* NrcKernel.h:27-35 */
+int tfg_launch_preparar_inferencia(const tfg_datos_mlp* d_buf, uint32_t n, uint32_t n_in, float* d_in,
+                                   tfg_scene_bounds bounds, tfg_stream stream);                    /* NrcKernel.h:37-38 */
+int tfg_launch_guardar_salida(const float* d_out, tfg_color* d_col, int n, tfg_stream stream);     /* NrcKernel.h:40-41 */
+
 #ifdef __cplusplus
 }
 #endif

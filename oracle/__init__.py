@@ -50,6 +50,8 @@ _SIGS = {
                           C.c_void_p, C.c_int]),
     "orc_render_rows": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                C.c_void_p, C.c_void_p, C.c_int]),
+    "orc_render_nrc": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                              C.c_void_p, C.c_int]),
     "orc_render_pixel": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                 C.c_void_p]),
 }
@@ -189,6 +191,14 @@ class Renderer:
         lib().orc_render_rows(C.byref(self.desc), self.width, self.height, y0, y1, frame0, spp, max_depth, rr_depth,
                               _p(acc), _p(rays), nthreads)
         return acc, rays
+
+    def render_nrc(self, frame, max_depth, rr_depth, mode, train_frac=0.05, nthreads=os.cpu_count() or 1):
+        """Per-pixel cache-mode outputs: dict of arrays (see orc_render_nrc)."""
+        out = np.zeros((self.height * self.width, 64), np.float32)
+        lib().orc_render_nrc(C.byref(self.desc), self.width, self.height, frame, max_depth, rr_depth, mode,
+                             train_frac, _p(out), nthreads)
+        return {"pixel": out[:, 0:3], "flags": out[:, 3].view(np.uint32), "head": out[:, 4:20],
+                "tail": out[:, 20:36], "lsuf": out[:, 36:39], "tsuf": out[:, 39:42], "norm": out[:, 42:45]}
 
     def pixel(self, x, y, frame, max_depth, rr_depth):
         out = np.zeros(3, np.float32)

@@ -688,6 +688,7 @@ inline D3 dld(const float* p) { return {p[0], p[1], p[2]}; }
 struct Col { float r, g, b; };   // color.h
 inline Col operator+(Col a, Col b) { return {a.r + b.r, a.g + b.g, a.b + b.b}; }
 inline Col operator*(Col a, Col b) { return {a.r * b.r, a.g * b.g, a.b * b.b}; }
+inline Col operator-(Col a, Col b) { return {a.r - b.r, a.g - b.g, a.b - b.b}; }
 inline Col operator*(Col a, float s) { return {a.r * s, a.g * s, a.b * s}; }
 inline Col cdiv(Col a, float s) { return s == 0.0f ? Col{0, 0, 0} : Col{a.r / s, a.g / s, a.b / s}; }   // color.h:42-48
 inline float cmax(Col c) { float m = c.r >= c.g ? c.r : c.g; return c.b > m ? c.b : m; }   // color.h:58-70
@@ -911,4 +912,216 @@ void orc_render_pixel(const orc_scene* sc, int width, int height, int x, int y, 
     out_rgb[2] = c.b;
 }
 
+}  // extern "C"
+
+// ======================================== NRC capture (render.h:33-262 cache branch)
+namespace {
+
+struct NrcOut {
+    float pixel[3];
+    uint32_t flags;        // bit0 training, bit1 head, bit2 query, bit3 tail set
+    float head[16];        // head (training) or query vertex
+    float tail[16];
+    float lsuf[3], tsuf[3], norm[3];
+};
+
+void pack_vtx(float* dst, D3 p, D3 dm, D3 nm, const OMat& m) {
+    dst[0] = (float)p.x; dst[1] = (float)p.y; dst[2] = (float)p.z; dst[3] = m.rough;
+    dst[4] = (float)dm.x; dst[5] = (float)dm.y; dst[6] = (float)dm.z; dst[7] = m.kd.r;
+    dst[8] = (float)nm.x; dst[9] = (float)nm.y; dst[10] = (float)nm.z; dst[11] = m.kd.g;
+    dst[12] = m.kd.b; dst[13] = m.ks.r; dst[14] = m.ks.g; dst[15] = m.ks.b;
+}
+
+void o_path_nrc(const orc_scene* sc, int W, int H, int x, int y, int frame, int max_depth, int rr_depth, int mode,
+                float train_frac, NrcOut& R) {
+    std::memset(&R, 0, sizeof(R));
+    const bool infer = mode & 1, train = mode & 2;
+    uint32_t s = seed_of(x, y, frame);
+    const float ox = pcg32_float(s);
+    const float oy = pcg32_float(s);
+    const tfg_camera& c = *sc->cam;
+    const float u = ((float)x + ox) / (float)W;
+    const float v = ((float)y + oy) / (float)H;
+    const float px = (2.0f * u - 1.0f) * (c.plane_w * 0.5f);
+    const float py = (1.0f - 2.0f * v) * (c.plane_h * 0.5f);
+    D3 o = dld(c.pos);
+    D3 d = dnorm(dld(c.front) * (double)c.focal + dld(c.left) * (double)px + dld(c.up) * (double)py);
+    // render.h:443 and :65
+    const bool training = train ? (infer ? (pcg32_float(s) < train_frac) : true) : false;
+    const bool unbiased = training && (pcg32_float(s) < 0.0625f);
+    Col color = {0, 0, 0}, camino = {1, 1, 1}, pre = {0, 0, 0}, tthr = {0, 0, 0};
+    double ior = 1.0;
+    float a0 = 0.f, acur = 0.f, asuf = 0.f, apre = 0.f, pdf_last = 1.0f;
+    bool head = false, query = false;
+    if (training) R.flags |= 1u;
+    const float kPiF = 3.14159265358979323846f;
+    for (int depth = 0; depth < max_depth; ++depth) {
+        tfg_hit h;
+        if (!o_closest(sc, o, d, h)) break;
+        const D3 p = o + d * (double)h.t;
+        const tfg_tri_shading& sh = sc->shading[h.prim];
+        const OMat m = omat(sc->materials[sh.material]);
+        D3 n;
+        bool have = false;
+        if (sh.flags & 1u) {
+            const float b0 = 1.0f - h.u - h.v;
+            const D3 ns = dld(sh.n0) * (double)b0 + dld(sh.n1) * (double)h.u + dld(sh.n2) * (double)h.v;
+            const float len = (float)std::sqrt(ddot(ns, ns));
+            if (len > 1e-6f) { n = ns * (double)(1.0f / len); have = true; }
+        }
+        if (!have) { n = dnorm(dld(sh.ng)); if (ddot(n, d) > 0.0) n = -n; }
+        const float cos_theta = (float)std::fabs(ddot(n, -d));
+        const bool diffuse = o_diffuse(m), glossy = o_glossy(m);
+        const bool delta = (cmax(m.ks) > kEpsF && m.rough <= 0.0f) || cmax(m.kt) > kEpsF;
+        // render.h:86-102
+        const float dist_sq = h.t * h.t;
+        if (depth == 0) { a0 = dist_sq / (4.0f * kPiF * std::max(cos_theta, 1e-4f)); acur = 0.f; }
+        else {
+            const float spread = dist_sq / (std::max(pdf_last, 1e-4f) * std::max(cos_theta, 1e-4f));
+            if (!head) acur += spread; else asuf += spread;
+        }
+        const bool foot_ok = acur >= a0 * 0.01f;
+        if (cmax(m.le) > 0.0f) {   // render.h:104-123
+            color = color + camino * m.le;
+            if (head) {
+                OMat z = m; z.kd = {0, 0, 0}; z.ks = {0, 0, 0};
+                pack_vtx(R.tail, p, -d, n, z);
+                R.flags |= 8u;
+            }
+            break;
+        }
+        const D3 wo = -d;
+        uint64_t nsh = 0;
+        if (diffuse || glossy) {
+            const Col nee = camino * o_direct(sc, p + n * kEpsF, n, wo, m, nsh);
+            color = color + nee;
+        }
+        const D3 dm = {wo.x * 0.5 + 0.5, wo.y * 0.5 + 0.5, wo.z * 0.5 + 0.5};
+        const D3 nm = {n.x * 0.5 + 0.5, n.y * 0.5 + 0.5, n.z * 0.5 + 0.5};
+        if (head) {   // render.h:134-161
+            if (!unbiased && asuf >= apre) {
+                pack_vtx(R.tail, p, dm, nm, m);
+                const Col rt = m.kd + m.ks + m.kt;
+                const Col ts = camino * rt;
+                R.tsuf[0] = ts.r; R.tsuf[1] = ts.g; R.tsuf[2] = ts.b;
+                R.flags |= 8u;
+                break;
+            }
+        }
+        if (diffuse && !delta && foot_ok) {   // render.h:166-202
+            Col refl = m.kd + m.ks + m.kt;
+            refl = {std::max(refl.r, 1e-3f), std::max(refl.g, 1e-3f), std::max(refl.b, 1e-3f)};
+            if (!training && infer) {
+                pack_vtx(R.head, p, dm, nm, m);
+                tthr = camino * refl;
+                query = true;
+                break;
+            } else if (training && !head) {
+                pack_vtx(R.head, p, dm, nm, m);
+                tthr = camino * refl;
+                pre = color;
+                apre = acur;
+                head = true;
+            }
+        }
+        const double mkd = cmax(m.kd), mks = cmax(m.ks), mkt = cmax(m.kt);
+        const double sum = mkd + mks + mkt;
+        if (!(sum > 0.0)) break;
+        const double pd = mkd / sum, ps = mks / sum, pt = 1.0 - pd - ps;
+        const double r = pcg32_float(s);
+        D3 wi;
+        bool dead = false;
+        if (r < pd) {
+            camino = cdiv(camino * m.kd, (float)pd);
+            const double r1 = pcg32_float(s), r2 = pcg32_float(s);
+            const double phi = 2.0 * M_PI * r1, th = std::acos(std::sqrt(1.0 - r2));
+            const D3 wl = {std::sin(th) * std::cos(phi), std::sin(th) * std::sin(phi), std::cos(th)};
+            if (wl.z <= kEpsF) dead = true;
+            else {
+                wi = o_world(oframe(n), wl);
+                pdf_last = (float)(std::max(std::fabs(ddot(wi, n)), 1e-4) / kPiD);
+            }
+        } else if (r < pd + ps) {
+            if (glossy) {
+                const double r1 = pcg32_float(s), r2 = pcg32_float(s);
+                const OFrame fr = oframe(n);
+                const D3 wol = o_local(fr, wo);
+                if (wol.z <= 0.0) dead = true;
+                else {
+                    const double alpha = ggx_alpha_d(m.rough);
+                    const D3 hm = ggx_vndf_d(wol, alpha, r1, r2);
+                    const double oh = ddot(wol, hm);
+                    const D3 wil = hm * (2.0 * oh) - wol;
+                    if (wil.z <= 0.0) dead = true;
+                    else {
+                        const double a2 = alpha * alpha;
+                        camino = camino * schlick_d(m.ks, oh) * (float)(ggx_G1_d(a2, wil.z) / ps);
+                        wi = dnorm(o_world(fr, wil));
+                        pdf_last = (float)(ggx_D_d(a2, hm.z) * ggx_G1_d(a2, wol.z) / (4.0 * wol.z));
+                    }
+                }
+            } else {
+                wi = dnorm(d - n * (2.0 * ddot(d, n)));
+                camino = cdiv(camino * m.ks, (float)ps);
+                pdf_last = 1e6f;
+            }
+        } else {
+            D3 N = n;
+            double cosi = std::max(-1.0, std::min(1.0, ddot(d, N)));
+            double etat;
+            if (cosi > 0.0) { N = -N; etat = 1.0; } else { cosi = -cosi; etat = m.ior; }
+            const double eta = ior / etat;
+            const double k = 1.0 - eta * eta * (1.0 - cosi * cosi);
+            if (k < 0.0) wi = dnorm(d - N * (2.0 * ddot(d, N)));
+            else { ior = etat; wi = dnorm(d * eta + N * (eta * cosi - std::sqrt(k))); }
+            camino = cdiv(camino * m.kt, (float)pt);
+            pdf_last = 1e6f;
+        }
+        if (dead) break;
+        if (depth > rr_depth) {
+            double prob = std::max((double)camino.r, std::max((double)camino.g, (double)camino.b));
+            if (prob > 0.95) prob = 0.95;
+            if (pcg32_float(s) > prob) break;
+            camino = cdiv(camino, (float)prob);
+        }
+        o = p + wi * kEpsF;
+        d = wi;
+    }
+    Col pix = color;
+    if (training && head) {
+        pix = pre;
+        R.flags |= 2u;
+        const Col ls = color - pre;
+        R.lsuf[0] = ls.r; R.lsuf[1] = ls.g; R.lsuf[2] = ls.b;
+        R.norm[0] = tthr.r; R.norm[1] = tthr.g; R.norm[2] = tthr.b;
+    }
+    if (query) R.flags |= 4u;
+    if (query || (training && head)) { R.norm[0] = tthr.r; R.norm[1] = tthr.g; R.norm[2] = tthr.b; }
+    R.pixel[0] = pix.r; R.pixel[1] = pix.g; R.pixel[2] = pix.b;
+}
+
+}  // namespace
+
+extern "C" {
+// Cache-mode paths of every pixel of one frame; out: W*H records of
+// 64 floats = pixel(3), flags(1, as bits), head(16), tail(16), lsuf(3),
+// tsuf(3), norm(3), pad.
+void orc_render_nrc(const orc_scene* sc, int width, int height, int frame, int max_depth, int rr_depth, int mode,
+                    float train_frac, float* out, int nthreads) {
+    const uint32_t n = (uint32_t)width * (uint32_t)height;
+    parallel_for(n, nthreads, [&](uint32_t i) {
+        NrcOut R;
+        o_path_nrc(sc, width, height, (int)(i % (uint32_t)width), (int)(i / (uint32_t)width), frame, max_depth,
+                   rr_depth, mode, train_frac, R);
+        float* o = out + 64 * (size_t)i;
+        std::memset(o, 0, 64 * sizeof(float));
+        o[0] = R.pixel[0]; o[1] = R.pixel[1]; o[2] = R.pixel[2];
+        std::memcpy(&o[3], &R.flags, 4);
+        std::memcpy(&o[4], R.head, 16 * 4);
+        std::memcpy(&o[20], R.tail, 16 * 4);
+        std::memcpy(&o[36], R.lsuf, 12);
+        std::memcpy(&o[39], R.tsuf, 12);
+        std::memcpy(&o[42], R.norm, 12);
+    });
+}
 }  // extern "C"

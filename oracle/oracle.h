@@ -89,6 +89,13 @@ void orc_render(const orc_scene* sc, int width, int height, int frame0, int spp,
 /* Rows [y0, y1) only; accum holds (y1 - y0) * width pixels. */
 void orc_render_rows(const orc_scene* sc, int width, int height, int y0, int y1, int frame0, int spp, int max_depth,
                      int rr_depth, double* accum, uint64_t* rays_out, int nthreads);
+/* Cache-mode paths (render.h:33-262 with usar_red_inferencia / entrenar_red
+ * from mode bits 1 / 2) of every pixel of one frame.  out: W*H x 64 floats:
+ * pixel(3), flags(1: bit0 training, bit1 head, bit2 query, bit3 tail),
+ * head or query vertex(16), tail vertex(16), L_suffix(3), T_suffix(3),
+ * throughput x reflectance at the head / query (3), pad. */
+void orc_render_nrc(const orc_scene* sc, int width, int height, int frame, int max_depth, int rr_depth, int mode,
+                    float train_frac, float* out, int nthreads);
 /* One path of pixel (x, y) at `frame`; out_rgb[3]. */
 void orc_render_pixel(const orc_scene* sc, int width, int height, int x, int y, int frame, int max_depth,
                       int rr_depth, float* out_rgb);

@@ -1,0 +1,168 @@
+"""GPU checks of the radiance-cache kernels against the FP64 oracle
+(oracle/mlp_oracle.py).
+
+Tolerances:
+  * forward, per layer, vs the oracle emulating the 16-bit operand rounding
+    (weights, inputs and inter-layer activations rounded like the kernel,
+    accumulation in FP64): median relative error <= 2e-3, 99th percentile
+    <= 3e-2 (residual: FP32 vs FP64 accumulation order flipping a 16-bit
+    rounding);
+  * forward vs the pure FP64 network: reported per layer, output layer
+    relative RMS error <= 5e-2 (BF16) / 1e-2 (FP16);
+  * weight gradient of one training step: cosine similarity >= 0.999 and
+    relative L2 error <= 5e-2 against the emulating oracle on the same
+    batch (same PCG32 batch indices);
+  * Adam/EMA update: master weights within 1e-5 relative of the oracle's
+    Adam step applied to the kernel's own gradient.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as M
+from tfg_amd import nrc
+
+pytestmark = pytest.mark.gpu
+
+BMIN, BMAX = (-6.0, -4.0, -10.0), (6.0, 6.0, 2.0)
+
+
+def _verts(n, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.uniform(0, 1, (n, 16)).astype(np.float32)
+    v[:, 0:3] = rng.uniform((-6, -4, -10), (6, 6, 2), (n, 3))
+    return v
+
+
+def _records(n, seed):
+    rng = np.random.default_rng(seed)
+    r = np.zeros((n, 11, 4), np.float32)
+    r[:, 0:4, :] = _verts(n, seed + 1).reshape(n, 4, 4)
+    r[:, 8, :3] = rng.uniform(0.0, 2.0, (n, 3))
+    r[:, 8, 3] = 1.0
+    r[:, 10, :3] = rng.uniform(0.2, 1.0, (n, 3))
+    return r.reshape(n, 44)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_forward_matches_fp64(torch_cuda, prec):
+    torch = torch_cuda
+    c = nrc.RadianceCache(precision=1 if prec == "fp16" else 0, seed=11)
+    n = 2048
+    v = _verts(n, 3)
+    out = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+    dump = torch.zeros((n, 6, 64), dtype=torch.float32, device="cuda")
+    c.eval(torch.from_numpy(v).cuda(), n, out, dump)
+    torch.cuda.synchronize()
+    g_dump = dump.cpu().numpy().astype(np.float64)
+    np.testing.assert_array_equal(out.cpu().numpy(), g_dump[:, 5, :3].astype(np.float32))
+    _, ema = c.weights()
+    x = M.encode(v, BMIN, BMAX)
+    emu, _ = M.forward(ema, x, emulate=prec)
+    exact, _ = M.forward(ema, x)
+    for l in range(6):
+        rows = 64 if l < 5 else 3
+        g = g_dump[:, l, :rows]
+        e = emu[l][:, :rows]
+        rms = np.sqrt(np.mean(e * e)) + 1e-12
+        rel = np.abs(g - e) / (np.abs(e) + 0.05 * rms)
+        ex = exact[l][:, :rows]
+        err_exact = np.sqrt(np.mean((g - ex) ** 2)) / (np.sqrt(np.mean(ex * ex)) + 1e-12)
+        print(f"{prec} layer {l}: emulated median {np.median(rel):.2e} p99 {np.percentile(rel, 99):.2e}; "
+              f"vs FP64 rel-RMS {err_exact:.2e}", flush=True)
+        assert np.median(rel) <= 2e-3
+        assert np.percentile(rel, 99) <= 3e-2
+        if l == 5:
+            assert err_exact <= (5e-2 if prec == "bf16" else 1e-2)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_gradient_and_adam_match_oracle(torch_cuda, prec):
+    torch = torch_cuda
+    batch = 1024
+    c = nrc.RadianceCache(precision=1 if prec == "fp16" else 0, seed=5, batch=batch)
+    w0, e0 = c.weights()
+    n_rec = 5000
+    rec = _records(n_rec, 7)
+    d_rec = torch.from_numpy(rec).cuda()
+    d_n = torch.tensor([n_rec], dtype=torch.int32, device="cuda")
+    c.train(d_rec, d_n, steps=1)
+    g = c.last_gradient().astype(np.float64)
+    st = c.stats()
+    assert st["steps"] == 1 and st["seed_counter"] == 1
+    idx = M.batch_indices(0, batch, n_rec)
+    x = M.encode(rec.reshape(n_rec, 11, 4)[idx, 0:4].reshape(batch, 16), BMIN, BMAX)
+    tgt = M.targets(rec[idx], np.zeros((batch, 3)))
+    loss, g_ref = M.loss_and_grad(w0, x, tgt, emulate=prec)
+    cos = g @ g_ref / (np.linalg.norm(g) * np.linalg.norm(g_ref))
+    rel = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    print(f"{prec}: loss gpu {st['last_loss']:.6f} oracle {loss:.6f}; grad cos {cos:.6f} rel {rel:.3e}", flush=True)
+    assert cos >= 0.999
+    assert rel <= 5e-2
+    assert abs(st["last_loss"] - loss) <= 1e-2 * loss
+    cfg = c.cfg
+    w1, _, _, e1 = M.adam_step(w0.astype(np.float64), 0, 0, e0.astype(np.float64), g, 1, cfg.lr, cfg.beta1,
+                               cfg.beta2, cfg.eps, cfg.ema_decay)
+    m1, me1 = c.weights()
+    np.testing.assert_allclose(m1, w1, rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(me1, e1, rtol=1e-5, atol=1e-7)
+
+
+def test_training_fits_a_smooth_function(torch_cuda):
+    torch = torch_cuda
+    c = nrc.RadianceCache(batch=4096, seed=2)
+    n_rec = 1 << 16
+    rec = _records(n_rec, 9).reshape(n_rec, 11, 4)
+    p = rec[:, 0, :3]
+    f = 0.5 + 0.4 * np.sin(p[:, 0]) * np.cos(0.5 * p[:, 2]) + 0.1 * rec[:, 1, 3:4].ravel() * 0
+    rec[:, 8, :3] = f[:, None] * np.array([1.0, 0.8, 0.6])
+    rec[:, 10, :3] = 1.0
+    d_rec = torch.from_numpy(rec.reshape(n_rec, 44)).cuda()
+    d_n = torch.tensor([n_rec], dtype=torch.int32, device="cuda")
+    c.train(d_rec, d_n, steps=1)
+    first = c.stats()["last_loss"]
+    c.train(d_rec, d_n, steps=300)
+    st = c.stats()
+    print("loss", first, "->", st["last_loss"], flush=True)
+    assert st["steps"] == 301 and st["skipped_nonfinite"] == 0
+    assert st["last_loss"] < first / 5
+
+
+def test_skips_without_data_and_on_nonfinite(torch_cuda):
+    torch = torch_cuda
+    c = nrc.RadianceCache(batch=256, seed=4)
+    w0, _ = c.weights()
+    d_rec = torch.from_numpy(_records(2000, 1)).cuda()
+    c.train(d_rec, torch.tensor([500], dtype=torch.int32, device="cuda"), steps=2)
+    st = c.stats()
+    assert st["skipped_no_data"] == 2 and st["steps"] == 0
+    np.testing.assert_array_equal(c.weights()[0], w0)
+    bad = w0.copy()
+    bad[4096 + 77] = np.nan
+    c.set_weights(bad)
+    c.train(d_rec, torch.tensor([2000], dtype=torch.int32, device="cuda"), steps=1)
+    st = c.stats()
+    assert st["skipped_nonfinite"] == 1 and st["steps"] == 0
+    m, _ = c.weights()
+    np.testing.assert_array_equal(np.isnan(m), np.isnan(bad))
+
+
+def test_checkpoint_roundtrip(torch_cuda, tmp_path):
+    torch = torch_cuda
+    c = nrc.RadianceCache(seed=8, batch=512)
+    d_rec = torch.from_numpy(_records(4000, 2)).cuda()
+    c.train(d_rec, torch.tensor([4000], dtype=torch.int32, device="cuda"), steps=3)
+    m, e = c.weights()
+    path = tmp_path / "cache.bin"
+    c.save(str(path))
+    c2 = nrc.RadianceCache(seed=99, batch=512)
+    c2.load(str(path))
+    m2, e2 = c2.weights()
+    np.testing.assert_array_equal(m, m2)
+    np.testing.assert_array_equal(e, e2)
+    v = _verts(256, 4)
+    o1 = torch.zeros((256, 3), device="cuda")
+    o2 = torch.zeros((256, 3), device="cuda")
+    c.eval(torch.from_numpy(v).cuda(), 256, o1)
+    c2.eval(torch.from_numpy(v).cuda(), 256, o2)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(o1.cpu().numpy(), o2.cpu().numpy())

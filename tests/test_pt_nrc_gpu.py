@@ -1,0 +1,120 @@
+"""Cache-mode path tracing on the GPU against the oracle's restatement of
+the cache branch of lanzarRayoIterativo (render.h:86-260, oracle
+orc_render_nrc): training records and cache queries are matched per path
+id and compared field by field; then the full frame (queries inferred and
+composited, records turned into targets, optimiser steps) runs inside the
+hipGraph.
+
+Bar: >= 95% of records / queries agree with the oracle (positions and
+throughputs within 1e-3 relative, mapped directions within 1e-4); the
+remaining ones are paths that diverge on an edge (FP32 vs FP64 geometry).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tfg_amd import nrc, pt, scenes
+
+pytestmark = pytest.mark.gpu
+
+W, H, MD, RR = 64, 64, 5, 2
+
+
+def _tracer(torch, sc, mode):
+    ds = pt.DeviceScene(sc, torch)
+    tr = pt.PathTracer(W, H, MD, RR, mode=mode, train_fraction=0.05, use_graph=True)
+    tr.set_scene(ds)
+    tr.set_camera(sc.camera.frame(W, H))
+    return ds, tr
+
+
+def _close(a, b, rtol, atol):
+    return np.all(np.abs(a - b) <= atol + rtol * np.abs(b), axis=1)
+
+
+def test_training_records_match_oracle(torch_cuda):
+    torch = torch_cuda
+    sc = scenes.cornell(level=3)
+    ds, tr = _tracer(torch, sc, 2)
+    img = torch.zeros((W * H, 3), device="cuda")
+    tr.render(0, img)
+    rec, _ = tr.read_cache_data()
+    ref = oracle.Renderer(sc, ds.host.nodes, ds.host.tris, W, H).render_nrc(0, MD, RR, 2)
+    heads = np.nonzero((ref["flags"] & 2) != 0)[0]
+    assert abs(rec.shape[0] - heads.size) <= 0.02 * heads.size
+    r = rec.reshape(-1, 11, 4)
+    pid = r[:, 9, 3].view(np.uint32)
+    assert np.unique(pid).size == pid.size
+    head_ok = _close(r[:, 0:4].reshape(-1, 16), ref["head"][pid], 1e-3, 1e-4)
+    lsuf_ok = _close(r[:, 8, :3], ref["lsuf"][pid], 1e-3, 1e-4)
+    norm_ok = _close(r[:, 10, :3], ref["norm"][pid], 1e-3, 1e-5)
+    tail_ok = _close(r[:, 4:8].reshape(-1, 16), ref["tail"][pid], 1e-3, 1e-4)
+    tsuf_ok = _close(r[:, 9, :3], ref["tsuf"][pid], 1e-3, 1e-5)
+    allok = head_ok & lsuf_ok & norm_ok & tail_ok & tsuf_ok
+    print("records", rec.shape[0], "agree", allok.mean(), head_ok.mean(), lsuf_ok.mean(), tail_ok.mean(), flush=True)
+    assert allok.mean() >= 0.95
+    # the training pixel colour is the prefix colour (render.h:251-253)
+    g = img.cpu().numpy()
+    pix_ok = _close(g[pid], ref["pixel"][pid], 1e-3, 1e-4)
+    assert pix_ok.mean() >= 0.95
+
+
+def test_queries_and_prefix_match_oracle(torch_cuda):
+    torch = torch_cuda
+    sc = scenes.cornell(level=3)
+    ds, tr = _tracer(torch, sc, 3)
+    cache = nrc.RadianceCache(batch=256)
+    cache.set_weights(np.zeros(21504, np.float32))   # predictions 0: image = prefix colour
+    tr.set_cache(cache)
+    img = torch.zeros((W * H, 3), device="cuda")
+    tr.render(3, img)
+    _, q = tr.read_cache_data()
+    ref = oracle.Renderer(sc, ds.host.nodes, ds.host.tris, W, H).render_nrc(3, MD, RR, 3)
+    need = np.nonzero((ref["flags"] & 4) | ((ref["flags"] & 3) == 3))[0]
+    assert abs(q.shape[0] - need.size) <= 0.02 * need.size
+    qq = q.reshape(-1, 5, 4)
+    pid = qq[:, 4, 3].view(np.uint32)
+    v_ok = _close(qq[:, 0:4].reshape(-1, 16), ref["head"][pid], 1e-3, 1e-4)
+    t_ok = _close(qq[:, 4, :3], ref["norm"][pid], 1e-3, 1e-5)
+    print("queries", q.shape[0], "agree", (v_ok & t_ok).mean(), flush=True)
+    assert (v_ok & t_ok).mean() >= 0.95
+    g = img.cpu().numpy()
+    assert _close(g, ref["pixel"], 1e-3, 1e-4).mean() >= 0.95
+
+
+def test_cache_frame_trains_and_composites(torch_cuda):
+    torch = torch_cuda
+    sc = scenes.cornell(level=3)
+    ds, tr = _tracer(torch, sc, 2)
+    cache = nrc.RadianceCache(batch=1024, steps_per_frame=4)
+    tr.set_cache(cache)
+    img = torch.zeros((W * H, 3), device="cuda")
+    for f in range(16):                     # warmup: 100% training paths
+        tr.render(f, img)
+    st = cache.stats()
+    assert st["steps"] == 64 and st["skipped_nonfinite"] == 0
+    tr.set_mode(3)
+    acc = torch.zeros((W * H, 3), dtype=torch.float64, device="cuda")
+    for f in range(16, 48):
+        tr.render(f, img, acc)
+    torch.cuda.synchronize()
+    st2 = cache.stats()
+    assert st2["steps"] > st["steps"] or st2["skipped_no_data"] > 0
+    a = acc.cpu().numpy() / 32
+    assert np.isfinite(a).all() and a.mean() > 0
+    # the cache adds light: the converged prefix-only image is darker
+    tr2 = pt.PathTracer(W, H, MD, RR, mode=3, use_graph=False)
+    tr2.set_scene(ds)
+    tr2.set_camera(sc.camera.frame(W, H))
+    zero = nrc.RadianceCache(batch=256)
+    zero.set_weights(np.zeros(21504, np.float32))
+    tr2.set_cache(zero)
+    acc2 = torch.zeros((W * H, 3), dtype=torch.float64, device="cuda")
+    for f in range(16, 48):
+        tr2.render(f, img, acc2)
+    torch.cuda.synchronize()
+    prefix = acc2.cpu().numpy() / 32
+    plain = oracle.Renderer(sc, ds.host.nodes, ds.host.tris, W, H).render(16, 32, MD, RR)[0] / 32
+    print("means: cache", a.mean(), "prefix only", prefix.mean(), "plain", plain.mean(), flush=True)
+    assert a.mean() > prefix.mean()
+    assert abs(a.mean() - plain.mean()) < 0.35 * plain.mean()

@@ -29,6 +29,12 @@ class CwbvhGpu(C.Structure):
     _fields_ = [("d_nodos", C.c_void_p), ("d_tris", C.c_void_p), ("num_nodos", C.c_uint32), ("num_tris", C.c_uint32)]
 
 
+class SceneBounds(C.Structure):
+    """SceneBounds (mlp_types.h:26-29), passed by value."""
+
+    _fields_ = [("min", C.c_double * 3), ("max", C.c_double * 3)]
+
+
 class PtConfig(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
                 ("mode", C.c_uint32), ("train_fraction", C.c_float), ("use_graph", C.c_uint32)]
@@ -46,6 +52,22 @@ class PtProfile(C.Structure):
                 ("ext_nodes", C.c_uint64), ("ext_tris", C.c_uint64), ("shadow_nodes", C.c_uint64),
                 ("shadow_tris", C.c_uint64)]
 
+
+class NrcConfig(C.Structure):
+    _fields_ = [("batch", C.c_uint32), ("steps_per_frame", C.c_uint32), ("min_records", C.c_uint32),
+                ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("ema_decay", C.c_float), ("loss_scale", C.c_float), ("bounds_min", C.c_float * 3),
+                ("bounds_max", C.c_float * 3), ("seed", C.c_uint32), ("precision", C.c_uint32)]
+
+
+class NrcStats(C.Structure):
+    _fields_ = [("steps", C.c_uint32), ("skipped_nonfinite", C.c_uint32), ("skipped_no_data", C.c_uint32),
+                ("seed_counter", C.c_uint32), ("last_loss", C.c_float)]
+
+
+TFG_NRC_BF16 = 0
+TFG_NRC_FP16 = 1
+TFG_NRC_PARAMS = 21504
 
 TFG_PT_INFER = 1
 TFG_PT_TRAIN = 2
@@ -87,6 +109,31 @@ _SIGS = {
     "tfg_pt_get_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                     C.POINTER(C.c_uint32)]),
     "tfg_pt_destroy": (C.c_int, [C.c_void_p]),
+    "tfg_pt_set_nrc": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tfg_pt_read_cache_data": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                         C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "tfg_pt_set_mode": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "tfg_pt_nrc_buffers": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                     C.POINTER(C.c_void_p)]),
+    "tfg_nrc_default_config": (C.c_int, [C.POINTER(NrcConfig)]),
+    "tfg_nrc_create": (C.c_int, [C.POINTER(NrcConfig), C.POINTER(C.c_void_p)]),
+    "tfg_nrc_destroy": (C.c_int, [C.c_void_p]),
+    "tfg_nrc_set_weights": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tfg_nrc_get_weights": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tfg_nrc_get_stats": (C.c_int, [C.c_void_p, C.POINTER(NrcStats)]),
+    "tfg_nrc_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "tfg_nrc_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "tfg_nrc_last_gradient": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tfg_launch_inicializar_camara": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "tfg_launch_composite": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "tfg_launch_calcular_targets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "tfg_launch_preparar_inferencia_tail": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "tfg_launch_generar_indices": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_void_p]),
+    "tfg_launch_preparar_entrenamiento": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                                    SceneBounds, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tfg_launch_preparar_inferencia": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, SceneBounds,
+                                                 C.c_void_p]),
+    "tfg_launch_guardar_salida": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
 }
 
 

@@ -1,14 +1,646 @@
-// nrc.hip - radiance cache (placeholder until the MLP kernels land).
+// nrc.hip - online-trained radiance cache: fused MFMA inference, fused
+// training step (tail inference + targets + forward + backward + per-block
+// weight gradients), gradient reduction and Adam/EMA on FP32 master weights.
+//
+// Restates mlp.cpp:95-170 (train_step / inference), NrcKernel.cu:10-126
+// (batch indices, input packing, output sanitising) and
+// RenderKernel.cu:92-166 (tail inference and training targets) as one
+// graph-capturable sequence per frame.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <random>
+#include <vector>
+
 #include "host_common.h"
+#include "nrc_device.h"
 #include "nrc_iface.h"
+#include "pt_internal.h"
+#include "shade_device.h"
+
+using namespace tfg;
 
 struct tfg_nrc {
-    int unused;
+    tfg_nrc_config cfg{};
+    NrcBounds bounds{};
+    // FP32 optimiser state (kNrcParams each)
+    float* master = nullptr;
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    float* ema = nullptr;
+    float* grad = nullptr;
+    float* partial = nullptr;   // (batch / 64) x kNrcParams
+    // operand layouts, 16-bit (bf16 or fp16 by cfg.precision)
+    void* fop_train = nullptr;
+    void* bop_train = nullptr;
+    void* fop_infer = nullptr;
+    // device scalars: [0] step count t, [1] seed counter, [2] flags of the
+    // current step, [3] skipped (non-finite) steps, [4] no-data steps,
+    // [5] loss of the last step (float bits), [6] steps taken
+    uint32_t* state = nullptr;
+    float* loss_accum = nullptr;
+    int n_cu = 0;
 };
 
 namespace tfg {
-int nrc_frame_enqueue(tfg_nrc*, const tfg_nrc_frame&, tfg_color*, hipStream_t) {
-    set_error("radiance cache not available");
-    return TFG_E_STATE;
+namespace {
+
+constexpr int kStLt = 0, kStSeed = 1, kStFlags = 2, kStSkipNan = 3, kStSkipData = 4, kStLoss = 5, kStSteps = 6;
+constexpr size_t kOpBytes16 = 16;   // one operand tile entry: 8 x 16-bit per lane
+
+struct Hyper {
+    float lr, beta1, beta2, eps, ema_decay, inv_loss_scale;
+};
+
+__device__ __forceinline__ uint32_t pcg32_step(uint32_t s) {
+    s = s * 747796405u + 2891336453u;
+    const uint32_t word = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (word >> 22u) ^ word;
 }
+
+// Element offsets (in 16-bit units) of parameter p in the operand layouts.
+__host__ __device__ inline void slot_of(int idx, int& c, int& g, int& j) {
+    c = idx / 32;
+    const int r = idx % 32;
+    if (r < 16) { g = r / 4; j = r % 4; }
+    else { g = (r - 16) / 4; j = 4 + (r - 16) % 4; }
+}
+__host__ __device__ inline int param_layer(int p) { return p < 5 * 4096 ? p / 4096 : 5; }
+__host__ __device__ inline size_t fop_pos(int p) {
+    const int l = param_layer(p);
+    const int r = p - nrc_layer_offset(l);
+    const int n = r / 64, k = r % 64;
+    int c, g, j;
+    slot_of(k, c, g, j);
+    const int lane = 16 * g + n % 16;
+    return ((size_t)(fop_tile_base(l) + 2 * (n / 16) + c) * 64 + lane) * 8 + j;
+}
+__host__ __device__ inline size_t bop_pos(int p) {   // layers 1..5
+    const int l = param_layer(p);
+    const int r = p - nrc_layer_offset(l);
+    const int n = r / 64, k = r % 64;
+    int c, g, j;
+    slot_of(n, c, g, j);
+    const int lane = 16 * g + k % 16;
+    const int u = k / 16;
+    const int tile = bop_tile_base(l) + (l < 5 ? 2 * u + c : u);
+    return ((size_t)tile * 64 + lane) * 8 + j;
+}
+
+template <typename T>
+__device__ __forceinline__ void load_ops(typename MfmaOps<T>::vec* dst, const typename MfmaOps<T>::vec* src,
+                                         int tiles) {
+    for (int i = threadIdx.x; i < tiles * 64; i += blockDim.x) dst[i] = src[i];
+}
+
+// ------------------------------------------------------------- inference
+// Queries (kQueryF4 float4: vertex + (throughput, path id)) -> cache
+// prediction, sanitised, composited into the image (RenderKernel.cu:49-66).
+template <typename T, int G>
+__global__ __launch_bounds__(256) void k_nrc_infer(const float4* __restrict__ queries, const uint32_t* __restrict__ count,
+                                                   tfg_color* __restrict__ image,
+                                                   const typename MfmaOps<T>::vec* __restrict__ fop, NrcBounds b) {
+    typedef typename MfmaOps<T>::vec V;
+    __shared__ V W[kFopTiles * 64];
+    load_ops<T>(W, fop, kFopTiles);
+    __syncthreads();
+    const uint32_t n = *count;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, ml = lane & 15;
+    for (uint32_t base = (blockIdx.x * 4 + w) * 16u * G; base < n; base += gridDim.x * 4 * 16u * G) {
+        V x[G][2];
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const uint32_t s = base + 16 * q + ml;
+            if (s < n) {
+                const float4* Q = queries + (size_t)kQueryF4 * s;
+                encode<T>(Q[0], Q[1], Q[2], Q[3], b, g, x[q][0], x[q][1]);
+            } else {
+                x[q][0] = V{};
+                x[q][1] = V{};
+            }
+        }
+        floatx4 out[G];
+        mlp_forward<T, G>(W, lane, x, out);
+        if (g == 0) {
+#pragma unroll
+            for (int q = 0; q < G; ++q) {
+                const uint32_t s = base + 16 * q + ml;
+                if (s >= n) continue;
+                const float4 th = queries[(size_t)kQueryF4 * s + 4];
+                if (th.x > 0.0f || th.y > 0.0f || th.z > 0.0f) {
+                    const uint32_t pid = __float_as_uint(th.w);
+                    tfg_color c = image[pid];
+                    c.r += th.x * sanitize_out(out[q][0]);
+                    c.g += th.y * sanitize_out(out[q][1]);
+                    c.b += th.z * sanitize_out(out[q][2]);
+                    image[pid] = c;
+                }
+            }
+        }
+    }
+}
+
+// Vertices (4 float4 each) -> raw outputs (3 floats each); optionally the
+// pre-activation of every layer (n x 6 x 64 floats) for the oracle checks.
+template <typename T>
+__global__ __launch_bounds__(256) void k_nrc_eval(const float4* __restrict__ verts, uint32_t n, float* __restrict__ out,
+                                                  float* __restrict__ dump,
+                                                  const typename MfmaOps<T>::vec* __restrict__ fop, NrcBounds b) {
+    typedef MfmaOps<T> M;
+    typedef typename M::vec V;
+    __shared__ V W[kFopTiles * 64];
+    load_ops<T>(W, fop, kFopTiles);
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, ml = lane & 15;
+    for (uint32_t base = (blockIdx.x * 4 + w) * 16u; base < n; base += gridDim.x * 64u) {
+        const uint32_t s = base + ml;
+        V x[2] = {V{}, V{}};
+        if (s < n) encode<T>(verts[4 * (size_t)s], verts[4 * (size_t)s + 1], verts[4 * (size_t)s + 2],
+                             verts[4 * (size_t)s + 3], b, g, x[0], x[1]);
+        for (int l = 0; l < kNrcLayers; ++l) {
+            const int ntiles = l < 5 ? 4 : 1;
+            floatx4 acc[4];
+            for (int t = 0; t < 4; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+            for (int t = 0; t < ntiles; ++t)
+                for (int c = 0; c < 2; ++c) acc[t] = M::mfma(W[(fop_tile_base(l) + 2 * t + c) * 64 + lane], x[c], acc[t]);
+            if (dump && s < n)
+                for (int t = 0; t < ntiles; ++t)
+                    for (int i = 0; i < 4; ++i) dump[((size_t)s * kNrcLayers + l) * 64 + 16 * t + 4 * g + i] = acc[t][i];
+            if (l < 5) {
+                x[0] = pack_relu<T>(acc[0], acc[1]);
+                x[1] = pack_relu<T>(acc[2], acc[3]);
+            } else if (g == 0 && s < n) {
+                for (int i = 0; i < 3; ++i) out[3 * (size_t)s + i] = acc[0][i];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------- training
+struct TrainArgs {
+    const float4* records;
+    const uint32_t* n_rec;
+    uint32_t min_records;
+    uint32_t* state;
+    uint32_t batch;
+    float grad_scale;   // loss_scale / (3 * batch)
+    float loss_norm;    // 1 / (3 * batch)
+    NrcBounds b;
+    const void* fop_train;
+    const void* bop_train;
+    const void* fop_infer;
+    float* partial;
+    float* loss_accum;
+};
+
+constexpr size_t kTrainLds = (size_t)(2 * kFopTiles + kBopTiles) * 64 * kOpBytes16 + 2 * 64 * 64 * 2;
+
+// One block = 64 training samples (4 waves x 16).  Tail inference with the
+// inference (EMA) weights, targets (RenderKernel.cu:92-137), head forward
+// with the training weights, RelativeL2Luminance gradient, backward through
+// all layers and this block's weight gradient (summed over its samples).
+template <typename T>
+__global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
+    typedef MfmaOps<T> M;
+    typedef typename M::vec V;
+    extern __shared__ __align__(16) unsigned char smem[];
+    V* s_fop = reinterpret_cast<V*>(smem);
+    V* s_inf = s_fop + kFopTiles * 64;
+    V* s_bop = s_inf + kFopTiles * 64;
+    T* s_dyT = reinterpret_cast<T*>(s_bop + kBopTiles * 64);   // [64 neurons][64 samples]
+    T* s_xT = s_dyT + 64 * 64;                                 // [64 inputs][64 samples]
+    const uint32_t n_rec = *A.n_rec;
+    if (n_rec <= A.min_records) return;
+    load_ops<T>(s_fop, static_cast<const V*>(A.fop_train), kFopTiles);
+    load_ops<T>(s_inf, static_cast<const V*>(A.fop_infer), kFopTiles);
+    load_ops<T>(s_bop, static_cast<const V*>(A.bop_train), kBopTiles);
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, ml = lane & 15;
+    const int m = 16 * w + ml;   // sample within the block
+    const uint32_t i = blockIdx.x * 64 + m;
+    // generarIndicesAleatorios (NrcKernel.cu:10-16): seed = step counter
+    const uint32_t idx = pcg32_step(A.state[kStSeed] ^ i) % n_rec;
+    const float4* R = A.records + (size_t)kRecF4 * idx;
+
+    V x[1][2];
+    floatx4 out[1];
+    encode<T>(R[4], R[5], R[6], R[7], A.b, g, x[0][0], x[0][1]);
+    mlp_forward<T, 1>(s_inf, lane, x, out);
+    float tgt[3] = {0.f, 0.f, 0.f};
+    if (g == 0) {
+        const float4 ls = R[8], ts = R[9], nm = R[10];
+        const float lsv[3] = {ls.x, ls.y, ls.z}, tsv[3] = {ts.x, ts.y, ts.z}, nmv[3] = {nm.x, nm.y, nm.z};
+        for (int c = 0; c < 3; ++c) {
+            float t = (lsv[c] + tsv[c] * sanitize_out(out[0][c])) / fmaxf(nmv[c], 1e-8f);
+            t = fminf(t, 100000.0f);
+            if (isnan(t) || isinf(t)) t = 0.0f;
+            tgt[c] = t;
+        }
+    }
+    encode<T>(R[0], R[1], R[2], R[3], A.b, g, x[0][0], x[0][1]);
+    V xs[1][kNrcLayers][2];
+    mlp_forward<T, 1>(s_fop, lane, x, out, xs);
+    floatx4 dy5 = floatx4{0.f, 0.f, 0.f, 0.f};
+    float loss = 0.0f;
+    if (g == 0) {   // RelativeL2Luminance: (p - t)^2 / (lum(p)^2 + 0.01), lum held constant
+        const float lum = kLumR * out[0][0] + kLumG * out[0][1] + kLumB * out[0][2];
+        const float inv = 1.0f / (lum * lum + 0.01f);
+        for (int c = 0; c < 3; ++c) {
+            const float d = out[0][c] - tgt[c];
+            loss += d * d * inv;
+            dy5[c] = 2.0f * d * inv * A.grad_scale;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o);
+    if (lane == 0) atomicAdd(A.loss_accum, loss * A.loss_norm);
+
+    float* part = A.partial + (size_t)blockIdx.x * kNrcParams;
+    // ---- layer 5 (16 x 64) ----
+    for (int k = threadIdx.x; k < 16 * 64; k += 256) s_dyT[k] = (T)0.0f;   // rows 4..15 stay zero
+    __syncthreads();
+    for (int ii = 0; ii < 4; ++ii) s_dyT[(4 * g + ii) * 64 + m] = (T)dy5[ii];
+    for (int c = 0; c < 2; ++c)
+        for (int j = 0; j < 8; ++j) s_xT[nrc_perm(c, g, j) * 64 + m] = xs[0][5][c][j];
+    __syncthreads();
+    {
+        const int u = w;   // k-tile of this wave
+        floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < 2; ++c) {
+            const V a = *reinterpret_cast<const V*>(&s_dyT[ml * 64 + 32 * c + 8 * g]);
+            const V bb = *reinterpret_cast<const V*>(&s_xT[(16 * u + ml) * 64 + 32 * c + 8 * g]);
+            acc = M::mfma(a, bb, acc);
+        }
+        for (int ii = 0; ii < 4; ++ii) part[nrc_layer_offset(5) + (4 * g + ii) * 64 + 16 * u + ml] = acc[ii];
+    }
+    floatx4 dy[4];
+    {
+        V bdy;
+        for (int j = 0; j < 4; ++j) { bdy[j] = (T)dy5[j]; bdy[4 + j] = (T)0.0f; }
+        for (int u = 0; u < 4; ++u) {
+            dy[u] = M::mfma(s_bop[(bop_tile_base(5) + u) * 64 + lane], bdy, floatx4{0.f, 0.f, 0.f, 0.f});
+            for (int ii = 0; ii < 4; ++ii)
+                if (!((float)xs[0][5][u >> 1][(u & 1) * 4 + ii] > 0.0f)) dy[u][ii] = 0.0f;
+        }
+    }
+    __syncthreads();
+    // ---- layers 4..0 ----
+    for (int l = 4; l >= 0; --l) {
+        for (int t = 0; t < 4; ++t)
+            for (int ii = 0; ii < 4; ++ii) s_dyT[(16 * t + 4 * g + ii) * 64 + m] = (T)dy[t][ii];
+        for (int c = 0; c < 2; ++c)
+            for (int j = 0; j < 8; ++j) s_xT[nrc_perm(c, g, j) * 64 + m] = xs[0][l][c][j];
+        __syncthreads();
+        for (int u = 0; u < 4; ++u) {
+            floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+            for (int c = 0; c < 2; ++c) {
+                const V a = *reinterpret_cast<const V*>(&s_dyT[(16 * w + ml) * 64 + 32 * c + 8 * g]);
+                const V bb = *reinterpret_cast<const V*>(&s_xT[(16 * u + ml) * 64 + 32 * c + 8 * g]);
+                acc = M::mfma(a, bb, acc);
+            }
+            for (int ii = 0; ii < 4; ++ii) part[nrc_layer_offset(l) + (16 * w + 4 * g + ii) * 64 + 16 * u + ml] = acc[ii];
+        }
+        if (l > 0) {
+            V b0, b1;
+            for (int ii = 0; ii < 4; ++ii) {
+                b0[ii] = (T)dy[0][ii];
+                b0[4 + ii] = (T)dy[1][ii];
+                b1[ii] = (T)dy[2][ii];
+                b1[4 + ii] = (T)dy[3][ii];
+            }
+            for (int u = 0; u < 4; ++u) {
+                floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+                acc = M::mfma(s_bop[(bop_tile_base(l) + 2 * u) * 64 + lane], b0, acc);
+                acc = M::mfma(s_bop[(bop_tile_base(l) + 2 * u + 1) * 64 + lane], b1, acc);
+                for (int ii = 0; ii < 4; ++ii)
+                    if (!((float)xs[0][l][u >> 1][(u & 1) * 4 + ii] > 0.0f)) acc[ii] = 0.0f;
+                dy[u] = acc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Sums the per-block gradients; flags non-finite values or a skipped step.
+__global__ __launch_bounds__(256) void k_nrc_grad_reduce(const float* __restrict__ partial, uint32_t nblk,
+                                                         float* __restrict__ grad, uint32_t* state,
+                                                         const uint32_t* __restrict__ n_rec, uint32_t min_records,
+                                                         float inv_scale) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (*n_rec <= min_records) {
+        if (p == 0) atomicOr(&state[kStFlags], 2u);
+        return;
+    }
+    if (p >= kNrcParams) return;
+    float s = 0.0f;
+    for (uint32_t b = 0; b < nblk; ++b) s += partial[(size_t)b * kNrcParams + p];
+    s *= inv_scale;
+    grad[p] = s;
+    if (!isfinite(s)) atomicOr(&state[kStFlags], 1u);
+}
+
+template <typename T>
+__device__ __forceinline__ void put16(void* base, size_t pos, float v) { static_cast<T*>(base)[pos] = (T)v; }
+
+// Adam (Kingma & Ba 2015) on the FP32 master weights, EMA of the weights for
+// inference (mlp.cpp:61-69: EMA(0.99) over Adam(lr 1e-2)), then the 16-bit
+// operand layouts.  Skipped entirely when the step was flagged.
+template <typename T>
+__global__ __launch_bounds__(256) void k_nrc_adam(float* __restrict__ master, float* __restrict__ am,
+                                                  float* __restrict__ av, float* __restrict__ ema,
+                                                  const float* __restrict__ grad, const uint32_t* state, Hyper h,
+                                                  void* fop_train, void* bop_train, void* fop_infer) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kNrcParams || state[kStFlags] != 0u) return;
+    const float t = (float)(state[kStLt] + 1u);
+    const float g = grad[p];
+    const float m = h.beta1 * am[p] + (1.0f - h.beta1) * g;
+    const float v = h.beta2 * av[p] + (1.0f - h.beta2) * g * g;
+    const float mh = m / (1.0f - powf(h.beta1, t));
+    const float vh = v / (1.0f - powf(h.beta2, t));
+    const float w = master[p] - h.lr * mh / (sqrtf(vh) + h.eps);
+    const float e = h.ema_decay * ema[p] + (1.0f - h.ema_decay) * w;
+    am[p] = m;
+    av[p] = v;
+    master[p] = w;
+    ema[p] = e;
+    put16<T>(fop_train, fop_pos(p), w);
+    if (param_layer(p) >= 1) put16<T>(bop_train, bop_pos(p), w);
+    put16<T>(fop_infer, fop_pos(p), e);
+}
+
+__global__ void k_nrc_step_begin(uint32_t* state, float* loss_accum) {
+    if (threadIdx.x == 0) {
+        state[kStFlags] = 0u;
+        *loss_accum = 0.0f;
+    }
+}
+
+__global__ void k_nrc_step_end(uint32_t* state, const float* loss_accum) {
+    if (threadIdx.x == 0) {
+        const uint32_t f = state[kStFlags];
+        state[kStSeed] += 1u;   // mlp.cpp:102 step_counter++ on every call
+        if (f == 0u) {
+            state[kStLt] += 1u;
+            state[kStSteps] += 1u;
+            state[kStLoss] = __float_as_uint(*loss_accum);
+        } else if (f & 1u) {
+            state[kStSkipNan] += 1u;
+        } else {
+            state[kStSkipData] += 1u;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_nrc_publish(const float* __restrict__ master, const float* __restrict__ ema,
+                                                     void* fop_train, void* bop_train, void* fop_infer) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kNrcParams) return;
+    put16<T>(fop_train, fop_pos(p), master[p]);
+    if (param_layer(p) >= 1) put16<T>(bop_train, bop_pos(p), master[p]);
+    put16<T>(fop_infer, fop_pos(p), ema[p]);
+}
+
+template <typename T>
+int enqueue_train_steps(tfg_nrc* nc, const float4* records, const uint32_t* n_rec, uint32_t steps, hipStream_t s) {
+    const uint32_t nblk = nc->cfg.batch / 64;
+    TrainArgs A;
+    A.records = records;
+    A.n_rec = n_rec;
+    A.min_records = nc->cfg.min_records;
+    A.state = nc->state;
+    A.batch = nc->cfg.batch;
+    A.grad_scale = nc->cfg.loss_scale / (3.0f * (float)nc->cfg.batch);
+    A.loss_norm = 1.0f / (3.0f * (float)nc->cfg.batch);
+    A.b = nc->bounds;
+    A.fop_train = nc->fop_train;
+    A.bop_train = nc->bop_train;
+    A.fop_infer = nc->fop_infer;
+    A.partial = nc->partial;
+    A.loss_accum = nc->loss_accum;
+    const Hyper h{nc->cfg.lr, nc->cfg.beta1, nc->cfg.beta2, nc->cfg.eps, nc->cfg.ema_decay, 1.0f / nc->cfg.loss_scale};
+    const int pblocks = (kNrcParams + 255) / 256;
+    for (uint32_t k = 0; k < steps; ++k) {
+        hipLaunchKernelGGL(k_nrc_step_begin, dim3(1), dim3(64), 0, s, nc->state, nc->loss_accum);
+        hipLaunchKernelGGL(k_nrc_train<T>, dim3(nblk), dim3(256), kTrainLds, s, A);
+        hipLaunchKernelGGL(k_nrc_grad_reduce, dim3(pblocks), dim3(256), 0, s, nc->partial, nblk, nc->grad, nc->state,
+                           n_rec, nc->cfg.min_records, h.inv_loss_scale);
+        hipLaunchKernelGGL(k_nrc_adam<T>, dim3(pblocks), dim3(256), 0, s, nc->master, nc->adam_m, nc->adam_v, nc->ema,
+                           nc->grad, nc->state, h, nc->fop_train, nc->bop_train, nc->fop_infer);
+        hipLaunchKernelGGL(k_nrc_step_end, dim3(1), dim3(64), 0, s, nc->state, nc->loss_accum);
+    }
+    TFG_HIP_CHECK(hipGetLastError());
+    return TFG_OK;
+}
+
+template <typename T>
+int enqueue_infer(tfg_nrc* nc, const float4* queries, const uint32_t* count, tfg_color* image, hipStream_t s) {
+    typedef typename MfmaOps<T>::vec V;
+    hipLaunchKernelGGL((k_nrc_infer<T, 4>), dim3(nc->n_cu * 4), dim3(256), 0, s, queries, count, image,
+                       static_cast<const V*>(nc->fop_infer), nc->bounds);
+    TFG_HIP_CHECK(hipGetLastError());
+    return TFG_OK;
+}
+
+template <typename T>
+int setup_lds() {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_nrc_train<T>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTrainLds);
+    return e == hipSuccess ? TFG_OK : hip_fail(e, "hipFuncSetAttribute(k_nrc_train)");
+}
+
+}  // namespace
+
+int nrc_frame_enqueue(tfg_nrc* nc, const tfg_nrc_frame& f, tfg_color* image, hipStream_t s) {
+    const bool fp16 = nc->cfg.precision == TFG_NRC_FP16;
+    int rc = TFG_OK;
+    if (f.mode & MODE_INFER) {
+        rc = fp16 ? enqueue_infer<_Float16>(nc, f.queries, f.counters + CNT_QUERIES, image, s)
+                  : enqueue_infer<__bf16>(nc, f.queries, f.counters + CNT_QUERIES, image, s);
+        if (rc != TFG_OK) return rc;
+    }
+    if ((f.mode & MODE_TRAIN) && nc->cfg.steps_per_frame > 0) {
+        rc = fp16 ? enqueue_train_steps<_Float16>(nc, f.records, f.counters + CNT_RECORDS, nc->cfg.steps_per_frame, s)
+                  : enqueue_train_steps<__bf16>(nc, f.records, f.counters + CNT_RECORDS, nc->cfg.steps_per_frame, s);
+    }
+    return rc;
+}
+
 }  // namespace tfg
+
+// ===================================================================== C ABI
+extern "C" {
+
+int tfg_nrc_default_config(tfg_nrc_config* c) {
+    if (!c) { set_error("tfg_nrc_default_config: NULL"); return TFG_E_INVALID; }
+    std::memset(c, 0, sizeof(*c));
+    c->batch = 16384;            // main.cpp:939
+    c->steps_per_frame = 4;      // 4 x 2^14 = 2^16 training samples per frame
+    c->min_records = 1024;       // main.cpp:1043
+    c->lr = 1e-2f;               // mlp.cpp:66
+    c->beta1 = 0.9f;
+    c->beta2 = 0.99f;
+    c->eps = 1e-8f;
+    c->ema_decay = 0.99f;        // mlp.cpp:62
+    c->loss_scale = 128.0f;
+    c->bounds_min[0] = -6.0f; c->bounds_min[1] = -4.0f; c->bounds_min[2] = -10.0f;   // main.cpp:870-872
+    c->bounds_max[0] = 6.0f;  c->bounds_max[1] = 6.0f;  c->bounds_max[2] = 2.0f;
+    c->seed = 1337u;
+    c->precision = TFG_NRC_BF16;
+    return TFG_OK;
+}
+
+int tfg_nrc_create(const tfg_nrc_config* cfg, tfg_nrc** out) {
+    if (!cfg || !out) { set_error("tfg_nrc_create: NULL argument"); return TFG_E_INVALID; }
+    *out = nullptr;
+    if (cfg->batch == 0 || cfg->batch % 64 != 0 || cfg->batch > (1u << 22)) {
+        set_error("tfg_nrc_create: batch must be a positive multiple of 64 (<= 2^22)");
+        return TFG_E_INVALID;
+    }
+    if (cfg->precision != TFG_NRC_BF16 && cfg->precision != TFG_NRC_FP16) {
+        set_error("tfg_nrc_create: unknown precision %u", cfg->precision);
+        return TFG_E_INVALID;
+    }
+    if (!(cfg->loss_scale > 0.0f) || !(cfg->lr > 0.0f)) { set_error("tfg_nrc_create: bad hyper-parameters"); return TFG_E_INVALID; }
+    for (int a = 0; a < 3; ++a)
+        if (!(cfg->bounds_max[a] > cfg->bounds_min[a])) { set_error("tfg_nrc_create: empty scene bounds"); return TFG_E_INVALID; }
+    auto* nc = new (std::nothrow) tfg_nrc();
+    if (!nc) { set_error("tfg_nrc_create: out of memory"); return TFG_E_NOMEM; }
+    nc->cfg = *cfg;
+    for (int a = 0; a < 3; ++a) {
+        nc->bounds.mn[a] = cfg->bounds_min[a];
+        nc->bounds.inv_ext[a] = 1.0f / (cfg->bounds_max[a] - cfg->bounds_min[a]);
+    }
+    const size_t pb = kNrcParams * sizeof(float);
+    hipError_t e = hipSuccess;
+    void** bufs[] = {(void**)&nc->master, (void**)&nc->adam_m, (void**)&nc->adam_v, (void**)&nc->ema, (void**)&nc->grad};
+    for (void** b : bufs)
+        if (e == hipSuccess) e = hipMalloc(b, pb);
+    if (e == hipSuccess) e = hipMalloc((void**)&nc->partial, (size_t)(cfg->batch / 64) * pb);
+    if (e == hipSuccess) e = hipMalloc(&nc->fop_train, (size_t)kFopTiles * 64 * kOpBytes16);
+    if (e == hipSuccess) e = hipMalloc(&nc->bop_train, (size_t)kBopTiles * 64 * kOpBytes16);
+    if (e == hipSuccess) e = hipMalloc(&nc->fop_infer, (size_t)kFopTiles * 64 * kOpBytes16);
+    if (e == hipSuccess) e = hipMalloc((void**)&nc->state, 16 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&nc->loss_accum, sizeof(float));
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) {
+        tfg_nrc_destroy(nc);
+        return hip_fail(e, "tfg_nrc_create");
+    }
+    nc->n_cu = prop.multiProcessorCount;
+    int rc = cfg->precision == TFG_NRC_FP16 ? setup_lds<_Float16>() : setup_lds<__bf16>();
+    if (rc != TFG_OK) {
+        tfg_nrc_destroy(nc);
+        return rc;
+    }
+    // Xavier-uniform initialisation (seeded); output rows 3..15 are padding.
+    std::vector<float> w(kNrcParams, 0.0f);
+    std::mt19937 gen(cfg->seed);
+    for (int l = 0; l < kNrcLayers; ++l) {
+        const int rows = l < 5 ? 64 : kNrcOut;
+        const float a = std::sqrt(6.0f / (64.0f + (float)(l < 5 ? 64 : kNrcOut)));
+        std::uniform_real_distribution<float> U(-a, a);
+        for (int n = 0; n < rows; ++n)
+            for (int k = 0; k < 64; ++k) w[nrc_layer_offset(l) + n * 64 + k] = U(gen);
+    }
+    rc = tfg_nrc_set_weights(nc, w.data(), w.data());
+    if (rc != TFG_OK) {
+        tfg_nrc_destroy(nc);
+        return rc;
+    }
+    *out = nc;
+    return TFG_OK;
+}
+
+int tfg_nrc_set_weights(tfg_nrc* nc, const float* master, const float* ema) {
+    if (!nc || !master) { set_error("tfg_nrc_set_weights: NULL argument"); return TFG_E_INVALID; }
+    const size_t pb = kNrcParams * sizeof(float);
+    TFG_HIP_CHECK(hipMemcpy(nc->master, master, pb, hipMemcpyHostToDevice));
+    TFG_HIP_CHECK(hipMemcpy(nc->ema, ema ? ema : master, pb, hipMemcpyHostToDevice));
+    TFG_HIP_CHECK(hipMemset(nc->adam_m, 0, pb));
+    TFG_HIP_CHECK(hipMemset(nc->adam_v, 0, pb));
+    TFG_HIP_CHECK(hipMemset(nc->fop_train, 0, (size_t)kFopTiles * 64 * kOpBytes16));
+    TFG_HIP_CHECK(hipMemset(nc->bop_train, 0, (size_t)kBopTiles * 64 * kOpBytes16));
+    TFG_HIP_CHECK(hipMemset(nc->fop_infer, 0, (size_t)kFopTiles * 64 * kOpBytes16));
+    TFG_HIP_CHECK(hipMemset(nc->state, 0, 16 * sizeof(uint32_t)));
+    const int pblocks = (kNrcParams + 255) / 256;
+    if (nc->cfg.precision == TFG_NRC_FP16)
+        hipLaunchKernelGGL(k_nrc_publish<_Float16>, dim3(pblocks), dim3(256), 0, nullptr, nc->master, nc->ema,
+                           nc->fop_train, nc->bop_train, nc->fop_infer);
+    else
+        hipLaunchKernelGGL(k_nrc_publish<__bf16>, dim3(pblocks), dim3(256), 0, nullptr, nc->master, nc->ema,
+                           nc->fop_train, nc->bop_train, nc->fop_infer);
+    TFG_HIP_CHECK(hipGetLastError());
+    TFG_HIP_CHECK(hipDeviceSynchronize());
+    return TFG_OK;
+}
+
+int tfg_nrc_get_weights(tfg_nrc* nc, float* master, float* ema) {
+    if (!nc) { set_error("tfg_nrc_get_weights: NULL"); return TFG_E_INVALID; }
+    TFG_HIP_CHECK(hipDeviceSynchronize());
+    const size_t pb = kNrcParams * sizeof(float);
+    if (master) TFG_HIP_CHECK(hipMemcpy(master, nc->master, pb, hipMemcpyDeviceToHost));
+    if (ema) TFG_HIP_CHECK(hipMemcpy(ema, nc->ema, pb, hipMemcpyDeviceToHost));
+    return TFG_OK;
+}
+
+int tfg_nrc_get_stats(tfg_nrc* nc, tfg_nrc_stats* st) {
+    if (!nc || !st) { set_error("tfg_nrc_get_stats: NULL argument"); return TFG_E_INVALID; }
+    uint32_t s[16];
+    TFG_HIP_CHECK(hipDeviceSynchronize());
+    TFG_HIP_CHECK(hipMemcpy(s, nc->state, sizeof(s), hipMemcpyDeviceToHost));
+    st->steps = s[kStSteps];
+    st->skipped_nonfinite = s[kStSkipNan];
+    st->skipped_no_data = s[kStSkipData];
+    std::memcpy(&st->last_loss, &s[kStLoss], sizeof(float));
+    st->seed_counter = s[kStSeed];
+    return TFG_OK;
+}
+
+int tfg_nrc_eval(tfg_nrc* nc, const float* d_verts, uint32_t n, float* d_out, float* d_dump, int use_train_weights,
+                 tfg_stream stream) {
+    if (!nc || !d_verts || !d_out) { set_error("tfg_nrc_eval: NULL argument"); return TFG_E_INVALID; }
+    if (n == 0) return TFG_OK;
+    const void* fop = use_train_weights ? nc->fop_train : nc->fop_infer;
+    const dim3 grid((n + 63) / 64);
+    if (nc->cfg.precision == TFG_NRC_FP16)
+        hipLaunchKernelGGL(k_nrc_eval<_Float16>, grid, dim3(256), 0, stream, reinterpret_cast<const float4*>(d_verts),
+                           n, d_out, d_dump, static_cast<const f16x8*>(fop), nc->bounds);
+    else
+        hipLaunchKernelGGL(k_nrc_eval<__bf16>, grid, dim3(256), 0, stream, reinterpret_cast<const float4*>(d_verts), n,
+                           d_out, d_dump, static_cast<const bf16x8*>(fop), nc->bounds);
+    TFG_HIP_CHECK(hipGetLastError());
+    return TFG_OK;
+}
+
+int tfg_nrc_train(tfg_nrc* nc, const float* d_records, const uint32_t* d_n_records, uint32_t steps, tfg_stream stream) {
+    if (!nc || !d_records || !d_n_records) { set_error("tfg_nrc_train: NULL argument"); return TFG_E_INVALID; }
+    const float4* r = reinterpret_cast<const float4*>(d_records);
+    return nc->cfg.precision == TFG_NRC_FP16 ? enqueue_train_steps<_Float16>(nc, r, d_n_records, steps, stream)
+                                             : enqueue_train_steps<__bf16>(nc, r, d_n_records, steps, stream);
+}
+
+int tfg_nrc_last_gradient(tfg_nrc* nc, float* host_grad) {
+    if (!nc || !host_grad) { set_error("tfg_nrc_last_gradient: NULL argument"); return TFG_E_INVALID; }
+    TFG_HIP_CHECK(hipDeviceSynchronize());
+    TFG_HIP_CHECK(hipMemcpy(host_grad, nc->grad, kNrcParams * sizeof(float), hipMemcpyDeviceToHost));
+    return TFG_OK;
+}
+
+int tfg_nrc_destroy(tfg_nrc* nc) {
+    if (!nc) return TFG_OK;
+    (void)hipDeviceSynchronize();
+    void* bufs[] = {nc->master, nc->adam_m, nc->adam_v, nc->ema, nc->grad, nc->partial, nc->fop_train,
+                    nc->bop_train, nc->fop_infer, nc->state, nc->loss_accum};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    delete nc;
+    return TFG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,196 @@
+// nrc_device.h - radiance-cache MLP on gfx950 matrix cores.
+//
+// Network (restating the reference's tcnn model, mlp.cpp:22-57, with the
+// 12-frequency position encoding the reference keeps as its commented-out
+// alternative, mlp.cpp:26-30): 64 encoded inputs -> 5 hidden layers of 64
+// ReLU neurons -> 3 linear outputs, no biases (a constant-1 input feature
+// gives the first layer an offset).  Weights are held in FP32 (master),
+// used as BF16 or FP16 operands with FP32 accumulation.
+//
+// Matrix-core mapping.  Every layer is computed transposed, Y^T = W X^T,
+// with v_mfma_f32_16x16x32_{bf16,f16}: A = a 16-neuron x 32-input weight
+// tile, B = a 32-input x 16-sample activation tile.  The result tile leaves
+// lane l holding sample l%16, neurons 4*(l/16)+i.  Reading the input slots of
+// the next layer in the permuted order
+//     slot (c, g, j) -> neuron 32c + (j < 4 ? 4g + j : 16 + 4g + j - 4)
+// makes two result tiles exactly the next B operand of the same lane, so
+// activations never leave registers between layers; the weight operands are
+// stored pre-permuted (see nrc.hip) and staged once per workgroup in LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tfg {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kNrcWidth = 64;
+constexpr int kNrcHidden = 5;                         // hidden 64x64 matrices: layers 0..4
+constexpr int kNrcLayers = 6;                         // + output layer 5 (16 x 64, 3 used)
+constexpr int kNrcOut = 3;
+constexpr int kNrcParams = 5 * 64 * 64 + 16 * 64;     // 21504 (output rows padded to 16)
+constexpr int kFopTiles = 5 * 8 + 2;                  // forward operand tiles (16 B per lane each)
+constexpr int kBopTiles = 4 * 8 + 4;                  // backward (transposed) operand tiles, layers 1..5
+constexpr float kLumR = 0.299f, kLumG = 0.587f, kLumB = 0.114f;
+
+__host__ __device__ constexpr int nrc_layer_offset(int l) { return l * 64 * 64; }
+__host__ __device__ constexpr int nrc_layer_rows(int l) { return l < 5 ? 64 : 16; }
+__host__ __device__ constexpr int fop_tile_base(int l) { return l * 8; }
+__host__ __device__ constexpr int bop_tile_base(int l) { return (l - 1) * 8; }   // l = 1..5
+
+// input slot (c, g, j) -> input neuron / feature index
+__host__ __device__ constexpr int nrc_perm(int c, int g, int j) { return 32 * c + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4); }
+
+template <typename T>
+struct MfmaOps;
+template <>
+struct MfmaOps<__bf16> {
+    typedef bf16x8 vec;
+    static __device__ __forceinline__ floatx4 mfma(vec a, vec b, floatx4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct MfmaOps<_Float16> {
+    typedef f16x8 vec;
+    static __device__ __forceinline__ floatx4 mfma(vec a, vec b, floatx4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+// ---------------------------------------------------------------- encoding
+// Vertex record (4 float4): (pos, roughness), (dir mapped to [0,1], kd.r),
+// (normal mapped to [0,1], kd.g), (kd.b, ks.rgb).
+struct NrcBounds {
+    float mn[3], inv_ext[3];
+};
+
+__device__ __forceinline__ float oneblob(float s, int bin) {
+    const float d = (s - ((float)bin + 0.5f) * 0.25f) * 4.0f;
+    return __expf(-0.5f * d * d);
+}
+__device__ __forceinline__ float sin_pi_pow2(float x, int k) {
+    // sin(2^k * pi * x) for x in [0, 1]: 2^(k-1) x revolutions, reduced
+    // exactly by fract (scaling by a power of two is exact).
+    float r = x * __uint_as_float((uint32_t)(126 + k) << 23);
+    r = r - floorf(r);
+    return __builtin_amdgcn_sinf(r);
+}
+__device__ __forceinline__ void sph01(float mx, float my, float mz, float& th, float& ph) {
+    const float x = 2.0f * mx - 1.0f, y = 2.0f * my - 1.0f, z = 2.0f * mz - 1.0f;
+    th = acosf(fminf(1.0f, fmaxf(-1.0f, z))) * 0.31830988618379067f;
+    ph = atan2f(y, x) * 0.15915494309189535f + 0.5f;
+}
+
+// The 16 input features lane-group g of a sample needs: chunk c, slot j.
+// Slot meaning by (c, j), parameter g (0..3):
+//  c0: j0 pos.x f=g, j1 pos.x f=4+g, j2 pos.x f=8+g, j3 pos.y f=g, j4 pos.y f=4+g,
+//      j5 pos.y f=8+g, j6 pos.z f=g, j7 pos.z f=4+g
+//  c1: j0 pos.z f=8+g, j1 dir theta blob g, j2 dir phi blob g, j3 normal theta
+//      blob g, j4 normal phi blob g, j5 roughness blob g, j6 kd[g] (g=3: 1),
+//      j7 ks[g] (g=3: 0)
+// with pos features sin(2^f pi x), x = position normalised to [0,1] by the
+// scene bounds (NrcKernel.cu:5-8).
+template <typename T>
+__device__ __forceinline__ void encode(const float4 v0, const float4 v1, const float4 v2, const float4 v3,
+                                       const NrcBounds& b, int g, typename MfmaOps<T>::vec& x0,
+                                       typename MfmaOps<T>::vec& x1) {
+    const float px = fminf(fmaxf((v0.x - b.mn[0]) * b.inv_ext[0], 0.0f), 1.0f);
+    const float py = fminf(fmaxf((v0.y - b.mn[1]) * b.inv_ext[1], 0.0f), 1.0f);
+    const float pz = fminf(fmaxf((v0.z - b.mn[2]) * b.inv_ext[2], 0.0f), 1.0f);
+    x0[0] = (T)sin_pi_pow2(px, g);
+    x0[1] = (T)sin_pi_pow2(px, 4 + g);
+    x0[2] = (T)sin_pi_pow2(px, 8 + g);
+    x0[3] = (T)sin_pi_pow2(py, g);
+    x0[4] = (T)sin_pi_pow2(py, 4 + g);
+    x0[5] = (T)sin_pi_pow2(py, 8 + g);
+    x0[6] = (T)sin_pi_pow2(pz, g);
+    x0[7] = (T)sin_pi_pow2(pz, 4 + g);
+    float dth, dph, nth, nph;
+    sph01(v1.x, v1.y, v1.z, dth, dph);
+    sph01(v2.x, v2.y, v2.z, nth, nph);
+    const float kd[3] = {v1.w, v2.w, v3.x};
+    const float ks[3] = {v3.y, v3.z, v3.w};
+    x1[0] = (T)sin_pi_pow2(pz, 8 + g);
+    x1[1] = (T)oneblob(dth, g);
+    x1[2] = (T)oneblob(dph, g);
+    x1[3] = (T)oneblob(nth, g);
+    x1[4] = (T)oneblob(nph, g);
+    x1[5] = (T)oneblob(v0.w, g);
+    x1[6] = (T)(g < 3 ? kd[g] : 1.0f);
+    x1[7] = (T)(g < 3 ? ks[g] : 0.0f);
+}
+
+// D tiles (4 per 64 neurons) -> B operands of the next layer, with ReLU.
+template <typename T>
+__device__ __forceinline__ typename MfmaOps<T>::vec pack_relu(floatx4 lo, floatx4 hi) {
+    typename MfmaOps<T>::vec v;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        v[i] = (T)fmaxf(lo[i], 0.0f);
+        v[4 + i] = (T)fmaxf(hi[i], 0.0f);
+    }
+    return v;
+}
+
+// Forward through all layers for `G` 16-sample groups of one wave.  W is the
+// forward operand array in LDS (kFopTiles x 64 lanes).  On return out[g] holds
+// the output tile: lanes with l/16 == 0 own outputs 0..2 of sample l%16.
+// If xs != nullptr the B operands of every layer are kept (training).
+template <typename T, int G>
+__device__ __forceinline__ void mlp_forward(const typename MfmaOps<T>::vec* __restrict__ W, int lane,
+                                            typename MfmaOps<T>::vec (&x)[G][2], floatx4 (&out)[G],
+                                            typename MfmaOps<T>::vec (*xs)[kNrcLayers][2] = nullptr) {
+    typedef MfmaOps<T> M;
+#pragma unroll 1
+    for (int l = 0; l < kNrcHidden; ++l) {
+        if (xs) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) { xs[g][l][0] = x[g][0]; xs[g][l][1] = x[g][1]; }
+        }
+        floatx4 acc[G][4];
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[g][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const typename M::vec a = W[(fop_tile_base(l) + 2 * t + c) * 64 + lane];
+#pragma unroll
+                for (int g = 0; g < G; ++g) acc[g][t] = M::mfma(a, x[g][c], acc[g][t]);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            x[g][0] = pack_relu<T>(acc[g][0], acc[g][1]);
+            x[g][1] = pack_relu<T>(acc[g][2], acc[g][3]);
+        }
+    }
+    if (xs) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) { xs[g][5][0] = x[g][0]; xs[g][5][1] = x[g][1]; }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) out[g] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const typename M::vec a = W[(fop_tile_base(5) + c) * 64 + lane];
+#pragma unroll
+        for (int g = 0; g < G; ++g) out[g] = M::mfma(a, x[g][c], out[g]);
+    }
+}
+
+// guardarSalidaInferencia (NrcKernel.cu:99-126)
+__device__ __forceinline__ float sanitize_out(float r) {
+    if (r < 0.0f) r = 0.0f;
+    if (isnan(r)) r = 0.0f;
+    if (r > 100000.0f) r = 100000.0f;
+    if (isinf(r)) r = 100000.0f;
+    return r;
+}
+
+}  // namespace tfg

@@ -2,9 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "../../include/tfg_types.h"
+#include "../../include/tfg.h"
 
-struct tfg_nrc;   // defined in nrc.hip
 
 // Device buffers of a path tracer frame that the cache reads.
 struct tfg_nrc_frame {

@@ -438,6 +438,54 @@ int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_g
     return TFG_OK;
 }
 
+int tfg_pt_set_nrc(tfg_pt* pt, tfg_nrc* nrc) {
+    if (!pt) { set_error("tfg_pt_set_nrc: NULL"); return TFG_E_INVALID; }
+    if (nrc && pt->cfg.mode == 0) { set_error("tfg_pt_set_nrc: tracer was created in plain mode"); return TFG_E_STATE; }
+    pt->nrc = nrc;
+    pt->nrc_frame.records = pt->P.records;
+    pt->nrc_frame.queries = pt->P.queries;
+    pt->nrc_frame.counters = pt->P.counters;
+    pt->nrc_frame.n_paths = pt->P.n_paths;
+    pt->nrc_frame.mode = pt->P.mode;
+    drop_graph(pt);
+    return TFG_OK;
+}
+
+int tfg_pt_set_mode(tfg_pt* pt, uint32_t mode) {
+    if (!pt || mode > 3) { set_error("tfg_pt_set_mode: bad argument"); return TFG_E_INVALID; }
+    if (mode && pt->cfg.mode == 0) { set_error("tfg_pt_set_mode: tracer was created in plain mode"); return TFG_E_STATE; }
+    if (pt->P.mode != mode) drop_graph(pt);
+    pt->P.mode = mode;
+    pt->nrc_frame.mode = mode;
+    return TFG_OK;
+}
+
+int tfg_pt_nrc_buffers(tfg_pt* pt, float** d_records, float** d_queries, uint32_t** d_counters) {
+    if (!pt) { set_error("tfg_pt_nrc_buffers: NULL"); return TFG_E_INVALID; }
+    if (d_records) *d_records = reinterpret_cast<float*>(pt->P.records);
+    if (d_queries) *d_queries = reinterpret_cast<float*>(pt->P.queries);
+    if (d_counters) *d_counters = pt->P.counters;
+    return TFG_OK;
+}
+
+int tfg_pt_read_cache_data(tfg_pt* pt, float* records, uint32_t max_records, float* queries, uint32_t max_queries,
+                           uint32_t* n_records, uint32_t* n_queries) {
+    if (!pt) { set_error("tfg_pt_read_cache_data: NULL"); return TFG_E_INVALID; }
+    if (pt->cfg.mode == 0) { set_error("tfg_pt_read_cache_data: tracer has no cache buffers"); return TFG_E_STATE; }
+    TFG_HIP_CHECK(hipStreamSynchronize(pt->stream));
+    uint32_t c[CNT_TOTAL];
+    TFG_HIP_CHECK(hipMemcpy(c, pt->P.counters, sizeof(c), hipMemcpyDeviceToHost));
+    const uint32_t nr = c[CNT_RECORDS] < max_records ? c[CNT_RECORDS] : max_records;
+    const uint32_t nq = c[CNT_QUERIES] < max_queries ? c[CNT_QUERIES] : max_queries;
+    if (records && nr)
+        TFG_HIP_CHECK(hipMemcpy(records, pt->P.records, (size_t)nr * kRecF4 * sizeof(float4), hipMemcpyDeviceToHost));
+    if (queries && nq)
+        TFG_HIP_CHECK(hipMemcpy(queries, pt->P.queries, (size_t)nq * kQueryF4 * sizeof(float4), hipMemcpyDeviceToHost));
+    if (n_records) *n_records = c[CNT_RECORDS];
+    if (n_queries) *n_queries = c[CNT_QUERIES];
+    return TFG_OK;
+}
+
 int tfg_pt_destroy(tfg_pt* pt) {
     if (!pt) return TFG_OK;
     if (pt->stream) (void)hipStreamSynchronize(pt->stream);

@@ -438,7 +438,9 @@ __global__ __launch_bounds__(256) void k_finalize(PtParams P, tfg_color* image, 
             const float4 pr = P.pre[pid];
             const float4 r = P.rad[pid];
             R[8] = make_float4(r.x - pr.x, r.y - pr.y, r.z - pr.z, 1.0f);
-            R[9] = tl ? P.tsuf[pid] : make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 ts = tl ? P.tsuf[pid] : make_float4(0.f, 0.f, 0.f, 0.f);
+            ts.w = __uint_as_float(pid);   // path id, for record-level checks
+            R[9] = ts;
             R[10] = P.tthr[pid];
         }
         const uint32_t qs = wave_append(want_q, &P.counters[CNT_QUERIES]);

@@ -1,0 +1,117 @@
+"""Equal-time comparison of the cache-accelerated and the plain path tracer.
+
+The frame schedule restates main.cpp:958-1089: ``warmup`` frames with 100%
+training paths and no inference, then frames in which 5% of the pixels
+train and the rest query the cache at their first diffuse vertex with a
+large enough footprint.  The reference image is plain path tracing with FP64
+accumulation; relMSE = mean over pixels and channels of
+(I - R)^2 / (R^2 + 0.01).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import nrc, pt
+
+
+def relmse(img: np.ndarray, ref: np.ndarray) -> float:
+    return float(np.mean((img - ref) ** 2 / (ref * ref + 0.01)))
+
+
+def _sync_time(torch):
+    torch.cuda.synchronize()
+    return time.perf_counter()
+
+
+def reference_image(torch, dscene, W, H, max_depth, rr_depth, spp, frame0=1_000_000):
+    tr = pt.PathTracer(W, H, max_depth, rr_depth, mode=0, use_graph=True)
+    tr.set_scene(dscene)
+    tr.set_camera(dscene.scene.camera.frame(W, H))
+    img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
+    acc = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
+    for k in range(spp):
+        tr.render(frame0 + k, img, acc)
+    torch.cuda.synchronize()
+    return (acc / spp).cpu().numpy()
+
+
+def plain_run(torch, dscene, W, H, max_depth, rr_depth, budget_s, frame0=0):
+    """Plain FP32 PT for budget_s seconds of wall time; returns (mean image, frames, seconds)."""
+    tr = pt.PathTracer(W, H, max_depth, rr_depth, mode=0, use_graph=True)
+    tr.set_scene(dscene)
+    tr.set_camera(dscene.scene.camera.frame(W, H))
+    img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
+    acc = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
+    tr.render(frame0, img)          # capture the graph outside the timed region
+    t0 = _sync_time(torch)
+    n = 0
+    while True:
+        tr.render(frame0 + 1 + n, img, acc)
+        n += 1
+        t = _sync_time(torch)
+        if t - t0 >= budget_s:
+            break
+    return (acc / n).cpu().numpy(), n, t - t0
+
+
+def nrc_run(torch, dscene, W, H, max_depth, rr_depth, budget_s, warmup=48, precision=0, train_fraction=0.05,
+            frame0=0, cache_cfg=None, count_warmup=True):
+    """Cache-accelerated PT: `warmup` training-only frames, then train+infer
+    frames until the budget (wall time including warmup when
+    count_warmup) is spent.  Returns (mean image, frames, seconds, cache stats)."""
+    tr = pt.PathTracer(W, H, max_depth, rr_depth, mode=pt_mode_train(), train_fraction=train_fraction,
+                       use_graph=True)
+    tr.set_scene(dscene)
+    tr.set_camera(dscene.scene.camera.frame(W, H))
+    sc = dscene.scene
+    lo = sc.verts.reshape(-1, 3).min(0) - 0.5
+    hi = sc.verts.reshape(-1, 3).max(0) + 0.5
+    kw = dict(precision=precision, bounds_min=tuple(map(float, lo)), bounds_max=tuple(map(float, hi)))
+    if cache_cfg:
+        kw.update(cache_cfg)
+    cache = nrc.RadianceCache(**kw)
+    tr.set_cache(cache)
+    img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
+    acc = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
+    t0 = _sync_time(torch)
+    for k in range(warmup):
+        tr.render(frame0 + k, img)
+    t_w = _sync_time(torch)
+    tr.set_mode(pt_mode_train() | pt_mode_infer())
+    start = t0 if count_warmup else t_w
+    n = 0
+    while True:
+        tr.render(frame0 + warmup + n, img, acc)
+        n += 1
+        t = _sync_time(torch)
+        if t - start >= budget_s and n >= 1:
+            break
+    return (acc / n).cpu().numpy(), n, t - start, cache.stats(), t_w - t0
+
+
+def pt_mode_train():
+    return 2
+
+
+def pt_mode_infer():
+    return 1
+
+
+def equal_time(torch, scene, W, H, max_depth, rr_depth, ref_spp, budget_s, warmup=48, precision=0):
+    dscene = pt.DeviceScene(scene, torch)
+    ref = reference_image(torch, dscene, W, H, max_depth, rr_depth, ref_spp)
+    p_img, p_n, p_t = plain_run(torch, dscene, W, H, max_depth, rr_depth, budget_s)
+    n_img, n_n, n_t, st, w_t = nrc_run(torch, dscene, W, H, max_depth, rr_depth, budget_s, warmup=warmup,
+                                       precision=precision)
+    rp = relmse(p_img, ref)
+    rn = relmse(n_img, ref)
+    return {
+        "ref_spp": ref_spp,
+        "budget_s": budget_s,
+        "plain": {"relmse": rp, "frames": p_n, "seconds": p_t},
+        "nrc": {"relmse": rn, "frames": n_n, "seconds": n_t, "warmup_frames": warmup, "warmup_s": w_t,
+                "cache": st},
+        "relmse_ratio_plain_over_nrc": rp / rn if rn > 0 else None,
+    }

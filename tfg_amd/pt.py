@@ -46,6 +46,28 @@ class PathTracer:
         assert c.size == 16
         check(lib().tfg_pt_set_camera(self._h, c.ctypes.data), "tfg_pt_set_camera")
 
+    def set_cache(self, cache):
+        self._cache = cache
+        check(lib().tfg_pt_set_nrc(self._h, cache.handle if cache is not None else None), "tfg_pt_set_nrc")
+
+    def set_mode(self, mode: int):
+        check(lib().tfg_pt_set_mode(self._h, mode), "tfg_pt_set_mode")
+
+    def cache_buffers(self):
+        r, q, c = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib().tfg_pt_nrc_buffers(self._h, C.byref(r), C.byref(q), C.byref(c)), "tfg_pt_nrc_buffers")
+        return r.value, q.value, c.value
+
+    def read_cache_data(self):
+        """(records (n, 44), queries (m, 20)) of the last frame, float32."""
+        n = self.width * self.height
+        rec = np.zeros((n, 44), np.float32)
+        q = np.zeros((n, 20), np.float32)
+        nr, nq = C.c_uint32(), C.c_uint32()
+        check(lib().tfg_pt_read_cache_data(self._h, rec.ctypes.data, n, q.ctypes.data, n, C.byref(nr), C.byref(nq)),
+              "tfg_pt_read_cache_data")
+        return rec[:nr.value], q[:nq.value]
+
     def render(self, frame: int, image, accum=None, stream=None):
         check(lib().tfg_pt_render(self._h, int(frame), image.data_ptr(),
                                   accum.data_ptr() if accum is not None else None, stream), "tfg_pt_render")
