@@ -99,6 +99,8 @@ typedef struct tfg_pt_scene {
     const tfg_material* d_materials;
     const tfg_point_light* d_lights;
     uint32_t n_lights;
+    const tfg_analytic* d_analytic;     /* may be NULL */
+    uint32_t n_analytic;                /* <= 64 */
 } tfg_pt_scene;
 
 #define TFG_PT_NCOUNTERS 72
@@ -215,6 +217,30 @@ int tfg_launch_preparar_entrenamiento(const tfg_datos_mlp* d_buf, const int* d_i
 int tfg_launch_preparar_inferencia(const tfg_datos_mlp* d_buf, uint32_t n, uint32_t n_in, float* d_in,
                                    tfg_scene_bounds bounds, tfg_stream stream);                    /* NrcKernel.h:37-38 */
 int tfg_launch_guardar_salida(const float* d_out, tfg_color* d_col, int n, tfg_stream stream);     /* NrcKernel.h:40-41 */
+
+
+/* ---- launchKernelRenderTiny drop-in ----------------------------------------
+ * RenderKernel.h:55-61 with the reference's argument list (grid/block
+ * dropped, bool flags as int, plus a stream): renders one frame of the
+ * scene given by the reference's buffers and writes the image, the cache
+ * queries (buffer_inference / buffer_throughput, throughput 0 where no
+ * query, render.h:452-458) and the training records
+ * (buffer_training[atomicAdd(counter_train, 1) % max_cap_train],
+ * render.h:461-464) in the reference layouts.  The CWBVH must come from this
+ * library (tfg_cwbvh_upload / the TinyBVH shim).  Converted scene data are
+ * cached per scene buffer; call tfg_dropin_invalidate() after rewriting a
+ * scene buffer in place.  At most 64 analytic primitives and 64 lights. */
+int tfg_launch_render_tiny(const tfg_camara* d_camara, const tfg_primitiva* d_prims, int n_prims,
+                           const tfg_luz_puntual* d_luces, int n_luces, const tfg_primitiva* d_prims_malla,
+                           int n_prims_malla, int width, int height, int spp, int frame, tfg_cwbvh_gpu bvh,
+                           const tfg_primitiva* d_prims_bvh, int n_prims_bvh, tfg_imagen_gpu imagen,
+                           unsigned int* d_dev_counter, tfg_registro_entrenamiento* d_recs,
+                           unsigned int* d_counter_train, int max_cap_train, tfg_datos_mlp* d_inference,
+                           tfg_color* d_throughput, int usar_red_inferencia, int entrenar_red, tfg_stream stream);
+/* Path length and Russian-roulette depth of the drop-in frames (defaults
+ * 20 and 10, render.h:63,210). */
+int tfg_dropin_configure(uint32_t max_depth, uint32_t rr_depth);
+int tfg_dropin_invalidate(void);
 
 #ifdef __cplusplus
 }

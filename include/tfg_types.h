@@ -124,6 +124,16 @@ typedef struct tfg_tri_shading {
     float ng[3]; uint32_t pad1;
 } tfg_tri_shading;
 
+/* Analytic primitive (48 B) tested linearly next to the CWBVH, like the
+ * reference's Primitiva list (escenario.h:323-334, primitiva.h:43-200).
+ * Its hit id is 0x80000000 | index. */
+enum { TFG_AN_SPHERE = 0, TFG_AN_PLANE = 1, TFG_AN_TRIANGLE = 2 };
+typedef struct tfg_analytic {
+    float a[3]; uint32_t type;       /* sphere: centre; plane: normal; triangle: v0 */
+    float b[3]; float s;             /* sphere: s = radius; plane: s = distance; triangle: b = v1 */
+    float c[3]; uint32_t material;   /* triangle: c = v2 */
+} tfg_analytic;
+
 typedef struct tfg_point_light {
     float pos[3]; float pad0;
     float intensity[3]; float pad1;

@@ -162,7 +162,8 @@ class Bvh2:
 
 class OrcScene(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("tris", C.c_void_p), ("shading", C.c_void_p), ("materials", C.c_void_p),
-                ("lights", C.c_void_p), ("n_lights", C.c_uint32), ("cam", C.c_void_p)]
+                ("lights", C.c_void_p), ("n_lights", C.c_uint32), ("cam", C.c_void_p), ("analytic", C.c_void_p),
+                ("n_analytic", C.c_uint32)]
 
 
 class Renderer:
@@ -174,8 +175,12 @@ class Renderer:
                       np.ascontiguousarray(scene.materials, np.float32),
                       np.ascontiguousarray(scene.lights, np.float32),
                       np.ascontiguousarray(scene.camera.frame(width, height), np.float32)]
+        an = getattr(scene, "analytic", None)
+        n_an = 0 if an is None else an.shape[0]
+        self._keep.append(np.ascontiguousarray(an if n_an else np.zeros((1, 12), np.float32)))
         k = self._keep
-        self.desc = OrcScene(_p(k[0]), _p(k[1]), _p(k[2]), _p(k[3]), _p(k[4]), scene.lights.shape[0], _p(k[5]))
+        self.desc = OrcScene(_p(k[0]), _p(k[1]), _p(k[2]), _p(k[3]), _p(k[4]), scene.lights.shape[0], _p(k[5]),
+                             _p(k[6]), n_an)
         self.width, self.height = width, height
 
     def render(self, frame0, spp, max_depth, rr_depth, nthreads=os.cpu_count() or 1):

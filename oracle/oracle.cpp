@@ -743,17 +743,88 @@ D3 ggx_vndf_d(D3 wo, double alpha, double u1, double u2) {   // Heitz 2018
     return dnorm(d3(alpha * nh.x, alpha * nh.y, std::max(0.0, nh.z)));
 }
 
+// primitiva.h:137-187: vectors in FP64, the scalars the reference keeps in
+// float rounded to float.  Returns t > 1e-6 or -1.
+float o_analytic_t(const tfg_analytic& q, D3 o, D3 d) {
+    if (q.type == TFG_AN_SPHERE) {
+        const D3 oc = o - dld(q.a);
+        const float a = (float)ddot(d, d);
+        const float b = (float)(2.0f * ddot(oc, d));
+        const float c = (float)(ddot(oc, oc) - (double)(q.s * q.s));
+        const float disc = b * b - 4.0f * a * c;
+        if (disc < 0) return -1.0f;
+        const float sq = std::sqrt(disc);
+        const float t1 = (-b - sq) / (2.0f * a), t2 = (-b + sq) / (2.0f * a);
+        if (t1 > 1e-6f && t1 < t2) return t1;
+        if (t2 > 1e-6f) return t2;
+        return -1.0f;
+    }
+    if (q.type == TFG_AN_PLANE) {
+        const D3 n = dld(q.a);
+        const float denom = (float)ddot(n, d);
+        if (std::fabs(denom) > 1e-6f) {
+            const float t = (float)(ddot(n * (double)q.s - o, n) / denom);
+            return t > 1e-6f ? t : -1.0f;
+        }
+        return -1.0f;
+    }
+    const V3 v0 = {q.a[0], q.a[1], q.a[2]}, v1 = {q.b[0], q.b[1], q.b[2]}, v2 = {q.c[0], q.c[1], q.c[2]};
+    float t;
+    if (!moller_trumbore(v0, v1, v2, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, t)) return -1.0f;
+    return t;
+}
+
 bool o_closest(const orc_scene* sc, D3 o, D3 d, tfg_hit& h) {
     tfg_ray r = {(float)o.x, (float)o.y, (float)o.z, 0.0f, (float)d.x, (float)d.y, (float)d.z,
                  std::numeric_limits<float>::infinity()};
     const RayPre rp = make_ray(r);
-    return traverse<false>(sc->nodes, sc->tris, 0, rp, h, nullptr);
+    bool hit = traverse<false>(sc->nodes, sc->tris, 0, rp, h, nullptr);
+    for (uint32_t k = 0; k < sc->n_analytic; ++k) {   // escenario.h:323-334
+        const float ta = o_analytic_t(sc->analytic[k], o, d);
+        if (ta > 0.0f && ta < h.t) {
+            h.t = ta;
+            h.u = h.v = 0.0f;
+            h.prim = 0x80000000u | k;
+            hit = true;
+        }
+    }
+    return hit;
 }
 bool o_occluded(const orc_scene* sc, D3 o, D3 d, float tmax) {
+    for (uint32_t k = 0; k < sc->n_analytic; ++k) {
+        const float ta = o_analytic_t(sc->analytic[k], o, d);
+        if (ta > 0.0f && ta < tmax) return true;
+    }
     tfg_ray r = {(float)o.x, (float)o.y, (float)o.z, 0.0f, (float)d.x, (float)d.y, (float)d.z, tmax};
     const RayPre rp = make_ray(r);
     tfg_hit h;
     return traverse<true>(sc->nodes, sc->tris, 0, rp, h, nullptr);
+}
+
+// Shading normal and material of a hit (escenario.h:123-142 for mesh
+// triangles, primitiva.h:190-220 for analytic primitives).
+void o_surface(const orc_scene* sc, const tfg_hit& h, D3 p, D3 d, D3& n, uint32_t& mat) {
+    if (h.prim & 0x80000000u) {
+        const tfg_analytic& q = sc->analytic[h.prim & 0x7FFFFFFFu];
+        mat = q.material;
+        if (q.type == TFG_AN_SPHERE) n = dnorm(p - dld(q.a));
+        else if (q.type == TFG_AN_PLANE) n = dld(q.a);
+        else {
+            n = dnorm(dcross(dld(q.b) - dld(q.a), dld(q.c) - dld(q.a)));
+            if (ddot(n, d) > 0.0) n = -n;
+        }
+        return;
+    }
+    const tfg_tri_shading& sh = sc->shading[h.prim];
+    mat = sh.material;
+    if (sh.flags & 1u) {   // primitiva.h:84-101
+        const float b0 = 1.0f - h.u - h.v;
+        const D3 ns = dld(sh.n0) * (double)b0 + dld(sh.n1) * (double)h.u + dld(sh.n2) * (double)h.v;
+        const float len = (float)std::sqrt(ddot(ns, ns));
+        if (len > 1e-6f) { n = ns * (double)(1.0f / len); return; }
+    }
+    n = dnorm(dld(sh.ng));
+    if (ddot(n, d) > 0.0) n = -n;
 }
 
 // escenario.h:144-184 with the mesh occluding (the reference's CWBVH mode
@@ -801,20 +872,10 @@ Col o_path(const orc_scene* sc, int W, int H, int x, int y, int frame, int max_d
         ++next;
         if (!o_closest(sc, o, d, h)) break;   // black background, render.h:227-239
         const D3 p = o + d * (double)h.t;
-        const tfg_tri_shading& sh = sc->shading[h.prim];
-        const OMat m = omat(sc->materials[sh.material]);
         D3 n;
-        bool have = false;
-        if (sh.flags & 1u) {   // primitiva.h:84-101
-            const float b0 = 1.0f - h.u - h.v;
-            const D3 ns = dld(sh.n0) * (double)b0 + dld(sh.n1) * (double)h.u + dld(sh.n2) * (double)h.v;
-            const float len = (float)std::sqrt(ddot(ns, ns));
-            if (len > 1e-6f) { n = ns * (double)(1.0f / len); have = true; }
-        }
-        if (!have) {
-            n = dnorm(dld(sh.ng));
-            if (ddot(n, d) > 0.0) n = -n;
-        }
+        uint32_t mat_id;
+        o_surface(sc, h, p, d, n, mat_id);
+        const OMat m = omat(sc->materials[mat_id]);
         if (cmax(m.le) > 0.0f) { color = color + camino * m.le; break; }
         const D3 wo = -d;
         if (o_diffuse(m) || o_glossy(m)) color = color + camino * o_direct(sc, p + n * kEpsF, n, wo, m, nshadow);
@@ -959,17 +1020,10 @@ void o_path_nrc(const orc_scene* sc, int W, int H, int x, int y, int frame, int 
         tfg_hit h;
         if (!o_closest(sc, o, d, h)) break;
         const D3 p = o + d * (double)h.t;
-        const tfg_tri_shading& sh = sc->shading[h.prim];
-        const OMat m = omat(sc->materials[sh.material]);
         D3 n;
-        bool have = false;
-        if (sh.flags & 1u) {
-            const float b0 = 1.0f - h.u - h.v;
-            const D3 ns = dld(sh.n0) * (double)b0 + dld(sh.n1) * (double)h.u + dld(sh.n2) * (double)h.v;
-            const float len = (float)std::sqrt(ddot(ns, ns));
-            if (len > 1e-6f) { n = ns * (double)(1.0f / len); have = true; }
-        }
-        if (!have) { n = dnorm(dld(sh.ng)); if (ddot(n, d) > 0.0) n = -n; }
+        uint32_t mat_id;
+        o_surface(sc, h, p, d, n, mat_id);
+        const OMat m = omat(sc->materials[mat_id]);
         const float cos_theta = (float)std::fabs(ddot(n, -d));
         const bool diffuse = o_diffuse(m), glossy = o_glossy(m);
         const bool delta = (cmax(m.ks) > kEpsF && m.rough <= 0.0f) || cmax(m.kt) > kEpsF;

@@ -83,6 +83,8 @@ typedef struct orc_scene {
     const tfg_point_light* lights;
     uint32_t n_lights;
     const tfg_camera* cam;
+    const tfg_analytic* analytic;   /* may be NULL; hit id 0x80000000 | index */
+    uint32_t n_analytic;
 } orc_scene;
 void orc_render(const orc_scene* sc, int width, int height, int frame0, int spp, int max_depth, int rr_depth,
                 double* accum, uint64_t* rays_out, int nthreads);

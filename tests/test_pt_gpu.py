@@ -26,12 +26,15 @@ def _setup(torch, sc, W, H, max_depth, rr_depth, use_graph=True):
     return ds, tracer, img
 
 
-@pytest.mark.parametrize("name", ["cornell", "instances_small"])
+@pytest.mark.parametrize("name", ["cornell", "instances_small", "reference_analytic"])
 def test_pt_per_path_matches_oracle(torch_cuda, name):
     torch = torch_cuda
     if name == "cornell":
         sc = scenes.cornell(level=3)
         W, H, md, rr = 96, 64, 5, 2
+    elif name == "reference_analytic":
+        sc = scenes.reference_scene()      # planes, glass and plastic spheres (main.cpp:164-345)
+        W, H, md, rr = 96, 64, 8, 5
     else:
         sc = scenes.instances(n_inst=8, sphere_res=(32, 16), box_res=4, seed=5)
         W, H, md, rr = 96, 64, 8, 3

@@ -29,6 +29,12 @@ class CwbvhGpu(C.Structure):
     _fields_ = [("d_nodos", C.c_void_p), ("d_tris", C.c_void_p), ("num_nodos", C.c_uint32), ("num_tris", C.c_uint32)]
 
 
+class ImagenGpu(C.Structure):
+    """ImagenGPU (imagen_gpu.h:14-17), passed by value."""
+
+    _fields_ = [("anchura", C.c_int32), ("altura", C.c_int32), ("datos", C.c_void_p)]
+
+
 class SceneBounds(C.Structure):
     """SceneBounds (mlp_types.h:26-29), passed by value."""
 
@@ -42,7 +48,7 @@ class PtConfig(C.Structure):
 
 class PtScene(C.Structure):
     _fields_ = [("bvh", CwbvhGpu), ("d_shading", C.c_void_p), ("d_materials", C.c_void_p), ("d_lights", C.c_void_p),
-                ("n_lights", C.c_uint32)]
+                ("n_lights", C.c_uint32), ("d_analytic", C.c_void_p), ("n_analytic", C.c_uint32)]
 
 
 class PtProfile(C.Structure):
@@ -134,6 +140,12 @@ _SIGS = {
     "tfg_launch_preparar_inferencia": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, SceneBounds,
                                                  C.c_void_p]),
     "tfg_launch_guardar_salida": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "tfg_launch_render_tiny": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                         C.c_int, C.c_int, C.c_int, C.c_int, CwbvhGpu, C.c_void_p, C.c_int,
+                                         ImagenGpu, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                         C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "tfg_dropin_configure": (C.c_int, [C.c_uint32, C.c_uint32]),
+    "tfg_dropin_invalidate": (C.c_int, []),
 }
 
 

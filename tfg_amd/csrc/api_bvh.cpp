@@ -24,18 +24,25 @@ void set_error(const char* fmt, ...) {
 // Depth of every tree uploaded by this library, keyed by its node buffer,
 // so the traversal entry points can refuse trees deeper than their stack.
 static std::mutex g_depth_mu;
-static std::unordered_map<const void*, uint32_t> g_depth;
+static std::unordered_map<const void*, std::pair<uint32_t, uint32_t>> g_depth;   // depth, format
 
 uint32_t known_tree_depth(const void* d_nodes) {
     std::lock_guard<std::mutex> lk(g_depth_mu);
     auto it = g_depth.find(d_nodes);
-    return it == g_depth.end() ? 0u : it->second;
+    return it == g_depth.end() ? 0u : it->second.first;
 }
 
-void register_tree_depth(const void* d_nodes, uint32_t depth) {
+// Triangle format of a tree uploaded by this library, or 0xFFFFFFFF.
+uint32_t known_tree_format(const void* d_nodes) {
+    std::lock_guard<std::mutex> lk(g_depth_mu);
+    auto it = g_depth.find(d_nodes);
+    return it == g_depth.end() ? 0xFFFFFFFFu : it->second.second;
+}
+
+void register_tree_depth(const void* d_nodes, uint32_t depth, uint32_t format = 0) {
     std::lock_guard<std::mutex> lk(g_depth_mu);
     if (depth == 0) g_depth.erase(d_nodes);
-    else g_depth[d_nodes] = depth;
+    else g_depth[d_nodes] = {depth, format};
 }
 
 }  // namespace tfg
@@ -110,7 +117,7 @@ int tfg_cwbvh_upload(const tfg_cwbvh_host* h, tfg_cwbvh_gpu* out) {
     out->d_tris = dt;
     out->num_nodos = h->b.num_nodes;
     out->num_tris = h->b.num_tris;
-    tfg::register_tree_depth(dn, h->b.max_depth);
+    tfg::register_tree_depth(dn, h->b.max_depth, h->b.format);
     return TFG_OK;
 }
 

@@ -29,3 +29,4 @@ static_assert(sizeof(tfg_cwbvh_gpu) == 24, "TinyBVHD_GPU");
 static_assert(sizeof(tfg_ray) == 32 && sizeof(tfg_hit) == 16, "ray/hit");
 static_assert(sizeof(tfg_material) == 64 && sizeof(tfg_tri_shading) == 64, "material/shading");
 static_assert(sizeof(tfg_point_light) == 32 && sizeof(tfg_camera) == 64, "light/camera");
+static_assert(sizeof(tfg_analytic) == 48, "analytic");

@@ -33,9 +33,11 @@ struct tfg_pt {
     const void* key_image = nullptr;
     const void* key_accum = nullptr;
     const void* key_nrc = nullptr;
+    uint32_t key_mode = 0;
     // radiance cache attached to the frame (nrc.hip)
     tfg_nrc* nrc = nullptr;
     tfg_nrc_frame nrc_frame{};
+    bool allow_no_cache = false;   // drop-in export frames: queries leave the frame uncomposited
 };
 
 namespace tfg {
@@ -139,6 +141,23 @@ int enqueue_frame(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum, St
 }  // namespace
 }  // namespace tfg
 
+namespace tfg {
+// Frame without cache inference even in INFER mode: the caller exports the
+// captured queries (drop-in path).  Camera already in device memory.
+int pt_render_export(tfg_pt* pt, int32_t frame, tfg_color* d_image, hipStream_t stream) {
+    pt->camera_set = true;
+    tfg_nrc* saved = pt->nrc;
+    pt->nrc = nullptr;
+    pt->allow_no_cache = true;
+    const int rc = tfg_pt_render(pt, frame, d_image, nullptr, stream);
+    pt->allow_no_cache = false;
+    pt->nrc = saved;
+    return rc;
+}
+const PtParams& pt_params(const tfg_pt* pt) { return pt->P; }
+hipStream_t pt_stream(const tfg_pt* pt) { return pt->stream; }
+}  // namespace tfg
+
 using namespace tfg;
 
 extern "C" {
@@ -238,6 +257,10 @@ int tfg_pt_set_scene(tfg_pt* pt, const tfg_pt_scene* sc) {
         set_error("tfg_pt_set_scene: missing scene buffer");
         return TFG_E_INVALID;
     }
+    if (sc->n_analytic > kMaxAnalytic || (sc->n_analytic && !sc->d_analytic)) {
+        set_error("tfg_pt_set_scene: at most %u analytic primitives", kMaxAnalytic);
+        return TFG_E_LIMIT;
+    }
     if (known_tree_depth(sc->bvh.d_nodos) > TFG_MAX_TREE_DEPTH) {
         set_error("tfg_pt_set_scene: tree too deep");
         return TFG_E_LIMIT;
@@ -249,6 +272,8 @@ int tfg_pt_set_scene(tfg_pt* pt, const tfg_pt_scene* sc) {
     P.mats = sc->d_materials;
     P.lights = sc->d_lights;
     P.n_lights = sc->n_lights;
+    P.analytic = sc->d_analytic;
+    P.n_analytic = sc->n_analytic;
     pt->scene_set = true;
     drop_graph(pt);
     return TFG_OK;
@@ -261,10 +286,11 @@ int tfg_pt_set_camera(tfg_pt* pt, const tfg_camera* cam) {
     return TFG_OK;
 }
 
+
 int tfg_pt_render(tfg_pt* pt, int32_t frame, tfg_color* d_image, double* d_accum, tfg_stream stream) {
     if (!pt || !d_image) { set_error("tfg_pt_render: NULL argument"); return TFG_E_INVALID; }
     if (!pt->scene_set || !pt->camera_set) { set_error("tfg_pt_render: scene or camera not set"); return TFG_E_STATE; }
-    if (pt->P.mode && pt->nrc == nullptr && (pt->P.mode & MODE_INFER)) {
+    if ((pt->P.mode & MODE_INFER) && pt->nrc == nullptr && !pt->allow_no_cache) {
         set_error("tfg_pt_render: cache inference requested without an attached cache");
         return TFG_E_STATE;
     }
@@ -274,7 +300,8 @@ int tfg_pt_render(tfg_pt* pt, int32_t frame, tfg_color* d_image, double* d_accum
     hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(1), 0, s, const_cast<int32_t*>(pt->P.frame), frame);
     int rc = TFG_OK;
     if (pt->cfg.use_graph) {
-        if (pt->exec && (pt->key_image != d_image || pt->key_accum != d_accum || pt->key_nrc != pt->nrc)) {
+        if (pt->exec && (pt->key_image != d_image || pt->key_accum != d_accum || pt->key_nrc != pt->nrc ||
+                         pt->key_mode != pt->P.mode)) {
             drop_graph(pt);
         }
         if (!pt->exec) {
@@ -292,6 +319,7 @@ int tfg_pt_render(tfg_pt* pt, int32_t frame, tfg_color* d_image, double* d_accum
             pt->key_image = d_image;
             pt->key_accum = d_accum;
             pt->key_nrc = pt->nrc;
+            pt->key_mode = pt->P.mode;
         }
         TFG_HIP_CHECK(hipGraphLaunch(pt->exec, s));
     } else {

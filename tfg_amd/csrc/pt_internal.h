@@ -11,6 +11,8 @@ namespace tfg {
 
 constexpr uint32_t kMaxDepth = 32;
 constexpr uint32_t kMiss = 0xFFFFFFFFu;
+constexpr uint32_t kAnalyticBit = 0x80000000u;
+constexpr uint32_t kMaxAnalytic = 64;
 
 // counters[] layout (uint32, device)
 enum : uint32_t {
@@ -53,6 +55,8 @@ struct PtParams {
     const tfg_material* mats;
     const tfg_point_light* lights;
     uint32_t n_lights;
+    const tfg_analytic* analytic;
+    uint32_t n_analytic;
     const tfg_camera* cam;
     const int32_t* frame;   // device frame index (seed), read by generate
     uint32_t width, height, n_paths;

@@ -16,25 +16,9 @@
 #include "pt_internal.h"
 #include "shade_device.h"
 #include "trace_device.h"
+#include "wave_ops.h"
 
 namespace tfg {
-
-// One append slot per requesting lane; all lanes of the wave must call it.
-__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
-    const uint64_t m = __ballot(want);
-    if (m == 0) return 0;
-    const uint32_t lane = __lane_id();
-    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, (int)leader);
-    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-}
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    return x;
-}
 
 __device__ __forceinline__ float4 f4(V3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 __device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
@@ -122,6 +106,15 @@ __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint3
             uint32_t prim;
             traverse<0, false, kStack, 0>(P.nodes, P.tris, nullptr, r, stack + threadIdx.x, kBlock, t, u, v, prim,
                                          kStats ? &st : nullptr);
+            for (uint32_t k = 0; k < P.n_analytic; ++k) {   // escenario.h:323-334
+                const float ta = analytic_t(P.analytic[k], v3(a.x, a.y, a.z), v3(b.x, b.y, b.z));
+                if (ta > a.w && ta < t) {
+                    t = ta;
+                    u = 0.0f;
+                    v = 0.0f;
+                    prim = kAnalyticBit | k;
+                }
+            }
             P.hits[i] = make_float4(t, u, v, __uint_as_float(prim));
         }
         if (kStats) add_stats(P, 0, st);
@@ -168,9 +161,18 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
             if (prim != kMiss) {
                 const float t = h.x;
                 const V3 p = xyz(ro) + d * t;
-                const tfg_tri_shading sh = P.shading[prim];
-                const Mat m = load_mat(&P.mats[sh.material]);
-                const V3 n = shading_normal(sh, ld3(sh.ng), h.y, h.z, d);
+                uint32_t mat_id;
+                V3 n;
+                if (prim & kAnalyticBit) {
+                    const tfg_analytic q = P.analytic[prim & ~kAnalyticBit];
+                    mat_id = q.material;
+                    n = analytic_normal(q, p, d);
+                } else {
+                    const tfg_tri_shading sh = P.shading[prim];
+                    mat_id = sh.material;
+                    n = shading_normal(sh, ld3(sh.ng), h.y, h.z, d);
+                }
+                const Mat m = load_mat(&P.mats[mat_id]);
                 const float cos_theta = fabsf(dot(n, -d));
                 const bool diffuse = mat_diffuse(m), glossy = mat_glossy(m), delta = mat_delta(m);
                 bool foot_ok = false;
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
                     if (diffuse || glossy) {   // NEE request (render.h:126-131)
                         want_shadow = true;
                         e0 = f4(p + n * kEps, __uint_as_float(pid));
-                        e1 = f4(n, __uint_as_float(sh.material));
+                        e1 = f4(n, __uint_as_float(mat_id));
                         e2 = f4(wo, 0.0f);
                         e3 = f4(thr, 0.0f);
                     }
@@ -237,7 +239,7 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
                             P.pre[pid] = f4(rad, 0.0f);
                             foot.w = foot.y;
                             fl |= PF_HEAD;
-                            if (want_shadow) e1.w = __uint_as_float(sh.material | SF_HEAD_NEE);
+                            if (want_shadow) e1.w = __uint_as_float(mat_id | SF_HEAD_NEE);
                         }
                     }
                     if (!done) {   // calcularLuzIndirecta, render.h:264-335
@@ -375,12 +377,19 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) 
                 float dist;
                 if (!light_term(P.lights[l], ps, n, wo, m, contrib, dir, dist)) continue;
                 const V3 o = ps + dir * kEps;
-                const RayPre r = make_ray(o.x, o.y, o.z, dir.x, dir.y, dir.z, 0.0f, dist);
-                float t, u, v;
-                uint32_t prim;
                 ++traced;
-                const bool occ = traverse<0, true, kStack, 0>(P.nodes, P.tris, nullptr, r, stack + threadIdx.x, kBlock,
-                                                             t, u, v, prim, kStats ? &st : nullptr);
+                bool occ = false;
+                for (uint32_t k = 0; k < P.n_analytic && !occ; ++k) {   // escenario.h:255-266
+                    const float ta = analytic_t(P.analytic[k], o, dir);
+                    occ = ta > 0.0f && ta < dist;
+                }
+                if (!occ) {
+                    const RayPre r = make_ray(o.x, o.y, o.z, dir.x, dir.y, dir.z, 0.0f, dist);
+                    float t, u, v;
+                    uint32_t prim;
+                    occ = traverse<0, true, kStack, 0>(P.nodes, P.tris, nullptr, r, stack + threadIdx.x, kBlock, t, u,
+                                                       v, prim, kStats ? &st : nullptr);
+                }
                 if (!occ) sum = sum + contrib;
             }
             const V3 c = xyz(e3) * sum;

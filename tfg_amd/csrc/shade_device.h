@@ -134,6 +134,59 @@ __device__ __forceinline__ V3 ggx_sample_vndf(V3 wo, float alpha, float u1, floa
     return normalize(v3(alpha * nh.x, alpha * nh.y, fmaxf(0.0f, nh.z)));
 }
 
+// ------------------------------------------------------------ analytic
+// primitiva.h:137-187 in FP32: returns t (> 1e-6) or a negative value.
+__device__ __forceinline__ float analytic_t(const tfg_analytic& q, V3 o, V3 d) {
+    if (q.type == TFG_AN_SPHERE) {
+        const V3 oc = o - ld3(q.a);
+        const float a = dot(d, d);
+        const float b = 2.0f * dot(oc, d);
+        const float c = dot(oc, oc) - q.s * q.s;
+        const float disc = b * b - 4.0f * a * c;
+        if (disc < 0.0f) return -1.0f;
+        const float sq = sqrtf(disc);
+        const float t1 = (-b - sq) / (2.0f * a);
+        const float t2 = (-b + sq) / (2.0f * a);
+        if (t1 > 1e-6f && t1 < t2) return t1;
+        if (t2 > 1e-6f) return t2;
+        return -1.0f;
+    }
+    if (q.type == TFG_AN_PLANE) {
+        const V3 n = ld3(q.a);
+        const float denom = dot(n, d);
+        if (fabsf(denom) > 1e-6f) {
+            const float t = dot(n * q.s - o, n) / denom;
+            return t > 1e-6f ? t : -1.0f;
+        }
+        return -1.0f;
+    }
+    // Moller-Trumbore, primitiva.h:170-187
+    const V3 v0 = ld3(q.a), e1 = ld3(q.b) - v0, e2 = ld3(q.c) - v0;
+    const V3 pv = cross(d, e2);
+    const float det = dot(e1, pv);
+    if (fabsf(det) < 1e-6f) return -1.0f;
+    const float inv = 1.0f / det;
+    const V3 tv = o - v0;
+    const float u = dot(tv, pv) * inv;
+    if (u < 0.0f || u > 1.0f) return -1.0f;
+    const V3 qv = cross(tv, e1);
+    const float v = dot(d, qv) * inv;
+    if (v < 0.0f || u + v > 1.0f) return -1.0f;
+    const float t = dot(e2, qv) * inv;
+    return t > 1e-6f ? t : -1.0f;
+}
+
+// Surface normal of an analytic hit: sphere outward (primitiva.h:190-192),
+// plane as stored (primitiva.h:194-196), triangle geometric normal turned
+// toward the incoming ray.
+__device__ __forceinline__ V3 analytic_normal(const tfg_analytic& q, V3 p, V3 d) {
+    if (q.type == TFG_AN_SPHERE) return normalize(p - ld3(q.a));
+    if (q.type == TFG_AN_PLANE) return ld3(q.a);
+    V3 n = normalize(cross(ld3(q.b) - ld3(q.a), ld3(q.c) - ld3(q.a)));
+    if (dot(n, d) > 0.0f) n = -n;
+    return n;
+}
+
 // ------------------------------------------------------------ materials
 struct Mat {
     V3 kd, ks, kt, le;

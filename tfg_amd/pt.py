@@ -28,6 +28,11 @@ class DeviceScene:
         self.desc.d_materials = self.materials.data_ptr()
         self.desc.d_lights = self.lights.data_ptr()
         self.desc.n_lights = scene.lights.shape[0]
+        an = getattr(scene, "analytic", None)
+        if an is not None and an.shape[0]:
+            self.analytic = torch.from_numpy(np.ascontiguousarray(an).view(np.int32)).cuda()
+            self.desc.d_analytic = self.analytic.data_ptr()
+            self.desc.n_analytic = an.shape[0]
 
 
 class PathTracer:

@@ -88,6 +88,7 @@ class Scene:
     lights: np.ndarray       # (L, 8) float32: pos, pad, intensity, pad
     camera: Camera
     seed: int = 0
+    analytic: np.ndarray | None = None   # (K, 12) float32 tfg_analytic records (type/material as uint32 bits)
 
     @property
     def n_tris(self) -> int:
@@ -362,6 +363,55 @@ def instances(n_inst: int = 64, sphere_res=(256, 128), box_res: int = 35, seed: 
     cam = Camera(pos=(0.0, 4.0, 8.0), front=(0.0, -0.35, -1.0), up=(0.0, 1.0, 0.0), fov_deg=60.0)
     return Scene("instances", np.concatenate(verts).astype(np.float32), np.concatenate(norms).astype(np.float32),
                  np.concatenate(mids), np.stack(mats), lights, cam, seed)
+
+
+def analytic_records(items) -> np.ndarray:
+    """items: (type, a, b, s, c, material) tuples -> (K, 12) float32 tfg_analytic."""
+    out = np.zeros((len(items), 12), np.float32)
+    u = out.view(np.uint32)
+    for i, (typ, a, b, sval, c, mat) in enumerate(items):
+        out[i, 0:3] = a
+        u[i, 3] = typ
+        out[i, 4:7] = b
+        out[i, 7] = sval
+        out[i, 8:11] = c
+        u[i, 11] = mat
+    return out
+
+
+def reference_scene(mesh_level: int = 3, seed: int = 0) -> Scene:
+    """The reference's base scene (main.cpp:164-345): five coloured planes, seven
+    spheres (two glass, two plastic, three diffuse) and one point light, as
+    analytic primitives, plus a small icosphere mesh so the CWBVH path runs."""
+    PL, SP = 1, 0
+    mats, items = [], []
+
+    def add(kd=(0, 0, 0), ks=(0, 0, 0), kt=(0, 0, 0), ior=1.0):
+        mats.append(material(kd=kd, ks=ks, kt=kt, ior=ior))
+        return len(mats) - 1
+
+    items.append((PL, (1, 0, 0), (0, 0, 0), -3.0, (0, 0, 0), add(kd=(0.1, 0.8, 0.9))))     # left
+    items.append((PL, (-1, 0, 0), (0, 0, 0), -3.0, (0, 0, 0), add(kd=(0.9, 0.1, 0.6))))    # right
+    items.append((PL, (0, 1, 0), (0, 0, 0), -3.0, (0, 0, 0), add(kd=(0.8, 0.8, 0.8))))     # floor
+    items.append((PL, (0, -1, 0), (0, 0, 0), -3.0, (0, 0, 0), add(kd=(0.8, 0.8, 0.8))))    # ceiling
+    items.append((PL, (0, 0, 1), (0, 0, 0), -8.0, (0, 0, 0), add(kd=(0.8, 0.8, 0.8))))     # back
+    spheres = [((-1.5, -1.0, -3.8), 1.1, dict(ks=(0.04,) * 3, kt=(0.95,) * 3, ior=1.5)),
+               ((1.8, 1.5, -4.5), 0.8, dict(kd=(0.9, 0.1, 0.1), ks=(0.3,) * 3)),
+               ((0.2, -2.2, -4.0), 0.7, dict(kd=(0.3, 0.5, 0.2))),
+               ((-1.2, 1.8, -5.0), 0.6, dict(kd=(0.2, 0.8, 0.2), ks=(0.2,) * 3)),
+               ((2.0, -2.0, -4.2), 0.5, dict(ks=(0.04,) * 3, kt=(0.95,) * 3, ior=1.5)),
+               ((-0.5, 0.5, -4.8), 0.6, dict(kd=(0.1, 0.1, 0.9), ks=(0.4,) * 3)),
+               ((1.0, -0.5, -5.5), 0.9, dict(kd=(0.9, 0.8, 0.2)))]
+    for c, r, kw in spheres:
+        items.append((SP, c, (0, 0, 0), r, (0, 0, 0), add(**kw)))
+    mesh_mat = add(kd=(0.75, 0.75, 0.75))
+    v, n = icosphere(mesh_level, 0.5, center=(1.2, 0.6, -3.2))
+    lights = np.zeros((1, 8), np.float32)
+    lights[0, 0:3] = (0.0, 2.9, -4.0)
+    lights[0, 4:7] = (100.0, 100.0, 100.0)
+    cam = Camera(pos=(0.0, -0.75, 4.25), front=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0))
+    return Scene("reference", v, n, np.full(v.shape[0], mesh_mat, np.uint32), np.stack(mats), lights, cam, seed,
+                 analytic=analytic_records(items))
 
 
 def random_soup(n: int, seed: int, extent: float = 1.0) -> np.ndarray:
