@@ -190,6 +190,13 @@ int tfg_nrc_last_gradient(tfg_nrc* nrc, float* host_grad);
  * id in float 19) to host memory, at most max_* of each; synchronises. */
 int tfg_pt_read_cache_data(tfg_pt* pt, float* records, uint32_t max_records, float* queries, uint32_t max_queries,
                            uint32_t* n_records, uint32_t* n_queries);
+/* ColorMLP::inference (mlp.cpp:127-170) on DatosMLP records: sanitised
+ * predictions (NrcKernel.cu:99-126) into d_out[n]; synchronous. */
+int tfg_nrc_infer_datos(tfg_nrc* nrc, const tfg_datos_mlp* d_in, uint32_t n, tfg_color* d_out, tfg_stream stream);
+/* ColorMLP::train_step (mlp.cpp:95-125): one optimiser step on the
+ * configured batch drawn (NrcKernel.cu:10-16) from d_in[0..n) with target =
+ * DatosMLP.color; no-op with *loss = 0 when n < 100; synchronous. */
+int tfg_nrc_train_datos(tfg_nrc* nrc, const tfg_datos_mlp* d_in, uint32_t n, float* loss, tfg_stream stream);
 /* Attaches the cache to a path tracer created with a nonzero mode. */
 int tfg_pt_set_nrc(tfg_pt* pt, tfg_nrc* nrc);
 /* Switches the frame mode (0 = plain, TFG_PT_INFER | TFG_PT_TRAIN); a

@@ -42,6 +42,12 @@ struct tfg_nrc {
     uint32_t* state = nullptr;
     float* loss_accum = nullptr;
     int n_cu = 0;
+    // DatosMLP entry points (ColorMLP::train_step / inference): converted
+    // inputs, outputs and the sample count, grown on demand
+    float4* dm_buf = nullptr;
+    float* dm_out = nullptr;
+    uint32_t* dm_count = nullptr;
+    size_t dm_cap = 0;
 };
 
 namespace tfg {
@@ -404,12 +410,13 @@ __global__ __launch_bounds__(256) void k_nrc_publish(const float* __restrict__ m
 }
 
 template <typename T>
-int enqueue_train_steps(tfg_nrc* nc, const float4* records, const uint32_t* n_rec, uint32_t steps, hipStream_t s) {
+int enqueue_train_steps(tfg_nrc* nc, const float4* records, const uint32_t* n_rec, uint32_t steps, hipStream_t s,
+                        uint32_t min_records) {
     const uint32_t nblk = nc->cfg.batch / 64;
     TrainArgs A;
     A.records = records;
     A.n_rec = n_rec;
-    A.min_records = nc->cfg.min_records;
+    A.min_records = min_records;
     A.state = nc->state;
     A.batch = nc->cfg.batch;
     A.grad_scale = nc->cfg.loss_scale / (3.0f * (float)nc->cfg.batch);
@@ -426,7 +433,7 @@ int enqueue_train_steps(tfg_nrc* nc, const float4* records, const uint32_t* n_re
         hipLaunchKernelGGL(k_nrc_step_begin, dim3(1), dim3(64), 0, s, nc->state, nc->loss_accum);
         hipLaunchKernelGGL(k_nrc_train<T>, dim3(nblk), dim3(256), kTrainLds, s, A);
         hipLaunchKernelGGL(k_nrc_grad_reduce, dim3(pblocks), dim3(256), 0, s, nc->partial, nblk, nc->grad, nc->state,
-                           n_rec, nc->cfg.min_records, h.inv_loss_scale);
+                           n_rec, min_records, h.inv_loss_scale);
         hipLaunchKernelGGL(k_nrc_adam<T>, dim3(pblocks), dim3(256), 0, s, nc->master, nc->adam_m, nc->adam_v, nc->ema,
                            nc->grad, nc->state, h, nc->fop_train, nc->bop_train, nc->fop_infer);
         hipLaunchKernelGGL(k_nrc_step_end, dim3(1), dim3(64), 0, s, nc->state, nc->loss_accum);
@@ -453,6 +460,47 @@ int setup_lds() {
 
 }  // namespace
 
+// DatosMLP (mlp_types.h:14-23) -> vertex record (roughness 0) or training
+// record (L_suffix = colour, T_suffix = 0, normaliser 1: target = colour).
+__global__ void k_from_datos(const tfg_datos_mlp* __restrict__ d, uint32_t n, float4* out, int as_record) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const tfg_datos_mlp x = d[i];
+        float4* o = out + (size_t)(as_record ? kRecF4 : kVertF4) * i;
+        o[0] = make_float4((float)x.posicion.x, (float)x.posicion.y, (float)x.posicion.z, 0.0f);
+        o[1] = make_float4((float)x.direccion.x, (float)x.direccion.y, (float)x.direccion.z, x.difuso.r);
+        o[2] = make_float4((float)x.normal.x, (float)x.normal.y, (float)x.normal.z, x.difuso.g);
+        o[3] = make_float4(x.difuso.b, x.especular.r, x.especular.g, x.especular.b);
+        if (as_record) {
+            for (int k = 4; k < 8; ++k) o[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            o[8] = make_float4(x.color.r, x.color.g, x.color.b, 1.0f);
+            o[9] = make_float4(0.f, 0.f, 0.f, 0.f);
+            o[10] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        }
+    }
+}
+
+__global__ void k_to_color(const float* __restrict__ raw, uint32_t n, tfg_color* out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = tfg_color{sanitize_out(raw[3 * (size_t)i]), sanitize_out(raw[3 * (size_t)i + 1]),
+                           sanitize_out(raw[3 * (size_t)i + 2])};
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+int dm_reserve(tfg_nrc* nc, size_t n) {
+    if (n <= nc->dm_cap) return TFG_OK;
+    if (nc->dm_buf) (void)hipFree(nc->dm_buf);
+    if (nc->dm_out) (void)hipFree(nc->dm_out);
+    nc->dm_buf = nullptr;
+    nc->dm_out = nullptr;
+    nc->dm_cap = 0;
+    TFG_HIP_CHECK(hipMalloc((void**)&nc->dm_buf, n * kRecF4 * sizeof(float4)));
+    TFG_HIP_CHECK(hipMalloc((void**)&nc->dm_out, n * 3 * sizeof(float)));
+    if (!nc->dm_count) TFG_HIP_CHECK(hipMalloc((void**)&nc->dm_count, sizeof(uint32_t)));
+    nc->dm_cap = n;
+    return TFG_OK;
+}
+
 int nrc_frame_enqueue(tfg_nrc* nc, const tfg_nrc_frame& f, tfg_color* image, hipStream_t s) {
     const bool fp16 = nc->cfg.precision == TFG_NRC_FP16;
     int rc = TFG_OK;
@@ -462,8 +510,10 @@ int nrc_frame_enqueue(tfg_nrc* nc, const tfg_nrc_frame& f, tfg_color* image, hip
         if (rc != TFG_OK) return rc;
     }
     if ((f.mode & MODE_TRAIN) && nc->cfg.steps_per_frame > 0) {
-        rc = fp16 ? enqueue_train_steps<_Float16>(nc, f.records, f.counters + CNT_RECORDS, nc->cfg.steps_per_frame, s)
-                  : enqueue_train_steps<__bf16>(nc, f.records, f.counters + CNT_RECORDS, nc->cfg.steps_per_frame, s);
+        rc = fp16 ? enqueue_train_steps<_Float16>(nc, f.records, f.counters + CNT_RECORDS, nc->cfg.steps_per_frame, s,
+                                                  nc->cfg.min_records)
+                  : enqueue_train_steps<__bf16>(nc, f.records, f.counters + CNT_RECORDS, nc->cfg.steps_per_frame, s,
+                                                nc->cfg.min_records);
     }
     return rc;
 }
@@ -621,8 +671,9 @@ int tfg_nrc_eval(tfg_nrc* nc, const float* d_verts, uint32_t n, float* d_out, fl
 int tfg_nrc_train(tfg_nrc* nc, const float* d_records, const uint32_t* d_n_records, uint32_t steps, tfg_stream stream) {
     if (!nc || !d_records || !d_n_records) { set_error("tfg_nrc_train: NULL argument"); return TFG_E_INVALID; }
     const float4* r = reinterpret_cast<const float4*>(d_records);
-    return nc->cfg.precision == TFG_NRC_FP16 ? enqueue_train_steps<_Float16>(nc, r, d_n_records, steps, stream)
-                                             : enqueue_train_steps<__bf16>(nc, r, d_n_records, steps, stream);
+    return nc->cfg.precision == TFG_NRC_FP16
+               ? enqueue_train_steps<_Float16>(nc, r, d_n_records, steps, stream, nc->cfg.min_records)
+               : enqueue_train_steps<__bf16>(nc, r, d_n_records, steps, stream, nc->cfg.min_records);
 }
 
 int tfg_nrc_last_gradient(tfg_nrc* nc, float* host_grad) {
@@ -632,11 +683,51 @@ int tfg_nrc_last_gradient(tfg_nrc* nc, float* host_grad) {
     return TFG_OK;
 }
 
+// ColorMLP::inference (mlp.cpp:127-170): DatosMLP -> sanitised Color, synchronous.
+int tfg_nrc_infer_datos(tfg_nrc* nc, const tfg_datos_mlp* d_in, uint32_t n, tfg_color* d_out, tfg_stream stream) {
+    if (!nc || !d_in || !d_out) { set_error("tfg_nrc_infer_datos: NULL argument"); return TFG_E_INVALID; }
+    if (n == 0) return TFG_OK;
+    int rc = dm_reserve(nc, n);
+    if (rc != TFG_OK) return rc;
+    hipLaunchKernelGGL(k_from_datos, dim3(1024), dim3(256), 0, stream, d_in, n, nc->dm_buf, 0);
+    rc = tfg_nrc_eval(nc, reinterpret_cast<const float*>(nc->dm_buf), n, nc->dm_out, nullptr, 0, stream);
+    if (rc != TFG_OK) return rc;
+    hipLaunchKernelGGL(k_to_color, dim3(1024), dim3(256), 0, stream, nc->dm_out, n, d_out);
+    TFG_HIP_CHECK(hipGetLastError());
+    TFG_HIP_CHECK(hipStreamSynchronize(stream));
+    return TFG_OK;
+}
+
+// ColorMLP::train_step (mlp.cpp:95-125): one optimiser step on `batch`
+// samples drawn from the first n entries; returns 0 loss and does nothing
+// when n < 100 (mlp.cpp:96).  Synchronous.
+int tfg_nrc_train_datos(tfg_nrc* nc, const tfg_datos_mlp* d_in, uint32_t n, float* loss, tfg_stream stream) {
+    if (!nc || !d_in) { set_error("tfg_nrc_train_datos: NULL argument"); return TFG_E_INVALID; }
+    if (loss) *loss = 0.0f;
+    if (n < 100) return TFG_OK;
+    int rc = dm_reserve(nc, n);
+    if (rc != TFG_OK) return rc;
+    hipLaunchKernelGGL(k_from_datos, dim3(1024), dim3(256), 0, stream, d_in, n, nc->dm_buf, 1);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, stream, nc->dm_count, n);
+    TFG_HIP_CHECK(hipGetLastError());
+    rc = nc->cfg.precision == TFG_NRC_FP16
+             ? enqueue_train_steps<_Float16>(nc, nc->dm_buf, nc->dm_count, 1, stream, 99u)
+             : enqueue_train_steps<__bf16>(nc, nc->dm_buf, nc->dm_count, 1, stream, 99u);
+    if (rc != TFG_OK) return rc;
+    TFG_HIP_CHECK(hipStreamSynchronize(stream));
+    if (loss) {
+        uint32_t s[16];
+        TFG_HIP_CHECK(hipMemcpy(s, nc->state, sizeof(s), hipMemcpyDeviceToHost));
+        std::memcpy(loss, &s[kStLoss], sizeof(float));
+    }
+    return TFG_OK;
+}
+
 int tfg_nrc_destroy(tfg_nrc* nc) {
     if (!nc) return TFG_OK;
     (void)hipDeviceSynchronize();
     void* bufs[] = {nc->master, nc->adam_m, nc->adam_v, nc->ema, nc->grad, nc->partial, nc->fop_train,
-                    nc->bop_train, nc->fop_infer, nc->state, nc->loss_accum};
+                    nc->bop_train, nc->fop_infer, nc->state, nc->loss_accum, nc->dm_buf, nc->dm_out, nc->dm_count};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete nc;
