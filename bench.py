@@ -1,6 +1,13 @@
 """Benchmark driver: prints ONE JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload traversal]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pt1080|pt|traversal]
+
+Workload ``pt1080`` (default, the metric's configuration): the plain FP32
+wavefront path tracer at 1920x1080 on the 64-instance scene (value = its
+Mrays/s over K hipGraph frames), followed by the equal-time relMSE
+comparison of the cache-accelerated tracer against it.
+
+Workload ``pt``: the Mrays/s part alone.
 
 Workload ``traversal``: the traversal microbenchmark on a 2^20-triangle
 Perlin-displaced sphere (CWBVH built by this library).  One step = 2^21
@@ -226,6 +233,30 @@ def bench_pt(args, torch):
     return res
 
 
+def bench_pt1080(args, torch):
+    """The metric's configuration: the plain FP32 wavefront PT of bench_pt at
+    1920x1080 (value = its Mrays/s), then the equal-time comparison of the
+    cache-accelerated tracer (BF16 MLP, 48 warmup frames counted in its time)
+    against the plain one, both measured by relMSE against a plain reference
+    accumulated in FP64."""
+    from tfg_amd import experiments, scenes
+
+    res = bench_pt(args, torch)
+    sc = res["_cpu_inputs"][0]
+    W, H, MD, RR = 1920, 1080, 8, 3
+    t0 = time.time()
+    cmp = experiments.equal_time(torch, sc, W, H, MD, RR, ref_spp=args.ref_spp, budget_s=args.budget,
+                                 warmup=48, precision=0)
+    _log(f"equal-time comparison: {json.dumps(cmp)} ({time.time() - t0:.1f}s)")
+    res["relmse"] = cmp
+    res["config"]["workload"] = ("1080p path tracer (64 instances, 2.57M tris, 4 lights, Lambertian+GGX, depth 8, "
+                                 "RR after 3): plain FP32 wavefront PT Mrays/s; equal-time relMSE of "
+                                 "NRC-BF16 vs plain against a plain FP64-accumulated reference")
+    res["config"]["relmse_reference_spp"] = args.ref_spp
+    res["config"]["equal_time_budget_s"] = args.budget
+    return res
+
+
 def cpu_baseline_pt(sc, host, W, H, MD, RR):
     import oracle
 
@@ -276,7 +307,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="traversal", choices=["traversal", "pt"])
+    ap.add_argument("--workload", default="pt1080", choices=["pt1080", "pt", "traversal"])
+    ap.add_argument("--ref-spp", type=int, default=512, help="reference spp of the relMSE comparison")
+    ap.add_argument("--budget", type=float, default=2.0, help="equal-time budget in seconds per method")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -288,8 +321,8 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "pt":
-        res = bench_pt(args, torch)
+    if args.workload in ("pt", "pt1080"):
+        res = bench_pt1080(args, torch) if args.workload == "pt1080" else bench_pt(args, torch)
         cpu_in = res.pop("_cpu_inputs")
         cpu = None if args.no_cpu else cpu_baseline_pt(*cpu_in)
     else:
@@ -315,6 +348,8 @@ def main():
         "roofline": res["roofline"],
         "cpu_baseline": cpu,
     }
+    if "relmse" in res:
+        out["relmse"] = res["relmse"]
     print(json.dumps(out), flush=True)
 
 

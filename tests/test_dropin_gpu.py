@@ -22,36 +22,33 @@ REG = np.dtype([("head", GEOM), ("tail", GEOM), ("luz", COL), ("thr", COL), ("no
                align=True)
 
 
+# Primitiva (primitiva.h:17-39) with the union's members at their offsets.
+PRIM = np.dtype({"names": ["tipo", "emision", "difuso", "especular", "transmision", "ior", "centro", "radio",
+                           "pnormal", "pdist", "tri"],
+                 "formats": [np.int32, COL, COL, COL, COL, np.float32, V3, np.float32, V3, np.float32, (np.float64, 18)],
+                 "offsets": [0, 4, 16, 28, 40, 52, 56, 80, 56, 80, 56], "itemsize": 200})
+
+
 def _prims_from_scene(sc):
     """Reference Primitiva records: analytic list + per-primIdx mesh copy."""
-    P = oracle.PRIM_DTYPE
     mats = sc.materials
-    an = np.zeros(sc.analytic.shape[0], P)
     au = sc.analytic.view(np.uint32)
+    an = np.zeros(sc.analytic.shape[0], PRIM)
     for i, rec in enumerate(sc.analytic):
         typ, m = int(au[i, 3]), mats[au[i, 11]]
         an[i]["tipo"] = typ
-        an[i]["difuso"], an[i]["especular"], an[i]["transmision"] = m[0:3], m[4:7], m[8:11]
-        an[i]["emision"] = m[12:15]
-        an[i]["ior"] = m[7]
-        g = np.zeros(18)
+        an[i]["emision"], an[i]["difuso"], an[i]["especular"] = m[12:15], m[0:3], m[4:7]
+        an[i]["transmision"], an[i]["ior"] = m[8:11], m[7]
         if typ == 0:
-            g[0:3], g[3] = rec[0:3], 0.0
-            an[i]["g"] = g
-            an[i]["g"][3] = 0
-            raw = an[i:i + 1].view(np.uint8).copy()
-            raw[0, 56 + 24:56 + 28] = np.frombuffer(np.float32(rec[7]).tobytes(), np.uint8)
-            an[i:i + 1] = raw.view(P)
-        elif typ == 1:
-            g[0:3] = rec[0:3]
-            an[i]["g"] = g
-            raw = an[i:i + 1].view(np.uint8).copy()
-            raw[0, 56 + 24:56 + 28] = np.frombuffer(np.float32(rec[7]).tobytes(), np.uint8)
-            an[i:i + 1] = raw.view(P)
-    mesh = oracle.primitivas_from_verts(sc.verts, sc.normals)
+            an[i]["centro"], an[i]["radio"] = rec[0:3], rec[7]
+        else:
+            an[i]["pnormal"], an[i]["pdist"] = rec[0:3], rec[7]
+    mesh = np.zeros(sc.n_tris, PRIM)
+    mesh["tipo"] = 2
     m = mats[sc.mat_id]
-    mesh["difuso"], mesh["especular"], mesh["transmision"] = m[:, 0:3], m[:, 4:7], m[:, 8:11]
-    mesh["ior"] = m[:, 7]
+    mesh["difuso"], mesh["especular"], mesh["transmision"], mesh["ior"] = m[:, 0:3], m[:, 4:7], m[:, 8:11], m[:, 7]
+    mesh["tri"][:, 0:9] = sc.verts
+    mesh["tri"][:, 9:18] = sc.normals
     return an, mesh
 
 

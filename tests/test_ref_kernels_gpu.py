@@ -43,7 +43,7 @@ def test_camera(torch_cuda):
     np.testing.assert_array_equal(c["pos"], [0, -0.75, 4.25])
     np.testing.assert_array_equal(c["front"], [0, 0, -1])
     np.testing.assert_array_equal(c["up"], [0, 1, 0])
-    np.testing.assert_array_equal(c["left"], [-1, 0, 0])   # front x up
+    np.testing.assert_array_equal(c["left"], [1, 0, 0])    # front x up (camara.h:36)
     h = np.float32(2 * np.tan(np.float32(np.float32(45 * np.pi / 180) / 2)))
     assert abs(c["h"] - h) <= 1e-6 and abs(c["w"] - h * np.float32(1920 / 1080)) <= 1e-6 and c["f"] == 1
 
@@ -104,7 +104,9 @@ def test_targets_tail_and_composite(torch_cuda):
     any_pos = (thr > 0).any(1)
     ref = img.copy()
     ref[any_pos] += thr[any_pos] * pred[any_pos]
-    np.testing.assert_array_equal(d_img.cpu().numpy(), ref)
+    # the kernel may fuse the multiply-add (as nvcc does by default): 1 ulp
+    np.testing.assert_allclose(d_img.cpu().numpy(), ref, rtol=2.5e-7, atol=1e-30)
+    np.testing.assert_array_equal(d_img.cpu().numpy()[~any_pos], img[~any_pos])
 
 
 def test_pack_and_save_output(torch_cuda):

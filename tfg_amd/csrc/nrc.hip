@@ -246,8 +246,8 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
         }
     }
     encode<T>(R[0], R[1], R[2], R[3], A.b, g, x[0][0], x[0][1]);
-    V xs[1][kNrcLayers][2];
-    mlp_forward<T, 1>(s_fop, lane, x, out, xs);
+    V xs[kNrcLayers][2];
+    mlp_forward_keep<T>(s_fop, lane, x, out, xs);
     floatx4 dy5 = floatx4{0.f, 0.f, 0.f, 0.f};
     float loss = 0.0f;
     if (g == 0) {   // RelativeL2Luminance: (p - t)^2 / (lum(p)^2 + 0.01), lum held constant
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
     __syncthreads();
     for (int ii = 0; ii < 4; ++ii) s_dyT[(4 * g + ii) * 64 + m] = (T)dy5[ii];
     for (int c = 0; c < 2; ++c)
-        for (int j = 0; j < 8; ++j) s_xT[nrc_perm(c, g, j) * 64 + m] = xs[0][5][c][j];
+        for (int j = 0; j < 8; ++j) s_xT[nrc_perm(c, g, j) * 64 + m] = xs[5][c][j];
     __syncthreads();
     {
         const int u = w;   // k-tile of this wave
@@ -287,16 +287,17 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
         for (int u = 0; u < 4; ++u) {
             dy[u] = M::mfma(s_bop[(bop_tile_base(5) + u) * 64 + lane], bdy, floatx4{0.f, 0.f, 0.f, 0.f});
             for (int ii = 0; ii < 4; ++ii)
-                if (!((float)xs[0][5][u >> 1][(u & 1) * 4 + ii] > 0.0f)) dy[u][ii] = 0.0f;
+                if (!((float)xs[5][u >> 1][(u & 1) * 4 + ii] > 0.0f)) dy[u][ii] = 0.0f;
         }
     }
     __syncthreads();
-    // ---- layers 4..0 ----
+    // ---- layers 4..0 (unrolled: the saved operands stay in registers) ----
+#pragma unroll
     for (int l = 4; l >= 0; --l) {
         for (int t = 0; t < 4; ++t)
             for (int ii = 0; ii < 4; ++ii) s_dyT[(16 * t + 4 * g + ii) * 64 + m] = (T)dy[t][ii];
         for (int c = 0; c < 2; ++c)
-            for (int j = 0; j < 8; ++j) s_xT[nrc_perm(c, g, j) * 64 + m] = xs[0][l][c][j];
+            for (int j = 0; j < 8; ++j) s_xT[nrc_perm(c, g, j) * 64 + m] = xs[l][c][j];
         __syncthreads();
         for (int u = 0; u < 4; ++u) {
             floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
                 acc = M::mfma(s_bop[(bop_tile_base(l) + 2 * u) * 64 + lane], b0, acc);
                 acc = M::mfma(s_bop[(bop_tile_base(l) + 2 * u + 1) * 64 + lane], b1, acc);
                 for (int ii = 0; ii < 4; ++ii)
-                    if (!((float)xs[0][l][u >> 1][(u & 1) * 4 + ii] > 0.0f)) acc[ii] = 0.0f;
+                    if (!((float)xs[l][u >> 1][(u & 1) * 4 + ii] > 0.0f)) acc[ii] = 0.0f;
                 dy[u] = acc;
             }
         }

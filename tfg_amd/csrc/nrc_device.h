@@ -135,45 +135,36 @@ __device__ __forceinline__ typename MfmaOps<T>::vec pack_relu(floatx4 lo, floatx
     return v;
 }
 
-// Forward through all layers for `G` 16-sample groups of one wave.  W is the
-// forward operand array in LDS (kFopTiles x 64 lanes).  On return out[g] holds
-// the output tile: lanes with l/16 == 0 own outputs 0..2 of sample l%16.
-// If xs != nullptr the B operands of every layer are kept (training).
+// One hidden layer for G 16-sample groups: x <- relu(W_l x).
 template <typename T, int G>
-__device__ __forceinline__ void mlp_forward(const typename MfmaOps<T>::vec* __restrict__ W, int lane,
-                                            typename MfmaOps<T>::vec (&x)[G][2], floatx4 (&out)[G],
-                                            typename MfmaOps<T>::vec (*xs)[kNrcLayers][2] = nullptr) {
+__device__ __forceinline__ void mlp_layer(const typename MfmaOps<T>::vec* __restrict__ W, int lane, int l,
+                                          typename MfmaOps<T>::vec (&x)[G][2]) {
     typedef MfmaOps<T> M;
-#pragma unroll 1
-    for (int l = 0; l < kNrcHidden; ++l) {
-        if (xs) {
+    floatx4 acc[G][4];
 #pragma unroll
-            for (int g = 0; g < G; ++g) { xs[g][l][0] = x[g][0]; xs[g][l][1] = x[g][1]; }
-        }
-        floatx4 acc[G][4];
+    for (int g = 0; g < G; ++g)
 #pragma unroll
-        for (int g = 0; g < G; ++g)
+        for (int t = 0; t < 4; ++t) acc[g][t] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int t = 0; t < 4; ++t) acc[g][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < 2; ++c) {
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
+        for (int t = 0; t < 4; ++t) {
+            const typename M::vec a = W[(fop_tile_base(l) + 2 * t + c) * 64 + lane];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const typename M::vec a = W[(fop_tile_base(l) + 2 * t + c) * 64 + lane];
-#pragma unroll
-                for (int g = 0; g < G; ++g) acc[g][t] = M::mfma(a, x[g][c], acc[g][t]);
-            }
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            x[g][0] = pack_relu<T>(acc[g][0], acc[g][1]);
-            x[g][1] = pack_relu<T>(acc[g][2], acc[g][3]);
+            for (int g = 0; g < G; ++g) acc[g][t] = M::mfma(a, x[g][c], acc[g][t]);
         }
     }
-    if (xs) {
 #pragma unroll
-        for (int g = 0; g < G; ++g) { xs[g][5][0] = x[g][0]; xs[g][5][1] = x[g][1]; }
+    for (int g = 0; g < G; ++g) {
+        x[g][0] = pack_relu<T>(acc[g][0], acc[g][1]);
+        x[g][1] = pack_relu<T>(acc[g][2], acc[g][3]);
     }
+}
+
+template <typename T, int G>
+__device__ __forceinline__ void mlp_output(const typename MfmaOps<T>::vec* __restrict__ W, int lane,
+                                           const typename MfmaOps<T>::vec (&x)[G][2], floatx4 (&out)[G]) {
+    typedef MfmaOps<T> M;
 #pragma unroll
     for (int g = 0; g < G; ++g) out[g] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -182,6 +173,36 @@ __device__ __forceinline__ void mlp_forward(const typename MfmaOps<T>::vec* __re
 #pragma unroll
         for (int g = 0; g < G; ++g) out[g] = M::mfma(a, x[g][c], out[g]);
     }
+}
+
+// Forward through all layers for `G` 16-sample groups of one wave.  W is the
+// forward operand array in LDS (kFopTiles x 64 lanes).  On return out[g] holds
+// the output tile: lanes with l/16 == 0 own outputs 0..2 of sample l%16.
+// The layer loop is kept rolled so the weights of one layer at a time are
+// live.
+template <typename T, int G>
+__device__ __forceinline__ void mlp_forward(const typename MfmaOps<T>::vec* __restrict__ W, int lane,
+                                            typename MfmaOps<T>::vec (&x)[G][2], floatx4 (&out)[G]) {
+#pragma unroll 1
+    for (int l = 0; l < kNrcHidden; ++l) mlp_layer<T, G>(W, lane, l, x);
+    mlp_output<T, G>(W, lane, x, out);
+}
+
+// Training forward for one group: also keeps the B operands (layer inputs)
+// of every layer in xs; unrolled so xs stays in registers.
+template <typename T>
+__device__ __forceinline__ void mlp_forward_keep(const typename MfmaOps<T>::vec* __restrict__ W, int lane,
+                                                 typename MfmaOps<T>::vec (&x)[1][2], floatx4 (&out)[1],
+                                                 typename MfmaOps<T>::vec (&xs)[kNrcLayers][2]) {
+#pragma unroll
+    for (int l = 0; l < kNrcHidden; ++l) {
+        xs[l][0] = x[0][0];
+        xs[l][1] = x[0][1];
+        mlp_layer<T, 1>(W, lane, l, x);
+    }
+    xs[5][0] = x[0][0];
+    xs[5][1] = x[0][1];
+    mlp_output<T, 1>(W, lane, x, out);
 }
 
 // guardarSalidaInferencia (NrcKernel.cu:99-126)

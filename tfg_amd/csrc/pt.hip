@@ -22,6 +22,7 @@ struct tfg_pt {
     int n_cu = 0;
     // launch shapes
     int trace_block = 128;
+    int trace_mult = 1;
     int trace_grid = 0;
     int shade_grid = 0;
     int gen_grid = 0;
@@ -61,18 +62,25 @@ void drop_graph(tfg_pt* pt) {
     pt->graph = nullptr;
 }
 
+constexpr uint32_t kTopNodes = 73;   // root + 8 + 64 nodes staged in LDS
+
+size_t trace_lds(const tfg_pt* pt, int block) {
+    return (size_t)pt->P.n_top * 80 + (size_t)pt->P.stack_depth * block * 8;
+}
+
 template <int B>
-int occupancy_extend(int* blocks) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_extend<B>, B, 0) == hipSuccess ? 0 : -1;
+int occupancy_extend(const tfg_pt* pt, int* blocks) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_extend<B>, B, trace_lds(pt, B)) == hipSuccess ? 0 : -1;
 }
 
 int set_trace_shape(tfg_pt* pt, int block, int mult) {
     int per_cu = 0;
-    if (block == 64) occupancy_extend<64>(&per_cu);
-    else if (block == 128) occupancy_extend<128>(&per_cu);
-    else occupancy_extend<256>(&per_cu);
+    if (block == 64) occupancy_extend<64>(pt, &per_cu);
+    else if (block == 128) occupancy_extend<128>(pt, &per_cu);
+    else occupancy_extend<256>(pt, &per_cu);
     if (per_cu <= 0) per_cu = 1;
     pt->trace_block = block;
+    pt->trace_mult = mult;
     pt->trace_grid = pt->n_cu * per_cu * mult;
     return TFG_OK;
 }
@@ -80,20 +88,22 @@ int set_trace_shape(tfg_pt* pt, int block, int mult) {
 template <bool kStats>
 void launch_extend(tfg_pt* pt, hipStream_t s, uint32_t q, uint32_t d) {
     const dim3 g(pt->trace_grid);
+    const size_t lds = trace_lds(pt, pt->trace_block);
     switch (pt->trace_block) {
-        case 64: hipLaunchKernelGGL((k_extend<64, kStats>), g, dim3(64), 0, s, pt->P, q, d); break;
-        case 256: hipLaunchKernelGGL((k_extend<256, kStats>), g, dim3(256), 0, s, pt->P, q, d); break;
-        default: hipLaunchKernelGGL((k_extend<128, kStats>), g, dim3(128), 0, s, pt->P, q, d); break;
+        case 64: hipLaunchKernelGGL((k_extend<64, kStats>), g, dim3(64), lds, s, pt->P, q, d); break;
+        case 256: hipLaunchKernelGGL((k_extend<256, kStats>), g, dim3(256), lds, s, pt->P, q, d); break;
+        default: hipLaunchKernelGGL((k_extend<128, kStats>), g, dim3(128), lds, s, pt->P, q, d); break;
     }
 }
 
 template <bool kStats>
 void launch_connect(tfg_pt* pt, hipStream_t s, uint32_t d) {
     const dim3 g(pt->trace_grid);
+    const size_t lds = trace_lds(pt, pt->trace_block);
     switch (pt->trace_block) {
-        case 64: hipLaunchKernelGGL((k_connect<64, kStats>), g, dim3(64), 0, s, pt->P, d); break;
-        case 256: hipLaunchKernelGGL((k_connect<256, kStats>), g, dim3(256), 0, s, pt->P, d); break;
-        default: hipLaunchKernelGGL((k_connect<128, kStats>), g, dim3(128), 0, s, pt->P, d); break;
+        case 64: hipLaunchKernelGGL((k_connect<64, kStats>), g, dim3(64), lds, s, pt->P, d); break;
+        case 256: hipLaunchKernelGGL((k_connect<256, kStats>), g, dim3(256), lds, s, pt->P, d); break;
+        default: hipLaunchKernelGGL((k_connect<128, kStats>), g, dim3(128), lds, s, pt->P, d); break;
     }
 }
 
@@ -268,6 +278,9 @@ int tfg_pt_set_scene(tfg_pt* pt, const tfg_pt_scene* sc) {
     PtParams& P = pt->P;
     P.nodes = static_cast<const float4*>(sc->bvh.d_nodos);
     P.tris = static_cast<const float4*>(sc->bvh.d_tris);
+    const uint32_t depth = known_tree_depth(sc->bvh.d_nodos);
+    P.stack_depth = depth ? (int)(depth > 1 ? depth - 1 : 1) : TFG_MAX_TREE_DEPTH - 1;
+    P.n_top = sc->bvh.num_nodos < kTopNodes ? sc->bvh.num_nodos : kTopNodes;
     P.shading = sc->d_shading;
     P.mats = sc->d_materials;
     P.lights = sc->d_lights;
@@ -276,6 +289,7 @@ int tfg_pt_set_scene(tfg_pt* pt, const tfg_pt_scene* sc) {
     P.n_analytic = sc->n_analytic;
     pt->scene_set = true;
     drop_graph(pt);
+    set_trace_shape(pt, pt->trace_block, pt->trace_mult);
     return TFG_OK;
 }
 

@@ -51,6 +51,8 @@ struct PtParams {
     // scene
     const float4* nodes;
     const float4* tris;
+    uint32_t n_top;          // leading nodes staged in LDS by the traversal stages
+    int stack_depth;         // LDS stack entries per lane (tree depth - 1)
     const tfg_tri_shading* shading;
     const tfg_material* mats;
     const tfg_point_light* lights;

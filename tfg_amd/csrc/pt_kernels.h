@@ -84,10 +84,18 @@ __device__ __forceinline__ void add_stats(PtParams& P, int slot, const TraceStat
 }
 
 // kStats: also count decoded nodes and tested triangles (profiling frames).
+// Dynamic LDS: [n_top nodes x 5 float4][stack_depth x kBlock uint2].
+__device__ __forceinline__ void stage_top_nodes(const PtParams& P, float4* top) {
+    for (uint32_t k = threadIdx.x; k < 5 * P.n_top; k += blockDim.x) top[k] = P.nodes[k];
+    __syncthreads();
+}
+
 template <int kBlock, bool kStats = false>
 __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint32_t depth) {
-    constexpr int kStack = TFG_MAX_TREE_DEPTH - 1;
-    __shared__ uint2 stack[kStack * kBlock];
+    extern __shared__ __align__(16) unsigned char lds[];
+    float4* top = reinterpret_cast<float4*>(lds);
+    uint2* stack = reinterpret_cast<uint2*>(top + 5 * P.n_top);
+    stage_top_nodes(P, top);
     const uint32_t count = P.counters[q];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         P.counters[CNT_EXT + depth] = count;
@@ -104,8 +112,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint3
             const RayPre r = make_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
             float t, u, v;
             uint32_t prim;
-            traverse<0, false, kStack, 0>(P.nodes, P.tris, nullptr, r, stack + threadIdx.x, kBlock, t, u, v, prim,
-                                         kStats ? &st : nullptr);
+            traverse<0, false>(P.nodes, P.tris, top, P.n_top, r, stack + threadIdx.x, kBlock, P.stack_depth, t, u, v,
+                               prim, kStats ? &st : nullptr);
             for (uint32_t k = 0; k < P.n_analytic; ++k) {   // escenario.h:323-334
                 const float ta = analytic_t(P.analytic[k], v3(a.x, a.y, a.z), v3(b.x, b.y, b.z));
                 if (ta > a.w && ta < t) {
@@ -355,8 +363,10 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
 // ---------------------------------------------------------------- connect
 template <int kBlock, bool kStats = false>
 __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) {
-    constexpr int kStack = TFG_MAX_TREE_DEPTH - 1;
-    __shared__ uint2 stack[kStack * kBlock];
+    extern __shared__ __align__(16) unsigned char lds[];
+    float4* top = reinterpret_cast<float4*>(lds);
+    uint2* stack = reinterpret_cast<uint2*>(top + 5 * P.n_top);
+    stage_top_nodes(P, top);
     const uint32_t count = P.counters[CNT_SHADOW];
     for (uint32_t b = blockIdx.x * kBlock; b < count; b += gridDim.x * kBlock) {
         const uint32_t i = b + threadIdx.x;
@@ -387,8 +397,8 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) 
                     const RayPre r = make_ray(o.x, o.y, o.z, dir.x, dir.y, dir.z, 0.0f, dist);
                     float t, u, v;
                     uint32_t prim;
-                    occ = traverse<0, true, kStack, 0>(P.nodes, P.tris, nullptr, r, stack + threadIdx.x, kBlock, t, u,
-                                                       v, prim, kStats ? &st : nullptr);
+                    occ = traverse<0, true>(P.nodes, P.tris, top, P.n_top, r, stack + threadIdx.x, kBlock,
+                                            P.stack_depth, t, u, v, prim, kStats ? &st : nullptr);
                 }
                 if (!occ) sum = sum + contrib;
             }

@@ -10,14 +10,19 @@ uint32_t known_tree_depth(const void* d_nodes);
 namespace {
 
 constexpr int kBlock = 128;
-constexpr int kStack = TFG_MAX_TREE_DEPTH - 1;
+constexpr uint32_t kTopNodes = 73;   // root + 8 + 64: the first three levels when fully branched
 
+// Dynamic LDS: [n_top nodes x 5 float4][stack_depth x kBlock uint2].
 template <int kFormat, bool kAnyHit, bool kStats>
 __global__ __launch_bounds__(kBlock) void k_trace(const float4* __restrict__ nodes, const float4* __restrict__ tris,
                                                   const float4* __restrict__ rays, float4* __restrict__ hits,
                                                   uint32_t* __restrict__ occluded, uint32_t* __restrict__ counts,
-                                                  uint32_t n) {
-    __shared__ uint2 stack[kStack * kBlock];
+                                                  uint32_t n, uint32_t n_top, int stack_depth) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    float4* top = reinterpret_cast<float4*>(lds);
+    uint2* stack = reinterpret_cast<uint2*>(top + 5 * n_top);
+    for (uint32_t k = threadIdx.x; k < 5 * n_top; k += kBlock) top[k] = nodes[k];
+    __syncthreads();
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const float4 a = rays[2 * (size_t)i];
@@ -26,8 +31,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(const float4* __restrict__ nod
     TraceStats st;
     float t, u, v;
     uint32_t prim;
-    const bool hit = traverse<kFormat, kAnyHit, kStack, 0>(nodes, tris, nullptr, r, stack + threadIdx.x, kBlock, t, u,
-                                                           v, prim, kStats ? &st : nullptr);
+    const bool hit = traverse<kFormat, kAnyHit>(nodes, tris, top, n_top, r, stack + threadIdx.x, kBlock, stack_depth,
+                                               t, u, v, prim, kStats ? &st : nullptr);
     if (kAnyHit) {
         occluded[i] = hit ? 1u : 0u;
     } else {
@@ -54,13 +59,17 @@ int launch(const tfg_cwbvh_gpu* bvh, uint32_t format, const tfg_ray* rays, void*
         set_error("%s: tree depth %u exceeds %d", name, depth, TFG_MAX_TREE_DEPTH);
         return TFG_E_LIMIT;
     }
+    const int stack_depth = depth ? (int)(depth > 1 ? depth - 1 : 1) : TFG_MAX_TREE_DEPTH - 1;
+    const uint32_t n_top = bvh->num_nodos < kTopNodes ? bvh->num_nodos : kTopNodes;
+    const size_t lds = (size_t)n_top * 80 + (size_t)stack_depth * kBlock * 8;
     const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
     const float4* nd = static_cast<const float4*>(bvh->d_nodos);
     const float4* tr = static_cast<const float4*>(bvh->d_tris);
     const float4* ry = reinterpret_cast<const float4*>(rays);
     float4* h = kAnyHit ? nullptr : static_cast<float4*>(out);
     uint32_t* o = kAnyHit ? static_cast<uint32_t*>(out) : nullptr;
-#define TFG_LAUNCH(F, S) hipLaunchKernelGGL((k_trace<F, kAnyHit, S>), grid, block, 0, stream, nd, tr, ry, h, o, counts, n)
+#define TFG_LAUNCH(F, S) \
+    hipLaunchKernelGGL((k_trace<F, kAnyHit, S>), grid, block, lds, stream, nd, tr, ry, h, o, counts, n, n_top, stack_depth)
     if (format == 0) {
         if (counts) TFG_LAUNCH(0, true); else TFG_LAUNCH(0, false);
     } else {

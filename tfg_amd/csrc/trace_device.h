@@ -178,12 +178,16 @@ struct TraceStats {
     uint32_t nodes = 0, tris = 0, overflow = 0;
 };
 
-// Per-lane LDS stack: entry k of this lane lives at base[k * stride].
-template <int kFormat, bool kAnyHit, int kStackDepth, int kTopNodes>
+// Per-lane LDS stack: entry k of this lane lives at base[k * stride]; it
+// holds at most stack_depth entries (tree depth - 1 suffices: one pending
+// node group per level).  Nodes with index < n_top are read from the LDS
+// copy top_nodes (the first levels of the breadth-first node array, shared
+// by every ray of the block).
+template <int kFormat, bool kAnyHit>
 __device__ __forceinline__ bool traverse(const float4* __restrict__ nodes, const float4* __restrict__ tris,
-                                         const float4* __restrict__ top_nodes, const RayPre& r, uint2* stack,
-                                         int stride, float& t_out, float& u_out, float& v_out, uint32_t& prim_out,
-                                         TraceStats* stats) {
+                                         const float4* __restrict__ top_nodes, uint32_t n_top, const RayPre& r,
+                                         uint2* stack, int stride, int stack_depth, float& t_out, float& u_out,
+                                         float& v_out, uint32_t& prim_out, TraceStats* stats) {
     int sp = 0;
     uint32_t ng_x = 0, ng_y = 0x80000000u;
     float tcur = r.tmax;
@@ -197,7 +201,7 @@ __device__ __forceinline__ bool traverse(const float4* __restrict__ nodes, const
             const int bit = highest_bit(ng_y & 0xFF000000u);
             ng_y &= ~(1u << bit);
             if (ng_y > 0x00FFFFFFu) {
-                if (sp < kStackDepth) {
+                if (sp < stack_depth) {
                     stack[sp * stride] = make_uint2(ng_x, ng_y);
                     ++sp;
                 } else if (stats) {
@@ -208,7 +212,7 @@ __device__ __forceinline__ bool traverse(const float4* __restrict__ nodes, const
             const uint32_t rel = (uint32_t)__popc(imask & ~(0xFFFFFFFFu << slot));
             const uint32_t ni = ng_x + rel;
             float4 n0, n1, n2, n3, n4;
-            if (kTopNodes > 0 && ni < (uint32_t)kTopNodes) {
+            if (ni < n_top) {
                 const float4* n = top_nodes + 5 * ni;
                 n0 = n[0]; n1 = n[1]; n2 = n[2]; n3 = n[3]; n4 = n[4];
             } else {
