@@ -9,6 +9,10 @@ comparison of the cache-accelerated tracer against it.
 
 Workload ``pt``: the Mrays/s part alone.
 
+Workload ``copy``: a plain device-to-device copy of 1 GiB (torch ``copy_``),
+the calibration run for the HBM byte counters (FETCH_SIZE / WRITE_SIZE per
+dispatch of a kernel that moves a known number of bytes).
+
 Workload ``traversal``: the traversal microbenchmark on a 2^20-triangle
 Perlin-displaced sphere (CWBVH built by this library).  One step = 2^21
 primary closest-hit rays + 2^21 cosine bounce closest-hit rays + 2^21
@@ -193,10 +197,17 @@ def bench_pt(args, torch):
     ms_ext = sum(prof["ms_extend"])
     n_ext = sum(prof["ext_rays"])
     ext_bytes = 48 * n_ext + 80 * prof["ext_nodes"] + 48 * prof["ext_tris"]
-    achieved = ext_bytes / (ms_ext * 1e-3) / 1e9
     ms_con = sum(prof["ms_connect"])
     n_shd = sum(prof["shadow_rays"])
     con_bytes = 48 * n_shd + 80 * prof["shadow_nodes"] + 48 * prof["shadow_tris"]
+    # the roofline object describes whichever traversal stage takes longer
+    stages = {
+        "extend": ("k_extend (closest-hit stage, all depths of one frame)", ext_bytes, ms_ext),
+        "connect": ("k_connect (shadow any-hit stage, all depths of one frame)", con_bytes, ms_con),
+    }
+    dom = max(stages, key=lambda k: stages[k][2])
+    name, dom_bytes, dom_ms = stages[dom]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     res = {
         "value": value,
         "ms_per_step": ms_frame,
@@ -204,15 +215,17 @@ def bench_pt(args, torch):
                           "connect": ms_con, "finalize": prof["ms_finalize"], "profiled_frame": prof["ms_total"]},
         "mrays_per_kernel": {"extend": n_ext / (ms_ext * 1e-3) / 1e6, "connect": n_shd / (ms_con * 1e-3) / 1e6},
         "roofline": {
-            "kernel": "k_extend (closest-hit stage, all depths of one frame)",
+            "kernel": name,
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "algorithmic_bytes": int(ext_bytes),
-            "connect_achieved_gbs": con_bytes / (ms_con * 1e-3) / 1e9,
+            "algorithmic_bytes": int(dom_bytes),
+            "launches_per_frame": MD,
+            "avg_launch_ms": dom_ms / MD,
+            "stage_gbs": {k: v[1] / (v[2] * 1e-3) / 1e9 for k, v in stages.items()},
         },
         "config": {
             "workload": "plain FP32 wavefront PT: 64 seeded instances (32 displaced spheres + 32 boxes, "
@@ -257,48 +270,101 @@ def bench_pt1080(args, torch):
     return res
 
 
+def bench_copy(args, torch):
+    """1 GiB device-to-device copy; one step = one copy (1 GiB read + 1 GiB
+    written)."""
+    n = 1 << 28
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    stream = torch.cuda.current_stream()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    a.record(stream)
+    for _ in range(args.steps):
+        dst.copy_(src)
+    b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    assert torch.equal(dst[-4:], src[-4:])
+    ms = a.elapsed_time(b) / args.steps
+    gbs = 2 * 4 * n / (ms * 1e-3) / 1e9
+    return {"metric": "HBM copy bandwidth (byte-counter calibration)", "value": gbs, "unit": "GB/s",
+            "ms_per_step": ms, "bytes_read": 4 * n, "bytes_written": 4 * n, "wall_s": wall}
+
+
+def _host_cores():
+    """Threads for the CPU baseline: the cores this process may run on, capped
+    by OMP_NUM_THREADS when that is set (the host's share for one GPU)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
 def cpu_baseline_pt(sc, host, W, H, MD, RR):
+    """The oracle integrator on one full frame of the same workload (all host
+    threads), plus a single-thread band of rows."""
     import oracle
 
-    cores = os.cpu_count() or 1
+    cores = _host_cores()
     R = oracle.Renderer(sc, host.nodes, host.tris, W, H)
-    y0, y1 = 520, 560
     t0 = time.perf_counter()
-    _, rays = R.render_rows(y0, y1, 1000, 1, MD, RR, nthreads=cores)
+    _, rays = R.render_rows(0, H, 1000, 1, MD, RR, nthreads=cores)
     dt = time.perf_counter() - t0
+    y0, y1 = 500, 580
+    t1 = time.perf_counter()
+    _, rays1 = R.render_rows(y0, y1, 1000, 1, MD, RR, nthreads=1)
+    dt1 = time.perf_counter() - t1
     return {
         "value": float(rays.sum()) / dt / 1e6,
         "unit": "Mrays/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"rows {y0}-{y1} of the same 1080p frame (1 spp, {int(rays[0])} extension + {int(rays[1])} "
-                  f"shadow rays) through the oracle integrator on {cores} threads, {dt:.2f} s",
+        "sample": f"one full 1080p frame of the same workload (frame seed 1000, {int(rays[0])} extension + "
+                  f"{int(rays[1])} shadow rays) through the oracle integrator on {cores} threads, {dt:.2f} s",
+        "single_thread": {"value": float(rays1.sum()) / dt1 / 1e6, "unit": "Mrays/s",
+                          "sample": f"rows {y0}-{y1} of the same frame on 1 thread, {dt1:.2f} s"},
     }
 
 
 def cpu_baseline_traversal(host, prim, bounce, shadow):
+    """The oracle CWBVH traversal on the full ray sets (all host threads), plus
+    a single-thread sample of 2^16 rays per set."""
     import oracle
 
-    cores = os.cpu_count() or 1
-    sample = 1 << 17
+    cores = _host_cores()
+
+    def run(sets, nthreads):
+        total = 0
+        t0 = time.perf_counter()
+        for rays, kind in sets:
+            if kind == "closest":
+                oracle.cwbvh_trace(host.nodes, host.tris, 0, rays, nthreads=nthreads)
+            else:
+                oracle.cwbvh_occluded(host.nodes, host.tris, 0, rays, nthreads=nthreads)
+            total += rays.shape[0]
+        return total, time.perf_counter() - t0
+
+    full = ((prim, "closest"), (bounce, "closest"), (shadow, "any"))
+    total, dt = run(full, cores)
     rng = np.random.default_rng(0)
-    total_rays = 0
-    t0 = time.perf_counter()
-    for rays, kind in ((prim, "closest"), (bounce, "closest"), (shadow, "any")):
-        sel = np.ascontiguousarray(rays[rng.choice(rays.shape[0], sample, replace=False)])
-        if kind == "closest":
-            oracle.cwbvh_trace(host.nodes, host.tris, 0, sel, nthreads=cores)
-        else:
-            oracle.cwbvh_occluded(host.nodes, host.tris, 0, sel, nthreads=cores)
-        total_rays += sample
-    dt = time.perf_counter() - t0
+    sample = 1 << 16
+    sub = tuple((np.ascontiguousarray(r[rng.choice(r.shape[0], sample, replace=False)]), k) for r, k in full)
+    total1, dt1 = run(sub, 1)
     return {
-        "value": total_rays / dt / 1e6,
+        "value": total / dt / 1e6,
         "unit": "Mrays/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"3 x 2^17 rays (primary, bounce closest-hit; shadow any-hit) drawn from the same sets, "
+        "sample": f"the full step (2^21 primary + 2^21 bounce closest-hit + 2^21 shadow any-hit rays), "
                   f"oracle CWBVH traversal on {cores} threads, {dt:.2f} s",
+        "single_thread": {"value": total1 / dt1 / 1e6, "unit": "Mrays/s",
+                          "sample": f"3 x 2^16 rays drawn from the same sets on 1 thread, {dt1:.2f} s"},
     }
 
 
@@ -307,7 +373,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt1080", choices=["pt1080", "pt", "traversal"])
+    ap.add_argument("--workload", default="pt1080", choices=["pt1080", "pt", "traversal", "copy"])
     ap.add_argument("--ref-spp", type=int, default=512, help="reference spp of the relMSE comparison")
     ap.add_argument("--budget", type=float, default=2.0, help="equal-time budget in seconds per method")
     ap.add_argument("--no-cpu", action="store_true")
@@ -321,6 +387,11 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "copy":
+        res = bench_copy(args, torch)
+        res.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True})
+        print(json.dumps(res), flush=True)
+        return
     if args.workload in ("pt", "pt1080"):
         res = bench_pt1080(args, torch) if args.workload == "pt1080" else bench_pt(args, torch)
         cpu_in = res.pop("_cpu_inputs")

@@ -128,10 +128,17 @@ typedef struct tfg_pt_profile {
     uint64_t ext_nodes, ext_tris, shadow_nodes, shadow_tris;
 } tfg_pt_profile;
 int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_profile* out);
-/* Times candidate launch shapes of the traversal stages on the current scene
- * and keeps the fastest (drops any captured graph). */
+/* Times candidate launch shapes of the traversal stages (and overlap on/off)
+ * on the current scene and keeps the fastest (drops any captured graph). */
 int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream);
 int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_grid, uint32_t* shade_grid);
+/* Overlap (default on): the shadow-ray stage of depth d runs on a second
+ * stream concurrently with the closest-hit and shading stages of depth d+1.
+ * Results are identical either way (NEE contributions are summed in depth
+ * order at the end of the frame).  No reference counterpart: the reference
+ * traces shadow rays inline in its megakernel (escenario.h:144-184). */
+int tfg_pt_set_overlap(tfg_pt* pt, int enable);
+int tfg_pt_get_overlap(const tfg_pt* pt, int* enabled);
 int tfg_pt_destroy(tfg_pt* pt);
 
 

@@ -114,6 +114,38 @@ def test_pt_graph_matches_eager_and_replays(torch_cuda):
     assert ext >= 4 * W * H
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_pt_overlap_matches_serial(torch_cuda, use_graph):
+    """Shadow rays on the side stream (overlapping the next depth) give the
+    same image bit for bit as the serial schedule, for the plain and the
+    cache-capture frames, and the same ray counts."""
+    torch = torch_cuda
+    sc = scenes.instances(n_inst=8, sphere_res=(32, 16), box_res=4, seed=5)
+    W, H = 96, 64
+    ds = pt.DeviceScene(sc, torch)
+    for mode in (0, 2):
+        imgs, counts = [], []
+        for overlap in (False, True):
+            tr = pt.PathTracer(W, H, 8, 3, mode=mode, use_graph=use_graph)
+            tr.set_scene(ds)
+            tr.set_camera(sc.camera.frame(W, H))
+            tr.set_overlap(overlap)
+            assert tr.launch_shape()["overlap"] == overlap
+            img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
+            out = []
+            for f in (7, 8, 7):
+                tr.render(f, img)
+                torch.cuda.synchronize()
+                out.append(img.cpu().numpy().copy())
+            imgs.append(out)
+            counts.append(tr.rays_last_frame())
+            tr.close()
+        for a, b in zip(imgs[0], imgs[1]):
+            np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(imgs[1][0], imgs[1][2])
+        assert counts[0] == counts[1]
+
+
 def test_pt_rejects_bad_config(torch_cuda):
     from tfg_amd import _lib
 
@@ -125,3 +157,25 @@ def test_pt_rejects_bad_config(torch_cuda):
     img = torch_cuda.zeros((64, 3), device="cuda")
     with pytest.raises(_lib.TfgError):
         t.render(0, img)   # no scene yet
+
+
+def test_unit_lobe_draw_stays_finite(torch_cuda):
+    """A lobe draw that rounds to 1.0 selects the transmission lobe of a
+    material with no transmission (probability 0); the reference's colour
+    division by zero gives black (color.h:42-48), so the pixel stays finite
+    and equal to the oracle's.  The listed (frame, pixel) pairs of the
+    1080p 64-instance workload each contain such a draw."""
+    torch = torch_cuda
+    sc = scenes.instances()
+    W, H, md, rr = 1920, 1080, 8, 3
+    ds, tracer, img = _setup(torch, sc, W, H, md, rr)
+    ref = oracle.Renderer(sc, ds.host.nodes, ds.host.tris, W, H)
+    cases = [(134, 1165, 597), (135, 792, 364), (1000284, 1025, 545), (1000378, 1082, 617), (1000485, 1038, 566)]
+    for frame, x, y in cases:
+        print(f"frame={frame} pixel=({x},{y})", flush=True)
+        tracer.render(frame, img)
+        torch.cuda.synchronize()
+        assert bool(torch.isfinite(img).all())
+        g = img[y * W + x].cpu().numpy().astype(np.float64)
+        c = ref.pixel(x, y, frame, md, rr).astype(np.float64)
+        np.testing.assert_allclose(g, c, rtol=1e-3, atol=1e-3)

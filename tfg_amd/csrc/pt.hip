@@ -28,7 +28,11 @@ struct tfg_pt {
     int gen_grid = 0;
     // graph
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;    // shadow-ray stage (connect), overlapped with the next depth
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipEvent_t ev_shaded[tfg::kMaxDepth] = {};   // shade of depth d done: its shadow queue is complete
+    hipEvent_t ev_joined = nullptr;              // every connect of the frame done
+    bool overlap = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const void* key_image = nullptr;
@@ -107,32 +111,80 @@ void launch_connect(tfg_pt* pt, hipStream_t s, uint32_t d) {
     }
 }
 
-// Per-stage events of a profiled frame (nullptr: plain frame).
+// Per-stage events of a profiled frame (nullptr: plain frame): a start and
+// an end event around each stage, recorded on the stream the stage runs on.
 struct StageEvents {
-    hipEvent_t ev[4 + 3 * kMaxDepth + 8];
-    int n = 0;
-    void mark(hipStream_t s) { (void)hipEventRecord(ev[n++], s); }
+    hipEvent_t gen[2] = {}, fin[2] = {};
+    hipEvent_t ext[kMaxDepth][2] = {}, shd[kMaxDepth][2] = {}, con[kMaxDepth][2] = {};
+    int create(uint32_t depth) {
+        hipError_t e = hipSuccess;
+        auto mk = [&](hipEvent_t* p) {
+            if (e == hipSuccess) e = hipEventCreate(p);
+        };
+        for (int k = 0; k < 2; ++k) {
+            mk(&gen[k]);
+            mk(&fin[k]);
+            for (uint32_t d = 0; d < depth; ++d) {
+                mk(&ext[d][k]);
+                mk(&shd[d][k]);
+                mk(&con[d][k]);
+            }
+        }
+        return e == hipSuccess ? TFG_OK : hip_fail(e, "hipEventCreate(profile)");
+    }
+    ~StageEvents() {
+        for (int k = 0; k < 2; ++k) {
+            if (gen[k]) (void)hipEventDestroy(gen[k]);
+            if (fin[k]) (void)hipEventDestroy(fin[k]);
+            for (uint32_t d = 0; d < kMaxDepth; ++d) {
+                if (ext[d][k]) (void)hipEventDestroy(ext[d][k]);
+                if (shd[d][k]) (void)hipEventDestroy(shd[d][k]);
+                if (con[d][k]) (void)hipEventDestroy(con[d][k]);
+            }
+        }
+    }
 };
 
+inline void mark(StageEvents* pe, hipEvent_t e, hipStream_t s) {
+    if (pe) (void)hipEventRecord(e, s);
+}
+
 // The frame body: every launch reads its work size from device counters.
+// The main stream runs generate, then extend(d) -> shade(d) per depth, then
+// finalize; connect(d) runs on the side stream after shade(d), overlapping
+// extend/shade of the following depths, and finalize waits for the last
+// connect.  Without overlap every stage runs on the main stream.
 int enqueue_frame(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum, StageEvents* pe = nullptr) {
     const bool nrc = pt->P.mode != 0;
-    if (pe) pe->mark(s);
+    hipStream_t sc = pt->overlap ? pt->side : s;
+    mark(pe, pe ? pe->gen[0] : nullptr, s);
     hipLaunchKernelGGL(k_frame_reset, dim3(1), dim3(128), 0, s, pt->P);
     hipLaunchKernelGGL(k_generate, dim3(pt->gen_grid), dim3(256), 0, s, pt->P);
+    mark(pe, pe ? pe->gen[1] : nullptr, s);
     for (uint32_t d = 0; d < pt->cfg.max_depth; ++d) {
         const uint32_t q = d & 1u;
-        if (pe) pe->mark(s);
+        mark(pe, pe ? pe->ext[d][0] : nullptr, s);
         if (pe) launch_extend<true>(pt, s, q, d);
         else launch_extend<false>(pt, s, q, d);
-        if (pe) pe->mark(s);
+        mark(pe, pe ? pe->ext[d][1] : nullptr, s);
+        mark(pe, pe ? pe->shd[d][0] : nullptr, s);
         if (nrc) hipLaunchKernelGGL(k_shade<true>, dim3(pt->shade_grid), dim3(256), 0, s, pt->P, q, d);
         else hipLaunchKernelGGL(k_shade<false>, dim3(pt->shade_grid), dim3(256), 0, s, pt->P, q, d);
-        if (pe) pe->mark(s);
-        if (pe) launch_connect<true>(pt, s, d);
-        else launch_connect<false>(pt, s, d);
+        mark(pe, pe ? pe->shd[d][1] : nullptr, s);
+        if (pt->overlap) {
+            TFG_HIP_CHECK(hipEventRecord(pt->ev_shaded[d], s));
+            TFG_HIP_CHECK(hipStreamWaitEvent(sc, pt->ev_shaded[d], 0));
+        }
+        mark(pe, pe ? pe->con[d][0] : nullptr, sc);
+        if (pe) launch_connect<true>(pt, sc, d);
+        else launch_connect<false>(pt, sc, d);
+        mark(pe, pe ? pe->con[d][1] : nullptr, sc);
     }
-    if (pe) pe->mark(s);
+    if (pt->overlap) {
+        TFG_HIP_CHECK(hipEventRecord(pt->ev_joined, sc));
+        TFG_HIP_CHECK(hipStreamWaitEvent(s, pt->ev_joined, 0));
+    }
+    mark(pe, pe ? pe->fin[0] : nullptr, s);
     if (nrc) {
         hipLaunchKernelGGL(k_finalize<true>, dim3(pt->gen_grid), dim3(256), 0, s, pt->P, image, accum);
         if (pt->nrc) {
@@ -143,7 +195,7 @@ int enqueue_frame(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum, St
     } else {
         hipLaunchKernelGGL(k_finalize<false>, dim3(pt->gen_grid), dim3(256), 0, s, pt->P, image, accum);
     }
-    if (pe) pe->mark(s);
+    mark(pe, pe ? pe->fin[1] : nullptr, s);
     TFG_HIP_CHECK(hipGetLastError());
     return TFG_OK;
 }
@@ -209,7 +261,9 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
     ALLOC(qpath[0], uint32_t, n)
     ALLOC(qpath[1], uint32_t, n)
     ALLOC(hits, float4, n)
-    ALLOC(shadow, float4, 4 * (size_t)n)
+    ALLOC(shadow, float4, 4 * (size_t)n * cfg->max_depth)
+    ALLOC(nee, float4, (size_t)n * cfg->max_depth)
+    ALLOC(neemask, uint32_t, n)
     ALLOC(counters, uint32_t, CNT_ALLOC)
     ALLOC(thr, float4, n)
     ALLOC(rad, float4, n)
@@ -240,8 +294,12 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
         hipError_t e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&pt->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&pt->side, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_in, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_out, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_joined, hipEventDisableTiming);
+        for (uint32_t d = 0; d < cfg->max_depth && e == hipSuccess; ++d)
+            e = hipEventCreateWithFlags(&pt->ev_shaded[d], hipEventDisableTiming);
         if (e != hipSuccess) rc = hip_fail(e, "tfg_pt_create");
         else pt->n_cu = prop.multiProcessorCount;
     }
@@ -384,32 +442,25 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
     TFG_HIP_CHECK(hipStreamSynchronize(s));
     TFG_HIP_CHECK(hipMemset(pt->P.counters + CNT_STAT, 0, 8 * sizeof(uint32_t)));
     StageEvents pe;
-    const int n_ev = 4 + 3 * (int)pt->cfg.max_depth;
-    for (int k = 0; k < n_ev; ++k) TFG_HIP_CHECK(hipEventCreate(&pe.ev[k]));
+    int rc = pe.create(pt->cfg.max_depth);
+    if (rc != TFG_OK) return rc;
     hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(1), 0, s, const_cast<int32_t*>(pt->P.frame), frame);
-    int rc = enqueue_frame(pt, s, d_image, nullptr, &pe);
+    rc = enqueue_frame(pt, s, d_image, nullptr, &pe);
     if (rc == TFG_OK) {
         TFG_HIP_CHECK(hipStreamSynchronize(s));
-        float ms = 0.f;
-        int k = 0;
-        (void)hipEventElapsedTime(&ms, pe.ev[k], pe.ev[k + 1]);
-        out->ms_generate = ms;
-        ++k;
+        auto span = [](hipEvent_t a, hipEvent_t b) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            return ms;
+        };
+        out->ms_generate = span(pe.gen[0], pe.gen[1]);
         for (uint32_t d = 0; d < pt->cfg.max_depth; ++d) {
-            (void)hipEventElapsedTime(&ms, pe.ev[k], pe.ev[k + 1]);
-            out->ms_extend[d] = ms;
-            ++k;
-            (void)hipEventElapsedTime(&ms, pe.ev[k], pe.ev[k + 1]);
-            out->ms_shade[d] = ms;
-            ++k;
-            (void)hipEventElapsedTime(&ms, pe.ev[k], pe.ev[k + 1]);
-            out->ms_connect[d] = ms;
-            ++k;
+            out->ms_extend[d] = span(pe.ext[d][0], pe.ext[d][1]);
+            out->ms_shade[d] = span(pe.shd[d][0], pe.shd[d][1]);
+            out->ms_connect[d] = span(pe.con[d][0], pe.con[d][1]);
         }
-        (void)hipEventElapsedTime(&ms, pe.ev[k], pe.ev[k + 1]);
-        out->ms_finalize = ms;
-        (void)hipEventElapsedTime(&ms, pe.ev[0], pe.ev[pe.n - 1]);
-        out->ms_total = ms;
+        out->ms_finalize = span(pe.fin[0], pe.fin[1]);
+        out->ms_total = span(pe.gen[0], pe.fin[1]);
         uint32_t c[CNT_ALLOC];
         TFG_HIP_CHECK(hipMemcpy(c, pt->P.counters, sizeof(c), hipMemcpyDeviceToHost));
         for (uint32_t d = 0; d < kMaxDepth; ++d) {
@@ -423,7 +474,6 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
         out->shadow_nodes = st[2];
         out->shadow_tris = st[3];
     }
-    for (int k = 0; k < n_ev; ++k) (void)hipEventDestroy(pe.ev[k]);
     return rc;
 }
 
@@ -440,12 +490,15 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     const int mults[3] = {1, 2, 4};
     float best = 1e30f;
     int bb = 128, bm = 1;
+    bool bo = true;
     const uint32_t saved_mode = pt->P.mode;
     pt->P.mode = 0;   // time the plain frame: the traversal stages dominate it
     int rc = TFG_OK;
+    for (int oi = 0; oi < 2 && rc == TFG_OK; ++oi)
     for (int bi = 0; bi < 3 && rc == TFG_OK; ++bi) {
         for (int mi = 0; mi < 3 && rc == TFG_OK; ++mi) {
             set_trace_shape(pt, blocks[bi], mults[mi]);
+            pt->overlap = oi == 0;
             float ms_tot = 0.f;
             for (int rep = 0; rep < 3 && rc == TFG_OK; ++rep) {
                 (void)hipEventRecord(a, pt->stream);
@@ -460,11 +513,13 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
                 best = ms_tot;
                 bb = blocks[bi];
                 bm = mults[mi];
+                bo = pt->overlap;
             }
         }
     }
     pt->P.mode = saved_mode;
     set_trace_shape(pt, bb, bm);
+    pt->overlap = bo;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     (void)hipFree(img);
@@ -477,6 +532,19 @@ int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_g
     if (trace_block) *trace_block = (uint32_t)pt->trace_block;
     if (trace_grid) *trace_grid = (uint32_t)pt->trace_grid;
     if (shade_grid) *shade_grid = (uint32_t)pt->shade_grid;
+    return TFG_OK;
+}
+
+int tfg_pt_set_overlap(tfg_pt* pt, int enable) {
+    if (!pt) { set_error("tfg_pt_set_overlap: NULL"); return TFG_E_INVALID; }
+    if (pt->overlap != (enable != 0)) drop_graph(pt);
+    pt->overlap = enable != 0;
+    return TFG_OK;
+}
+
+int tfg_pt_get_overlap(const tfg_pt* pt, int* enabled) {
+    if (!pt || !enabled) { set_error("tfg_pt_get_overlap: NULL"); return TFG_E_INVALID; }
+    *enabled = pt->overlap ? 1 : 0;
     return TFG_OK;
 }
 
@@ -531,10 +599,15 @@ int tfg_pt_read_cache_data(tfg_pt* pt, float* records, uint32_t max_records, flo
 int tfg_pt_destroy(tfg_pt* pt) {
     if (!pt) return TFG_OK;
     if (pt->stream) (void)hipStreamSynchronize(pt->stream);
+    if (pt->side) (void)hipStreamSynchronize(pt->side);
     drop_graph(pt);
     for (void* p : pt->allocs) (void)hipFree(p);
     if (pt->ev_in) (void)hipEventDestroy(pt->ev_in);
     if (pt->ev_out) (void)hipEventDestroy(pt->ev_out);
+    if (pt->ev_joined) (void)hipEventDestroy(pt->ev_joined);
+    for (hipEvent_t e : pt->ev_shaded)
+        if (e) (void)hipEventDestroy(e);
+    if (pt->side) (void)hipStreamDestroy(pt->side);
     if (pt->stream) (void)hipStreamDestroy(pt->stream);
     delete pt;
     return TFG_OK;

@@ -18,7 +18,7 @@ constexpr uint32_t kMaxAnalytic = 64;
 enum : uint32_t {
     CNT_Q0 = 0,           // ray queue 0 length
     CNT_Q1 = 1,           // ray queue 1 length
-    CNT_SHADOW = 2,       // shadow queue length
+    CNT_RESERVED = 2,
     CNT_RECORDS = 3,      // training records written this frame
     CNT_QUERIES = 4,      // cache queries written this frame
     CNT_EXT = 8,          // + depth: extension rays traced at that depth
@@ -26,7 +26,8 @@ enum : uint32_t {
     CNT_TOTAL = 8 + 2 * kMaxDepth,
     CNT_CUM = 80,         // 2 x uint64 at counters + CNT_CUM: rays of all previous frames
     CNT_STAT = 84,        // 4 x uint64: extend nodes, extend tris, shadow nodes, shadow tris
-    CNT_ALLOC = 92,
+    CNT_SHQ = 96,         // + depth: shadow queue length of that depth
+    CNT_ALLOC = CNT_SHQ + kMaxDepth,
 };
 
 // path flags
@@ -37,9 +38,7 @@ enum : uint32_t {
     PF_QUERY = 1u << 3,      // cache query captured (render.h:182-190)
     PF_TAIL = 1u << 4,       // suffix ended at a surface: tail data set
 };
-
-// shadow entry flag (top byte of the material word)
-constexpr uint32_t SF_HEAD_NEE = 1u << 24;   // also add this NEE to the head's prefix colour
+constexpr uint32_t PF_HEAD_DEPTH_SHIFT = 8;   // flags bits 8..12: depth of the head vertex
 
 // frame mode bits
 enum : uint32_t {
@@ -68,11 +67,13 @@ struct PtParams {
     float4* rays[2];        // 2 float4 per ray (tfg_ray)
     uint32_t* qpath[2];
     float4* hits;           // 1 float4 per queue entry
-    float4* shadow;         // 4 float4 per entry
+    float4* shadow;         // per depth: n_paths entries of 4 float4 (depth-major)
     uint32_t* counters;
     // path state (indexed by path id = pixel)
     float4* thr;            // throughput rgb, pdf of the last bounce
-    float4* rad;            // radiance rgb, current ior
+    float4* rad;            // emitted radiance rgb (totals after finalize), current ior
+    float4* nee;            // per depth: n_paths NEE contributions (rgb), written by connect
+    uint32_t* neemask;      // bit d: the path has a NEE contribution at depth d
     uint32_t* rng;
     uint32_t* flags;
     float4* foot;           // a0, a_current, a_suffix, a_prefix (render.h:50-54)

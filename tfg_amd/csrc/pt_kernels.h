@@ -25,7 +25,8 @@ __device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
 
 // ------------------------------------------------------------------ reset
 // One block of 128 threads: folds the previous frame's ray counts into the
-// 64-bit running totals, then clears the per-frame counters.
+// 64-bit running totals, then clears the per-frame counters and the per-depth
+// shadow queue lengths.
 __global__ void k_frame_reset(PtParams P) {
     const uint32_t i = threadIdx.x;
     if (i < 64) {
@@ -39,6 +40,7 @@ __global__ void k_frame_reset(PtParams P) {
     }
     __syncthreads();
     if (i < CNT_TOTAL) P.counters[i] = (i == CNT_Q0) ? P.n_paths : 0u;
+    if (i < kMaxDepth) P.counters[CNT_SHQ + i] = 0u;
 }
 
 __global__ void k_set_frame(int32_t* frame, int32_t value) { *frame = value; }
@@ -69,6 +71,7 @@ __global__ __launch_bounds__(256) void k_generate(PtParams P) {
         P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
         P.rng[i] = s;
         P.flags[i] = fl;
+        P.neemask[i] = 0u;
         if (P.mode) P.foot[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 }
@@ -100,7 +103,6 @@ __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint3
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         P.counters[CNT_EXT + depth] = count;
         P.counters[q ^ 1u] = 0;
-        P.counters[CNT_SHADOW] = 0;
     }
     const float4* rays = P.rays[q];
     for (uint32_t b0 = blockIdx.x * kBlock; b0 < count; b0 += gridDim.x * kBlock) {
@@ -246,8 +248,7 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
                             P.tthr[pid] = f4(thr * refl, 0.0f);
                             P.pre[pid] = f4(rad, 0.0f);
                             foot.w = foot.y;
-                            fl |= PF_HEAD;
-                            if (want_shadow) e1.w = __uint_as_float(mat_id | SF_HEAD_NEE);
+                            fl |= PF_HEAD | (depth << PF_HEAD_DEPTH_SHIFT);
                         }
                     }
                     if (!done) {   // calcularLuzIndirecta, render.h:264-335
@@ -260,7 +261,7 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
                             const float pd = mkd / sum, ps = mks / sum, pt = 1.0f - pd - ps;
                             const float r = pcg32_float(s);
                             if (r < pd) {
-                                thr = thr * m.kd * (1.0f / pd);
+                                thr = cdiv(thr * m.kd, pd);
                                 const float r1 = pcg32_float(s);
                                 const float r2 = pcg32_float(s);
                                 const V3 wl = cosine_sample(r1, r2);
@@ -294,7 +295,7 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
                                     }
                                 } else {   // render.h:337-340
                                     wi = normalize(d - n * (2.0f * dot(d, n)));
-                                    thr = thr * m.ks * (1.0f / ps);
+                                    thr = cdiv(thr * m.ks, ps);
                                     pdf = 1e6f;
                                 }
                             } else {   // render.h:342-376
@@ -316,7 +317,7 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
                                     ior = etat;
                                     wi = normalize(d * eta + N * (eta * cosi - sqrtf(k)));
                                 }
-                                thr = thr * m.kt * (1.0f / pt);
+                                thr = cdiv(thr * m.kt, pt);
                                 pdf = 1e6f;
                             }
                         }
@@ -324,7 +325,7 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
                         if (alive && depth > P.rr_depth) {
                             const float prob = fminf(vmax(thr), 0.95f);
                             if (pcg32_float(s) > prob) alive = false;
-                            else thr = thr * (1.0f / prob);
+                            else thr = cdiv(thr, prob);
                         }
                         if (alive && !last) {
                             want_ray = true;
@@ -343,13 +344,14 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
                 if (fl != fl_in) P.flags[pid] = fl;
             }
         }
-        const uint32_t ss = wave_append(want_shadow, &P.counters[CNT_SHADOW]);
+        const uint32_t ss = wave_append(want_shadow, &P.counters[CNT_SHQ + depth]);
         if (want_shadow) {
-            float4* e = P.shadow + 4 * (size_t)ss;
+            float4* e = P.shadow + 4 * ((size_t)depth * P.n_paths + ss);
             e[0] = e0;
             e[1] = e1;
             e[2] = e2;
             e[3] = e3;
+            P.neemask[pid] |= 1u << depth;
         }
         const uint32_t rs = wave_append(want_ray, &P.counters[qo]);
         if (want_ray) {
@@ -361,19 +363,25 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
 }
 
 // ---------------------------------------------------------------- connect
+// Shadow rays of the NEE requests made at one depth.  The stage reads only the
+// shadow entries of its depth and writes only nee[depth][pid], so connect of
+// depth d can run on a second stream while extend/shade of depth d+1 run;
+// finalize adds the contributions in depth order.
 template <int kBlock, bool kStats = false>
 __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) {
     extern __shared__ __align__(16) unsigned char lds[];
     float4* top = reinterpret_cast<float4*>(lds);
     uint2* stack = reinterpret_cast<uint2*>(top + 5 * P.n_top);
     stage_top_nodes(P, top);
-    const uint32_t count = P.counters[CNT_SHADOW];
+    const uint32_t count = P.counters[CNT_SHQ + depth];
+    const float4* entries = P.shadow + 4 * (size_t)depth * P.n_paths;
+    float4* nee = P.nee + (size_t)depth * P.n_paths;
     for (uint32_t b = blockIdx.x * kBlock; b < count; b += gridDim.x * kBlock) {
         const uint32_t i = b + threadIdx.x;
         uint32_t traced = 0;
         TraceStats st;
         if (i < count) {
-            const float4* e = P.shadow + 4 * (size_t)i;
+            const float4* e = entries + 4 * (size_t)i;
             const float4 e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3];
             const V3 ps = xyz(e0);
             const uint32_t pid = __float_as_uint(e0.w);
@@ -402,13 +410,7 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) 
                 }
                 if (!occ) sum = sum + contrib;
             }
-            const V3 c = xyz(e3) * sum;
-            float4 rd = P.rad[pid];
-            P.rad[pid] = make_float4(rd.x + c.x, rd.y + c.y, rd.z + c.z, rd.w);
-            if (mw & SF_HEAD_NEE) {
-                const float4 pr = P.pre[pid];
-                P.pre[pid] = make_float4(pr.x + c.x, pr.y + c.y, pr.z + c.z, pr.w);
-            }
+            nee[pid] = f4(xyz(e3) * sum, 0.0f);
         }
         const uint32_t tot = wave_sum(traced);
         if (__lane_id() == 0 && tot) atomicAdd(&P.counters[CNT_SHD + depth], tot);
@@ -417,10 +419,14 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) 
 }
 
 // --------------------------------------------------------------- finalize
-// Plain mode: pixel = path radiance.  Cache modes: training pixels keep the
-// colour before their head (render.h:251-260); every captured head/query
-// vertex becomes a cache query; every training path with a head becomes a
-// record with L_suffix = radiance - prefix colour.
+// Path radiance = emitted radiance + the NEE contributions of every depth,
+// added in depth order.  Plain mode: pixel = path radiance.  Cache modes:
+// training pixels keep the colour before their head (render.h:251-260):
+// the emission collected before the head plus the NEE of depths up to and
+// including the head's; every captured head/query vertex becomes a cache
+// query; every training path with a head becomes a record with
+// L_suffix = radiance - prefix colour.  Cache modes store the totals back
+// into rad / pre for the drop-in export.
 template <bool kNrc>
 __global__ __launch_bounds__(256) void k_finalize(PtParams P, tfg_color* image, double* accum) {
     for (uint32_t b = blockIdx.x * blockDim.x; b < P.n_paths; b += gridDim.x * blockDim.x) {
@@ -428,13 +434,30 @@ __global__ __launch_bounds__(256) void k_finalize(PtParams P, tfg_color* image, 
         bool want_rec = false, want_q = false;
         uint32_t fl = 0;
         V3 pix = v3(0.f, 0.f, 0.f);
+        V3 tot = v3(0.f, 0.f, 0.f), pre = v3(0.f, 0.f, 0.f);
         if (pid < P.n_paths) {
             const float4 r = P.rad[pid];
-            pix = xyz(r);
+            tot = xyz(r);
+            bool head = false;
+            uint32_t head_depth = 0;
             if (kNrc) {
                 fl = P.flags[pid];
-                const bool head = (fl & PF_TRAIN) && (fl & PF_HEAD);
-                if (head) pix = xyz(P.pre[pid]);
+                head = (fl & PF_TRAIN) && (fl & PF_HEAD);
+                head_depth = (fl >> PF_HEAD_DEPTH_SHIFT) & (kMaxDepth - 1);
+                if (head) pre = xyz(P.pre[pid]);
+            }
+            uint32_t mask = P.neemask[pid];
+            while (mask) {
+                const uint32_t d = (uint32_t)__ffs(mask) - 1u;
+                mask &= mask - 1u;
+                const V3 c = xyz(P.nee[(size_t)d * P.n_paths + pid]);
+                tot = tot + c;
+                if (head && d <= head_depth) pre = pre + c;
+            }
+            pix = head ? pre : tot;
+            if (kNrc) {
+                P.rad[pid] = f4(tot, r.w);
+                if (head) P.pre[pid] = f4(pre, 0.0f);
                 want_rec = head;
                 want_q = head || (fl & PF_QUERY);
             }
@@ -454,9 +477,7 @@ __global__ __launch_bounds__(256) void k_finalize(PtParams P, tfg_color* image, 
             const bool tl = fl & PF_TAIL;
             const float4* tv = P.tail + (size_t)kVertF4 * pid;
             for (int k = 0; k < kVertF4; ++k) R[kVertF4 + k] = tl ? tv[k] : make_float4(0.f, 0.f, 0.f, 0.f);
-            const float4 pr = P.pre[pid];
-            const float4 r = P.rad[pid];
-            R[8] = make_float4(r.x - pr.x, r.y - pr.y, r.z - pr.z, 1.0f);
+            R[8] = make_float4(tot.x - pre.x, tot.y - pre.y, tot.z - pre.z, 1.0f);
             float4 ts = tl ? P.tsuf[pid] : make_float4(0.f, 0.f, 0.f, 0.f);
             ts.w = __uint_as_float(pid);   // path id, for record-level checks
             R[9] = ts;

@@ -55,6 +55,11 @@ __device__ __forceinline__ V3 normalize(V3 a) {
     return l > 0.0f ? a * (1.0f / l) : a;
 }
 __device__ __forceinline__ float vmax(V3 a) { return fmaxf(fmaxf(a.x, a.y), a.z); }
+// Colour divided by a scalar with the reference's rule that a zero divisor
+// gives black (color.h:42-48): a lobe drawn with probability 0 (a uniform
+// draw that rounds to 1.0 picks transmission on a material without it)
+// zeroes the throughput instead of producing 0 * inf.
+__device__ __forceinline__ V3 cdiv(V3 a, float s) { return s == 0.0f ? v3(0.0f, 0.0f, 0.0f) : a * (1.0f / s); }
 __device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
 
 // ------------------------------------------------------------ camera

@@ -113,7 +113,12 @@ class PathTracer:
     def launch_shape(self):
         b, g, s = C.c_uint32(), C.c_uint32(), C.c_uint32()
         check(lib().tfg_pt_get_launch(self._h, C.byref(b), C.byref(g), C.byref(s)))
-        return {"trace_block": b.value, "trace_grid": g.value, "shade_grid": s.value}
+        o = C.c_int()
+        check(lib().tfg_pt_get_overlap(self._h, C.byref(o)), "tfg_pt_get_overlap")
+        return {"trace_block": b.value, "trace_grid": g.value, "shade_grid": s.value, "overlap": bool(o.value)}
+
+    def set_overlap(self, enable: bool):
+        check(lib().tfg_pt_set_overlap(self._h, 1 if enable else 0), "tfg_pt_set_overlap")
 
     def close(self):
         if self._h:

@@ -326,7 +326,7 @@ def instances(n_inst: int = 64, sphere_res=(256, 128), box_res: int = 35, seed: 
     """`n_inst` seeded instances (half displaced spheres, half boxes) on a ground quad,
     4 point lights, Lambertian albedo U[0.2,0.8] and GGX roughness U[0.1,0.8].
 
-    At the defaults: 32 x 65,024 + 32 x 14,700 + 2 = 2,551,170 triangles."""
+    At the defaults: 32 x 65,536 + 32 x 14,700 + 2 = 2,567,554 triangles."""
     rng = np.random.default_rng(seed)
     verts, norms, mids, mats = [], [], [], []
     side = int(math.ceil(math.sqrt(n_inst)))
@@ -363,6 +363,31 @@ def instances(n_inst: int = 64, sphere_res=(256, 128), box_res: int = 35, seed: 
     cam = Camera(pos=(0.0, 4.0, 8.0), front=(0.0, -0.35, -1.0), up=(0.0, 1.0, 0.0), fov_deg=60.0)
     return Scene("instances", np.concatenate(verts).astype(np.float32), np.concatenate(norms).astype(np.float32),
                  np.concatenate(mids), np.stack(mats), lights, cam, seed)
+
+
+VIEW_RESOLUTIONS = ((256, 256), (384, 216), (512, 288), (640, 360), (768, 432), (854, 480), (960, 540),
+                    (1024, 576), (1152, 648), (1280, 720), (1366, 768), (1440, 810), (1600, 900), (1680, 945),
+                    (1792, 1008), (1920, 1080))
+
+
+def views(n: int = 16, seed: int = 11):
+    """`n` seeded camera poses over the instances scene, with resolutions from
+    256x256 to 1920x1080 (VIEW_RESOLUTIONS, cycled): list of (Camera, W, H).
+
+    Poses orbit the instance grid (centre (0, 0, -13)) at radius U[9, 16] and
+    height U[1.5, 7], looking at a seeded point near the grid centre."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        phi = rng.uniform(0.0, 2.0 * math.pi)
+        rad = rng.uniform(9.0, 16.0)
+        pos = np.array([rad * math.cos(phi), rng.uniform(1.5, 7.0), -13.0 + rad * math.sin(phi)])
+        target = np.array([rng.uniform(-3, 3), rng.uniform(-1, 1), -13.0 + rng.uniform(-3, 3)])
+        front = target - pos
+        W, H = VIEW_RESOLUTIONS[i % len(VIEW_RESOLUTIONS)]
+        out.append((Camera(pos=tuple(pos), front=tuple(front / np.linalg.norm(front)), up=(0.0, 1.0, 0.0),
+                           fov_deg=float(rng.uniform(40.0, 70.0))), W, H))
+    return out
 
 
 def analytic_records(items) -> np.ndarray:
