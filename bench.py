@@ -174,6 +174,21 @@ def bench_pt(args, torch):
     _log(f"autotuned launch shape {shape}")
     img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
     accum = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
+    # the tuner times eager frames; pick the tile count on graph frames
+    sweep = {}
+    for tiles in (1, 2, 4):
+        tracer.set_tiles(tiles)
+        tracer.render(0, img)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(5):
+            tracer.render(1 + f, img)
+        torch.cuda.synchronize()
+        sweep[tiles] = (time.perf_counter() - t0) / 5 * 1e3
+    best_tiles = min(sweep, key=sweep.get)
+    tracer.set_tiles(best_tiles)
+    shape = tracer.launch_shape()
+    _log(f"graph-frame ms by tile count {sweep}: using {best_tiles}")
     for f in range(args.warmup):
         tracer.render(f, img, accum)
     torch.cuda.synchronize()
@@ -236,7 +251,11 @@ def bench_pt(args, torch):
             "rays_per_frame": rays_frame,
             "ext_rays_per_depth": prof["ext_rays"],
             "shadow_rays_per_depth": prof["shadow_rays"],
+            "ms_per_depth": {"extend": [round(x, 4) for x in prof["ms_extend"]],
+                             "shade": [round(x, 4) for x in prof["ms_shade"]],
+                             "connect": [round(x, 4) for x in prof["ms_connect"]]},
             "launch": shape,
+            "graph_ms_by_tiles": {str(k): round(v, 4) for k, v in sweep.items()},
             "seeds": {"scene": sc.seed, "frames": [1000, 1000 + args.steps - 1]},
         },
         "wall_s": wall,

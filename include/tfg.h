@@ -98,7 +98,7 @@ typedef struct tfg_pt_scene {
     const tfg_tri_shading* d_shading;   /* per primIdx */
     const tfg_material* d_materials;
     const tfg_point_light* d_lights;
-    uint32_t n_lights;
+    uint32_t n_lights;                  /* <= 64 */
     const tfg_analytic* d_analytic;     /* may be NULL */
     uint32_t n_analytic;                /* <= 64 */
 } tfg_pt_scene;
@@ -128,8 +128,9 @@ typedef struct tfg_pt_profile {
     uint64_t ext_nodes, ext_tris, shadow_nodes, shadow_tris;
 } tfg_pt_profile;
 int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_profile* out);
-/* Times candidate launch shapes of the traversal stages (and overlap on/off)
- * on the current scene and keeps the fastest (drops any captured graph). */
+/* Times candidate launch shapes of the traversal stages, then tile counts and
+ * overlap on/off, on the current scene and keeps the fastest (drops any
+ * captured graph). */
 int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream);
 int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_grid, uint32_t* shade_grid);
 /* Overlap (default on): the shadow-ray stage of depth d runs on a second
@@ -139,6 +140,14 @@ int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_g
  * traces shadow rays inline in its megakernel (escenario.h:144-184). */
 int tfg_pt_set_overlap(tfg_pt* pt, int enable);
 int tfg_pt_get_overlap(const tfg_pt* pt, int* enabled);
+/* Pixel tiles (1..4, default 1): the frame's pixels are split into that many
+ * contiguous ranges, each traced by its own chain of launches on its own
+ * streams, so that the latency-bound small launches of late depths of
+ * different tiles overlap.  Results are identical for any tile count.  No
+ * reference counterpart (the reference launches one megakernel per frame,
+ * RenderKernel.cu:185-200). */
+int tfg_pt_set_tiles(tfg_pt* pt, uint32_t tiles);
+int tfg_pt_get_tiles(const tfg_pt* pt, uint32_t* tiles);
 int tfg_pt_destroy(tfg_pt* pt);
 
 

@@ -115,22 +115,27 @@ def test_pt_graph_matches_eager_and_replays(torch_cuda):
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_pt_overlap_matches_serial(torch_cuda, use_graph):
-    """Shadow rays on the side stream (overlapping the next depth) give the
-    same image bit for bit as the serial schedule, for the plain and the
-    cache-capture frames, and the same ray counts."""
+def test_pt_schedules_match_serial(torch_cuda, use_graph):
+    """Shadow rays on side streams (overlapping the next depth) and the frame
+    split into 2 or 4 pixel tiles on their own streams give the same image
+    bit for bit as one serial chain, for plain and cache-capture frames, with
+    the same ray counts and the same captured records and queries."""
     torch = torch_cuda
     sc = scenes.instances(n_inst=8, sphere_res=(32, 16), box_res=4, seed=5)
     W, H = 96, 64
     ds = pt.DeviceScene(sc, torch)
+    schedules = [(1, False), (1, True), (2, True), (4, False), (4, True)]
     for mode in (0, 2):
-        imgs, counts = [], []
-        for overlap in (False, True):
+        imgs, counts, caps = [], [], []
+        for tiles, overlap in schedules:
+            print(f"mode={mode} tiles={tiles} overlap={overlap}", flush=True)
             tr = pt.PathTracer(W, H, 8, 3, mode=mode, use_graph=use_graph)
             tr.set_scene(ds)
             tr.set_camera(sc.camera.frame(W, H))
             tr.set_overlap(overlap)
-            assert tr.launch_shape()["overlap"] == overlap
+            tr.set_tiles(tiles)
+            shape = tr.launch_shape()
+            assert shape["overlap"] == overlap and shape["tiles"] == tiles
             img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
             out = []
             for f in (7, 8, 7):
@@ -139,11 +144,19 @@ def test_pt_overlap_matches_serial(torch_cuda, use_graph):
                 out.append(img.cpu().numpy().copy())
             imgs.append(out)
             counts.append(tr.rays_last_frame())
+            if mode:
+                rec, qry = tr.read_cache_data()
+                # append order differs between schedules: sort by pixel index
+                caps.append((rec[np.argsort(rec[:, 39].view(np.uint32))], qry[np.argsort(qry[:, 19].view(np.uint32))]))
             tr.close()
-        for a, b in zip(imgs[0], imgs[1]):
-            np.testing.assert_array_equal(a, b)
+        for k in range(1, len(schedules)):
+            for a, b in zip(imgs[0], imgs[k]):
+                np.testing.assert_array_equal(a, b)
+            assert counts[0] == counts[k]
+            if mode:
+                np.testing.assert_array_equal(caps[0][0], caps[k][0])
+                np.testing.assert_array_equal(caps[0][1], caps[k][1])
         np.testing.assert_array_equal(imgs[1][0], imgs[1][2])
-        assert counts[0] == counts[1]
 
 
 def test_pt_rejects_bad_config(torch_cuda):

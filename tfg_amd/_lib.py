@@ -111,6 +111,8 @@ _SIGS = {
     "tfg_pt_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tfg_pt_set_overlap": (C.c_int, [C.c_void_p, C.c_int]),
     "tfg_pt_get_overlap": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "tfg_pt_set_tiles": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "tfg_pt_get_tiles": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "tfg_pt_ray_totals": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tfg_pt_profile_frame": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(PtProfile)]),
     "tfg_pt_reset_totals": (C.c_int, [C.c_void_p]),

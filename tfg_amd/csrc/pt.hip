@@ -26,15 +26,25 @@ struct tfg_pt {
     int trace_grid = 0;
     int shade_grid = 0;
     int gen_grid = 0;
-    // graph
-    hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;    // shadow-ray stage (connect), overlapped with the next depth
-    hipEvent_t ev_in = nullptr, ev_out = nullptr;
-    hipEvent_t ev_shaded[tfg::kMaxDepth] = {};   // shade of depth d done: its shadow queue is complete
-    hipEvent_t ev_joined = nullptr;              // every connect of the frame done
+    // pixel tiles: each runs its own chain of stages on its own streams
+    int tiles = 1;
+    tfg::PtParams TP[tfg::kMaxTiles]{};
+    hipStream_t tstream[tfg::kMaxTiles] = {};   // generate, extend, shade, finalize of a tile
+    hipStream_t tside[tfg::kMaxTiles] = {};     // the tile's shadow-ray stage (connect)
+    hipEvent_t ev_shaded[tfg::kMaxTiles][tfg::kMaxDepth] = {};   // shade(d) of a tile done
+    hipEvent_t ev_cjoin[tfg::kMaxTiles] = {};   // every connect of a tile done
+    hipEvent_t ev_tdone[tfg::kMaxTiles] = {};   // a tile finalized
+    hipEvent_t ev_start = nullptr;              // frame counters reset
     bool overlap = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // graphs: one per tile (its stream chain and side stream) and one for the
+    // cache work after the tiles (single stream)
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraph_t graph_tile[tfg::kMaxTiles] = {};
+    hipGraphExec_t exec_tile[tfg::kMaxTiles] = {};
+    hipGraph_t graph_post = nullptr;
+    hipGraphExec_t exec_post = nullptr;
+    bool graphs_ready = false;
     const void* key_image = nullptr;
     const void* key_accum = nullptr;
     const void* key_nrc = nullptr;
@@ -60,10 +70,17 @@ int dalloc(tfg_pt* pt, T** p, size_t count) {
 }
 
 void drop_graph(tfg_pt* pt) {
-    if (pt->exec) (void)hipGraphExecDestroy(pt->exec);
-    if (pt->graph) (void)hipGraphDestroy(pt->graph);
-    pt->exec = nullptr;
-    pt->graph = nullptr;
+    for (uint32_t t = 0; t < kMaxTiles; ++t) {
+        if (pt->exec_tile[t]) (void)hipGraphExecDestroy(pt->exec_tile[t]);
+        if (pt->graph_tile[t]) (void)hipGraphDestroy(pt->graph_tile[t]);
+        pt->exec_tile[t] = nullptr;
+        pt->graph_tile[t] = nullptr;
+    }
+    if (pt->exec_post) (void)hipGraphExecDestroy(pt->exec_post);
+    if (pt->graph_post) (void)hipGraphDestroy(pt->graph_post);
+    pt->exec_post = nullptr;
+    pt->graph_post = nullptr;
+    pt->graphs_ready = false;
 }
 
 constexpr uint32_t kTopNodes = 73;   // root + 8 + 64 nodes staged in LDS
@@ -90,114 +107,246 @@ int set_trace_shape(tfg_pt* pt, int block, int mult) {
 }
 
 template <bool kStats>
-void launch_extend(tfg_pt* pt, hipStream_t s, uint32_t q, uint32_t d) {
+void launch_extend(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t q, uint32_t d) {
     const dim3 g(pt->trace_grid);
     const size_t lds = trace_lds(pt, pt->trace_block);
     switch (pt->trace_block) {
-        case 64: hipLaunchKernelGGL((k_extend<64, kStats>), g, dim3(64), lds, s, pt->P, q, d); break;
-        case 256: hipLaunchKernelGGL((k_extend<256, kStats>), g, dim3(256), lds, s, pt->P, q, d); break;
-        default: hipLaunchKernelGGL((k_extend<128, kStats>), g, dim3(128), lds, s, pt->P, q, d); break;
+        case 64: hipLaunchKernelGGL((k_extend<64, kStats>), g, dim3(64), lds, s, T, q, d); break;
+        case 256: hipLaunchKernelGGL((k_extend<256, kStats>), g, dim3(256), lds, s, T, q, d); break;
+        default: hipLaunchKernelGGL((k_extend<128, kStats>), g, dim3(128), lds, s, T, q, d); break;
     }
 }
 
 template <bool kStats>
-void launch_connect(tfg_pt* pt, hipStream_t s, uint32_t d) {
+void launch_connect(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t d) {
     const dim3 g(pt->trace_grid);
     const size_t lds = trace_lds(pt, pt->trace_block);
     switch (pt->trace_block) {
-        case 64: hipLaunchKernelGGL((k_connect<64, kStats>), g, dim3(64), lds, s, pt->P, d); break;
-        case 256: hipLaunchKernelGGL((k_connect<256, kStats>), g, dim3(256), lds, s, pt->P, d); break;
-        default: hipLaunchKernelGGL((k_connect<128, kStats>), g, dim3(128), lds, s, pt->P, d); break;
+        case 64: hipLaunchKernelGGL((k_connect<64, kStats>), g, dim3(64), lds, s, T, d); break;
+        case 256: hipLaunchKernelGGL((k_connect<256, kStats>), g, dim3(256), lds, s, T, d); break;
+        default: hipLaunchKernelGGL((k_connect<128, kStats>), g, dim3(128), lds, s, T, d); break;
+    }
+}
+
+// Split the frame's paths into pt->tiles contiguous pixel ranges (multiples
+// of 256 pixels) and derive each tile's parameters: slices of every per-path
+// and queue array, its own counter block, the shared frame counter block and
+// frame-wide cache I/O.
+void make_tiles(tfg_pt* pt) {
+    const PtParams& P = pt->P;
+    const uint32_t n = P.n_paths;
+    const uint32_t k = (uint32_t)pt->tiles;
+    for (uint32_t t = 0; t < k; ++t) {
+        const uint32_t b = t == 0 ? 0u : (uint32_t)(((uint64_t)n * t / k) & ~255ull);
+        const uint32_t e = t + 1 == k ? n : (uint32_t)(((uint64_t)n * (t + 1) / k) & ~255ull);
+        PtParams T = P;
+        T.n_paths = e - b;
+        T.pix_base = b;
+        T.depth_stride = n;
+        T.counters = P.counters + (size_t)CNT_ALLOC * (1 + t);
+        T.gcounters = P.counters;
+        T.rays[0] = P.rays[0] + 2 * (size_t)b;
+        T.rays[1] = P.rays[1] + 2 * (size_t)b;
+        T.qpath[0] = P.qpath[0] + b;
+        T.qpath[1] = P.qpath[1] + b;
+        T.hits = P.hits + b;
+        T.shadow = P.shadow + 4 * (size_t)b;
+        T.nee = P.nee + b;
+        T.neemask = P.neemask + b;
+        T.thr = P.thr + b;
+        T.rad = P.rad + b;
+        T.rng = P.rng + b;
+        T.flags = P.flags + b;
+        if (P.mode || P.foot) {
+            T.foot = P.foot ? P.foot + b : nullptr;
+            T.pre = P.pre ? P.pre + b : nullptr;
+            T.tthr = P.tthr ? P.tthr + b : nullptr;
+            T.head = P.head ? P.head + (size_t)kVertF4 * b : nullptr;
+            T.tail = P.tail ? P.tail + (size_t)kVertF4 * b : nullptr;
+            T.tsuf = P.tsuf ? P.tsuf + b : nullptr;
+        }
+        pt->TP[t] = T;
     }
 }
 
 // Per-stage events of a profiled frame (nullptr: plain frame): a start and
-// an end event around each stage, recorded on the stream the stage runs on.
+// an end event around each launch, recorded on the stream it runs on.
 struct StageEvents {
-    hipEvent_t gen[2] = {}, fin[2] = {};
-    hipEvent_t ext[kMaxDepth][2] = {}, shd[kMaxDepth][2] = {}, con[kMaxDepth][2] = {};
-    int create(uint32_t depth) {
+    hipEvent_t frame[2] = {};
+    hipEvent_t gen[kMaxTiles][2] = {}, fin[kMaxTiles][2] = {};
+    hipEvent_t ext[kMaxTiles][kMaxDepth][2] = {}, shd[kMaxTiles][kMaxDepth][2] = {},
+               con[kMaxTiles][kMaxDepth][2] = {};
+    int create(uint32_t tiles, uint32_t depth) {
         hipError_t e = hipSuccess;
         auto mk = [&](hipEvent_t* p) {
             if (e == hipSuccess) e = hipEventCreate(p);
         };
         for (int k = 0; k < 2; ++k) {
-            mk(&gen[k]);
-            mk(&fin[k]);
-            for (uint32_t d = 0; d < depth; ++d) {
-                mk(&ext[d][k]);
-                mk(&shd[d][k]);
-                mk(&con[d][k]);
+            mk(&frame[k]);
+            for (uint32_t t = 0; t < tiles; ++t) {
+                mk(&gen[t][k]);
+                mk(&fin[t][k]);
+                for (uint32_t d = 0; d < depth; ++d) {
+                    mk(&ext[t][d][k]);
+                    mk(&shd[t][d][k]);
+                    mk(&con[t][d][k]);
+                }
             }
         }
         return e == hipSuccess ? TFG_OK : hip_fail(e, "hipEventCreate(profile)");
     }
     ~StageEvents() {
+        auto rm = [](hipEvent_t e) {
+            if (e) (void)hipEventDestroy(e);
+        };
         for (int k = 0; k < 2; ++k) {
-            if (gen[k]) (void)hipEventDestroy(gen[k]);
-            if (fin[k]) (void)hipEventDestroy(fin[k]);
-            for (uint32_t d = 0; d < kMaxDepth; ++d) {
-                if (ext[d][k]) (void)hipEventDestroy(ext[d][k]);
-                if (shd[d][k]) (void)hipEventDestroy(shd[d][k]);
-                if (con[d][k]) (void)hipEventDestroy(con[d][k]);
+            rm(frame[k]);
+            for (uint32_t t = 0; t < kMaxTiles; ++t) {
+                rm(gen[t][k]);
+                rm(fin[t][k]);
+                for (uint32_t d = 0; d < kMaxDepth; ++d) {
+                    rm(ext[t][d][k]);
+                    rm(shd[t][d][k]);
+                    rm(con[t][d][k]);
+                }
             }
         }
     }
 };
 
-inline void mark(StageEvents* pe, hipEvent_t e, hipStream_t s) {
-    if (pe) (void)hipEventRecord(e, s);
+inline void mark(hipEvent_t e, hipStream_t s) {
+    if (e) (void)hipEventRecord(e, s);
 }
 
-// The frame body: every launch reads its work size from device counters.
-// The main stream runs generate, then extend(d) -> shade(d) per depth, then
-// finalize; connect(d) runs on the side stream after shade(d), overlapping
-// extend/shade of the following depths, and finalize waits for the last
-// connect.  Without overlap every stage runs on the main stream.
-int enqueue_frame(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum, StageEvents* pe = nullptr) {
+// One tile's chain; every launch reads its work size from device counters.
+// On the tile stream ts: generate, then extend(d) -> shade(d) per depth,
+// then finalize.  The tile's connect(d) runs on its side stream after
+// shade(d), overlapping extend/shade of the following depths (without
+// overlap it runs on ts); ts waits for the last connect before finalize.
+int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, tfg_color* image, double* accum, StageEvents* pe) {
     const bool nrc = pt->P.mode != 0;
-    hipStream_t sc = pt->overlap ? pt->side : s;
-    mark(pe, pe ? pe->gen[0] : nullptr, s);
-    hipLaunchKernelGGL(k_frame_reset, dim3(1), dim3(128), 0, s, pt->P);
-    hipLaunchKernelGGL(k_generate, dim3(pt->gen_grid), dim3(256), 0, s, pt->P);
-    mark(pe, pe ? pe->gen[1] : nullptr, s);
+    hipStream_t cs = pt->overlap ? pt->tside[t] : ts;
+    const PtParams& T = pt->TP[t];
+    mark(pe ? pe->gen[t][0] : nullptr, ts);
+    hipLaunchKernelGGL(k_generate, dim3(pt->gen_grid), dim3(256), 0, ts, T);
+    mark(pe ? pe->gen[t][1] : nullptr, ts);
     for (uint32_t d = 0; d < pt->cfg.max_depth; ++d) {
         const uint32_t q = d & 1u;
-        mark(pe, pe ? pe->ext[d][0] : nullptr, s);
-        if (pe) launch_extend<true>(pt, s, q, d);
-        else launch_extend<false>(pt, s, q, d);
-        mark(pe, pe ? pe->ext[d][1] : nullptr, s);
-        mark(pe, pe ? pe->shd[d][0] : nullptr, s);
-        if (nrc) hipLaunchKernelGGL(k_shade<true>, dim3(pt->shade_grid), dim3(256), 0, s, pt->P, q, d);
-        else hipLaunchKernelGGL(k_shade<false>, dim3(pt->shade_grid), dim3(256), 0, s, pt->P, q, d);
-        mark(pe, pe ? pe->shd[d][1] : nullptr, s);
+        mark(pe ? pe->ext[t][d][0] : nullptr, ts);
+        if (pe) launch_extend<true>(pt, ts, T, q, d);
+        else launch_extend<false>(pt, ts, T, q, d);
+        mark(pe ? pe->ext[t][d][1] : nullptr, ts);
+        mark(pe ? pe->shd[t][d][0] : nullptr, ts);
+        if (nrc) hipLaunchKernelGGL(k_shade<true>, dim3(pt->shade_grid), dim3(256), 0, ts, T, q, d);
+        else hipLaunchKernelGGL(k_shade<false>, dim3(pt->shade_grid), dim3(256), 0, ts, T, q, d);
+        mark(pe ? pe->shd[t][d][1] : nullptr, ts);
         if (pt->overlap) {
-            TFG_HIP_CHECK(hipEventRecord(pt->ev_shaded[d], s));
-            TFG_HIP_CHECK(hipStreamWaitEvent(sc, pt->ev_shaded[d], 0));
+            TFG_HIP_CHECK(hipEventRecord(pt->ev_shaded[t][d], ts));
+            TFG_HIP_CHECK(hipStreamWaitEvent(cs, pt->ev_shaded[t][d], 0));
         }
-        mark(pe, pe ? pe->con[d][0] : nullptr, sc);
-        if (pe) launch_connect<true>(pt, sc, d);
-        else launch_connect<false>(pt, sc, d);
-        mark(pe, pe ? pe->con[d][1] : nullptr, sc);
+        mark(pe ? pe->con[t][d][0] : nullptr, cs);
+        if (pe) launch_connect<true>(pt, cs, T, d);
+        else launch_connect<false>(pt, cs, T, d);
+        mark(pe ? pe->con[t][d][1] : nullptr, cs);
     }
     if (pt->overlap) {
-        TFG_HIP_CHECK(hipEventRecord(pt->ev_joined, sc));
-        TFG_HIP_CHECK(hipStreamWaitEvent(s, pt->ev_joined, 0));
+        TFG_HIP_CHECK(hipEventRecord(pt->ev_cjoin[t], cs));
+        TFG_HIP_CHECK(hipStreamWaitEvent(ts, pt->ev_cjoin[t], 0));
     }
-    mark(pe, pe ? pe->fin[0] : nullptr, s);
-    if (nrc) {
-        hipLaunchKernelGGL(k_finalize<true>, dim3(pt->gen_grid), dim3(256), 0, s, pt->P, image, accum);
-        if (pt->nrc) {
-            const int rc = nrc_frame_enqueue(pt->nrc, pt->nrc_frame, image, s);
-            if (rc != TFG_OK) return rc;
-        }
-        if (accum) hipLaunchKernelGGL(k_accumulate, dim3(pt->gen_grid), dim3(256), 0, s, image, accum, pt->P.n_paths);
-    } else {
-        hipLaunchKernelGGL(k_finalize<false>, dim3(pt->gen_grid), dim3(256), 0, s, pt->P, image, accum);
-    }
-    mark(pe, pe ? pe->fin[1] : nullptr, s);
+    mark(pe ? pe->fin[t][0] : nullptr, ts);
+    if (nrc) hipLaunchKernelGGL(k_finalize<true>, dim3(pt->gen_grid), dim3(256), 0, ts, T, image, accum);
+    else hipLaunchKernelGGL(k_finalize<false>, dim3(pt->gen_grid), dim3(256), 0, ts, T, image, accum);
+    mark(pe ? pe->fin[t][1] : nullptr, ts);
     TFG_HIP_CHECK(hipGetLastError());
     return TFG_OK;
+}
+
+// After every tile: cache inference (composite) and training, then the
+// accumulation of the composited image (cache modes only).
+int enqueue_post(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum) {
+    if (pt->P.mode == 0) return TFG_OK;
+    if (pt->nrc) {
+        const int rc = nrc_frame_enqueue(pt->nrc, pt->nrc_frame, image, s);
+        if (rc != TFG_OK) return rc;
+    }
+    if (accum) hipLaunchKernelGGL(k_accumulate, dim3(pt->gen_grid), dim3(256), 0, s, image, accum, pt->P.n_paths);
+    TFG_HIP_CHECK(hipGetLastError());
+    return TFG_OK;
+}
+
+// Frame prologue on s (reset of the frame counters) and the fork of the tile
+// streams: each waits for the prologue.
+int fork_tiles(tfg_pt* pt, hipStream_t s) {
+    hipLaunchKernelGGL(k_frame_reset, dim3(1), dim3(128), 0, s, pt->P.counters);
+    TFG_HIP_CHECK(hipEventRecord(pt->ev_start, s));
+    for (int t = 0; t < pt->tiles; ++t) TFG_HIP_CHECK(hipStreamWaitEvent(pt->tstream[t], pt->ev_start, 0));
+    return TFG_OK;
+}
+
+int join_tiles(tfg_pt* pt, hipStream_t s) {
+    for (int t = 0; t < pt->tiles; ++t) {
+        TFG_HIP_CHECK(hipEventRecord(pt->ev_tdone[t], pt->tstream[t]));
+        TFG_HIP_CHECK(hipStreamWaitEvent(s, pt->ev_tdone[t], 0));
+    }
+    return TFG_OK;
+}
+
+// The whole frame, issued eagerly (no graph): prologue on s, the tiles on
+// their streams, the cache work on s after every tile.  Tiles are
+// independent, so the latency tails of their small late-depth launches
+// overlap.
+int enqueue_frame(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum, StageEvents* pe = nullptr) {
+    make_tiles(pt);
+    mark(pe ? pe->frame[0] : nullptr, s);
+    int rc = fork_tiles(pt, s);
+    for (int t = 0; t < pt->tiles && rc == TFG_OK; ++t) rc = enqueue_tile(pt, t, pt->tstream[t], image, accum, pe);
+    if (rc == TFG_OK) rc = join_tiles(pt, s);
+    if (rc == TFG_OK) rc = enqueue_post(pt, s, image, accum);
+    mark(pe ? pe->frame[1] : nullptr, s);
+    return rc;
+}
+
+// Captures one graph per tile (origin: the tile stream, plus its side
+// stream) and one for the cache work (origin: s, single stream).
+int capture_graphs(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum) {
+    make_tiles(pt);
+    for (int t = 0; t < pt->tiles; ++t) {
+        hipStream_t ts = pt->tstream[t];
+        TFG_HIP_CHECK(hipStreamBeginCapture(ts, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_tile(pt, t, ts, image, accum, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ts, &g);
+        if (rc != TFG_OK || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc != TFG_OK ? rc : hip_fail(e, "hipStreamEndCapture(tile)");
+        }
+        pt->graph_tile[t] = g;
+        TFG_HIP_CHECK(hipGraphInstantiate(&pt->exec_tile[t], g, nullptr, nullptr, 0));
+    }
+    if (pt->P.mode != 0) {
+        TFG_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_post(pt, s, image, accum);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc != TFG_OK || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc != TFG_OK ? rc : hip_fail(e, "hipStreamEndCapture(cache)");
+        }
+        pt->graph_post = g;
+        TFG_HIP_CHECK(hipGraphInstantiate(&pt->exec_post, g, nullptr, nullptr, 0));
+    }
+    pt->graphs_ready = true;
+    return TFG_OK;
+}
+
+// The frame from the captured graphs: prologue on s, each tile graph on its
+// tile stream, the cache graph on s after every tile.
+int launch_graphs(tfg_pt* pt, hipStream_t s) {
+    int rc = fork_tiles(pt, s);
+    for (int t = 0; t < pt->tiles && rc == TFG_OK; ++t) TFG_HIP_CHECK(hipGraphLaunch(pt->exec_tile[t], pt->tstream[t]));
+    if (rc == TFG_OK) rc = join_tiles(pt, s);
+    if (rc == TFG_OK && pt->exec_post) TFG_HIP_CHECK(hipGraphLaunch(pt->exec_post, s));
+    return rc;
 }
 
 }  // namespace
@@ -247,6 +396,8 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
     P.width = cfg->width;
     P.height = cfg->height;
     P.n_paths = n;
+    P.pix_base = 0;
+    P.depth_stride = n;
     P.max_depth = cfg->max_depth;
     P.rr_depth = cfg->rr_depth;
     P.mode = cfg->mode;
@@ -264,7 +415,7 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
     ALLOC(shadow, float4, 4 * (size_t)n * cfg->max_depth)
     ALLOC(nee, float4, (size_t)n * cfg->max_depth)
     ALLOC(neemask, uint32_t, n)
-    ALLOC(counters, uint32_t, CNT_ALLOC)
+    ALLOC(counters, uint32_t, (size_t)CNT_ALLOC * (1 + kMaxTiles))
     ALLOC(thr, float4, n)
     ALLOC(rad, float4, n)
     ALLOC(rng, uint32_t, n)
@@ -288,23 +439,29 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
     if (rc == TFG_OK) rc = dalloc<tfg_camera>(pt, &dcam, 1);
     P.frame = dframe;
     P.cam = dcam;
+    P.gcounters = P.counters;
     if (rc == TFG_OK) {
         int dev = 0;
         hipDeviceProp_t prop;
         hipError_t e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&pt->stream, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&pt->side, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_in, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_out, hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_joined, hipEventDisableTiming);
-        for (uint32_t d = 0; d < cfg->max_depth && e == hipSuccess; ++d)
-            e = hipEventCreateWithFlags(&pt->ev_shaded[d], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_start, hipEventDisableTiming);
+        for (uint32_t t = 0; t < kMaxTiles && e == hipSuccess; ++t) {
+            e = hipStreamCreateWithFlags(&pt->tstream[t], hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipStreamCreateWithFlags(&pt->tside[t], hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_cjoin[t], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_tdone[t], hipEventDisableTiming);
+            for (uint32_t d = 0; d < cfg->max_depth && e == hipSuccess; ++d)
+                e = hipEventCreateWithFlags(&pt->ev_shaded[t][d], hipEventDisableTiming);
+        }
         if (e != hipSuccess) rc = hip_fail(e, "tfg_pt_create");
         else pt->n_cu = prop.multiProcessorCount;
     }
     if (rc == TFG_OK) {
-        const hipError_t e = hipMemset(P.counters, 0, CNT_ALLOC * sizeof(uint32_t));
+        const hipError_t e = hipMemset(P.counters, 0, (size_t)CNT_ALLOC * (1 + kMaxTiles) * sizeof(uint32_t));
         if (e != hipSuccess) rc = hip_fail(e, "hipMemset(counters)");
     }
     if (rc != TFG_OK) {
@@ -325,6 +482,10 @@ int tfg_pt_set_scene(tfg_pt* pt, const tfg_pt_scene* sc) {
         set_error("tfg_pt_set_scene: missing scene buffer");
         return TFG_E_INVALID;
     }
+    if (sc->n_lights > kMaxLights) {
+        set_error("tfg_pt_set_scene: at most %u point lights", kMaxLights);
+        return TFG_E_LIMIT;
+    }
     if (sc->n_analytic > kMaxAnalytic || (sc->n_analytic && !sc->d_analytic)) {
         set_error("tfg_pt_set_scene: at most %u analytic primitives", kMaxAnalytic);
         return TFG_E_LIMIT;
@@ -343,6 +504,8 @@ int tfg_pt_set_scene(tfg_pt* pt, const tfg_pt_scene* sc) {
     P.mats = sc->d_materials;
     P.lights = sc->d_lights;
     P.n_lights = sc->n_lights;
+    P.light_shift = 0;
+    while ((1u << P.light_shift) < sc->n_lights) ++P.light_shift;
     P.analytic = sc->d_analytic;
     P.n_analytic = sc->n_analytic;
     pt->scene_set = true;
@@ -372,28 +535,23 @@ int tfg_pt_render(tfg_pt* pt, int32_t frame, tfg_color* d_image, double* d_accum
     hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(1), 0, s, const_cast<int32_t*>(pt->P.frame), frame);
     int rc = TFG_OK;
     if (pt->cfg.use_graph) {
-        if (pt->exec && (pt->key_image != d_image || pt->key_accum != d_accum || pt->key_nrc != pt->nrc ||
-                         pt->key_mode != pt->P.mode)) {
+        if (pt->graphs_ready && (pt->key_image != d_image || pt->key_accum != d_accum || pt->key_nrc != pt->nrc ||
+                                 pt->key_mode != pt->P.mode)) {
             drop_graph(pt);
         }
-        if (!pt->exec) {
-            TFG_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            rc = enqueue_frame(pt, s, d_image, d_accum);
-            hipGraph_t g = nullptr;
-            const hipError_t e = hipStreamEndCapture(s, &g);
+        if (!pt->graphs_ready) {
+            rc = capture_graphs(pt, s, d_image, d_accum);
             if (rc != TFG_OK) {
-                if (g) (void)hipGraphDestroy(g);
+                drop_graph(pt);
                 return rc;
             }
-            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-            pt->graph = g;
-            TFG_HIP_CHECK(hipGraphInstantiate(&pt->exec, g, nullptr, nullptr, 0));
             pt->key_image = d_image;
             pt->key_accum = d_accum;
             pt->key_nrc = pt->nrc;
             pt->key_mode = pt->P.mode;
         }
-        TFG_HIP_CHECK(hipGraphLaunch(pt->exec, s));
+        rc = launch_graphs(pt, s);
+        if (rc != TFG_OK) return rc;
     } else {
         rc = enqueue_frame(pt, s, d_image, d_accum);
         if (rc != TFG_OK) return rc;
@@ -430,7 +588,7 @@ int tfg_pt_ray_totals(tfg_pt* pt, uint64_t* out) {
 int tfg_pt_reset_totals(tfg_pt* pt) {
     if (!pt) { set_error("tfg_pt_reset_totals: NULL"); return TFG_E_INVALID; }
     TFG_HIP_CHECK(hipStreamSynchronize(pt->stream));
-    TFG_HIP_CHECK(hipMemset(pt->P.counters, 0, CNT_ALLOC * sizeof(uint32_t)));
+    TFG_HIP_CHECK(hipMemset(pt->P.counters, 0, (size_t)CNT_ALLOC * (1 + kMaxTiles) * sizeof(uint32_t)));
     return TFG_OK;
 }
 
@@ -442,7 +600,7 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
     TFG_HIP_CHECK(hipStreamSynchronize(s));
     TFG_HIP_CHECK(hipMemset(pt->P.counters + CNT_STAT, 0, 8 * sizeof(uint32_t)));
     StageEvents pe;
-    int rc = pe.create(pt->cfg.max_depth);
+    int rc = pe.create((uint32_t)pt->tiles, pt->cfg.max_depth);
     if (rc != TFG_OK) return rc;
     hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(1), 0, s, const_cast<int32_t*>(pt->P.frame), frame);
     rc = enqueue_frame(pt, s, d_image, nullptr, &pe);
@@ -453,14 +611,17 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
             (void)hipEventElapsedTime(&ms, a, b);
             return ms;
         };
-        out->ms_generate = span(pe.gen[0], pe.gen[1]);
-        for (uint32_t d = 0; d < pt->cfg.max_depth; ++d) {
-            out->ms_extend[d] = span(pe.ext[d][0], pe.ext[d][1]);
-            out->ms_shade[d] = span(pe.shd[d][0], pe.shd[d][1]);
-            out->ms_connect[d] = span(pe.con[d][0], pe.con[d][1]);
+        // per stage: the sum of its launch durations over the tiles
+        for (int t = 0; t < pt->tiles; ++t) {
+            out->ms_generate += span(pe.gen[t][0], pe.gen[t][1]);
+            for (uint32_t d = 0; d < pt->cfg.max_depth; ++d) {
+                out->ms_extend[d] += span(pe.ext[t][d][0], pe.ext[t][d][1]);
+                out->ms_shade[d] += span(pe.shd[t][d][0], pe.shd[t][d][1]);
+                out->ms_connect[d] += span(pe.con[t][d][0], pe.con[t][d][1]);
+            }
+            out->ms_finalize += span(pe.fin[t][0], pe.fin[t][1]);
         }
-        out->ms_finalize = span(pe.fin[0], pe.fin[1]);
-        out->ms_total = span(pe.gen[0], pe.fin[1]);
+        out->ms_total = span(pe.frame[0], pe.frame[1]);
         uint32_t c[CNT_ALLOC];
         TFG_HIP_CHECK(hipMemcpy(c, pt->P.counters, sizeof(c), hipMemcpyDeviceToHost));
         for (uint32_t d = 0; d < kMaxDepth; ++d) {
@@ -486,39 +647,60 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     hipEvent_t a, b;
     TFG_HIP_CHECK(hipEventCreate(&a));
     TFG_HIP_CHECK(hipEventCreate(&b));
+    const uint32_t saved_mode = pt->P.mode;
+    pt->P.mode = 0;   // time the plain frame: the traversal stages dominate it
+    int rc = TFG_OK;
+    auto time_frames = [&]() {
+        float ms_tot = 0.f;
+        for (int rep = 0; rep < 3 && rc == TFG_OK; ++rep) {
+            (void)hipEventRecord(a, pt->stream);
+            rc = enqueue_frame(pt, pt->stream, img, nullptr);
+            (void)hipEventRecord(b, pt->stream);
+            (void)hipEventSynchronize(b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            if (rep > 0) ms_tot += ms;
+        }
+        return ms_tot;
+    };
+    // launch shape of the traversal stages, with one tile and overlap on
     const int blocks[3] = {64, 128, 256};
     const int mults[3] = {1, 2, 4};
     float best = 1e30f;
     int bb = 128, bm = 1;
-    bool bo = true;
-    const uint32_t saved_mode = pt->P.mode;
-    pt->P.mode = 0;   // time the plain frame: the traversal stages dominate it
-    int rc = TFG_OK;
-    for (int oi = 0; oi < 2 && rc == TFG_OK; ++oi)
+    pt->tiles = 1;
+    pt->overlap = true;
     for (int bi = 0; bi < 3 && rc == TFG_OK; ++bi) {
         for (int mi = 0; mi < 3 && rc == TFG_OK; ++mi) {
             set_trace_shape(pt, blocks[bi], mults[mi]);
-            pt->overlap = oi == 0;
-            float ms_tot = 0.f;
-            for (int rep = 0; rep < 3 && rc == TFG_OK; ++rep) {
-                (void)hipEventRecord(a, pt->stream);
-                rc = enqueue_frame(pt, pt->stream, img, nullptr);
-                (void)hipEventRecord(b, pt->stream);
-                (void)hipEventSynchronize(b);
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, a, b);
-                if (rep > 0) ms_tot += ms;
-            }
-            if (ms_tot < best) {
-                best = ms_tot;
+            const float ms = time_frames();
+            if (ms < best) {
+                best = ms;
                 bb = blocks[bi];
                 bm = mults[mi];
+            }
+        }
+    }
+    set_trace_shape(pt, bb, bm);
+    // then the tile count and the overlap
+    const int tile_opts[3] = {1, 2, 4};
+    best = 1e30f;
+    int bt = 1;
+    bool bo = true;
+    for (int ti = 0; ti < 3 && rc == TFG_OK; ++ti) {
+        for (int oi = 0; oi < 2 && rc == TFG_OK; ++oi) {
+            pt->tiles = tile_opts[ti];
+            pt->overlap = oi == 0;
+            const float ms = time_frames();
+            if (ms < best) {
+                best = ms;
+                bt = pt->tiles;
                 bo = pt->overlap;
             }
         }
     }
     pt->P.mode = saved_mode;
-    set_trace_shape(pt, bb, bm);
+    pt->tiles = bt;
     pt->overlap = bo;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
@@ -532,6 +714,22 @@ int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_g
     if (trace_block) *trace_block = (uint32_t)pt->trace_block;
     if (trace_grid) *trace_grid = (uint32_t)pt->trace_grid;
     if (shade_grid) *shade_grid = (uint32_t)pt->shade_grid;
+    return TFG_OK;
+}
+
+int tfg_pt_set_tiles(tfg_pt* pt, uint32_t tiles) {
+    if (!pt || tiles == 0 || tiles > kMaxTiles) {
+        set_error("tfg_pt_set_tiles: tiles must be 1..%u", kMaxTiles);
+        return TFG_E_INVALID;
+    }
+    if ((uint32_t)pt->tiles != tiles) drop_graph(pt);
+    pt->tiles = (int)tiles;
+    return TFG_OK;
+}
+
+int tfg_pt_get_tiles(const tfg_pt* pt, uint32_t* tiles) {
+    if (!pt || !tiles) { set_error("tfg_pt_get_tiles: NULL"); return TFG_E_INVALID; }
+    *tiles = (uint32_t)pt->tiles;
     return TFG_OK;
 }
 
@@ -599,15 +797,23 @@ int tfg_pt_read_cache_data(tfg_pt* pt, float* records, uint32_t max_records, flo
 int tfg_pt_destroy(tfg_pt* pt) {
     if (!pt) return TFG_OK;
     if (pt->stream) (void)hipStreamSynchronize(pt->stream);
-    if (pt->side) (void)hipStreamSynchronize(pt->side);
+    for (uint32_t t = 0; t < kMaxTiles; ++t) {
+        if (pt->tstream[t]) (void)hipStreamSynchronize(pt->tstream[t]);
+        if (pt->tside[t]) (void)hipStreamSynchronize(pt->tside[t]);
+    }
     drop_graph(pt);
     for (void* p : pt->allocs) (void)hipFree(p);
     if (pt->ev_in) (void)hipEventDestroy(pt->ev_in);
     if (pt->ev_out) (void)hipEventDestroy(pt->ev_out);
-    if (pt->ev_joined) (void)hipEventDestroy(pt->ev_joined);
-    for (hipEvent_t e : pt->ev_shaded)
-        if (e) (void)hipEventDestroy(e);
-    if (pt->side) (void)hipStreamDestroy(pt->side);
+    if (pt->ev_start) (void)hipEventDestroy(pt->ev_start);
+    for (uint32_t t = 0; t < kMaxTiles; ++t) {
+        if (pt->ev_cjoin[t]) (void)hipEventDestroy(pt->ev_cjoin[t]);
+        if (pt->ev_tdone[t]) (void)hipEventDestroy(pt->ev_tdone[t]);
+        for (hipEvent_t e : pt->ev_shaded[t])
+            if (e) (void)hipEventDestroy(e);
+        if (pt->tside[t]) (void)hipStreamDestroy(pt->tside[t]);
+        if (pt->tstream[t]) (void)hipStreamDestroy(pt->tstream[t]);
+    }
     if (pt->stream) (void)hipStreamDestroy(pt->stream);
     delete pt;
     return TFG_OK;

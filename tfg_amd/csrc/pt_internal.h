@@ -13,22 +13,26 @@ constexpr uint32_t kMaxDepth = 32;
 constexpr uint32_t kMiss = 0xFFFFFFFFu;
 constexpr uint32_t kAnalyticBit = 0x80000000u;
 constexpr uint32_t kMaxAnalytic = 64;
+constexpr uint32_t kMaxLights = 64;   // one wave64 lane per light of a shadow entry
 
-// counters[] layout (uint32, device)
+// Counter blocks (uint32, device).  The frame block holds the frame-wide
+// counts; each pixel tile has its own block for its queue lengths.  Both use
+// this layout; a block uses only its own entries.
 enum : uint32_t {
-    CNT_Q0 = 0,           // ray queue 0 length
-    CNT_Q1 = 1,           // ray queue 1 length
+    CNT_Q0 = 0,           // tile: ray queue 0 length
+    CNT_Q1 = 1,           // tile: ray queue 1 length
     CNT_RESERVED = 2,
-    CNT_RECORDS = 3,      // training records written this frame
-    CNT_QUERIES = 4,      // cache queries written this frame
-    CNT_EXT = 8,          // + depth: extension rays traced at that depth
-    CNT_SHD = 8 + kMaxDepth,   // + depth: shadow rays traced at that depth
+    CNT_RECORDS = 3,      // frame: training records written this frame
+    CNT_QUERIES = 4,      // frame: cache queries written this frame
+    CNT_EXT = 8,          // frame, + depth: extension rays traced at that depth
+    CNT_SHD = 8 + kMaxDepth,   // frame, + depth: shadow rays traced at that depth
     CNT_TOTAL = 8 + 2 * kMaxDepth,
-    CNT_CUM = 80,         // 2 x uint64 at counters + CNT_CUM: rays of all previous frames
-    CNT_STAT = 84,        // 4 x uint64: extend nodes, extend tris, shadow nodes, shadow tris
-    CNT_SHQ = 96,         // + depth: shadow queue length of that depth
+    CNT_CUM = 80,         // frame: 2 x uint64, rays of all previous frames
+    CNT_STAT = 84,        // frame: 4 x uint64, extend nodes, extend tris, shadow nodes, shadow tris
+    CNT_SHQ = 96,         // tile, + depth: shadow queue length of that depth
     CNT_ALLOC = CNT_SHQ + kMaxDepth,
 };
+constexpr uint32_t kMaxTiles = 4;   // pixel tiles traced as independent stream chains
 
 // path flags
 enum : uint32_t {
@@ -56,23 +60,28 @@ struct PtParams {
     const tfg_material* mats;
     const tfg_point_light* lights;
     uint32_t n_lights;
+    uint32_t light_shift;    // lanes per shadow entry = 2^light_shift >= n_lights (<= 64)
     const tfg_analytic* analytic;
     uint32_t n_analytic;
     const tfg_camera* cam;
     const int32_t* frame;   // device frame index (seed), read by generate
-    uint32_t width, height, n_paths;
+    uint32_t width, height;
+    uint32_t n_paths;        // paths of this tile (path ids 0..n_paths-1)
+    uint32_t pix_base;       // pixel index of the tile's path 0
+    uint32_t depth_stride;   // pixels of the frame: stride of the per-depth shadow / nee arrays
     uint32_t max_depth, rr_depth, mode;
     float train_frac;
     // queues
     float4* rays[2];        // 2 float4 per ray (tfg_ray)
     uint32_t* qpath[2];
     float4* hits;           // 1 float4 per queue entry
-    float4* shadow;         // per depth: n_paths entries of 4 float4 (depth-major)
-    uint32_t* counters;
+    float4* shadow;         // per depth: depth_stride entries of 4 float4 (depth-major)
+    uint32_t* counters;     // this tile's counter block
+    uint32_t* gcounters;    // the frame counter block
     // path state (indexed by path id = pixel)
     float4* thr;            // throughput rgb, pdf of the last bounce
     float4* rad;            // emitted radiance rgb (totals after finalize), current ior
-    float4* nee;            // per depth: n_paths NEE contributions (rgb), written by connect
+    float4* nee;            // per depth: depth_stride NEE contributions (rgb), written by connect
     uint32_t* neemask;      // bit d: the path has a NEE contribution at depth d
     uint32_t* rng;
     uint32_t* flags;
@@ -82,7 +91,8 @@ struct PtParams {
     float4* head;           // 4 float4: head (training) or query vertex data
     float4* tail;           // 4 float4: tail vertex data
     float4* tsuf;           // suffix throughput x tail reflectance
-    // cache I/O
+    // cache I/O (frame-wide, appended through the frame counter block;
+    // the path ids they carry are pixel indices)
     float4* records;        // kRecF4 float4 per record
     float4* queries;        // kQueryF4 float4 per query
 };

@@ -24,33 +24,42 @@ __device__ __forceinline__ float4 f4(V3 a, float w) { return make_float4(a.x, a.
 __device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
 
 // ------------------------------------------------------------------ reset
-// One block of 128 threads: folds the previous frame's ray counts into the
-// 64-bit running totals, then clears the per-frame counters and the per-depth
-// shadow queue lengths.
-__global__ void k_frame_reset(PtParams P) {
+// One block of 128 threads on the frame counter block: folds the previous
+// frame's ray counts into the 64-bit running totals, then clears the
+// per-frame counts.  (Each tile's queue counters are set by its generate.)
+__global__ void k_frame_reset(uint32_t* gc) {
     const uint32_t i = threadIdx.x;
     if (i < 64) {
-        const uint32_t e = wave_sum(i < kMaxDepth ? P.counters[CNT_EXT + i] : 0u);
-        const uint32_t s = wave_sum(i < kMaxDepth ? P.counters[CNT_SHD + i] : 0u);
+        const uint32_t e = wave_sum(i < kMaxDepth ? gc[CNT_EXT + i] : 0u);
+        const uint32_t s = wave_sum(i < kMaxDepth ? gc[CNT_SHD + i] : 0u);
         if (i == 0) {
-            unsigned long long* cum = reinterpret_cast<unsigned long long*>(P.counters + CNT_CUM);
+            unsigned long long* cum = reinterpret_cast<unsigned long long*>(gc + CNT_CUM);
             cum[0] += e;
             cum[1] += s;
         }
     }
     __syncthreads();
-    if (i < CNT_TOTAL) P.counters[i] = (i == CNT_Q0) ? P.n_paths : 0u;
-    if (i < kMaxDepth) P.counters[CNT_SHQ + i] = 0u;
+    if (i < CNT_TOTAL) gc[i] = 0u;
 }
 
 __global__ void k_set_frame(int32_t* frame, int32_t value) { *frame = value; }
 
 // --------------------------------------------------------------- generate
+// Primary rays of the tile's pixels; block 0 also sets the tile's queue
+// counters (nothing else reads them while generate runs).
 __global__ __launch_bounds__(256) void k_generate(PtParams P) {
+    if (blockIdx.x == 0) {
+        for (uint32_t k = threadIdx.x; k < kMaxDepth; k += blockDim.x) P.counters[CNT_SHQ + k] = 0u;
+        if (threadIdx.x == 0) {
+            P.counters[CNT_Q0] = P.n_paths;
+            P.counters[CNT_Q1] = 0u;
+        }
+    }
     const tfg_camera cam = *P.cam;
     const int32_t frame = *P.frame;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n_paths; i += gridDim.x * blockDim.x) {
-        const int x = (int)(i % P.width), y = (int)(i / P.width);
+        const uint32_t px = P.pix_base + i;
+        const int x = (int)(px % P.width), y = (int)(px / P.width);
         uint32_t s = pixel_seed(x, y, frame);
         const float ox = pcg32_float(s);
         const float oy = pcg32_float(s);
@@ -80,7 +89,7 @@ __global__ __launch_bounds__(256) void k_generate(PtParams P) {
 __device__ __forceinline__ void add_stats(PtParams& P, int slot, const TraceStats& st) {
     const uint32_t n = wave_sum(st.nodes), t = wave_sum(st.tris);
     if (__lane_id() == 0 && (n | t)) {
-        unsigned long long* c = reinterpret_cast<unsigned long long*>(P.counters + CNT_STAT);
+        unsigned long long* c = reinterpret_cast<unsigned long long*>(P.gcounters + CNT_STAT);
         atomicAdd(&c[slot], (unsigned long long)n);
         atomicAdd(&c[slot + 1], (unsigned long long)t);
     }
@@ -101,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint3
     stage_top_nodes(P, top);
     const uint32_t count = P.counters[q];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        P.counters[CNT_EXT + depth] = count;
+        if (count) atomicAdd(&P.gcounters[CNT_EXT + depth], count);
         P.counters[q ^ 1u] = 0;
     }
     const float4* rays = P.rays[q];
@@ -346,7 +355,7 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
         }
         const uint32_t ss = wave_append(want_shadow, &P.counters[CNT_SHQ + depth]);
         if (want_shadow) {
-            float4* e = P.shadow + 4 * ((size_t)depth * P.n_paths + ss);
+            float4* e = P.shadow + 4 * ((size_t)depth * P.depth_stride + ss);
             e[0] = e0;
             e[1] = e1;
             e[2] = e2;
@@ -363,7 +372,11 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
 }
 
 // ---------------------------------------------------------------- connect
-// Shadow rays of the NEE requests made at one depth.  The stage reads only the
+// Shadow rays of the NEE requests made at one depth (escenario.h:144-184).
+// Each entry gets a group of 2^light_shift adjacent lanes, one per light, so
+// the rays of one shading point are traced side by side rather than one after
+// another; the group's first lane then sums the unoccluded contributions in
+// light order (the order of the reference's loop).  The stage reads only the
 // shadow entries of its depth and writes only nee[depth][pid], so connect of
 // depth d can run on a second stream while extend/shade of depth d+1 run;
 // finalize adds the contributions in depth order.
@@ -374,46 +387,60 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) 
     uint2* stack = reinterpret_cast<uint2*>(top + 5 * P.n_top);
     stage_top_nodes(P, top);
     const uint32_t count = P.counters[CNT_SHQ + depth];
-    const float4* entries = P.shadow + 4 * (size_t)depth * P.n_paths;
-    float4* nee = P.nee + (size_t)depth * P.n_paths;
-    for (uint32_t b = blockIdx.x * kBlock; b < count; b += gridDim.x * kBlock) {
-        const uint32_t i = b + threadIdx.x;
+    const float4* entries = P.shadow + 4 * (size_t)depth * P.depth_stride;
+    float4* nee = P.nee + (size_t)depth * P.depth_stride;
+    const uint32_t shift = P.light_shift;
+    const uint32_t group = 1u << shift;
+    const uint64_t work = (uint64_t)count << shift;
+    const uint32_t lane = __lane_id();
+    const int lead = (int)(lane & ~(group - 1u));
+    for (uint64_t b = (uint64_t)blockIdx.x * kBlock; b < work; b += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t w = b + threadIdx.x;
+        const uint32_t i = (uint32_t)(w >> shift);
+        const uint32_t l = (uint32_t)w & (group - 1u);
         uint32_t traced = 0;
         TraceStats st;
-        if (i < count) {
+        V3 c = v3(0.f, 0.f, 0.f);
+        float4 e0 = make_float4(0.f, 0.f, 0.f, 0.f), e3 = e0;
+        if (w < work) {
             const float4* e = entries + 4 * (size_t)i;
-            const float4 e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3];
-            const V3 ps = xyz(e0);
-            const uint32_t pid = __float_as_uint(e0.w);
-            const V3 n = xyz(e1);
-            const uint32_t mw = __float_as_uint(e1.w);
-            const Mat m = load_mat(&P.mats[mw & 0x00FFFFFFu]);
-            const V3 wo = xyz(e2);
-            V3 sum = v3(0.f, 0.f, 0.f);
-            for (uint32_t l = 0; l < P.n_lights; ++l) {
+            e0 = e[0];
+            e3 = e[3];
+            if (l < P.n_lights) {
+                const float4 e1 = e[1], e2 = e[2];
+                const V3 ps = xyz(e0);
+                const V3 n = xyz(e1);
+                const Mat m = load_mat(&P.mats[__float_as_uint(e1.w) & 0x00FFFFFFu]);
                 V3 contrib, dir;
                 float dist;
-                if (!light_term(P.lights[l], ps, n, wo, m, contrib, dir, dist)) continue;
-                const V3 o = ps + dir * kEps;
-                ++traced;
-                bool occ = false;
-                for (uint32_t k = 0; k < P.n_analytic && !occ; ++k) {   // escenario.h:255-266
-                    const float ta = analytic_t(P.analytic[k], o, dir);
-                    occ = ta > 0.0f && ta < dist;
+                if (light_term(P.lights[l], ps, n, xyz(e2), m, contrib, dir, dist)) {
+                    const V3 o = ps + dir * kEps;
+                    traced = 1;
+                    bool occ = false;
+                    for (uint32_t k = 0; k < P.n_analytic && !occ; ++k) {   // escenario.h:255-266
+                        const float ta = analytic_t(P.analytic[k], o, dir);
+                        occ = ta > 0.0f && ta < dist;
+                    }
+                    if (!occ) {
+                        const RayPre r = make_ray(o.x, o.y, o.z, dir.x, dir.y, dir.z, 0.0f, dist);
+                        float t, u, v;
+                        uint32_t prim;
+                        occ = traverse<0, true>(P.nodes, P.tris, top, P.n_top, r, stack + threadIdx.x, kBlock,
+                                                P.stack_depth, t, u, v, prim, kStats ? &st : nullptr);
+                    }
+                    if (!occ) c = contrib;
                 }
-                if (!occ) {
-                    const RayPre r = make_ray(o.x, o.y, o.z, dir.x, dir.y, dir.z, 0.0f, dist);
-                    float t, u, v;
-                    uint32_t prim;
-                    occ = traverse<0, true>(P.nodes, P.tris, top, P.n_top, r, stack + threadIdx.x, kBlock,
-                                            P.stack_depth, t, u, v, prim, kStats ? &st : nullptr);
-                }
-                if (!occ) sum = sum + contrib;
             }
-            nee[pid] = f4(xyz(e3) * sum, 0.0f);
         }
+        // ordered sum over the group's lights (wave-uniform trip count)
+        V3 sum = v3(0.f, 0.f, 0.f);
+        for (uint32_t k = 0; k < group; ++k) {
+            const int src = lead + (int)k;
+            sum = sum + v3(__shfl(c.x, src), __shfl(c.y, src), __shfl(c.z, src));
+        }
+        if (w < work && l == 0) nee[__float_as_uint(e0.w)] = f4(xyz(e3) * sum, 0.0f);
         const uint32_t tot = wave_sum(traced);
-        if (__lane_id() == 0 && tot) atomicAdd(&P.counters[CNT_SHD + depth], tot);
+        if (lane == 0 && tot) atomicAdd(&P.gcounters[CNT_SHD + depth], tot);
         if (kStats) add_stats(P, 2, st);
     }
 }
@@ -450,7 +477,7 @@ __global__ __launch_bounds__(256) void k_finalize(PtParams P, tfg_color* image, 
             while (mask) {
                 const uint32_t d = (uint32_t)__ffs(mask) - 1u;
                 mask &= mask - 1u;
-                const V3 c = xyz(P.nee[(size_t)d * P.n_paths + pid]);
+                const V3 c = xyz(P.nee[(size_t)d * P.depth_stride + pid]);
                 tot = tot + c;
                 if (head && d <= head_depth) pre = pre + c;
             }
@@ -461,15 +488,16 @@ __global__ __launch_bounds__(256) void k_finalize(PtParams P, tfg_color* image, 
                 want_rec = head;
                 want_q = head || (fl & PF_QUERY);
             }
-            image[pid] = tfg_color{pix.x, pix.y, pix.z};
+            const size_t px = (size_t)P.pix_base + pid;
+            image[px] = tfg_color{pix.x, pix.y, pix.z};
             if (!kNrc && accum) {
-                accum[3 * (size_t)pid] += (double)pix.x;
-                accum[3 * (size_t)pid + 1] += (double)pix.y;
-                accum[3 * (size_t)pid + 2] += (double)pix.z;
+                accum[3 * px] += (double)pix.x;
+                accum[3 * px + 1] += (double)pix.y;
+                accum[3 * px + 2] += (double)pix.z;
             }
         }
         if (!kNrc) continue;
-        const uint32_t rs = wave_append(want_rec, &P.counters[CNT_RECORDS]);
+        const uint32_t rs = wave_append(want_rec, &P.gcounters[CNT_RECORDS]);
         if (want_rec) {
             float4* R = P.records + (size_t)kRecF4 * rs;
             const float4* hd = P.head + (size_t)kVertF4 * pid;
@@ -479,17 +507,17 @@ __global__ __launch_bounds__(256) void k_finalize(PtParams P, tfg_color* image, 
             for (int k = 0; k < kVertF4; ++k) R[kVertF4 + k] = tl ? tv[k] : make_float4(0.f, 0.f, 0.f, 0.f);
             R[8] = make_float4(tot.x - pre.x, tot.y - pre.y, tot.z - pre.z, 1.0f);
             float4 ts = tl ? P.tsuf[pid] : make_float4(0.f, 0.f, 0.f, 0.f);
-            ts.w = __uint_as_float(pid);   // path id, for record-level checks
+            ts.w = __uint_as_float(P.pix_base + pid);   // pixel index, for record-level checks
             R[9] = ts;
             R[10] = P.tthr[pid];
         }
-        const uint32_t qs = wave_append(want_q, &P.counters[CNT_QUERIES]);
+        const uint32_t qs = wave_append(want_q, &P.gcounters[CNT_QUERIES]);
         if (want_q) {
             float4* Q = P.queries + (size_t)kQueryF4 * qs;
             const float4* hd = P.head + (size_t)kVertF4 * pid;
             for (int k = 0; k < kVertF4; ++k) Q[k] = hd[k];
             const float4 tt = P.tthr[pid];
-            Q[4] = make_float4(tt.x, tt.y, tt.z, __uint_as_float(pid));
+            Q[4] = make_float4(tt.x, tt.y, tt.z, __uint_as_float(P.pix_base + pid));
         }
     }
 }

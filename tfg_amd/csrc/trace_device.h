@@ -178,25 +178,111 @@ struct TraceStats {
     uint32_t nodes = 0, tris = 0, overflow = 0;
 };
 
+// Triangles of one node's leaf children (tg: base, 24-bit hit mask) against
+// the ray.  Watertight records: the vertices of two triangles are fetched
+// together, so a ray that tests many triangles pays one memory latency per
+// pair.  The accept rule (nearer t, then smaller primIdx) does not depend on
+// the test order.  Returns true for an any-hit ray that found an occluder.
+template <int kFormat, bool kAnyHit>
+__device__ __forceinline__ bool test_triangles(const float4* __restrict__ tris, uint32_t tg_x, uint32_t tg_y,
+                                               const RayPre& r, float& tcur, float& bu, float& bv,
+                                               uint32_t& best_prim, bool& found, TraceStats* stats) {
+    if (kFormat == 0) {
+        while (tg_y != 0) {
+            const int b0 = highest_bit(tg_y);
+            tg_y &= ~(1u << b0);
+            const bool two = tg_y != 0;
+            int b1 = b0;
+            if (two) {
+                b1 = highest_bit(tg_y);
+                tg_y &= ~(1u << b1);
+            }
+            const float4* T0 = tris + (size_t)(tg_x + 4u * (uint32_t)b0);
+            const float4* T1 = tris + (size_t)(tg_x + 4u * (uint32_t)b1);
+            const float4 a0 = T0[0], a1 = T0[1], a2 = T0[2];
+            const float4 c0 = T1[0], c1 = T1[1], c2 = T1[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                if (k == 1 && !two) break;
+                if (stats) stats->tris++;
+                float t, u, v;
+                const bool ok = (k == 0 ? watertight(a0, a1, a2, r, t, u, v) : watertight(c0, c1, c2, r, t, u, v)) &&
+                                t > r.tmin && t <= tcur;
+                if (!ok) continue;
+                const uint32_t prim = __float_as_uint((k == 0 ? T0 : T1)[3].w);
+                if (kAnyHit) {
+                    if (t < r.tmax) return true;
+                    continue;
+                }
+                if (t < tcur || prim < best_prim) {
+                    tcur = t;
+                    bu = u;
+                    bv = v;
+                    best_prim = prim;
+                    found = true;
+                }
+            }
+        }
+    } else {
+        while (tg_y != 0) {
+            const int b = highest_bit(tg_y);
+            tg_y &= ~(1u << b);
+            const float4* T = tris + (size_t)(tg_x + 4u * (uint32_t)b);
+            if (stats) stats->tris++;
+            float t, u, v;
+            if (!affine(T, r, tcur, t, u, v)) continue;
+            const uint32_t prim = __float_as_uint(T[3].w);
+            if (kAnyHit) {
+                if (t < r.tmax) return true;
+                continue;
+            }
+            if (t < tcur || prim < best_prim) {
+                tcur = t;
+                bu = u;
+                bv = v;
+                best_prim = prim;
+                found = true;
+            }
+        }
+    }
+    return false;
+}
+
 // Per-lane LDS stack: entry k of this lane lives at base[k * stride]; it
 // holds at most stack_depth entries (tree depth - 1 suffices: one pending
 // node group per level).  Nodes with index < n_top are read from the LDS
 // copy top_nodes (the first levels of the breadth-first node array, shared
 // by every ray of the block).
+//
+// Each step first selects the next node (from the current node group, or the
+// group on top of the stack) and issues its fetch, then tests the triangles
+// of the previous node while that fetch is in flight, then intersects the
+// new node's children with the ray.  The node order does not depend on the
+// triangle results, and each node's child test sees every triangle hit found
+// before it, so visits and results are those of testing a node's triangles
+// before selecting the next node.
 template <int kFormat, bool kAnyHit>
 __device__ __forceinline__ bool traverse(const float4* __restrict__ nodes, const float4* __restrict__ tris,
                                          const float4* __restrict__ top_nodes, uint32_t n_top, const RayPre& r,
                                          uint2* stack, int stride, int stack_depth, float& t_out, float& u_out,
                                          float& v_out, uint32_t& prim_out, TraceStats* stats) {
     int sp = 0;
-    uint32_t ng_x = 0, ng_y = 0x80000000u;
+    uint32_t ng_x = 0, ng_y = 0x80000000u;   // node group: child base, (hits << 24) | imask
+    uint32_t tg_x = 0, tg_y = 0;             // triangles of the previous node
     float tcur = r.tmax;
     uint32_t best_prim = 0xFFFFFFFFu;
     float bu = 0.f, bv = 0.f;
     bool found = false;
     for (;;) {
-        uint32_t tg_x = 0, tg_y = 0;
-        if (ng_y > 0x00FFFFFFu) {
+        if (ng_y <= 0x00FFFFFFu && sp > 0) {
+            --sp;
+            const uint2 e = stack[sp * stride];
+            ng_x = e.x;
+            ng_y = e.y;
+        }
+        const bool have = ng_y > 0x00FFFFFFu;
+        float4 n0, n1, n2, n3, n4;
+        if (have) {
             const uint32_t imask = ng_y;
             const int bit = highest_bit(ng_y & 0xFF000000u);
             ng_y &= ~(1u << bit);
@@ -211,57 +297,22 @@ __device__ __forceinline__ bool traverse(const float4* __restrict__ nodes, const
             const uint32_t slot = ((uint32_t)bit - 24u) ^ r.octinv;
             const uint32_t rel = (uint32_t)__popc(imask & ~(0xFFFFFFFFu << slot));
             const uint32_t ni = ng_x + rel;
-            float4 n0, n1, n2, n3, n4;
-            if (ni < n_top) {
-                const float4* n = top_nodes + 5 * ni;
-                n0 = n[0]; n1 = n[1]; n2 = n[2]; n3 = n[3]; n4 = n[4];
-            } else {
-                const float4* n = nodes + 5 * (size_t)ni;
-                n0 = n[0]; n1 = n[1]; n2 = n[2]; n3 = n[3]; n4 = n[4];
-            }
-            if (stats) stats->nodes++;
-            const uint32_t hm = node_hitmask(n0, n1, n2, n3, n4, r, tcur);
-            ng_x = __float_as_uint(n1.x);
-            ng_y = (hm & 0xFF000000u) | (__float_as_uint(n0.w) >> 24);
-            tg_x = __float_as_uint(n1.y);
-            tg_y = hm & 0x00FFFFFFu;
+            const float4* n = ni < n_top ? top_nodes + 5 * ni : nodes + 5 * (size_t)ni;
+            n0 = n[0];
+            n1 = n[1];
+            n2 = n[2];
+            n3 = n[3];
+            n4 = n[4];
         }
-        while (tg_y != 0) {
-            const int b = highest_bit(tg_y);
-            tg_y &= ~(1u << b);
-            const float4* T = tris + (size_t)(tg_x + 4u * (uint32_t)b);
-            if (stats) stats->tris++;
-            float t, u, v;
-            bool ok;
-            if (kFormat == 0) {
-                ok = watertight(T[0], T[1], T[2], r, t, u, v) && t > r.tmin && t <= tcur;
-            } else {
-                ok = affine(T, r, tcur, t, u, v);
-            }
-            if (!ok) continue;
-            const uint32_t prim = __float_as_uint(T[3].w);
-            if (kAnyHit) {
-                if (t < r.tmax) return true;
-                continue;
-            }
-            if (t < tcur || prim < best_prim) {
-                tcur = t;
-                bu = u;
-                bv = v;
-                best_prim = prim;
-                found = true;
-            }
-        }
-        if (ng_y <= 0x00FFFFFFu) {
-            if (sp > 0) {
-                --sp;
-                const uint2 e = stack[sp * stride];
-                ng_x = e.x;
-                ng_y = e.y;
-            } else {
-                break;
-            }
-        }
+        if (test_triangles<kFormat, kAnyHit>(tris, tg_x, tg_y, r, tcur, bu, bv, best_prim, found, stats))
+            return true;
+        if (!have) break;
+        if (stats) stats->nodes++;
+        const uint32_t hm = node_hitmask(n0, n1, n2, n3, n4, r, tcur);
+        ng_x = __float_as_uint(n1.x);
+        ng_y = (hm & 0xFF000000u) | (__float_as_uint(n0.w) >> 24);
+        tg_x = __float_as_uint(n1.y);
+        tg_y = hm & 0x00FFFFFFu;
     }
     if (kAnyHit) return false;
     t_out = tcur;

@@ -115,10 +115,16 @@ class PathTracer:
         check(lib().tfg_pt_get_launch(self._h, C.byref(b), C.byref(g), C.byref(s)))
         o = C.c_int()
         check(lib().tfg_pt_get_overlap(self._h, C.byref(o)), "tfg_pt_get_overlap")
-        return {"trace_block": b.value, "trace_grid": g.value, "shade_grid": s.value, "overlap": bool(o.value)}
+        t = C.c_uint32()
+        check(lib().tfg_pt_get_tiles(self._h, C.byref(t)), "tfg_pt_get_tiles")
+        return {"trace_block": b.value, "trace_grid": g.value, "shade_grid": s.value, "overlap": bool(o.value),
+                "tiles": t.value}
 
     def set_overlap(self, enable: bool):
         check(lib().tfg_pt_set_overlap(self._h, 1 if enable else 0), "tfg_pt_set_overlap")
+
+    def set_tiles(self, tiles: int):
+        check(lib().tfg_pt_set_tiles(self._h, int(tiles)), "tfg_pt_set_tiles")
 
     def close(self):
         if self._h:
