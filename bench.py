@@ -40,6 +40,16 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrays/s + equal-time relMSE, NRC vs plain PT, 1080p, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
 
+# Bytes per launch between L2 and memory, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs of this file's workloads, scaled by the same counters on a
+# 1 GiB copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Sources: the summaries in
+# profiles/ named in "traffic_source".
+MEASURED_TRAFFIC = {
+    "k_trace_bounce": (358.1e6, "profiles/k_trace_bounce_displaced_sphere_1M.md"),
+    "k_extend": (1515.0e6 / 8, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
+    "k_connect": (2451.4e6 / 8, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
+}
+
 
 def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -136,7 +146,8 @@ def bench_traversal(args, torch):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC["k_trace_bounce"][0],
+            "traffic_source": MEASURED_TRAFFIC["k_trace_bounce"][1],
             "algorithmic_bytes": bytes_per[dom],
         },
         "config": {
@@ -236,8 +247,10 @@ def bench_pt(args, torch):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC["k_" + dom][0],
+            "traffic_source": MEASURED_TRAFFIC["k_" + dom][1],
             "algorithmic_bytes": int(dom_bytes),
+            "algorithmic_bytes_per_launch": int(dom_bytes / MD),
             "launches_per_frame": MD,
             "avg_launch_ms": dom_ms / MD,
             "stage_gbs": {k: v[1] / (v[2] * 1e-3) / 1e9 for k, v in stages.items()},
@@ -393,7 +406,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="pt1080", choices=["pt1080", "pt", "traversal", "copy"])
-    ap.add_argument("--ref-spp", type=int, default=512, help="reference spp of the relMSE comparison")
+    ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the relMSE comparison")
     ap.add_argument("--budget", type=float, default=2.0, help="equal-time budget in seconds per method")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
