@@ -4,7 +4,7 @@
 
 Workload ``pt1080`` (default, the metric's configuration): the plain FP32
 wavefront path tracer at 1920x1080 on the 64-instance scene (value = its
-Mrays/s over K hipGraph frames), followed by the equal-time relMSE
+Mrays/s over K frames with the tuned schedule), followed by the equal-time relMSE
 comparison of the cache-accelerated tracer against it.
 
 Workload ``pt``: the Mrays/s part alone.
@@ -169,8 +169,9 @@ def bench_traversal(args, torch):
 
 def bench_pt(args, torch):
     """Plain FP32 wavefront path tracer, 64-instance scene, 1920x1080, 1 spp per
-    frame, max depth 8, Russian roulette after depth 3; hipGraph-captured
-    frames with autotuned traversal launch shapes.  One step = one frame."""
+    frame, max depth 8, Russian roulette after depth 3;
+    frames with the tuned schedule (launch shape, side streams, graph replay or
+    direct launches).  One step = one frame."""
     from tfg_amd import pt, scenes
 
     t0 = time.time()
@@ -178,28 +179,15 @@ def bench_pt(args, torch):
     ds = pt.DeviceScene(sc, torch)
     _log(f"scene {sc.n_tris} tris, {ds.host.num_nodes} nodes, depth {ds.host.max_depth} ({time.time() - t0:.1f}s)")
     W, H, MD, RR = 1920, 1080, 8, 3
-    tracer = pt.PathTracer(W, H, MD, RR, mode=0, use_graph=True)
-    tracer.set_scene(ds)
-    tracer.set_camera(sc.camera.frame(W, H))
-    shape = tracer.autotune()
-    _log(f"autotuned launch shape {shape}")
     img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
     accum = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
-    # the tuner times eager frames; pick the tile count on graph frames
-    sweep = {}
-    for tiles in (1, 2, 4):
-        tracer.set_tiles(tiles)
-        tracer.render(0, img)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for f in range(5):
-            tracer.render(1 + f, img)
-        torch.cuda.synchronize()
-        sweep[tiles] = (time.perf_counter() - t0) / 5 * 1e3
-    best_tiles = min(sweep, key=sweep.get)
-    tracer.set_tiles(best_tiles)
+    # direct launches: faster than graph replay on this workload
+    tracer = pt.PathTracer(W, H, MD, RR, mode=0, use_graph=False)
+    tracer.set_scene(ds)
+    tracer.set_camera(sc.camera.frame(W, H))
+    tracer.autotune()
     shape = tracer.launch_shape()
-    _log(f"graph-frame ms by tile count {sweep}: using {best_tiles}")
+    _log(f"tuned schedule {shape}")
     for f in range(args.warmup):
         tracer.render(f, img, accum)
     torch.cuda.synchronize()
@@ -258,7 +246,7 @@ def bench_pt(args, torch):
         "config": {
             "workload": "plain FP32 wavefront PT: 64 seeded instances (32 displaced spheres + 32 boxes, "
                         f"{sc.n_tris} tris) + ground, 4 point lights, Lambertian+GGX; 1920x1080, 1 spp/frame, "
-                        "max depth 8, RR after depth 3, hipGraph frame",
+                        "max depth 8, RR after depth 3; frame schedule (graph replay or direct launches, streams) chosen by the tuner, see launch",
             "tris": sc.n_tris,
             "cwbvh_nodes": ds.host.num_nodes,
             "rays_per_frame": rays_frame,
@@ -268,7 +256,6 @@ def bench_pt(args, torch):
                              "shade": [round(x, 4) for x in prof["ms_shade"]],
                              "connect": [round(x, 4) for x in prof["ms_connect"]]},
             "launch": shape,
-            "graph_ms_by_tiles": {str(k): round(v, 4) for k, v in sweep.items()},
             "seeds": {"scene": sc.seed, "frames": [1000, 1000 + args.steps - 1]},
         },
         "wall_s": wall,
@@ -291,7 +278,7 @@ def bench_pt1080(args, torch):
     W, H, MD, RR = 1920, 1080, 8, 3
     t0 = time.time()
     cmp = experiments.equal_time(torch, sc, W, H, MD, RR, ref_spp=args.ref_spp, budget_s=args.budget,
-                                 warmup=48, precision=0)
+                                 warmup=48, precision=0, schedule=res["config"]["launch"])
     _log(f"equal-time comparison: {json.dumps(cmp)} ({time.time() - t0:.1f}s)")
     res["relmse"] = cmp
     res["config"]["workload"] = ("1080p path tracer (64 instances, 2.57M tris, 4 lights, Lambertian+GGX, depth 8, "

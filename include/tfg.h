@@ -128,26 +128,33 @@ typedef struct tfg_pt_profile {
     uint64_t ext_nodes, ext_tris, shadow_nodes, shadow_tris;
 } tfg_pt_profile;
 int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_profile* out);
-/* Times candidate launch shapes of the traversal stages, then tile counts and
- * overlap on/off, on the current scene and keeps the fastest (drops any
- * captured graph). */
+/* Times candidate schedules (tfg_pt_schedule) on the current scene and keeps
+ * the fastest: first the traversal launch shape, then tiles, side streams and
+ * overlap, all timed on direct launches (the tuner captures no graph).  Drops
+ * any captured graph. */
 int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream);
 int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_grid, uint32_t* shade_grid);
-/* Overlap (default on): the shadow-ray stage of depth d runs on a second
- * stream concurrently with the closest-hit and shading stages of depth d+1.
- * Results are identical either way (NEE contributions are summed in depth
- * order at the end of the frame).  No reference counterpart: the reference
- * traces shadow rays inline in its megakernel (escenario.h:144-184). */
-int tfg_pt_set_overlap(tfg_pt* pt, int enable);
-int tfg_pt_get_overlap(const tfg_pt* pt, int* enabled);
-/* Pixel tiles (1..4, default 1): the frame's pixels are split into that many
- * contiguous ranges, each traced by its own chain of launches on its own
- * streams, so that the latency-bound small launches of late depths of
- * different tiles overlap.  Results are identical for any tile count.  No
- * reference counterpart (the reference launches one megakernel per frame,
- * RenderKernel.cu:185-200). */
-int tfg_pt_set_tiles(tfg_pt* pt, uint32_t tiles);
-int tfg_pt_get_tiles(const tfg_pt* pt, uint32_t* tiles);
+/* How a frame is launched.  None of these change the results: every
+ * schedule gives the same image bit for bit.  No reference counterpart (the
+ * reference launches one megakernel per frame, RenderKernel.cu:185-200).
+ *   trace_block, grid_mult: traversal stages' block size (64, 128, 256) and
+ *       persistent grid (resident blocks per CU x grid_mult x CUs);
+ *   tiles (1..4): the pixels split into that many contiguous ranges, each
+ *       traced by its own chain of launches on its own streams;
+ *   side_streams (1..4), overlap: with overlap on, the shadow-ray stage of
+ *       depth d runs on side stream d % side_streams of its tile, after the
+ *       shading stage of depth d and concurrently with the following depths
+ *       (NEE contributions are kept per depth and summed in depth order);
+ *   use_graph: replay captured hipGraphs (one per tile, plus one for the
+ *       cache work) instead of launching the kernels directly; read-only,
+ *       fixed by tfg_pt_config.use_graph (set_schedule must pass the same
+ *       value). */
+typedef struct tfg_pt_schedule {
+    uint32_t trace_block, grid_mult, tiles, side_streams;
+    int32_t overlap, use_graph;
+} tfg_pt_schedule;
+int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out);
+int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* schedule);
 int tfg_pt_destroy(tfg_pt* pt);
 
 

@@ -59,6 +59,11 @@ class PtProfile(C.Structure):
                 ("shadow_tris", C.c_uint64)]
 
 
+class PtSchedule(C.Structure):
+    _fields_ = [("trace_block", C.c_uint32), ("grid_mult", C.c_uint32), ("tiles", C.c_uint32),
+                ("side_streams", C.c_uint32), ("overlap", C.c_int32), ("use_graph", C.c_int32)]
+
+
 class NrcConfig(C.Structure):
     _fields_ = [("batch", C.c_uint32), ("steps_per_frame", C.c_uint32), ("min_records", C.c_uint32),
                 ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
@@ -109,10 +114,8 @@ _SIGS = {
     "tfg_pt_render": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "tfg_pt_counters": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tfg_pt_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
-    "tfg_pt_set_overlap": (C.c_int, [C.c_void_p, C.c_int]),
-    "tfg_pt_get_overlap": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
-    "tfg_pt_set_tiles": (C.c_int, [C.c_void_p, C.c_uint32]),
-    "tfg_pt_get_tiles": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "tfg_pt_get_schedule": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tfg_pt_set_schedule": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tfg_pt_ray_totals": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tfg_pt_profile_frame": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(PtProfile)]),
     "tfg_pt_reset_totals": (C.c_int, [C.c_void_p]),

@@ -30,12 +30,20 @@ struct tfg_pt {
     int tiles = 1;
     tfg::PtParams TP[tfg::kMaxTiles]{};
     hipStream_t tstream[tfg::kMaxTiles] = {};   // generate, extend, shade, finalize of a tile
-    hipStream_t tside[tfg::kMaxTiles] = {};     // the tile's shadow-ray stage (connect)
+    hipStream_t tside[tfg::kMaxTiles][tfg::kMaxSide] = {};   // the tile's shadow-ray stage (connect)
+    // Streams used only for graph capture, never for direct launches: a stream
+    // that took part in a capture is not reused for direct work (the graphs
+    // are replayed on tstream / stream).
+    hipStream_t cap_tile[tfg::kMaxTiles] = {};
+    hipStream_t cap_side[tfg::kMaxTiles][tfg::kMaxSide] = {};
+    hipStream_t cap_post = nullptr;
+    int sides = 1;                               // side streams per tile: connect(d) runs on side d % sides
     hipEvent_t ev_shaded[tfg::kMaxTiles][tfg::kMaxDepth] = {};   // shade(d) of a tile done
-    hipEvent_t ev_cjoin[tfg::kMaxTiles] = {};   // every connect of a tile done
+    hipEvent_t ev_cjoin[tfg::kMaxTiles][tfg::kMaxSide] = {};   // every connect of a side stream done
     hipEvent_t ev_tdone[tfg::kMaxTiles] = {};   // a tile finalized
     hipEvent_t ev_start = nullptr;              // frame counters reset
     bool overlap = true;
+    bool run_graph = false;   // replay captured graphs (cfg.use_graph, fixed at creation)
     // graphs: one per tile (its stream chain and side stream) and one for the
     // cache work after the tiles (single stream)
     hipStream_t stream = nullptr;
@@ -223,9 +231,10 @@ inline void mark(hipEvent_t e, hipStream_t s) {
 // then finalize.  The tile's connect(d) runs on its side stream after
 // shade(d), overlapping extend/shade of the following depths (without
 // overlap it runs on ts); ts waits for the last connect before finalize.
-int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, tfg_color* image, double* accum, StageEvents* pe) {
+int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, hipStream_t* side, tfg_color* image, double* accum,
+                 StageEvents* pe) {
     const bool nrc = pt->P.mode != 0;
-    hipStream_t cs = pt->overlap ? pt->tside[t] : ts;
+    const int sides = pt->overlap ? pt->sides : 0;
     const PtParams& T = pt->TP[t];
     mark(pe ? pe->gen[t][0] : nullptr, ts);
     hipLaunchKernelGGL(k_generate, dim3(pt->gen_grid), dim3(256), 0, ts, T);
@@ -240,7 +249,8 @@ int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, tfg_color* image, double* ac
         if (nrc) hipLaunchKernelGGL(k_shade<true>, dim3(pt->shade_grid), dim3(256), 0, ts, T, q, d);
         else hipLaunchKernelGGL(k_shade<false>, dim3(pt->shade_grid), dim3(256), 0, ts, T, q, d);
         mark(pe ? pe->shd[t][d][1] : nullptr, ts);
-        if (pt->overlap) {
+        hipStream_t cs = sides ? side[d % sides] : ts;
+        if (sides) {
             TFG_HIP_CHECK(hipEventRecord(pt->ev_shaded[t][d], ts));
             TFG_HIP_CHECK(hipStreamWaitEvent(cs, pt->ev_shaded[t][d], 0));
         }
@@ -249,9 +259,9 @@ int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, tfg_color* image, double* ac
         else launch_connect<false>(pt, cs, T, d);
         mark(pe ? pe->con[t][d][1] : nullptr, cs);
     }
-    if (pt->overlap) {
-        TFG_HIP_CHECK(hipEventRecord(pt->ev_cjoin[t], cs));
-        TFG_HIP_CHECK(hipStreamWaitEvent(ts, pt->ev_cjoin[t], 0));
+    for (int k = 0; k < sides && k < (int)pt->cfg.max_depth; ++k) {
+        TFG_HIP_CHECK(hipEventRecord(pt->ev_cjoin[t][k], side[k]));
+        TFG_HIP_CHECK(hipStreamWaitEvent(ts, pt->ev_cjoin[t][k], 0));
     }
     mark(pe ? pe->fin[t][0] : nullptr, ts);
     if (nrc) hipLaunchKernelGGL(k_finalize<true>, dim3(pt->gen_grid), dim3(256), 0, ts, T, image, accum);
@@ -299,21 +309,23 @@ int enqueue_frame(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum, St
     make_tiles(pt);
     mark(pe ? pe->frame[0] : nullptr, s);
     int rc = fork_tiles(pt, s);
-    for (int t = 0; t < pt->tiles && rc == TFG_OK; ++t) rc = enqueue_tile(pt, t, pt->tstream[t], image, accum, pe);
+    for (int t = 0; t < pt->tiles && rc == TFG_OK; ++t)
+        rc = enqueue_tile(pt, t, pt->tstream[t], pt->tside[t], image, accum, pe);
     if (rc == TFG_OK) rc = join_tiles(pt, s);
     if (rc == TFG_OK) rc = enqueue_post(pt, s, image, accum);
     mark(pe ? pe->frame[1] : nullptr, s);
     return rc;
 }
 
-// Captures one graph per tile (origin: the tile stream, plus its side
-// stream) and one for the cache work (origin: s, single stream).
-int capture_graphs(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum) {
+// Captures one graph per tile (origin: the tile's capture stream, plus its
+// side capture streams) and one for the cache work (single stream), all on
+// the capture-only streams.
+int capture_graphs(tfg_pt* pt, tfg_color* image, double* accum) {
     make_tiles(pt);
     for (int t = 0; t < pt->tiles; ++t) {
-        hipStream_t ts = pt->tstream[t];
+        hipStream_t ts = pt->cap_tile[t];
         TFG_HIP_CHECK(hipStreamBeginCapture(ts, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_tile(pt, t, ts, image, accum, nullptr);
+        const int rc = enqueue_tile(pt, t, ts, pt->cap_side[t], image, accum, nullptr);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(ts, &g);
         if (rc != TFG_OK || e != hipSuccess) {
@@ -324,10 +336,11 @@ int capture_graphs(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum) {
         TFG_HIP_CHECK(hipGraphInstantiate(&pt->exec_tile[t], g, nullptr, nullptr, 0));
     }
     if (pt->P.mode != 0) {
-        TFG_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_post(pt, s, image, accum);
+        hipStream_t cs = pt->cap_post;
+        TFG_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_post(pt, cs, image, accum);
         hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(s, &g);
+        const hipError_t e = hipStreamEndCapture(cs, &g);
         if (rc != TFG_OK || e != hipSuccess) {
             if (g) (void)hipGraphDestroy(g);
             return rc != TFG_OK ? rc : hip_fail(e, "hipStreamEndCapture(cache)");
@@ -449,10 +462,18 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
         if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_in, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_out, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_start, hipEventDisableTiming);
+        if (e == hipSuccess && cfg->use_graph) e = hipStreamCreateWithFlags(&pt->cap_post, hipStreamNonBlocking);
+        for (uint32_t t = 0; t < kMaxTiles && e == hipSuccess && cfg->use_graph; ++t) {
+            e = hipStreamCreateWithFlags(&pt->cap_tile[t], hipStreamNonBlocking);
+            for (uint32_t k = 0; k < kMaxSide && e == hipSuccess; ++k)
+                e = hipStreamCreateWithFlags(&pt->cap_side[t][k], hipStreamNonBlocking);
+        }
         for (uint32_t t = 0; t < kMaxTiles && e == hipSuccess; ++t) {
             e = hipStreamCreateWithFlags(&pt->tstream[t], hipStreamNonBlocking);
-            if (e == hipSuccess) e = hipStreamCreateWithFlags(&pt->tside[t], hipStreamNonBlocking);
-            if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_cjoin[t], hipEventDisableTiming);
+            for (uint32_t k = 0; k < kMaxSide && e == hipSuccess; ++k) {
+                e = hipStreamCreateWithFlags(&pt->tside[t][k], hipStreamNonBlocking);
+                if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_cjoin[t][k], hipEventDisableTiming);
+            }
             if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_tdone[t], hipEventDisableTiming);
             for (uint32_t d = 0; d < cfg->max_depth && e == hipSuccess; ++d)
                 e = hipEventCreateWithFlags(&pt->ev_shaded[t][d], hipEventDisableTiming);
@@ -469,6 +490,7 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
         return rc;
     }
     set_trace_shape(pt, 128, 1);
+    pt->run_graph = cfg->use_graph != 0;
     pt->shade_grid = pt->n_cu * 8;
     pt->gen_grid = pt->n_cu * 8;
     *out = pt;
@@ -534,13 +556,13 @@ int tfg_pt_render(tfg_pt* pt, int32_t frame, tfg_color* d_image, double* d_accum
     TFG_HIP_CHECK(hipStreamWaitEvent(s, pt->ev_in, 0));
     hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(1), 0, s, const_cast<int32_t*>(pt->P.frame), frame);
     int rc = TFG_OK;
-    if (pt->cfg.use_graph) {
+    if (pt->run_graph) {
         if (pt->graphs_ready && (pt->key_image != d_image || pt->key_accum != d_accum || pt->key_nrc != pt->nrc ||
                                  pt->key_mode != pt->P.mode)) {
             drop_graph(pt);
         }
         if (!pt->graphs_ready) {
-            rc = capture_graphs(pt, s, d_image, d_accum);
+            rc = capture_graphs(pt, d_image, d_accum);
             if (rc != TFG_OK) {
                 drop_graph(pt);
                 return rc;
@@ -669,6 +691,7 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     float best = 1e30f;
     int bb = 128, bm = 1;
     pt->tiles = 1;
+    pt->sides = 2;
     pt->overlap = true;
     for (int bi = 0; bi < 3 && rc == TFG_OK; ++bi) {
         for (int mi = 0; mi < 3 && rc == TFG_OK; ++mi) {
@@ -682,26 +705,53 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
         }
     }
     set_trace_shape(pt, bb, bm);
-    // then the tile count and the overlap
-    const int tile_opts[3] = {1, 2, 4};
-    best = 1e30f;
-    int bt = 1;
-    bool bo = true;
-    for (int ti = 0; ti < 3 && rc == TFG_OK; ++ti) {
-        for (int oi = 0; oi < 2 && rc == TFG_OK; ++oi) {
-            pt->tiles = tile_opts[ti];
-            pt->overlap = oi == 0;
-            const float ms = time_frames();
-            if (ms < best) {
-                best = ms;
-                bt = pt->tiles;
-                bo = pt->overlap;
+    // then the schedule, timed on direct launches (the tuner never captures
+    // graphs; a tracer that replays graphs captures once, at its next frame)
+    struct Cand {
+        int tiles, sides;
+        bool overlap;
+    };
+    const Cand cands[] = {{1, 1, false}, {1, 1, true}, {1, 2, true}, {1, 4, true},
+                          {2, 1, true},  {2, 2, true}, {2, 4, true}};
+    // two passes over the candidates; a candidate's time is its faster pass
+    constexpr int kCands = (int)(sizeof(cands) / sizeof(cands[0]));
+    float cand_ms[2][kCands];
+    for (int gi = 0; gi < 2; ++gi)
+        for (int ci = 0; ci < kCands; ++ci) cand_ms[gi][ci] = 1e30f;
+    const int n_graph = 1;
+    for (int pass = 0; pass < 2 && rc == TFG_OK; ++pass) {
+        for (int gi = 0; gi < n_graph && rc == TFG_OK; ++gi) {
+            for (int ci = 0; ci < kCands && rc == TFG_OK; ++ci) {
+                const Cand& c = cands[ci];
+                pt->tiles = c.tiles;
+                pt->sides = c.sides;
+                pt->overlap = c.overlap;
+                for (int w = 0; w < 2 && rc == TFG_OK; ++w) rc = enqueue_frame(pt, pt->stream, img, nullptr);
+                (void)hipEventRecord(a, pt->stream);
+                for (int rep = 0; rep < 6 && rc == TFG_OK; ++rep) rc = enqueue_frame(pt, pt->stream, img, nullptr);
+                (void)hipEventRecord(b, pt->stream);
+                (void)hipEventSynchronize(b);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, a, b);
+                if (ms < cand_ms[gi][ci]) cand_ms[gi][ci] = ms;
             }
         }
     }
+    best = 1e30f;
+    Cand bc = {1, 1, true};
+    for (int gi = 0; gi < n_graph; ++gi) {
+        for (int ci = 0; ci < kCands; ++ci) {
+            if (cand_ms[gi][ci] < best) {
+                best = cand_ms[gi][ci];
+                bc = cands[ci];
+            }
+        }
+    }
+    drop_graph(pt);
     pt->P.mode = saved_mode;
-    pt->tiles = bt;
-    pt->overlap = bo;
+    pt->tiles = bc.tiles;
+    pt->sides = bc.sides;
+    pt->overlap = bc.overlap;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     (void)hipFree(img);
@@ -717,32 +767,35 @@ int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_g
     return TFG_OK;
 }
 
-int tfg_pt_set_tiles(tfg_pt* pt, uint32_t tiles) {
-    if (!pt || tiles == 0 || tiles > kMaxTiles) {
-        set_error("tfg_pt_set_tiles: tiles must be 1..%u", kMaxTiles);
+int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out) {
+    if (!pt || !out) { set_error("tfg_pt_get_schedule: NULL argument"); return TFG_E_INVALID; }
+    out->trace_block = (uint32_t)pt->trace_block;
+    out->grid_mult = (uint32_t)pt->trace_mult;
+    out->tiles = (uint32_t)pt->tiles;
+    out->side_streams = (uint32_t)pt->sides;
+    out->overlap = pt->overlap ? 1 : 0;
+    out->use_graph = pt->run_graph ? 1 : 0;
+    return TFG_OK;
+}
+
+int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
+    if (!pt || !sc) { set_error("tfg_pt_set_schedule: NULL argument"); return TFG_E_INVALID; }
+    const uint32_t b = sc->trace_block;
+    if ((b != 64 && b != 128 && b != 256) || sc->grid_mult == 0 || sc->grid_mult > 64 || sc->tiles == 0 ||
+        sc->tiles > kMaxTiles || sc->side_streams == 0 || sc->side_streams > kMaxSide) {
+        set_error("tfg_pt_set_schedule: block 64/128/256, grid_mult 1..64, tiles 1..%u, side streams 1..%u",
+                  kMaxTiles, kMaxSide);
         return TFG_E_INVALID;
     }
-    if ((uint32_t)pt->tiles != tiles) drop_graph(pt);
-    pt->tiles = (int)tiles;
-    return TFG_OK;
-}
-
-int tfg_pt_get_tiles(const tfg_pt* pt, uint32_t* tiles) {
-    if (!pt || !tiles) { set_error("tfg_pt_get_tiles: NULL"); return TFG_E_INVALID; }
-    *tiles = (uint32_t)pt->tiles;
-    return TFG_OK;
-}
-
-int tfg_pt_set_overlap(tfg_pt* pt, int enable) {
-    if (!pt) { set_error("tfg_pt_set_overlap: NULL"); return TFG_E_INVALID; }
-    if (pt->overlap != (enable != 0)) drop_graph(pt);
-    pt->overlap = enable != 0;
-    return TFG_OK;
-}
-
-int tfg_pt_get_overlap(const tfg_pt* pt, int* enabled) {
-    if (!pt || !enabled) { set_error("tfg_pt_get_overlap: NULL"); return TFG_E_INVALID; }
-    *enabled = pt->overlap ? 1 : 0;
+    if ((sc->use_graph != 0) != pt->run_graph) {
+        set_error("tfg_pt_set_schedule: graph replay is fixed when the tracer is created");
+        return TFG_E_STATE;
+    }
+    drop_graph(pt);
+    set_trace_shape(pt, (int)b, (int)sc->grid_mult);
+    pt->tiles = (int)sc->tiles;
+    pt->sides = (int)sc->side_streams;
+    pt->overlap = sc->overlap != 0;
     return TFG_OK;
 }
 
@@ -799,7 +852,8 @@ int tfg_pt_destroy(tfg_pt* pt) {
     if (pt->stream) (void)hipStreamSynchronize(pt->stream);
     for (uint32_t t = 0; t < kMaxTiles; ++t) {
         if (pt->tstream[t]) (void)hipStreamSynchronize(pt->tstream[t]);
-        if (pt->tside[t]) (void)hipStreamSynchronize(pt->tside[t]);
+        for (hipStream_t q : pt->tside[t])
+            if (q) (void)hipStreamSynchronize(q);
     }
     drop_graph(pt);
     for (void* p : pt->allocs) (void)hipFree(p);
@@ -807,13 +861,19 @@ int tfg_pt_destroy(tfg_pt* pt) {
     if (pt->ev_out) (void)hipEventDestroy(pt->ev_out);
     if (pt->ev_start) (void)hipEventDestroy(pt->ev_start);
     for (uint32_t t = 0; t < kMaxTiles; ++t) {
-        if (pt->ev_cjoin[t]) (void)hipEventDestroy(pt->ev_cjoin[t]);
+        for (hipEvent_t e : pt->ev_cjoin[t])
+            if (e) (void)hipEventDestroy(e);
         if (pt->ev_tdone[t]) (void)hipEventDestroy(pt->ev_tdone[t]);
         for (hipEvent_t e : pt->ev_shaded[t])
             if (e) (void)hipEventDestroy(e);
-        if (pt->tside[t]) (void)hipStreamDestroy(pt->tside[t]);
+        for (hipStream_t q : pt->tside[t])
+            if (q) (void)hipStreamDestroy(q);
+        for (hipStream_t q : pt->cap_side[t])
+            if (q) (void)hipStreamDestroy(q);
+        if (pt->cap_tile[t]) (void)hipStreamDestroy(pt->cap_tile[t]);
         if (pt->tstream[t]) (void)hipStreamDestroy(pt->tstream[t]);
     }
+    if (pt->cap_post) (void)hipStreamDestroy(pt->cap_post);
     if (pt->stream) (void)hipStreamDestroy(pt->stream);
     delete pt;
     return TFG_OK;

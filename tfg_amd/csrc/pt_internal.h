@@ -33,6 +33,7 @@ enum : uint32_t {
     CNT_ALLOC = CNT_SHQ + kMaxDepth,
 };
 constexpr uint32_t kMaxTiles = 4;   // pixel tiles traced as independent stream chains
+constexpr uint32_t kMaxSide = 4;    // side streams per tile for the shadow-ray stage
 
 // path flags
 enum : uint32_t {

@@ -25,9 +25,19 @@ def _sync_time(torch):
     return time.perf_counter()
 
 
-def reference_image(torch, dscene, W, H, max_depth, rr_depth, spp, frame0=1_000_000):
-    tr = pt.PathTracer(W, H, max_depth, rr_depth, mode=0, use_graph=True)
+def _graph(schedule):
+    return bool(schedule.get("use_graph", True)) if schedule else True
+
+
+def _apply(tr, schedule):
+    if schedule:
+        tr.set_schedule(**{k: v for k, v in schedule.items() if k in tr.schedule()})
+
+
+def reference_image(torch, dscene, W, H, max_depth, rr_depth, spp, frame0=1_000_000, schedule=None):
+    tr = pt.PathTracer(W, H, max_depth, rr_depth, mode=0, use_graph=_graph(schedule))
     tr.set_scene(dscene)
+    _apply(tr, schedule)
     tr.set_camera(dscene.scene.camera.frame(W, H))
     img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
     acc = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
@@ -37,10 +47,11 @@ def reference_image(torch, dscene, W, H, max_depth, rr_depth, spp, frame0=1_000_
     return (acc / spp).cpu().numpy()
 
 
-def plain_run(torch, dscene, W, H, max_depth, rr_depth, budget_s, frame0=0):
+def plain_run(torch, dscene, W, H, max_depth, rr_depth, budget_s, frame0=0, schedule=None):
     """Plain FP32 PT for budget_s seconds of wall time; returns (mean image, frames, seconds)."""
-    tr = pt.PathTracer(W, H, max_depth, rr_depth, mode=0, use_graph=True)
+    tr = pt.PathTracer(W, H, max_depth, rr_depth, mode=0, use_graph=_graph(schedule))
     tr.set_scene(dscene)
+    _apply(tr, schedule)
     tr.set_camera(dscene.scene.camera.frame(W, H))
     img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
     acc = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
@@ -57,13 +68,14 @@ def plain_run(torch, dscene, W, H, max_depth, rr_depth, budget_s, frame0=0):
 
 
 def nrc_run(torch, dscene, W, H, max_depth, rr_depth, budget_s, warmup=48, precision=0, train_fraction=0.05,
-            frame0=0, cache_cfg=None, count_warmup=True):
+            frame0=0, cache_cfg=None, count_warmup=True, schedule=None):
     """Cache-accelerated PT: `warmup` training-only frames, then train+infer
     frames until the budget (wall time including warmup when
     count_warmup) is spent.  Returns (mean image, frames, seconds, cache stats)."""
     tr = pt.PathTracer(W, H, max_depth, rr_depth, mode=pt_mode_train(), train_fraction=train_fraction,
-                       use_graph=True)
+                       use_graph=_graph(schedule))
     tr.set_scene(dscene)
+    _apply(tr, schedule)
     tr.set_camera(dscene.scene.camera.frame(W, H))
     sc = dscene.scene
     lo = sc.verts.reshape(-1, 3).min(0) - 0.5
@@ -99,12 +111,13 @@ def pt_mode_infer():
     return 1
 
 
-def equal_time(torch, scene, W, H, max_depth, rr_depth, ref_spp, budget_s, warmup=48, precision=0):
+def equal_time(torch, scene, W, H, max_depth, rr_depth, ref_spp, budget_s, warmup=48, precision=0, schedule=None):
+    """Both methods (and the reference) run with the same frame schedule."""
     dscene = pt.DeviceScene(scene, torch)
-    ref = reference_image(torch, dscene, W, H, max_depth, rr_depth, ref_spp)
-    p_img, p_n, p_t = plain_run(torch, dscene, W, H, max_depth, rr_depth, budget_s)
+    ref = reference_image(torch, dscene, W, H, max_depth, rr_depth, ref_spp, schedule=schedule)
+    p_img, p_n, p_t = plain_run(torch, dscene, W, H, max_depth, rr_depth, budget_s, schedule=schedule)
     n_img, n_n, n_t, st, w_t = nrc_run(torch, dscene, W, H, max_depth, rr_depth, budget_s, warmup=warmup,
-                                       precision=precision)
+                                       precision=precision, schedule=schedule)
     rp = relmse(p_img, ref)
     rn = relmse(n_img, ref)
     return {

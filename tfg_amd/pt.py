@@ -110,21 +110,41 @@ class PathTracer:
         check(lib().tfg_pt_autotune(self._h, stream), "tfg_pt_autotune")
         return self.launch_shape()
 
+    def schedule(self) -> dict:
+        sc = _lib.PtSchedule()
+        check(lib().tfg_pt_get_schedule(self._h, C.byref(sc)), "tfg_pt_get_schedule")
+        return {"trace_block": sc.trace_block, "grid_mult": sc.grid_mult, "tiles": sc.tiles,
+                "side_streams": sc.side_streams, "overlap": bool(sc.overlap), "use_graph": bool(sc.use_graph)}
+
+    def set_schedule(self, **kw):
+        """Change any of trace_block, grid_mult, tiles, side_streams, overlap, use_graph."""
+        cur = self.schedule()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise TypeError(f"unknown schedule fields {sorted(unknown)}")
+        cur.update(kw)
+        sc = _lib.PtSchedule(cur["trace_block"], cur["grid_mult"], cur["tiles"], cur["side_streams"],
+                             1 if cur["overlap"] else 0, 1 if cur["use_graph"] else 0)
+        check(lib().tfg_pt_set_schedule(self._h, C.byref(sc)), "tfg_pt_set_schedule")
+
     def launch_shape(self):
         b, g, s = C.c_uint32(), C.c_uint32(), C.c_uint32()
         check(lib().tfg_pt_get_launch(self._h, C.byref(b), C.byref(g), C.byref(s)))
-        o = C.c_int()
-        check(lib().tfg_pt_get_overlap(self._h, C.byref(o)), "tfg_pt_get_overlap")
-        t = C.c_uint32()
-        check(lib().tfg_pt_get_tiles(self._h, C.byref(t)), "tfg_pt_get_tiles")
-        return {"trace_block": b.value, "trace_grid": g.value, "shade_grid": s.value, "overlap": bool(o.value),
-                "tiles": t.value}
+        out = {"trace_grid": g.value, "shade_grid": s.value}
+        out.update(self.schedule())
+        return out
 
     def set_overlap(self, enable: bool):
-        check(lib().tfg_pt_set_overlap(self._h, 1 if enable else 0), "tfg_pt_set_overlap")
+        self.set_schedule(overlap=bool(enable))
 
     def set_tiles(self, tiles: int):
-        check(lib().tfg_pt_set_tiles(self._h, int(tiles)), "tfg_pt_set_tiles")
+        self.set_schedule(tiles=int(tiles))
+
+    def set_sides(self, sides: int):
+        self.set_schedule(side_streams=int(sides))
+
+    def set_launch(self, trace_block: int, grid_mult: int):
+        self.set_schedule(trace_block=int(trace_block), grid_mult=int(grid_mult))
 
     def close(self):
         if self._h:
