@@ -130,8 +130,8 @@ typedef struct tfg_pt_profile {
 int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_profile* out);
 /* Times candidate schedules (tfg_pt_schedule) on the current scene and keeps
  * the fastest: first the traversal launch shape, then tiles, side streams and
- * overlap, all timed on direct launches (the tuner captures no graph).  Drops
- * any captured graph. */
+ * overlap, then the XCD super-chunk size, all timed on direct launches (the
+ * tuner captures no graph).  Drops any captured graph. */
 int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream);
 int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_grid, uint32_t* shade_grid);
 /* How a frame is launched.  None of these change the results: every
@@ -145,6 +145,10 @@ int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_g
  *       depth d runs on side stream d % side_streams of its tile, after the
  *       shading stage of depth d and concurrently with the following depths
  *       (NEE contributions are kept per depth and summed in depth order);
+ *   xcd_group: rays per XCD super-chunk in the traversal stages (0 = plain
+ *       grid stride; else a multiple of 256): consecutive queue entries are
+ *       grouped so that each group is traced by the workgroups of one XCD
+ *       (one L2), groups dealt to the 8 XCDs in turn;
  *   use_graph: replay captured hipGraphs (one per tile, plus one for the
  *       cache work) instead of launching the kernels directly; read-only,
  *       fixed by tfg_pt_config.use_graph (set_schedule must pass the same
@@ -152,6 +156,7 @@ int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_g
 typedef struct tfg_pt_schedule {
     uint32_t trace_block, grid_mult, tiles, side_streams;
     int32_t overlap, use_graph;
+    uint32_t xcd_group;
 } tfg_pt_schedule;
 int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out);
 int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* schedule);

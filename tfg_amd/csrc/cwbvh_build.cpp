@@ -1,6 +1,7 @@
-// cwbvh_build.cpp - binned-SAH BVH2, greedy 8-wide collapse with
-// octant-ordered child slots, and 8-bit quantisation of child bounds into
-// the 80 B CWBVH node format decoded by traverse_cwbvh.cuh:51-185.
+// cwbvh_build.cpp - binned-SAH BVH2, SAH-optimal 8-wide collapse (dynamic
+// programming over the BVH2, leaves of up to 3 triangles), octant-ordered
+// child slots, and 8-bit quantisation of child bounds into the 80 B CWBVH
+// node format decoded by traverse_cwbvh.cuh:51-185.
 //
 // Written from the decoder's semantics and from Ylitie, Karras, Laine,
 // "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs"
@@ -159,6 +160,83 @@ void build_rec(Ctx& c, int begin, int end, std::vector<Node2>& nodes, int depth)
     }
 }
 
+// SAH-optimal collapse of the BVH2 into 8-wide nodes (Ylitie, Karras, Laine
+// 2017, section 4.1).  cost[n][i] is the least SAH cost of representing the
+// subtree of BVH2 node n with at most i slots of its parent wide node:
+//   i = 1: a leaf slot (at most 3 triangles: A(n) * kTriCost * count) or an
+//          internal slot (A(n) * kNodeCost + best split of 8 slots between
+//          n's two children);
+//   i > 1: the best of i - 1 slots and every split of i slots between the
+//          children.
+// split[n][i] records the choice (0: i - 1 slots / the i = 1 case, k > 0:
+// k slots to the left child), wide[n] the split of an internal node's 8
+// slots.  Subtrees of at most 3 triangles are always leaf slots when given
+// one slot, and larger ones always internal, so a slot's kind follows from
+// its triangle count.
+struct Collapse {
+    std::vector<float> cost;      // 8 per BVH2 node (slots 1..8)
+    std::vector<uint8_t> split;   // 8 per BVH2 node
+    std::vector<uint8_t> wide;    // left share of an internal node's 8 slots
+};
+
+constexpr double kNodeCost = 1.0;   // SAH cost of a wide-node visit
+constexpr double kTriCost = 0.3;    // SAH cost of one triangle test
+
+void plan_collapse(const std::vector<Node2>& n2, Collapse& cp) {
+    const size_t nn = n2.size();
+    cp.cost.assign(8 * nn, 0.f);
+    cp.split.assign(8 * nn, 0);
+    cp.wide.assign(nn, 0);
+    // children always follow their parent in n2, so a reverse sweep is post-order
+    for (size_t idx = nn; idx-- > 0;) {
+        const Node2& n = n2[idx];
+        float* C = &cp.cost[8 * idx];
+        uint8_t* S = &cp.split[8 * idx];
+        const double area = n.box.area();
+        if (n.left < 0) {   // single triangle
+            for (int i = 0; i < 8; ++i) C[i] = (float)(area * kTriCost * n.count);
+            continue;
+        }
+        const float* CL = &cp.cost[8 * (size_t)n.left];
+        const float* CR = &cp.cost[8 * (size_t)n.right];
+        double dist[9];
+        int dk[9];
+        for (int j = 2; j <= 8; ++j) {
+            dist[j] = std::numeric_limits<double>::infinity();
+            dk[j] = 1;
+            for (int k = 1; k < j; ++k) {
+                const double c = (double)CL[k - 1] + (double)CR[j - k - 1];
+                if (c < dist[j]) { dist[j] = c; dk[j] = k; }
+            }
+        }
+        cp.wide[idx] = (uint8_t)dk[8];
+        const double c1 = n.count <= 3 ? area * kTriCost * n.count : area * kNodeCost + dist[8];
+        C[0] = (float)c1;
+        S[0] = 0;
+        for (int i = 2; i <= 8; ++i) {
+            if (dist[i] < (double)C[i - 2]) {
+                C[i - 1] = (float)dist[i];
+                S[i - 1] = (uint8_t)dk[i];
+            } else {
+                C[i - 1] = C[i - 2];
+                S[i - 1] = 0;
+            }
+        }
+    }
+}
+
+// The slots BVH2 node n takes when given at most i of them.
+void expand(const std::vector<Node2>& n2, const Collapse& cp, int n, int i, int* kids, int& nk) {
+    while (i > 1 && cp.split[8 * (size_t)n + i - 1] == 0) --i;
+    if (i <= 1) {
+        kids[nk++] = n;
+        return;
+    }
+    const int k = cp.split[8 * (size_t)n + i - 1];
+    expand(n2, cp, n2[n].left, k, kids, nk);
+    expand(n2, cp, n2[n].right, i - k, kids, nk);
+}
+
 inline uint32_t f2u(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
 inline float u2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 
@@ -243,6 +321,8 @@ bool build_cwbvh(const float* verts, uint32_t n_tris, uint32_t format, CwbvhHost
     std::vector<Node2> n2;
     n2.reserve(2 * (size_t)n_tris);
     build_rec(c, 0, (int)n_tris, n2, 0);
+    Collapse cp;
+    plan_collapse(n2, cp);
 
     // ---- collapse to 8-wide, breadth-first node order --------------------
     struct Pending { uint32_t slot8; int node2; uint32_t depth; };
@@ -268,22 +348,9 @@ bool build_cwbvh(const float* verts, uint32_t n_tris, uint32_t format, CwbvhHost
         if (root.left < 0) {
             kids[nk++] = pd.node2;   // single-triangle tree: one leaf child
         } else {
-            kids[nk++] = root.left;
-            kids[nk++] = root.right;
-            while (nk < 8) {
-                int best = -1;
-                double best_area = -1.0;
-                for (int k = 0; k < nk; ++k) {
-                    const Node2& cn = n2[kids[k]];
-                    if (cn.left < 0) continue;
-                    const double a = cn.box.area();
-                    if (a > best_area) { best_area = a; best = k; }
-                }
-                if (best < 0) break;
-                const Node2& cn = n2[kids[best]];
-                kids[best] = cn.left;
-                kids[nk++] = cn.right;
-            }
+            const int k8 = cp.wide[pd.node2];
+            expand(n2, cp, root.left, k8, kids, nk);
+            expand(n2, cp, root.right, 8 - k8, kids, nk);
         }
 
         // Octant-ordered slot assignment: slot s is visited first by rays

@@ -411,6 +411,7 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
     P.n_paths = n;
     P.pix_base = 0;
     P.depth_stride = n;
+    P.xcd_group = 4096;
     P.max_depth = cfg->max_depth;
     P.rr_depth = cfg->rr_depth;
     P.mode = cfg->mode;
@@ -747,11 +748,31 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
             }
         }
     }
-    drop_graph(pt);
-    pt->P.mode = saved_mode;
     pt->tiles = bc.tiles;
     pt->sides = bc.sides;
     pt->overlap = bc.overlap;
+    // then the XCD super-chunk size of the traversal stages
+    const uint32_t groups[4] = {0, 1024, 4096, 16384};
+    float group_ms[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+    for (int pass = 0; pass < 2 && rc == TFG_OK; ++pass) {
+        for (int gi = 0; gi < 4 && rc == TFG_OK; ++gi) {
+            pt->P.xcd_group = groups[gi];
+            for (int w = 0; w < 2 && rc == TFG_OK; ++w) rc = enqueue_frame(pt, pt->stream, img, nullptr);
+            (void)hipEventRecord(a, pt->stream);
+            for (int rep = 0; rep < 6 && rc == TFG_OK; ++rep) rc = enqueue_frame(pt, pt->stream, img, nullptr);
+            (void)hipEventRecord(b, pt->stream);
+            (void)hipEventSynchronize(b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            if (ms < group_ms[gi]) group_ms[gi] = ms;
+        }
+    }
+    int bgi = 0;
+    for (int gi = 1; gi < 4; ++gi)
+        if (group_ms[gi] < group_ms[bgi]) bgi = gi;
+    pt->P.xcd_group = groups[bgi];
+    drop_graph(pt);
+    pt->P.mode = saved_mode;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     (void)hipFree(img);
@@ -775,6 +796,7 @@ int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out) {
     out->side_streams = (uint32_t)pt->sides;
     out->overlap = pt->overlap ? 1 : 0;
     out->use_graph = pt->run_graph ? 1 : 0;
+    out->xcd_group = pt->P.xcd_group;
     return TFG_OK;
 }
 
@@ -782,9 +804,10 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
     if (!pt || !sc) { set_error("tfg_pt_set_schedule: NULL argument"); return TFG_E_INVALID; }
     const uint32_t b = sc->trace_block;
     if ((b != 64 && b != 128 && b != 256) || sc->grid_mult == 0 || sc->grid_mult > 64 || sc->tiles == 0 ||
-        sc->tiles > kMaxTiles || sc->side_streams == 0 || sc->side_streams > kMaxSide) {
-        set_error("tfg_pt_set_schedule: block 64/128/256, grid_mult 1..64, tiles 1..%u, side streams 1..%u",
-                  kMaxTiles, kMaxSide);
+        sc->tiles > kMaxTiles || sc->side_streams == 0 || sc->side_streams > kMaxSide ||
+        (sc->xcd_group != 0 && (sc->xcd_group % 256 != 0 || sc->xcd_group > (1u << 20)))) {
+        set_error("tfg_pt_set_schedule: block 64/128/256, grid_mult 1..64, tiles 1..%u, side streams 1..%u, "
+                  "xcd_group 0 or a multiple of 256 up to 2^20", kMaxTiles, kMaxSide);
         return TFG_E_INVALID;
     }
     if ((sc->use_graph != 0) != pt->run_graph) {
@@ -796,6 +819,7 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
     pt->tiles = (int)sc->tiles;
     pt->sides = (int)sc->side_streams;
     pt->overlap = sc->overlap != 0;
+    pt->P.xcd_group = sc->xcd_group;
     return TFG_OK;
 }
 

@@ -114,7 +114,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint3
         P.counters[q ^ 1u] = 0;
     }
     const float4* rays = P.rays[q];
-    for (uint32_t b0 = blockIdx.x * kBlock; b0 < count; b0 += gridDim.x * kBlock) {
+    for (ChunkIter it((count + kBlock - 1) / kBlock, P.xcd_group / kBlock, P.xcd_group != 0); it.valid(); it.next()) {
+        const uint32_t b0 = it.chunk * kBlock;
         const uint32_t i = b0 + threadIdx.x;
         TraceStats st;
         if (i < count) {
@@ -394,7 +395,9 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) 
     const uint64_t work = (uint64_t)count << shift;
     const uint32_t lane = __lane_id();
     const int lead = (int)(lane & ~(group - 1u));
-    for (uint64_t b = (uint64_t)blockIdx.x * kBlock; b < work; b += (uint64_t)gridDim.x * kBlock) {
+    for (ChunkIter it((uint32_t)((work + kBlock - 1) / kBlock), P.xcd_group / kBlock, P.xcd_group != 0); it.valid();
+         it.next()) {
+        const uint64_t b = (uint64_t)it.chunk * kBlock;
         const uint64_t w = b + threadIdx.x;
         const uint32_t i = (uint32_t)(w >> shift);
         const uint32_t l = (uint32_t)w & (group - 1u);

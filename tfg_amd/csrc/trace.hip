@@ -3,6 +3,7 @@
 
 #include "host_common.h"
 #include "trace_device.h"
+#include "wave_ops.h"
 
 namespace tfg {
 uint32_t known_tree_depth(const void* d_nodes);
@@ -23,24 +24,27 @@ __global__ __launch_bounds__(kBlock) void k_trace(const float4* __restrict__ nod
     uint2* stack = reinterpret_cast<uint2*>(top + 5 * n_top);
     for (uint32_t k = threadIdx.x; k < 5 * n_top; k += kBlock) top[k] = nodes[k];
     __syncthreads();
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const float4 a = rays[2 * (size_t)i];
-    const float4 b = rays[2 * (size_t)i + 1];
-    const RayPre r = make_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
-    TraceStats st;
-    float t, u, v;
-    uint32_t prim;
-    const bool hit = traverse<kFormat, kAnyHit>(nodes, tris, top, n_top, r, stack + threadIdx.x, kBlock, stack_depth,
-                                               t, u, v, prim, kStats ? &st : nullptr);
-    if (kAnyHit) {
-        occluded[i] = hit ? 1u : 0u;
-    } else {
-        hits[i] = make_float4(t, u, v, __uint_as_float(prim));
-    }
-    if (kStats) {
-        counts[2 * (size_t)i] = st.nodes | (st.overflow << 31);
-        counts[2 * (size_t)i + 1] = st.tris;
+    // one chunk of kBlock rays per workgroup, placed XCD-aware
+    for (ChunkIter it(gridDim.x, 4096 / kBlock); it.valid(); it.next()) {
+        const uint32_t i = it.chunk * kBlock + threadIdx.x;
+        if (i >= n) continue;
+        const float4 a = rays[2 * (size_t)i];
+        const float4 b = rays[2 * (size_t)i + 1];
+        const RayPre r = make_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
+        TraceStats st;
+        float t, u, v;
+        uint32_t prim;
+        const bool hit = traverse<kFormat, kAnyHit>(nodes, tris, top, n_top, r, stack + threadIdx.x, kBlock,
+                                                   stack_depth, t, u, v, prim, kStats ? &st : nullptr);
+        if (kAnyHit) {
+            occluded[i] = hit ? 1u : 0u;
+        } else {
+            hits[i] = make_float4(t, u, v, __uint_as_float(prim));
+        }
+        if (kStats) {
+            counts[2 * (size_t)i] = st.nodes | (st.overflow << 31);
+            counts[2 * (size_t)i + 1] = st.tris;
+        }
     }
 }
 

@@ -18,6 +18,38 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
     return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
 }
 
+// XCD-aware order of `chunks` work chunks over a grid.  The GPU hands
+// workgroups to its 8 XCDs round robin (workgroup b runs on XCD b % 8) and
+// each XCD has its own L2.  Chunks are grouped into super-chunks of `group`
+// consecutive chunks; super-chunk j belongs to XCD j % 8, whose workgroups
+// share out its chunks.  Neighbouring rays (neighbouring pixels, for the
+// queues here) thus meet on one L2, while every XCD still gets work from the
+// whole frame.  Usage:
+//     for (ChunkIter it(chunks, group); it.valid(); it.next()) ... it.chunk ...
+// Every chunk is visited exactly once whatever the real placement (with
+// enable false, or a grid that is not a multiple of 8, the plain grid stride).
+constexpr uint32_t kXcds = 8;
+struct ChunkIter {
+    uint32_t chunks, group, xcd, step, k, chunk;
+    bool xcd_order;
+    __device__ __forceinline__ ChunkIter(uint32_t n_chunks, uint32_t group_chunks, bool enable = true)
+        : chunks(n_chunks), group(group_chunks ? group_chunks : 1u) {
+        xcd_order = enable && gridDim.x % kXcds == 0;
+        xcd = blockIdx.x % kXcds;
+        step = xcd_order ? gridDim.x / kXcds : gridDim.x;
+        k = xcd_order ? blockIdx.x / kXcds : blockIdx.x;
+        map();
+    }
+    __device__ __forceinline__ void map() {
+        chunk = xcd_order ? (xcd + kXcds * (k / group)) * group + k % group : k;
+    }
+    __device__ __forceinline__ bool valid() const { return chunk < chunks; }
+    __device__ __forceinline__ void next() {
+        k += step;
+        map();
+    }
+};
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
     return x;

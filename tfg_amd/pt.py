@@ -114,17 +114,19 @@ class PathTracer:
         sc = _lib.PtSchedule()
         check(lib().tfg_pt_get_schedule(self._h, C.byref(sc)), "tfg_pt_get_schedule")
         return {"trace_block": sc.trace_block, "grid_mult": sc.grid_mult, "tiles": sc.tiles,
-                "side_streams": sc.side_streams, "overlap": bool(sc.overlap), "use_graph": bool(sc.use_graph)}
+                "side_streams": sc.side_streams, "overlap": bool(sc.overlap), "use_graph": bool(sc.use_graph),
+                "xcd_group": sc.xcd_group}
 
     def set_schedule(self, **kw):
-        """Change any of trace_block, grid_mult, tiles, side_streams, overlap, use_graph."""
+        """Change any of trace_block, grid_mult, tiles, side_streams, overlap, xcd_group
+        (use_graph is fixed when the tracer is created)."""
         cur = self.schedule()
         unknown = set(kw) - set(cur)
         if unknown:
             raise TypeError(f"unknown schedule fields {sorted(unknown)}")
         cur.update(kw)
         sc = _lib.PtSchedule(cur["trace_block"], cur["grid_mult"], cur["tiles"], cur["side_streams"],
-                             1 if cur["overlap"] else 0, 1 if cur["use_graph"] else 0)
+                             1 if cur["overlap"] else 0, 1 if cur["use_graph"] else 0, cur["xcd_group"])
         check(lib().tfg_pt_set_schedule(self._h, C.byref(sc)), "tfg_pt_set_schedule")
 
     def launch_shape(self):
