@@ -47,6 +47,18 @@ int tfg_version(void);
  * Runs on the host only; needs no GPU. */
 typedef struct tfg_cwbvh_host tfg_cwbvh_host;
 int tfg_cwbvh_build(const float* verts, uint32_t n_tris, uint32_t format, tfg_cwbvh_host** out);
+/* Builder settings (tfg_cwbvh_build uses the defaults): SAH costs of a
+ * wide-node visit and of a triangle test for the 8-wide collapse (1, 1),
+ * triangles per leaf slot (1..3, default 2) and SAH bins per axis (4..64,
+ * default 32).  Any setting gives a valid tree; they trade node visits
+ * against triangle tests. */
+typedef struct tfg_cwbvh_build_options {
+    float node_cost, tri_cost;
+    uint32_t max_leaf, bins;
+} tfg_cwbvh_build_options;
+void tfg_cwbvh_default_build_options(tfg_cwbvh_build_options* o);
+int tfg_cwbvh_build_ex(const float* verts, uint32_t n_tris, uint32_t format, const tfg_cwbvh_build_options* opts,
+                       tfg_cwbvh_host** out);
 int tfg_cwbvh_host_info(const tfg_cwbvh_host* h, uint32_t* num_nodes, uint32_t* num_tris, uint32_t* max_depth,
                         double* sah_cost);
 const float* tfg_cwbvh_host_nodes(const tfg_cwbvh_host* h); /* 20 floats per node */

@@ -36,6 +36,34 @@ def test_builder_valid(name, fmt):
     np.testing.assert_array_equal(prims, np.arange(v.shape[0], dtype=np.uint32))
 
 
+@pytest.mark.parametrize("opts", [dict(max_leaf=1), dict(max_leaf=2, bins=8), dict(node_cost=1.0, tri_cost=3.0),
+                                  dict(node_cost=1.0, tri_cost=0.05, bins=64)])
+def test_builder_options_give_valid_trees(opts):
+    """Every builder setting gives a valid tree whose traversal equals brute
+    force; leaf slots never hold more than max_leaf triangles."""
+    v = MESHES["sphere"]()
+    h = _lib.CwbvhHost(v, 0, **opts)
+    rc, depth = oracle.validate(h.nodes, h.tris, 0, v)
+    assert rc == 0 and depth == h.max_depth
+    meta = h.nodes[:, 6:8].copy().view(np.uint8).reshape(-1, 8)
+    imask = (h.nodes[:, 3].view(np.uint32) >> 24).astype(np.uint8)
+    slot_internal = (imask[:, None] >> np.arange(8, dtype=np.uint8)) & 1
+    leaf_counts = np.array([bin(m >> 5).count("1") for m in meta.ravel()]).reshape(-1, 8)
+    assert leaf_counts[slot_internal == 0].max() <= opts.get("max_leaf", 3)
+    rays = workloads.random_rays(3000, 17, extent=1.5)
+    a = oracle.cwbvh_trace(h.nodes, h.tris, 0, rays)
+    b = oracle.brute_trace(v, rays)
+    np.testing.assert_array_equal(a["prim"], b["prim"])
+    np.testing.assert_array_equal(a["t"].view(np.uint32), b["t"].view(np.uint32))
+
+
+def test_builder_rejects_bad_options():
+    v = MESHES["soup"]()
+    for bad in (dict(max_leaf=0), dict(max_leaf=4), dict(bins=2), dict(bins=65), dict(tri_cost=0.0)):
+        with pytest.raises(_lib.TfgError):
+            _lib.CwbvhHost(v, 0, **bad)
+
+
 @pytest.mark.parametrize("name", ["soup", "cornell", "sphere", "coplanar"])
 def test_traversal_equals_brute_force(name):
     v = MESHES[name]()

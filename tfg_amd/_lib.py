@@ -92,6 +92,8 @@ _SIGS = {
     "tfg_last_error": (C.c_char_p, []),
     "tfg_version": (C.c_int, []),
     "tfg_cwbvh_build": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "tfg_cwbvh_build_ex": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "tfg_cwbvh_default_build_options": (None, [C.c_void_p]),
     "tfg_cwbvh_host_info": (
         C.c_int,
         [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_double)],
@@ -192,13 +194,23 @@ def ptr(x) -> int:
     return x.data_ptr()
 
 
-class CwbvhHost:
-    """Host CWBVH built by tfg_cwbvh_build; nodes/tris exposed as numpy views."""
+class BuildOptions(C.Structure):
+    _fields_ = [("node_cost", C.c_float), ("tri_cost", C.c_float), ("max_leaf", C.c_uint32), ("bins", C.c_uint32)]
 
-    def __init__(self, verts: np.ndarray, fmt: int = TFG_TRI_WATERTIGHT):
+
+class CwbvhHost:
+    """Host CWBVH built by tfg_cwbvh_build; nodes/tris exposed as numpy views.
+    Builder settings (node_cost, tri_cost, max_leaf, bins) override the defaults."""
+
+    def __init__(self, verts: np.ndarray, fmt: int = TFG_TRI_WATERTIGHT, **options):
         v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 9)
         self._h = C.c_void_p()
-        check(lib().tfg_cwbvh_build(v.ctypes.data, v.shape[0], fmt, C.byref(self._h)), "tfg_cwbvh_build")
+        opts = BuildOptions()
+        lib().tfg_cwbvh_default_build_options(C.byref(opts))
+        for k, val in options.items():
+            setattr(opts, k, val)
+        check(lib().tfg_cwbvh_build_ex(v.ctypes.data, v.shape[0], fmt, C.byref(opts), C.byref(self._h)),
+              "tfg_cwbvh_build")
         self.format = fmt
         nn, nt, dp = C.c_uint32(), C.c_uint32(), C.c_uint32()
         sah = C.c_double()

@@ -57,7 +57,33 @@ const char* tfg_last_error(void) { return tfg::g_error.c_str(); }
 int tfg_version(void) { return 1; }
 
 int tfg_cwbvh_build(const float* verts, uint32_t n_tris, uint32_t format, tfg_cwbvh_host** out) {
+    return tfg_cwbvh_build_ex(verts, n_tris, format, nullptr, out);
+}
+
+void tfg_cwbvh_default_build_options(tfg_cwbvh_build_options* o) {
+    if (!o) return;
+    const tfg::BuildOptions d;
+    o->node_cost = (float)d.node_cost;
+    o->tri_cost = (float)d.tri_cost;
+    o->max_leaf = (uint32_t)d.max_leaf;
+    o->bins = (uint32_t)d.bins;
+}
+
+int tfg_cwbvh_build_ex(const float* verts, uint32_t n_tris, uint32_t format, const tfg_cwbvh_build_options* opts,
+                       tfg_cwbvh_host** out) {
     if (out == nullptr) { tfg::set_error("tfg_cwbvh_build: out is NULL"); return TFG_E_INVALID; }
+    tfg::BuildOptions bo;
+    if (opts) {
+        if (opts->max_leaf < 1 || opts->max_leaf > 3 || opts->bins < 4 || opts->bins > 64 ||
+            !(opts->node_cost > 0.0f) || !(opts->tri_cost > 0.0f)) {
+            tfg::set_error("tfg_cwbvh_build: options out of range (max_leaf 1..3, bins 4..64, costs > 0)");
+            return TFG_E_INVALID;
+        }
+        bo.node_cost = opts->node_cost;
+        bo.tri_cost = opts->tri_cost;
+        bo.max_leaf = (int)opts->max_leaf;
+        bo.bins = (int)opts->bins;
+    }
     *out = nullptr;
     if (verts == nullptr || n_tris == 0) { tfg::set_error("tfg_cwbvh_build: no triangles"); return TFG_E_INVALID; }
     if (format != TFG_TRI_WATERTIGHT && format != TFG_TRI_AFFINE) {
@@ -66,7 +92,7 @@ int tfg_cwbvh_build(const float* verts, uint32_t n_tris, uint32_t format, tfg_cw
     }
     auto* h = new (std::nothrow) tfg_cwbvh_host();
     if (!h) { tfg::set_error("tfg_cwbvh_build: out of host memory"); return TFG_E_NOMEM; }
-    if (!tfg::build_cwbvh(verts, n_tris, format, h->b)) {
+    if (!tfg::build_cwbvh(verts, n_tris, format, h->b, bo)) {
         delete h;
         tfg::set_error("tfg_cwbvh_build: invalid input (non-finite vertex or too many triangles)");
         return TFG_E_INVALID;

