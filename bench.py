@@ -9,6 +9,10 @@ comparison of the cache-accelerated tracer against it.
 
 Workload ``pt``: the Mrays/s part alone.
 
+Workload ``nrc``: the cache's inference and training kernels alone, on the
+data of a 1080p cache frame (also part of the default run's JSON line as
+``nrc_kernels``).
+
 Workload ``copy``: a plain device-to-device copy of 1 GiB (torch ``copy_``),
 the calibration run for the HBM byte counters (FETCH_SIZE / WRITE_SIZE per
 dispatch of a kernel that moves a known number of bytes).
@@ -281,12 +285,85 @@ def bench_pt1080(args, torch):
                                  warmup=48, precision=0, schedule=res["config"]["launch"])
     _log(f"equal-time comparison: {json.dumps(cmp)} ({time.time() - t0:.1f}s)")
     res["relmse"] = cmp
+    t0 = time.time()
+    res["nrc_kernels"] = bench_nrc_kernels(torch, sc, res["config"]["launch"])
+    _log(f"cache kernels: {json.dumps(res['nrc_kernels'])} ({time.time() - t0:.1f}s)")
     res["config"]["workload"] = ("1080p path tracer (64 instances, 2.57M tris, 4 lights, Lambertian+GGX, depth 8, "
                                  "RR after 3): plain FP32 wavefront PT Mrays/s; equal-time relMSE of "
                                  "NRC-BF16 vs plain against a plain FP64-accumulated reference")
     res["config"]["relmse_reference_spp"] = args.ref_spp
     res["config"]["equal_time_budget_s"] = args.budget
     return res
+
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+NRC_INFER_FLOP = 41_344            # 2 x (5 x 64 x 64 + 64 x 3) per query
+NRC_TRAIN_FLOP = 115_840 + 41_344  # per training sample: forward + dX + dW, plus the tail inference
+
+
+def bench_nrc_kernels(torch, sc, schedule, reps=20):
+    """The cache's two matrix-core kernels on the frame's own data: a cache
+    tracer (BF16, 5% training paths) runs 48 training-only frames and 4
+    train+infer frames; then the last frame's queries are inferred and
+    composited (k_nrc_infer) `reps` times and 4 optimiser steps on its
+    records (k_nrc_train + reduction + Adam) are run `reps` times, each timed
+    with HIP events on the launch stream."""
+    from tfg_amd import experiments, nrc, pt
+
+    W, H, MD, RR = 1920, 1080, 8, 3
+    ds = pt.DeviceScene(sc, torch)
+    tr = pt.PathTracer(W, H, MD, RR, mode=experiments.pt_mode_train(), use_graph=False)
+    tr.set_scene(ds)
+    tr.set_camera(sc.camera.frame(W, H))
+    tr.set_schedule(**{k: v for k, v in schedule.items() if k in tr.schedule() and k != "use_graph"})
+    lo = sc.verts.reshape(-1, 3).min(0) - 0.5
+    hi = sc.verts.reshape(-1, 3).max(0) + 0.5
+    cache = nrc.RadianceCache(precision=0, bounds_min=tuple(map(float, lo)), bounds_max=tuple(map(float, hi)))
+    tr.set_cache(cache)
+    img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
+    for f in range(48):
+        tr.render(f, img)
+    tr.set_mode(experiments.pt_mode_train() | experiments.pt_mode_infer())
+    for f in range(4):
+        tr.render(48 + f, img)
+    torch.cuda.synchronize()
+    rec, qry = tr.read_cache_data()
+    n_rec, n_q = rec.shape[0], qry.shape[0]
+    r_ptr, q_ptr, c_ptr = tr.cache_buffers()
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    ms_inf = timed(lambda: cache.infer_queries(q_ptr, c_ptr + 4 * 4, img.data_ptr(), stream=sh))
+    steps = int(cache.cfg.steps_per_frame)
+    batch = int(cache.cfg.batch)
+    ms_train = timed(lambda: cache.train_ptr(r_ptr, c_ptr + 4 * 3, steps=steps, stream=sh))
+    inf_tf = n_q * NRC_INFER_FLOP / (ms_inf * 1e-3) / 1e12
+    tr_tf = steps * batch * NRC_TRAIN_FLOP / (ms_train * 1e-3) / 1e12
+    out = {
+        "precision": "bf16 operands, fp32 accumulation",
+        "inference": {"kernel": "k_nrc_infer (+ composite)", "queries": int(n_q), "ms": ms_inf,
+                      "flop_per_query": NRC_INFER_FLOP, "achieved_tflops": inf_tf,
+                      "mfma_frac": inf_tf / MFMA_BF16_PEAK_TFLOPS},
+        "training": {"kernels": "k_nrc_train + k_nrc_grad_reduce + k_nrc_adam", "steps": steps, "batch": batch,
+                     "records_available": int(n_rec), "ms": ms_train, "flop_per_sample": NRC_TRAIN_FLOP,
+                     "achieved_tflops": tr_tf, "mfma_frac": tr_tf / MFMA_BF16_PEAK_TFLOPS},
+        "peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+    }
+    tr.close()
+    cache.close()
+    return out
 
 
 def bench_copy(args, torch):
@@ -392,7 +469,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt1080", choices=["pt1080", "pt", "traversal", "copy"])
+    ap.add_argument("--workload", default="pt1080", choices=["pt1080", "pt", "traversal", "copy", "nrc"])
     ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the relMSE comparison")
     ap.add_argument("--budget", type=float, default=2.0, help="equal-time budget in seconds per method")
     ap.add_argument("--no-cpu", action="store_true")
@@ -406,6 +483,14 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "nrc":
+        from tfg_amd import scenes
+
+        sched = {"trace_block": 256, "grid_mult": 4, "tiles": 1, "side_streams": 2, "overlap": True,
+                 "xcd_group": 4096}
+        res = bench_nrc_kernels(torch, scenes.instances(), sched, reps=args.steps)
+        print(json.dumps({"metric": "cache kernel throughput (MFMA)", **res}), flush=True)
+        return
     if args.workload == "copy":
         res = bench_copy(args, torch)
         res.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True})
@@ -440,6 +525,8 @@ def main():
     }
     if "relmse" in res:
         out["relmse"] = res["relmse"]
+    if "nrc_kernels" in res:
+        out["nrc_kernels"] = res["nrc_kernels"]
     print(json.dumps(out), flush=True)
 
 

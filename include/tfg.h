@@ -130,20 +130,25 @@ int tfg_pt_counters(tfg_pt* pt, uint32_t* out);
  * including the last frame (synchronises). */
 int tfg_pt_ray_totals(tfg_pt* pt, uint64_t* out2);
 int tfg_pt_reset_totals(tfg_pt* pt);
-/* One frame launched stage by stage (no graph) with HIP events around every
- * launch and node/triangle visit counters in the traversal stages: for the
- * roofline of the path tracer's kernels.  Synchronises. */
+/* For the roofline of the path tracer's kernels: the frame is run twice
+ * with direct launches, first with node/triangle visit counters in the
+ * traversal stages, then with the plain kernels and HIP events around every
+ * launch (ms_* = sum over tiles of that stage's launch durations; ms_total =
+ * the whole frame).  Synchronises. */
 typedef struct tfg_pt_profile {
     float ms_generate, ms_finalize, ms_total;
     float ms_extend[32], ms_shade[32], ms_connect[32];
     uint32_t ext_rays[32], shadow_rays[32];
     uint64_t ext_nodes, ext_tris, shadow_nodes, shadow_tris;
+    /* start of each of tile 0's launches, ms after the frame's start */
+    float at_extend[32], at_shade[32], at_connect[32], at_finalize;
 } tfg_pt_profile;
 int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_profile* out);
 /* Times candidate schedules (tfg_pt_schedule) on the current scene and keeps
  * the fastest: first the traversal launch shape, then tiles, side streams and
- * overlap, then the XCD super-chunk size, all timed on direct launches (the
- * tuner captures no graph).  Drops any captured graph. */
+ * overlap, then the XCD super-chunk size, octet_below and connect_per_cu,
+ * all timed on direct launches (the tuner captures no graph).  Drops any
+ * captured graph. */
 int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream);
 int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_grid, uint32_t* shade_grid);
 /* How a frame is launched.  None of these change the results: every
@@ -161,6 +166,13 @@ int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_g
  *       grid stride; else a multiple of 256): consecutive queue entries are
  *       grouped so that each group is traced by the workgroups of one XCD
  *       (one L2), groups dealt to the 8 XCDs in turn;
+ *   octet_below: traversal launches of at most this many rays trace each
+ *       ray with 8 lanes (one per child slot, up to 8 triangles at once),
+ *       which shortens each ray's latency when too few rays fill the GPU
+ *       (shadow rays: only with at most 8 lights);
+ *   connect_per_cu (0..64): the shadow stage's persistent grid, workgroups
+ *       per CU (0: the closest-hit stages' grid); a small grid leaves CU
+ *       slots to the closest-hit chain, the frame's critical path;
  *   use_graph: replay captured hipGraphs (one per tile, plus one for the
  *       cache work) instead of launching the kernels directly; read-only,
  *       fixed by tfg_pt_config.use_graph (set_schedule must pass the same
@@ -169,6 +181,8 @@ typedef struct tfg_pt_schedule {
     uint32_t trace_block, grid_mult, tiles, side_streams;
     int32_t overlap, use_graph;
     uint32_t xcd_group;
+    uint32_t octet_below;
+    uint32_t connect_per_cu;
 } tfg_pt_schedule;
 int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out);
 int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* schedule);
@@ -219,6 +233,13 @@ int tfg_nrc_get_stats(tfg_nrc* nrc, tfg_nrc_stats* out);
  * pre-activations of all 6 layers (n x 6 x 64 floats). */
 int tfg_nrc_eval(tfg_nrc* nrc, const float* d_verts, uint32_t n, float* d_out, float* d_dump, int use_train_weights,
                  tfg_stream stream);
+/* The frame's cache inference on its own: predictions for cache queries (5
+ * float4 each: vertex record, then throughput and pixel index) whose count
+ * is read from device memory at *d_count, composited into d_image
+ * (image[pixel] += throughput * prediction, RenderKernel.cu:49-66).  Used
+ * to time the inference kernel; tfg_pt_render runs the same kernel. */
+int tfg_nrc_infer_queries(tfg_nrc* nrc, const float* d_queries, const uint32_t* d_count, tfg_color* d_image,
+                          tfg_stream stream);
 /* `steps` optimiser steps on training records (11 float4 each) whose count
  * is read from device memory at *d_n_records. */
 int tfg_nrc_train(tfg_nrc* nrc, const float* d_records, const uint32_t* d_n_records, uint32_t steps,

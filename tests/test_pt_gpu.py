@@ -117,27 +117,30 @@ def test_pt_graph_matches_eager_and_replays(torch_cuda):
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_pt_schedules_match_serial(torch_cuda, use_graph):
     """Shadow rays on 1-3 side streams (overlapping the following depths), the
-    frame split into 2 or 4 pixel tiles on their own streams and any XCD
-    super-chunk size give the same image bit for bit as one serial chain, for
+    frame split into 2 or 4 pixel tiles on their own streams, any XCD
+    super-chunk size and 8-lane (octet) closest-hit traversal for all or some
+    launches give the same image bit for bit as one serial chain, for
     plain and cache-capture frames, with the same ray counts and the same
     captured records and queries."""
     torch = torch_cuda
     sc = scenes.instances(n_inst=8, sphere_res=(32, 16), box_res=4, seed=5)
     W, H = 96, 64
     ds = pt.DeviceScene(sc, torch)
-    schedules = [(1, False, 1, 0), (1, True, 1, 4096), (1, True, 2, 256), (2, True, 4, 1024), (4, False, 1, 0),
-                 (4, True, 3, 16384)]
+    big = 1 << 30
+    schedules = [(1, False, 1, 0, 0), (1, True, 1, 4096, 0), (1, True, 2, 256, big), (2, True, 4, 1024, 2000),
+                 (4, False, 1, 0, big), (4, True, 3, 16384, 0)]
     for mode in (0, 2):
         imgs, counts, caps = [], [], []
-        for tiles, overlap, sides, xcd in schedules:
-            print(f"mode={mode} tiles={tiles} overlap={overlap} sides={sides} xcd_group={xcd}", flush=True)
+        for tiles, overlap, sides, xcd, octet in schedules:
+            print(f"mode={mode} tiles={tiles} overlap={overlap} sides={sides} xcd_group={xcd} octet_below={octet}",
+                  flush=True)
             tr = pt.PathTracer(W, H, 8, 3, mode=mode, use_graph=use_graph)
             tr.set_scene(ds)
             tr.set_camera(sc.camera.frame(W, H))
-            tr.set_schedule(overlap=overlap, tiles=tiles, side_streams=sides, xcd_group=xcd)
+            tr.set_schedule(overlap=overlap, tiles=tiles, side_streams=sides, xcd_group=xcd, octet_below=octet)
             shape = tr.launch_shape()
             assert shape["overlap"] == overlap and shape["tiles"] == tiles and shape["side_streams"] == sides
-            assert shape["xcd_group"] == xcd
+            assert shape["xcd_group"] == xcd and shape["octet_below"] == octet
             assert shape["use_graph"] == use_graph
             img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
             out = []

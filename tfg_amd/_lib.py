@@ -56,13 +56,14 @@ class PtProfile(C.Structure):
                 ("ms_extend", C.c_float * 32), ("ms_shade", C.c_float * 32), ("ms_connect", C.c_float * 32),
                 ("ext_rays", C.c_uint32 * 32), ("shadow_rays", C.c_uint32 * 32),
                 ("ext_nodes", C.c_uint64), ("ext_tris", C.c_uint64), ("shadow_nodes", C.c_uint64),
-                ("shadow_tris", C.c_uint64)]
+                ("shadow_tris", C.c_uint64), ("at_extend", C.c_float * 32), ("at_shade", C.c_float * 32),
+                ("at_connect", C.c_float * 32), ("at_finalize", C.c_float)]
 
 
 class PtSchedule(C.Structure):
     _fields_ = [("trace_block", C.c_uint32), ("grid_mult", C.c_uint32), ("tiles", C.c_uint32),
                 ("side_streams", C.c_uint32), ("overlap", C.c_int32), ("use_graph", C.c_int32),
-                ("xcd_group", C.c_uint32)]
+                ("xcd_group", C.c_uint32), ("octet_below", C.c_uint32), ("connect_per_cu", C.c_uint32)]
 
 
 class NrcConfig(C.Structure):
@@ -138,6 +139,7 @@ _SIGS = {
     "tfg_nrc_get_weights": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tfg_nrc_get_stats": (C.c_int, [C.c_void_p, C.POINTER(NrcStats)]),
     "tfg_nrc_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "tfg_nrc_infer_queries": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "tfg_nrc_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "tfg_nrc_last_gradient": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tfg_nrc_infer_datos": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),

@@ -669,6 +669,14 @@ int tfg_nrc_eval(tfg_nrc* nc, const float* d_verts, uint32_t n, float* d_out, fl
     return TFG_OK;
 }
 
+int tfg_nrc_infer_queries(tfg_nrc* nc, const float* d_queries, const uint32_t* d_count, tfg_color* d_image,
+                          tfg_stream stream) {
+    if (!nc || !d_queries || !d_count || !d_image) { set_error("tfg_nrc_infer_queries: NULL argument"); return TFG_E_INVALID; }
+    const float4* q = reinterpret_cast<const float4*>(d_queries);
+    return nc->cfg.precision == TFG_NRC_FP16 ? enqueue_infer<_Float16>(nc, q, d_count, d_image, stream)
+                                             : enqueue_infer<__bf16>(nc, q, d_count, d_image, stream);
+}
+
 int tfg_nrc_train(tfg_nrc* nc, const float* d_records, const uint32_t* d_n_records, uint32_t steps, tfg_stream stream) {
     if (!nc || !d_records || !d_n_records) { set_error("tfg_nrc_train: NULL argument"); return TFG_E_INVALID; }
     const float4* r = reinterpret_cast<const float4*>(d_records);

@@ -24,6 +24,7 @@ struct tfg_pt {
     int trace_block = 128;
     int trace_mult = 1;
     int trace_grid = 0;
+    int connect_per_cu = 0;   // shadow-stage workgroups per CU (0: the closest-hit grid)
     int shade_grid = 0;
     int gen_grid = 0;
     // pixel tiles: each runs its own chain of stages on its own streams
@@ -127,7 +128,8 @@ void launch_extend(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t q, uin
 
 template <bool kStats>
 void launch_connect(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t d) {
-    const dim3 g(pt->trace_grid);
+    // a bounded persistent grid leaves CU slots to the closest-hit chain
+    const dim3 g(pt->connect_per_cu > 0 ? pt->n_cu * pt->connect_per_cu : pt->trace_grid);
     const size_t lds = trace_lds(pt, pt->trace_block);
     switch (pt->trace_block) {
         case 64: hipLaunchKernelGGL((k_connect<64, kStats>), g, dim3(64), lds, s, T, d); break;
@@ -232,7 +234,7 @@ inline void mark(hipEvent_t e, hipStream_t s) {
 // shade(d), overlapping extend/shade of the following depths (without
 // overlap it runs on ts); ts waits for the last connect before finalize.
 int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, hipStream_t* side, tfg_color* image, double* accum,
-                 StageEvents* pe) {
+                 StageEvents* pe, bool stats) {
     const bool nrc = pt->P.mode != 0;
     const int sides = pt->overlap ? pt->sides : 0;
     const PtParams& T = pt->TP[t];
@@ -242,7 +244,7 @@ int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, hipStream_t* side, tfg_color
     for (uint32_t d = 0; d < pt->cfg.max_depth; ++d) {
         const uint32_t q = d & 1u;
         mark(pe ? pe->ext[t][d][0] : nullptr, ts);
-        if (pe) launch_extend<true>(pt, ts, T, q, d);
+        if (stats) launch_extend<true>(pt, ts, T, q, d);
         else launch_extend<false>(pt, ts, T, q, d);
         mark(pe ? pe->ext[t][d][1] : nullptr, ts);
         mark(pe ? pe->shd[t][d][0] : nullptr, ts);
@@ -255,7 +257,7 @@ int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, hipStream_t* side, tfg_color
             TFG_HIP_CHECK(hipStreamWaitEvent(cs, pt->ev_shaded[t][d], 0));
         }
         mark(pe ? pe->con[t][d][0] : nullptr, cs);
-        if (pe) launch_connect<true>(pt, cs, T, d);
+        if (stats) launch_connect<true>(pt, cs, T, d);
         else launch_connect<false>(pt, cs, T, d);
         mark(pe ? pe->con[t][d][1] : nullptr, cs);
     }
@@ -305,12 +307,13 @@ int join_tiles(tfg_pt* pt, hipStream_t s) {
 // their streams, the cache work on s after every tile.  Tiles are
 // independent, so the latency tails of their small late-depth launches
 // overlap.
-int enqueue_frame(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum, StageEvents* pe = nullptr) {
+int enqueue_frame(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum, StageEvents* pe = nullptr,
+                  bool stats = false) {
     make_tiles(pt);
     mark(pe ? pe->frame[0] : nullptr, s);
     int rc = fork_tiles(pt, s);
     for (int t = 0; t < pt->tiles && rc == TFG_OK; ++t)
-        rc = enqueue_tile(pt, t, pt->tstream[t], pt->tside[t], image, accum, pe);
+        rc = enqueue_tile(pt, t, pt->tstream[t], pt->tside[t], image, accum, pe, stats);
     if (rc == TFG_OK) rc = join_tiles(pt, s);
     if (rc == TFG_OK) rc = enqueue_post(pt, s, image, accum);
     mark(pe ? pe->frame[1] : nullptr, s);
@@ -325,7 +328,7 @@ int capture_graphs(tfg_pt* pt, tfg_color* image, double* accum) {
     for (int t = 0; t < pt->tiles; ++t) {
         hipStream_t ts = pt->cap_tile[t];
         TFG_HIP_CHECK(hipStreamBeginCapture(ts, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_tile(pt, t, ts, pt->cap_side[t], image, accum, nullptr);
+        const int rc = enqueue_tile(pt, t, ts, pt->cap_side[t], image, accum, nullptr, false);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(ts, &g);
         if (rc != TFG_OK || e != hipSuccess) {
@@ -412,6 +415,7 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
     P.pix_base = 0;
     P.depth_stride = n;
     P.xcd_group = 4096;
+    P.octet_below = 1u << 18;
     P.max_depth = cfg->max_depth;
     P.rr_depth = cfg->rr_depth;
     P.mode = cfg->mode;
@@ -625,8 +629,16 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
     StageEvents pe;
     int rc = pe.create((uint32_t)pt->tiles, pt->cfg.max_depth);
     if (rc != TFG_OK) return rc;
+    // the frame twice: once with the counting kernels (node / triangle
+    // visits), then with the plain kernels, timed
     hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(1), 0, s, const_cast<int32_t*>(pt->P.frame), frame);
-    rc = enqueue_frame(pt, s, d_image, nullptr, &pe);
+    rc = enqueue_frame(pt, s, d_image, nullptr, nullptr, true);
+    uint64_t st[4] = {0, 0, 0, 0};
+    if (rc == TFG_OK) {
+        TFG_HIP_CHECK(hipStreamSynchronize(s));
+        TFG_HIP_CHECK(hipMemcpy(st, pt->P.counters + CNT_STAT, sizeof(st), hipMemcpyDeviceToHost));
+        rc = enqueue_frame(pt, s, d_image, nullptr, &pe, false);
+    }
     if (rc == TFG_OK) {
         TFG_HIP_CHECK(hipStreamSynchronize(s));
         auto span = [](hipEvent_t a, hipEvent_t b) {
@@ -644,6 +656,12 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
             }
             out->ms_finalize += span(pe.fin[t][0], pe.fin[t][1]);
         }
+        for (uint32_t d = 0; d < pt->cfg.max_depth; ++d) {
+            out->at_extend[d] = span(pe.frame[0], pe.ext[0][d][0]);
+            out->at_shade[d] = span(pe.frame[0], pe.shd[0][d][0]);
+            out->at_connect[d] = span(pe.frame[0], pe.con[0][d][0]);
+        }
+        out->at_finalize = span(pe.frame[0], pe.fin[0][0]);
         out->ms_total = span(pe.frame[0], pe.frame[1]);
         uint32_t c[CNT_ALLOC];
         TFG_HIP_CHECK(hipMemcpy(c, pt->P.counters, sizeof(c), hipMemcpyDeviceToHost));
@@ -651,8 +669,6 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
             out->ext_rays[d] = c[CNT_EXT + d];
             out->shadow_rays[d] = c[CNT_SHD + d];
         }
-        uint64_t st[4];
-        std::memcpy(st, c + CNT_STAT, sizeof(st));
         out->ext_nodes = st[0];
         out->ext_tris = st[1];
         out->shadow_nodes = st[2];
@@ -687,15 +703,18 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
         return ms_tot;
     };
     // launch shape of the traversal stages, with one tile and overlap on
+    // large multipliers give more workgroups than chunks: each then traces
+    // one chunk and retires, which lets launches on other streams start
+    // while a long launch is still running
     const int blocks[3] = {64, 128, 256};
-    const int mults[3] = {1, 2, 4};
+    const int mults[4] = {1, 4, 16, 64};
     float best = 1e30f;
     int bb = 128, bm = 1;
     pt->tiles = 1;
     pt->sides = 2;
     pt->overlap = true;
     for (int bi = 0; bi < 3 && rc == TFG_OK; ++bi) {
-        for (int mi = 0; mi < 3 && rc == TFG_OK; ++mi) {
+        for (int mi = 0; mi < 4 && rc == TFG_OK; ++mi) {
             set_trace_shape(pt, blocks[bi], mults[mi]);
             const float ms = time_frames();
             if (ms < best) {
@@ -771,6 +790,46 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     for (int gi = 1; gi < 4; ++gi)
         if (group_ms[gi] < group_ms[bgi]) bgi = gi;
     pt->P.xcd_group = groups[bgi];
+    // then the ray count below which closest-hit launches trace 8 lanes per ray
+    const uint32_t octs[4] = {0, 1u << 16, 1u << 18, 1u << 20};
+    float oct_ms[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+    for (int pass = 0; pass < 2 && rc == TFG_OK; ++pass) {
+        for (int oi = 0; oi < 4 && rc == TFG_OK; ++oi) {
+            pt->P.octet_below = octs[oi];
+            for (int w = 0; w < 2 && rc == TFG_OK; ++w) rc = enqueue_frame(pt, pt->stream, img, nullptr);
+            (void)hipEventRecord(a, pt->stream);
+            for (int rep = 0; rep < 6 && rc == TFG_OK; ++rep) rc = enqueue_frame(pt, pt->stream, img, nullptr);
+            (void)hipEventRecord(b, pt->stream);
+            (void)hipEventSynchronize(b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            if (ms < oct_ms[oi]) oct_ms[oi] = ms;
+        }
+    }
+    int boi = 0;
+    for (int oi = 1; oi < 4; ++oi)
+        if (oct_ms[oi] < oct_ms[boi]) boi = oi;
+    pt->P.octet_below = octs[boi];
+    // then the shadow stage's grid
+    const int cpcs[4] = {0, 1, 2, 3};
+    float cpc_ms[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+    for (int pass = 0; pass < 2 && rc == TFG_OK; ++pass) {
+        for (int ci = 0; ci < 4 && rc == TFG_OK; ++ci) {
+            pt->connect_per_cu = cpcs[ci];
+            for (int w = 0; w < 2 && rc == TFG_OK; ++w) rc = enqueue_frame(pt, pt->stream, img, nullptr);
+            (void)hipEventRecord(a, pt->stream);
+            for (int rep = 0; rep < 6 && rc == TFG_OK; ++rep) rc = enqueue_frame(pt, pt->stream, img, nullptr);
+            (void)hipEventRecord(b, pt->stream);
+            (void)hipEventSynchronize(b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            if (ms < cpc_ms[ci]) cpc_ms[ci] = ms;
+        }
+    }
+    int bci = 0;
+    for (int ci = 1; ci < 4; ++ci)
+        if (cpc_ms[ci] < cpc_ms[bci]) bci = ci;
+    pt->connect_per_cu = cpcs[bci];
     drop_graph(pt);
     pt->P.mode = saved_mode;
     (void)hipEventDestroy(a);
@@ -797,6 +856,8 @@ int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out) {
     out->overlap = pt->overlap ? 1 : 0;
     out->use_graph = pt->run_graph ? 1 : 0;
     out->xcd_group = pt->P.xcd_group;
+    out->octet_below = pt->P.octet_below;
+    out->connect_per_cu = (uint32_t)pt->connect_per_cu;
     return TFG_OK;
 }
 
@@ -805,9 +866,10 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
     const uint32_t b = sc->trace_block;
     if ((b != 64 && b != 128 && b != 256) || sc->grid_mult == 0 || sc->grid_mult > 64 || sc->tiles == 0 ||
         sc->tiles > kMaxTiles || sc->side_streams == 0 || sc->side_streams > kMaxSide ||
-        (sc->xcd_group != 0 && (sc->xcd_group % 256 != 0 || sc->xcd_group > (1u << 20)))) {
+        (sc->xcd_group != 0 && (sc->xcd_group % 256 != 0 || sc->xcd_group > (1u << 20))) ||
+        sc->connect_per_cu > 64) {
         set_error("tfg_pt_set_schedule: block 64/128/256, grid_mult 1..64, tiles 1..%u, side streams 1..%u, "
-                  "xcd_group 0 or a multiple of 256 up to 2^20", kMaxTiles, kMaxSide);
+                  "xcd_group 0 or a multiple of 256 up to 2^20, connect_per_cu 0..64", kMaxTiles, kMaxSide);
         return TFG_E_INVALID;
     }
     if ((sc->use_graph != 0) != pt->run_graph) {
@@ -820,6 +882,8 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
     pt->sides = (int)sc->side_streams;
     pt->overlap = sc->overlap != 0;
     pt->P.xcd_group = sc->xcd_group;
+    pt->P.octet_below = sc->octet_below;
+    pt->connect_per_cu = (int)sc->connect_per_cu;
     return TFG_OK;
 }
 

@@ -58,6 +58,7 @@ struct PtParams {
     uint32_t n_top;          // leading nodes staged in LDS by the traversal stages
     int stack_depth;         // LDS stack entries per lane (tree depth - 1)
     uint32_t xcd_group;      // rays per XCD super-chunk of the traversal stages (0: plain grid stride)
+    uint32_t octet_below;    // traversal launches with at most this many rays trace 8 lanes per ray
     const tfg_tri_shading* shading;
     const tfg_material* mats;
     const tfg_point_light* lights;

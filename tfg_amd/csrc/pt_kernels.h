@@ -107,14 +107,48 @@ __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint3
     extern __shared__ __align__(16) unsigned char lds[];
     float4* top = reinterpret_cast<float4*>(lds);
     uint2* stack = reinterpret_cast<uint2*>(top + 5 * P.n_top);
-    stage_top_nodes(P, top);
     const uint32_t count = P.counters[q];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (count) atomicAdd(&P.gcounters[CNT_EXT + depth], count);
         P.counters[q ^ 1u] = 0;
     }
     const float4* rays = P.rays[q];
-    for (ChunkIter it((count + kBlock - 1) / kBlock, P.xcd_group / kBlock, P.xcd_group != 0); it.valid(); it.next()) {
+    // few rays: 8 lanes per ray (traverse_octet), kBlock / 8 rays per chunk
+    const bool octet = count <= P.octet_below;
+    const uint32_t per_chunk = octet ? kBlock / 8 : kBlock;
+    ChunkIter it((count + per_chunk - 1) / per_chunk, P.xcd_group / per_chunk, P.xcd_group != 0);
+    if (!it.valid()) return;   // workgroups without a chunk leave before staging
+    stage_top_nodes(P, top);
+    if (octet) {
+        constexpr uint32_t kRays = kBlock / 8;
+        const uint32_t j = threadIdx.x & 7u;
+        for (; it.valid(); it.next()) {
+            const uint32_t i = it.chunk * kRays + (threadIdx.x >> 3);
+            TraceStats st;
+            if (i < count) {
+                const float4 a = rays[2 * (size_t)i];
+                const float4 b = rays[2 * (size_t)i + 1];
+                const RayPre r = make_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
+                float t, u, v;
+                uint32_t prim;
+                traverse_octet<false>(P.nodes, P.tris, top, P.n_top, r, stack + (threadIdx.x >> 3), kRays,
+                                      P.stack_depth, t, u, v, prim, kStats ? &st : nullptr);
+                for (uint32_t k = 0; k < P.n_analytic; ++k) {   // escenario.h:323-334
+                    const float ta = analytic_t(P.analytic[k], v3(a.x, a.y, a.z), v3(b.x, b.y, b.z));
+                    if (ta > a.w && ta < t) {
+                        t = ta;
+                        u = 0.0f;
+                        v = 0.0f;
+                        prim = kAnalyticBit | k;
+                    }
+                }
+                if (j == 0) P.hits[i] = make_float4(t, u, v, __uint_as_float(prim));
+            }
+            if (kStats) add_stats(P, 0, st);
+        }
+        return;
+    }
+    for (; it.valid(); it.next()) {
         const uint32_t b0 = it.chunk * kBlock;
         const uint32_t i = b0 + threadIdx.x;
         TraceStats st;
@@ -374,8 +408,9 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
 
 // ---------------------------------------------------------------- connect
 // Shadow rays of the NEE requests made at one depth (escenario.h:144-184).
-// Each entry gets a group of 2^light_shift adjacent lanes, one per light, so
-// the rays of one shading point are traced side by side rather than one after
+// Each entry gets a group of 2^light_shift adjacent lanes, one per light (or
+// 8 lanes per light when the launch has few rays, see octet_below), so the
+// rays of one shading point are traced side by side rather than one after
 // another; the group's first lane then sums the unoccluded contributions in
 // light order (the order of the reference's loop).  The stage reads only the
 // shadow entries of its depth and writes only nee[depth][pid], so connect of
@@ -386,21 +421,28 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) 
     extern __shared__ __align__(16) unsigned char lds[];
     float4* top = reinterpret_cast<float4*>(lds);
     uint2* stack = reinterpret_cast<uint2*>(top + 5 * P.n_top);
-    stage_top_nodes(P, top);
     const uint32_t count = P.counters[CNT_SHQ + depth];
     const float4* entries = P.shadow + 4 * (size_t)depth * P.depth_stride;
     float4* nee = P.nee + (size_t)depth * P.depth_stride;
     const uint32_t shift = P.light_shift;
     const uint32_t group = 1u << shift;
-    const uint64_t work = (uint64_t)count << shift;
+    // few shadow rays: 8 lanes per ray (traverse_octet); needs <= 8 lights so
+    // that an entry's lanes stay within one wave
+    const uint32_t os = ((uint64_t)count << shift) <= P.octet_below && shift <= 3 ? 3u : 0u;
+    const uint32_t sh = shift + os;
+    const uint64_t work = (uint64_t)count << sh;
     const uint32_t lane = __lane_id();
-    const int lead = (int)(lane & ~(group - 1u));
-    for (ChunkIter it((uint32_t)((work + kBlock - 1) / kBlock), P.xcd_group / kBlock, P.xcd_group != 0); it.valid();
-         it.next()) {
+    const int lead = (int)(lane & ~((1u << sh) - 1u));
+    const uint32_t rays_per_chunk = kBlock >> os;
+    ChunkIter it((uint32_t)((work + kBlock - 1) / kBlock), P.xcd_group / rays_per_chunk, P.xcd_group != 0);
+    if (!it.valid()) return;   // workgroups without a chunk leave before staging
+    stage_top_nodes(P, top);
+    for (; it.valid(); it.next()) {
         const uint64_t b = (uint64_t)it.chunk * kBlock;
         const uint64_t w = b + threadIdx.x;
-        const uint32_t i = (uint32_t)(w >> shift);
-        const uint32_t l = (uint32_t)w & (group - 1u);
+        const uint32_t i = (uint32_t)(w >> sh);
+        const uint32_t l = (uint32_t)(w >> os) & (group - 1u);
+        const uint32_t j = (uint32_t)w & ((1u << os) - 1u);
         uint32_t traced = 0;
         TraceStats st;
         V3 c = v3(0.f, 0.f, 0.f);
@@ -418,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) 
                 float dist;
                 if (light_term(P.lights[l], ps, n, xyz(e2), m, contrib, dir, dist)) {
                     const V3 o = ps + dir * kEps;
-                    traced = 1;
+                    traced = j == 0 ? 1u : 0u;
                     bool occ = false;
                     for (uint32_t k = 0; k < P.n_analytic && !occ; ++k) {   // escenario.h:255-266
                         const float ta = analytic_t(P.analytic[k], o, dir);
@@ -428,20 +470,25 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) 
                         const RayPre r = make_ray(o.x, o.y, o.z, dir.x, dir.y, dir.z, 0.0f, dist);
                         float t, u, v;
                         uint32_t prim;
-                        occ = traverse<0, true>(P.nodes, P.tris, top, P.n_top, r, stack + threadIdx.x, kBlock,
-                                                P.stack_depth, t, u, v, prim, kStats ? &st : nullptr);
+                        if (os)
+                            occ = traverse_octet<true>(P.nodes, P.tris, top, P.n_top, r, stack + (threadIdx.x >> 3),
+                                                       kBlock / 8, P.stack_depth, t, u, v, prim,
+                                                       kStats ? &st : nullptr);
+                        else
+                            occ = traverse<0, true>(P.nodes, P.tris, top, P.n_top, r, stack + threadIdx.x, kBlock,
+                                                    P.stack_depth, t, u, v, prim, kStats ? &st : nullptr);
                     }
                     if (!occ) c = contrib;
                 }
             }
         }
-        // ordered sum over the group's lights (wave-uniform trip count)
+        // ordered sum over the entry's lights (wave-uniform trip count)
         V3 sum = v3(0.f, 0.f, 0.f);
         for (uint32_t k = 0; k < group; ++k) {
-            const int src = lead + (int)k;
+            const int src = lead + (int)(k << os);
             sum = sum + v3(__shfl(c.x, src), __shfl(c.y, src), __shfl(c.z, src));
         }
-        if (w < work && l == 0) nee[__float_as_uint(e0.w)] = f4(xyz(e3) * sum, 0.0f);
+        if (w < work && l == 0 && j == 0) nee[__float_as_uint(e0.w)] = f4(xyz(e3) * sum, 0.0f);
         const uint32_t tot = wave_sum(traced);
         if (lane == 0 && tot) atomicAdd(&P.gcounters[CNT_SHD + depth], tot);
         if (kStats) add_stats(P, 2, st);

@@ -322,4 +322,151 @@ __device__ __forceinline__ bool traverse(const float4* __restrict__ nodes, const
     return found;
 }
 
+// ------------------------------------------------------------------------
+// Group traversal: 8 adjacent lanes (an aligned octet of the wave) trace one
+// ray together.  Lane j of the octet tests child slot j of each node and, per
+// round, the j-th triangle of the node's pending leaf triangles; the octet
+// combines the child bits by OR and the triangle hits by a (t, primIdx)
+// minimum over shuffles.  The octet's traversal state is uniform (every lane
+// holds the same copy); its stack lives in LDS, stack_depth entries of
+// `stride` uint2, written identically by all 8 lanes.  A ray's sequence of
+// dependent steps is the same as in traverse(), but each step does an eighth
+// of the box tests and up to 8 triangle tests at once, so a ray's latency is
+// shorter: the form for launches with too few rays to fill the GPU.
+// Results (t, u, v, primIdx; occlusion) equal traverse()'s: the accept rule
+// does not depend on the test order.
+__device__ __forceinline__ uint32_t octet_or(uint32_t x) {
+    x |= __shfl_xor(x, 1);
+    x |= __shfl_xor(x, 2);
+    x |= __shfl_xor(x, 4);
+    return x;
+}
+
+template <bool kAnyHit>
+__device__ __forceinline__ bool traverse_octet(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                               const float4* __restrict__ top_nodes, uint32_t n_top, const RayPre& r,
+                                               uint2* stack, int stride, int stack_depth, float& t_out, float& u_out,
+                                               float& v_out, uint32_t& prim_out, TraceStats* stats) {
+    const uint32_t j = __lane_id() & 7u;
+    int sp = 0;
+    uint32_t ng_x = 0, ng_y = 0x80000000u;
+    uint32_t tg_x = 0, tg_y = 0;
+    float tcur = r.tmax;
+    uint32_t best_prim = 0xFFFFFFFFu;
+    float bu = 0.f, bv = 0.f;
+    bool found = false;
+    for (;;) {
+        if (ng_y <= 0x00FFFFFFu && sp > 0) {
+            --sp;
+            const uint2 e = stack[sp * stride];
+            ng_x = e.x;
+            ng_y = e.y;
+        }
+        const bool have = ng_y > 0x00FFFFFFu;
+        float4 n0, n1, n2, n3, n4;
+        if (have) {
+            const uint32_t imask = ng_y;
+            const int bit = highest_bit(ng_y & 0xFF000000u);
+            ng_y &= ~(1u << bit);
+            if (ng_y > 0x00FFFFFFu) {
+                if (sp < stack_depth) {
+                    stack[sp * stride] = make_uint2(ng_x, ng_y);
+                    ++sp;
+                } else if (stats && j == 0) {
+                    stats->overflow = 1;
+                }
+            }
+            const uint32_t slot = ((uint32_t)bit - 24u) ^ r.octinv;
+            const uint32_t rel = (uint32_t)__popc(imask & ~(0xFFFFFFFFu << slot));
+            const uint32_t ni = ng_x + rel;
+            const float4* n = ni < n_top ? top_nodes + 5 * ni : nodes + 5 * (size_t)ni;
+            n0 = n[0];
+            n1 = n[1];
+            n2 = n[2];
+            n3 = n[3];
+            n4 = n[4];
+        }
+        // triangles of the previous node, up to 8 per round: lane j takes
+        // the (j+1)-th highest pending bit
+        while (tg_y != 0) {
+            uint32_t m = tg_y;
+            for (uint32_t k = 0; k < j && m != 0; ++k) m &= ~(1u << highest_bit(m));
+            const bool mine = m != 0;
+            bool ok = false;
+            float t = 0.f, u = 0.f, v = 0.f;
+            uint32_t prim = 0xFFFFFFFFu;
+            if (mine) {
+                const int b = highest_bit(m);
+                const float4* T = tris + (size_t)(tg_x + 4u * (uint32_t)b);
+                if (stats) stats->tris++;
+                ok = watertight(T[0], T[1], T[2], r, t, u, v) && t > r.tmin && t <= tcur;
+                if (ok) prim = __float_as_uint(T[3].w);
+            }
+            // drop the (up to) 8 highest pending bits
+            for (int k = 0; k < 8 && tg_y != 0; ++k) tg_y &= ~(1u << highest_bit(tg_y));
+            if (kAnyHit) {
+                if (octet_or((ok && t < r.tmax) ? 1u : 0u)) return true;
+                continue;
+            }
+            // octet minimum of (t, primIdx) over accepted candidates
+            float bt = ok ? t : __int_as_float(0x7f800000);
+            uint32_t bp = ok ? prim : 0xFFFFFFFFu;
+            float uu = u, vv = v;
+#pragma unroll
+            for (int o = 1; o < 8; o <<= 1) {
+                const float ot = __shfl_xor(bt, o);
+                const uint32_t op = __shfl_xor(bp, o);
+                const float ou = __shfl_xor(uu, o), ov = __shfl_xor(vv, o);
+                if (ot < bt || (ot == bt && op < bp)) {
+                    bt = ot;
+                    bp = op;
+                    uu = ou;
+                    vv = ov;
+                }
+            }
+            if (bp != 0xFFFFFFFFu && (bt < tcur || bp < best_prim)) {
+                tcur = bt;
+                bu = uu;
+                bv = vv;
+                best_prim = bp;
+                found = true;
+            }
+        }
+        if (!have) break;
+        if (stats && j == 0) stats->nodes++;
+        // lane j: child slot j
+        const uint32_t w = __float_as_uint(n0.w);
+        const int ex = (int)(int8_t)(w & 0xFFu), ey = (int)(int8_t)((w >> 8) & 0xFFu),
+                  ez = (int)(int8_t)((w >> 16) & 0xFFu);
+        const float idirx = __uint_as_float((uint32_t)(ex + 127) << 23) * r.rD[0];
+        const float idiry = __uint_as_float((uint32_t)(ey + 127) << 23) * r.rD[1];
+        const float idirz = __uint_as_float((uint32_t)(ez + 127) << 23) * r.rD[2];
+        const float origx = (n0.x - r.O[0]) * r.rD[0];
+        const float origy = (n0.y - r.O[1]) * r.rD[1];
+        const float origz = (n0.z - r.O[2]) * r.rD[2];
+        const int h = (int)(j >> 2), jj = (int)(j & 3u);
+        const uint32_t meta = ((h ? __float_as_uint(n1.w) : __float_as_uint(n1.z)) >> (8 * jj)) & 0xFFu;
+        const uint32_t lx = h ? __float_as_uint(n2.y) : __float_as_uint(n2.x);
+        const uint32_t ly = h ? __float_as_uint(n2.w) : __float_as_uint(n2.z);
+        const uint32_t lz = h ? __float_as_uint(n3.y) : __float_as_uint(n3.x);
+        const uint32_t hx = h ? __float_as_uint(n3.w) : __float_as_uint(n3.z);
+        const uint32_t hy = h ? __float_as_uint(n4.y) : __float_as_uint(n4.x);
+        const uint32_t hz = h ? __float_as_uint(n4.w) : __float_as_uint(n4.z);
+        const uint32_t bits = child_bits(meta, ubyte_f(lx, jj), ubyte_f(ly, jj), ubyte_f(lz, jj), ubyte_f(hx, jj),
+                                         ubyte_f(hy, jj), ubyte_f(hz, jj), idirx, idiry, idirz, origx, origy, origz,
+                                         r, tcur);
+        const uint32_t hm = octet_or(bits);
+        ng_x = __float_as_uint(n1.x);
+        ng_y = (hm & 0xFF000000u) | (__float_as_uint(n0.w) >> 24);
+        tg_x = __float_as_uint(n1.y);
+        tg_y = hm & 0x00FFFFFFu;
+    }
+    if (kAnyHit) return false;
+    t_out = tcur;
+    u_out = found ? bu : 0.f;
+    v_out = found ? bv : 0.f;
+    prim_out = found ? best_prim : 0xFFFFFFFFu;
+    return found;
+}
+
 }  // namespace tfg

@@ -64,6 +64,15 @@ class RadianceCache:
                                  dump_dev.data_ptr() if dump_dev is not None else None, 1 if train_weights else 0,
                                  stream), "tfg_nrc_eval")
 
+    def infer_queries(self, queries_ptr, count_ptr, image_ptr, stream=None):
+        """Cache inference + composite over raw device pointers (the tracer's
+        query buffer and counter, an image)."""
+        check(lib().tfg_nrc_infer_queries(self._h, queries_ptr, count_ptr, image_ptr, stream),
+              "tfg_nrc_infer_queries")
+
+    def train_ptr(self, records_ptr, count_ptr, steps=1, stream=None):
+        check(lib().tfg_nrc_train(self._h, records_ptr, count_ptr, steps, stream), "tfg_nrc_train")
+
     def train(self, records_dev, n_records_dev, steps=1, stream=None):
         check(lib().tfg_nrc_train(self._h, records_dev.data_ptr(), n_records_dev.data_ptr(), steps, stream),
               "tfg_nrc_train")
