@@ -189,6 +189,17 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* schedule);
 int tfg_pt_destroy(tfg_pt* pt);
 
 
+/* ---- frame output ----------------------------------------------------------
+ * Replaces the host loop of main.cpp:1070-1071 / 1109-1115 and the byte
+ * conversion of guardarPNG (imagen_utils.h:58-62): per pixel, the mean of
+ * an FP64 accumulation buffer (3 doubles per pixel, as tfg_pt_render
+ * accumulates) over `samples` frames, optionally the ACES filmic curve of
+ * Imagen::filmic (imagen.h:194-215), then round(clamp(v, 0, 1) * 255) into
+ * d_rgb8 (3 bytes per pixel, nullable) and the FP32 mean into d_mean
+ * (nullable).  tfg_amd/image.py writes the bytes as an 8-bit RGB PNG. */
+int tfg_image_resolve(const double* d_accum, uint32_t n_pixels, uint32_t samples, int tonemap, uint8_t* d_rgb8,
+                      float* d_mean, tfg_stream stream);
+
 /* ---- radiance cache (NRC) -------------------------------------------------
  * Replaces ColorMLP (mlp.h:19-70, mlp.cpp:16-170) and the NRC data kernels
  * (NrcKernel.cu, RenderKernel.cu:92-166).  64 encoded inputs -> 5 x 64 ReLU

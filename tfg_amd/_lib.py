@@ -139,6 +139,8 @@ _SIGS = {
     "tfg_nrc_get_weights": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tfg_nrc_get_stats": (C.c_int, [C.c_void_p, C.POINTER(NrcStats)]),
     "tfg_nrc_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "tfg_image_resolve": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p,
+                                    C.c_void_p]),
     "tfg_nrc_infer_queries": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "tfg_nrc_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "tfg_nrc_last_gradient": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -680,6 +680,7 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
 int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     if (!pt) { set_error("tfg_pt_autotune: NULL"); return TFG_E_INVALID; }
     if (!pt->scene_set || !pt->camera_set) { set_error("tfg_pt_autotune: scene or camera not set"); return TFG_E_STATE; }
+    (void)stream;
     drop_graph(pt);
     tfg_color* img = nullptr;
     TFG_HIP_CHECK(hipMalloc(&img, sizeof(tfg_color) * pt->P.n_paths));
@@ -689,153 +690,61 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     const uint32_t saved_mode = pt->P.mode;
     pt->P.mode = 0;   // time the plain frame: the traversal stages dominate it
     int rc = TFG_OK;
-    auto time_frames = [&]() {
-        float ms_tot = 0.f;
-        for (int rep = 0; rep < 3 && rc == TFG_OK; ++rep) {
-            (void)hipEventRecord(a, pt->stream);
-            rc = enqueue_frame(pt, pt->stream, img, nullptr);
-            (void)hipEventRecord(b, pt->stream);
-            (void)hipEventSynchronize(b);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, a, b);
-            if (rep > 0) ms_tot += ms;
-        }
-        return ms_tot;
-    };
-    // launch shape of the traversal stages, with one tile and overlap on
-    // large multipliers give more workgroups than chunks: each then traces
-    // one chunk and retires, which lets launches on other streams start
-    // while a long launch is still running
-    const int blocks[3] = {64, 128, 256};
-    const int mults[4] = {1, 4, 16, 64};
-    float best = 1e30f;
-    int bb = 128, bm = 1;
-    pt->tiles = 1;
-    pt->sides = 2;
-    pt->overlap = true;
-    for (int bi = 0; bi < 3 && rc == TFG_OK; ++bi) {
-        for (int mi = 0; mi < 4 && rc == TFG_OK; ++mi) {
-            set_trace_shape(pt, blocks[bi], mults[mi]);
-            const float ms = time_frames();
-            if (ms < best) {
-                best = ms;
-                bb = blocks[bi];
-                bm = mults[mi];
-            }
-        }
-    }
-    set_trace_shape(pt, bb, bm);
-    // then the schedule, timed on direct launches (the tuner never captures
-    // graphs; a tracer that replays graphs captures once, at its next frame)
-    struct Cand {
-        int tiles, sides;
-        bool overlap;
-    };
-    const Cand cands[] = {{1, 1, false}, {1, 1, true}, {1, 2, true}, {1, 4, true},
-                          {2, 1, true},  {2, 2, true}, {2, 4, true}};
-    // two passes over the candidates; a candidate's time is its faster pass
-    constexpr int kCands = (int)(sizeof(cands) / sizeof(cands[0]));
-    float cand_ms[2][kCands];
-    for (int gi = 0; gi < 2; ++gi)
-        for (int ci = 0; ci < kCands; ++ci) cand_ms[gi][ci] = 1e30f;
-    const int n_graph = 1;
-    for (int pass = 0; pass < 2 && rc == TFG_OK; ++pass) {
-        for (int gi = 0; gi < n_graph && rc == TFG_OK; ++gi) {
-            for (int ci = 0; ci < kCands && rc == TFG_OK; ++ci) {
-                const Cand& c = cands[ci];
-                pt->tiles = c.tiles;
-                pt->sides = c.sides;
-                pt->overlap = c.overlap;
+    // Coordinate search: each step sweeps one setting with the others held;
+    // a candidate's time is the faster of two interleaved passes of 2 untimed
+    // and 8 timed frames (direct launches; the tuner captures no graph).
+    auto sweep = [&](int n, auto&& apply) {
+        float best_ms[16];
+        for (int i = 0; i < n; ++i) best_ms[i] = 1e30f;
+        for (int pass = 0; pass < 2 && rc == TFG_OK; ++pass) {
+            for (int i = 0; i < n && rc == TFG_OK; ++i) {
+                apply(i);
                 for (int w = 0; w < 2 && rc == TFG_OK; ++w) rc = enqueue_frame(pt, pt->stream, img, nullptr);
                 (void)hipEventRecord(a, pt->stream);
-                for (int rep = 0; rep < 6 && rc == TFG_OK; ++rep) rc = enqueue_frame(pt, pt->stream, img, nullptr);
+                for (int f = 0; f < 8 && rc == TFG_OK; ++f) rc = enqueue_frame(pt, pt->stream, img, nullptr);
                 (void)hipEventRecord(b, pt->stream);
                 (void)hipEventSynchronize(b);
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, a, b);
-                if (ms < cand_ms[gi][ci]) cand_ms[gi][ci] = ms;
+                if (ms < best_ms[i]) best_ms[i] = ms;
             }
         }
-    }
-    best = 1e30f;
-    Cand bc = {1, 1, true};
-    for (int gi = 0; gi < n_graph; ++gi) {
-        for (int ci = 0; ci < kCands; ++ci) {
-            if (cand_ms[gi][ci] < best) {
-                best = cand_ms[gi][ci];
-                bc = cands[ci];
-            }
-        }
-    }
-    pt->tiles = bc.tiles;
-    pt->sides = bc.sides;
-    pt->overlap = bc.overlap;
-    // then the XCD super-chunk size of the traversal stages
+        int bi = 0;
+        for (int i = 1; i < n; ++i)
+            if (best_ms[i] < best_ms[bi]) bi = i;
+        apply(bi);
+    };
+    pt->tiles = 1;
+    pt->sides = 2;
+    pt->overlap = true;
+    // 1. traversal launch shape (large multipliers give more workgroups than
+    //    chunks: each then traces one chunk and retires)
+    const int blocks[3] = {64, 128, 256};
+    const int mults[4] = {1, 4, 16, 64};
+    sweep(12, [&](int i) { set_trace_shape(pt, blocks[i / 4], mults[i % 4]); });
+    // 2. tiles, side streams and overlap
+    struct Cand {
+        int tiles, sides;
+        bool overlap;
+    };
+    const Cand cands[] = {{1, 1, false}, {1, 1, true}, {1, 2, true}, {1, 4, true}, {2, 2, true}, {2, 4, true}};
+    sweep(6, [&](int i) {
+        pt->tiles = cands[i].tiles;
+        pt->sides = cands[i].sides;
+        pt->overlap = cands[i].overlap;
+    });
+    // 3. XCD super-chunks, octet threshold, shadow-stage grid
     const uint32_t groups[4] = {0, 1024, 4096, 16384};
-    float group_ms[4] = {1e30f, 1e30f, 1e30f, 1e30f};
-    for (int pass = 0; pass < 2 && rc == TFG_OK; ++pass) {
-        for (int gi = 0; gi < 4 && rc == TFG_OK; ++gi) {
-            pt->P.xcd_group = groups[gi];
-            for (int w = 0; w < 2 && rc == TFG_OK; ++w) rc = enqueue_frame(pt, pt->stream, img, nullptr);
-            (void)hipEventRecord(a, pt->stream);
-            for (int rep = 0; rep < 6 && rc == TFG_OK; ++rep) rc = enqueue_frame(pt, pt->stream, img, nullptr);
-            (void)hipEventRecord(b, pt->stream);
-            (void)hipEventSynchronize(b);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, a, b);
-            if (ms < group_ms[gi]) group_ms[gi] = ms;
-        }
-    }
-    int bgi = 0;
-    for (int gi = 1; gi < 4; ++gi)
-        if (group_ms[gi] < group_ms[bgi]) bgi = gi;
-    pt->P.xcd_group = groups[bgi];
-    // then the ray count below which closest-hit launches trace 8 lanes per ray
+    sweep(4, [&](int i) { pt->P.xcd_group = groups[i]; });
     const uint32_t octs[4] = {0, 1u << 16, 1u << 18, 1u << 20};
-    float oct_ms[4] = {1e30f, 1e30f, 1e30f, 1e30f};
-    for (int pass = 0; pass < 2 && rc == TFG_OK; ++pass) {
-        for (int oi = 0; oi < 4 && rc == TFG_OK; ++oi) {
-            pt->P.octet_below = octs[oi];
-            for (int w = 0; w < 2 && rc == TFG_OK; ++w) rc = enqueue_frame(pt, pt->stream, img, nullptr);
-            (void)hipEventRecord(a, pt->stream);
-            for (int rep = 0; rep < 6 && rc == TFG_OK; ++rep) rc = enqueue_frame(pt, pt->stream, img, nullptr);
-            (void)hipEventRecord(b, pt->stream);
-            (void)hipEventSynchronize(b);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, a, b);
-            if (ms < oct_ms[oi]) oct_ms[oi] = ms;
-        }
-    }
-    int boi = 0;
-    for (int oi = 1; oi < 4; ++oi)
-        if (oct_ms[oi] < oct_ms[boi]) boi = oi;
-    pt->P.octet_below = octs[boi];
-    // then the shadow stage's grid
-    const int cpcs[4] = {0, 1, 2, 3};
-    float cpc_ms[4] = {1e30f, 1e30f, 1e30f, 1e30f};
-    for (int pass = 0; pass < 2 && rc == TFG_OK; ++pass) {
-        for (int ci = 0; ci < 4 && rc == TFG_OK; ++ci) {
-            pt->connect_per_cu = cpcs[ci];
-            for (int w = 0; w < 2 && rc == TFG_OK; ++w) rc = enqueue_frame(pt, pt->stream, img, nullptr);
-            (void)hipEventRecord(a, pt->stream);
-            for (int rep = 0; rep < 6 && rc == TFG_OK; ++rep) rc = enqueue_frame(pt, pt->stream, img, nullptr);
-            (void)hipEventRecord(b, pt->stream);
-            (void)hipEventSynchronize(b);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, a, b);
-            if (ms < cpc_ms[ci]) cpc_ms[ci] = ms;
-        }
-    }
-    int bci = 0;
-    for (int ci = 1; ci < 4; ++ci)
-        if (cpc_ms[ci] < cpc_ms[bci]) bci = ci;
-    pt->connect_per_cu = cpcs[bci];
+    sweep(4, [&](int i) { pt->P.octet_below = octs[i]; });
+    const int cpcs[3] = {0, 2, 3};
+    sweep(3, [&](int i) { pt->connect_per_cu = cpcs[i]; });
     drop_graph(pt);
     pt->P.mode = saved_mode;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     (void)hipFree(img);
-    (void)stream;
     return rc;
 }
 
