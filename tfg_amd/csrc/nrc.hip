@@ -330,21 +330,35 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
 }
 
 // Sums the per-block gradients; flags non-finite values or a skipped step.
+// Sum of the per-block partial gradients.  A block covers 64 parameters
+// with 4 waves; wave k sums the partials of blocks b = k, k + 4, ... (lanes
+// read 64 consecutive parameters), and the 4 sums are added in wave order
+// through LDS, so the result is deterministic.
+constexpr int kReduceParams = 64;
 __global__ __launch_bounds__(256) void k_nrc_grad_reduce(const float* __restrict__ partial, uint32_t nblk,
                                                          float* __restrict__ grad, uint32_t* state,
                                                          const uint32_t* __restrict__ n_rec, uint32_t min_records,
                                                          float inv_scale) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float sums[4][kReduceParams];
+    const int lane = (int)(threadIdx.x & 63u), slice = (int)(threadIdx.x >> 6);
+    const int p = blockIdx.x * kReduceParams + lane;
     if (*n_rec <= min_records) {
-        if (p == 0) atomicOr(&state[kStFlags], 2u);
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&state[kStFlags], 2u);
         return;
     }
-    if (p >= kNrcParams) return;
     float s = 0.0f;
-    for (uint32_t b = 0; b < nblk; ++b) s += partial[(size_t)b * kNrcParams + p];
-    s *= inv_scale;
-    grad[p] = s;
-    if (!isfinite(s)) atomicOr(&state[kStFlags], 1u);
+    if (p < kNrcParams) {
+#pragma unroll 8
+        for (uint32_t b = (uint32_t)slice; b < nblk; b += 4) s += partial[(size_t)b * kNrcParams + p];
+    }
+    sums[slice][lane] = s;
+    __syncthreads();
+    if (slice == 0 && p < kNrcParams) {
+        const float t = ((sums[0][lane] + sums[1][lane]) + sums[2][lane]) + sums[3][lane];
+        const float g = t * inv_scale;
+        grad[p] = g;
+        if (!isfinite(g)) atomicOr(&state[kStFlags], 1u);
+    }
 }
 
 template <typename T>
@@ -433,8 +447,8 @@ int enqueue_train_steps(tfg_nrc* nc, const float4* records, const uint32_t* n_re
     for (uint32_t k = 0; k < steps; ++k) {
         hipLaunchKernelGGL(k_nrc_step_begin, dim3(1), dim3(64), 0, s, nc->state, nc->loss_accum);
         hipLaunchKernelGGL(k_nrc_train<T>, dim3(nblk), dim3(256), kTrainLds, s, A);
-        hipLaunchKernelGGL(k_nrc_grad_reduce, dim3(pblocks), dim3(256), 0, s, nc->partial, nblk, nc->grad, nc->state,
-                           n_rec, min_records, h.inv_loss_scale);
+        hipLaunchKernelGGL(k_nrc_grad_reduce, dim3((kNrcParams + kReduceParams - 1) / kReduceParams), dim3(256), 0,
+                           s, nc->partial, nblk, nc->grad, nc->state, n_rec, min_records, h.inv_loss_scale);
         hipLaunchKernelGGL(k_nrc_adam<T>, dim3(pblocks), dim3(256), 0, s, nc->master, nc->adam_m, nc->adam_v, nc->ema,
                            nc->grad, nc->state, h, nc->fop_train, nc->bop_train, nc->fop_infer);
         hipLaunchKernelGGL(k_nrc_step_end, dim3(1), dim3(64), 0, s, nc->state, nc->loss_accum);

@@ -388,7 +388,8 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
                 if (fl != fl_in) P.flags[pid] = fl;
             }
         }
-        const uint32_t ss = wave_append(want_shadow, &P.counters[CNT_SHQ + depth]);
+        uint32_t ss, rs;
+        block_append2<256>(want_shadow, &P.counters[CNT_SHQ + depth], ss, want_ray, &P.counters[qo], rs);
         if (want_shadow) {
             float4* e = P.shadow + 4 * ((size_t)depth * P.depth_stride + ss);
             e[0] = e0;
@@ -397,7 +398,6 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
             e[3] = e3;
             P.neemask[pid] |= 1u << depth;
         }
-        const uint32_t rs = wave_append(want_ray, &P.counters[qo]);
         if (want_ray) {
             P.rays[qo][2 * (size_t)rs] = n0;
             P.rays[qo][2 * (size_t)rs + 1] = n1;
@@ -547,7 +547,8 @@ __global__ __launch_bounds__(256) void k_finalize(PtParams P, tfg_color* image, 
             }
         }
         if (!kNrc) continue;
-        const uint32_t rs = wave_append(want_rec, &P.gcounters[CNT_RECORDS]);
+        uint32_t rs, qs;
+        block_append2<256>(want_rec, &P.gcounters[CNT_RECORDS], rs, want_q, &P.gcounters[CNT_QUERIES], qs);
         if (want_rec) {
             float4* R = P.records + (size_t)kRecF4 * rs;
             const float4* hd = P.head + (size_t)kVertF4 * pid;
@@ -561,7 +562,6 @@ __global__ __launch_bounds__(256) void k_finalize(PtParams P, tfg_color* image, 
             R[9] = ts;
             R[10] = P.tthr[pid];
         }
-        const uint32_t qs = wave_append(want_q, &P.gcounters[CNT_QUERIES]);
         if (want_q) {
             float4* Q = P.queries + (size_t)kQueryF4 * qs;
             const float4* hd = P.head + (size_t)kVertF4 * pid;

@@ -1,4 +1,4 @@
-// wave_ops.h - wave64 helpers shared by the frame kernels.
+// wave_ops.h - wave64 / block helpers shared by the frame kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -49,6 +49,39 @@ struct ChunkIter {
         map();
     }
 };
+
+// Two appends aggregated over the whole block: one ballot per wave, wave
+// offsets by an LDS scan, one atomic per block and counter (kBlock / 64
+// times fewer atomics on the two counters than wave_append).  Every thread
+// of the block must call it: it synchronises the block.
+template <int kBlock>
+__device__ __forceinline__ void block_append2(bool want_a, uint32_t* counter_a, uint32_t& slot_a, bool want_b,
+                                              uint32_t* counter_b, uint32_t& slot_b) {
+    constexpr int kWaves = kBlock / 64;
+    __shared__ uint32_t wave_off[2][kWaves];
+    __shared__ uint32_t block_base[2];
+    const uint64_t ma = __ballot(want_a), mb = __ballot(want_b);
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / 64u;
+    if (lane == 0) {
+        wave_off[0][wave] = (uint32_t)__popcll(ma);
+        wave_off[1][wave] = (uint32_t)__popcll(mb);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t tot = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = wave_off[threadIdx.x][w];
+            wave_off[threadIdx.x][w] = tot;
+            tot += c;
+        }
+        block_base[threadIdx.x] = tot ? atomicAdd(threadIdx.x ? counter_b : counter_a, tot) : 0u;
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    slot_a = block_base[0] + wave_off[0][wave] + (uint32_t)__popcll(ma & below);
+    slot_b = block_base[1] + wave_off[1][wave] + (uint32_t)__popcll(mb & below);
+    __syncthreads();   // the LDS words are reused by the next call
+}
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
