@@ -225,6 +225,11 @@ def bench_pt(args, torch):
     }
     dom = max(stages, key=lambda k: stages[k][2])
     name, dom_bytes, dom_ms = stages[dom]
+    # launches per frame: one per depth and tile, except that connect_batch
+    # k > 0 traces the shadow rays of depths k..MD-1 in one launch
+    cb = shape.get("connect_batch", 0)
+    per_tile = MD if dom == "extend" or cb == 0 or cb >= MD else cb + 1
+    n_launch = per_tile * shape.get("tiles", 1)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     res = {
         "value": value,
@@ -242,9 +247,9 @@ def bench_pt(args, torch):
             "traffic": MEASURED_TRAFFIC["k_" + dom][0],
             "traffic_source": MEASURED_TRAFFIC["k_" + dom][1],
             "algorithmic_bytes": int(dom_bytes),
-            "algorithmic_bytes_per_launch": int(dom_bytes / MD),
-            "launches_per_frame": MD,
-            "avg_launch_ms": dom_ms / MD,
+            "algorithmic_bytes_per_launch": int(dom_bytes / n_launch),
+            "launches_per_frame": n_launch,
+            "avg_launch_ms": dom_ms / n_launch,
             "stage_gbs": {k: v[1] / (v[2] * 1e-3) / 1e9 for k, v in stages.items()},
         },
         "config": {

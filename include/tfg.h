@@ -146,9 +146,9 @@ typedef struct tfg_pt_profile {
 int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_profile* out);
 /* Times candidate schedules (tfg_pt_schedule) on the current scene and keeps
  * the fastest: first the traversal launch shape, then tiles, side streams and
- * overlap, then the XCD super-chunk size, octet_below and connect_per_cu,
- * all timed on direct launches (the tuner captures no graph).  Drops any
- * captured graph. */
+ * overlap, then the XCD super-chunk size, octet_below, connect_per_cu and
+ * connect_batch, all timed on direct launches (the tuner captures no graph).
+ * Drops any captured graph. */
 int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream);
 int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_grid, uint32_t* shade_grid);
 /* How a frame is launched.  None of these change the results: every
@@ -173,6 +173,10 @@ int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_g
  *   connect_per_cu (0..64): the shadow stage's persistent grid, workgroups
  *       per CU (0: the closest-hit stages' grid); a small grid leaves CU
  *       slots to the closest-hit chain, the frame's critical path;
+ *   connect_batch (0..32): 0 gives every depth's shadow stage its own
+ *       launch; k > 0 traces the shadow rays of depths k..max_depth-1 in one
+ *       launch after the last shading stage (late depths have few rays, and
+ *       one launch of them replaces a chain of small launches);
  *   use_graph: replay captured hipGraphs (one per tile, plus one for the
  *       cache work) instead of launching the kernels directly; read-only,
  *       fixed by tfg_pt_config.use_graph (set_schedule must pass the same
@@ -183,6 +187,7 @@ typedef struct tfg_pt_schedule {
     uint32_t xcd_group;
     uint32_t octet_below;
     uint32_t connect_per_cu;
+    uint32_t connect_batch;
 } tfg_pt_schedule;
 int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out);
 int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* schedule);

@@ -25,6 +25,7 @@ struct tfg_pt {
     int trace_mult = 1;
     int trace_grid = 0;
     int connect_per_cu = 0;   // shadow-stage workgroups per CU (0: the closest-hit grid)
+    uint32_t connect_batch = 0;   // depths >= this (if > 0) share one shadow-stage launch
     int shade_grid = 0;
     int gen_grid = 0;
     // pixel tiles: each runs its own chain of stages on its own streams
@@ -127,14 +128,14 @@ void launch_extend(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t q, uin
 }
 
 template <bool kStats>
-void launch_connect(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t d) {
+void launch_connect(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t d_lo, uint32_t d_hi) {
     // a bounded persistent grid leaves CU slots to the closest-hit chain
     const dim3 g(pt->connect_per_cu > 0 ? pt->n_cu * pt->connect_per_cu : pt->trace_grid);
     const size_t lds = trace_lds(pt, pt->trace_block);
     switch (pt->trace_block) {
-        case 64: hipLaunchKernelGGL((k_connect<64, kStats>), g, dim3(64), lds, s, T, d); break;
-        case 256: hipLaunchKernelGGL((k_connect<256, kStats>), g, dim3(256), lds, s, T, d); break;
-        default: hipLaunchKernelGGL((k_connect<128, kStats>), g, dim3(128), lds, s, T, d); break;
+        case 64: hipLaunchKernelGGL((k_connect<64, kStats>), g, dim3(64), lds, s, T, d_lo, d_hi); break;
+        case 256: hipLaunchKernelGGL((k_connect<256, kStats>), g, dim3(256), lds, s, T, d_lo, d_hi); break;
+        default: hipLaunchKernelGGL((k_connect<128, kStats>), g, dim3(128), lds, s, T, d_lo, d_hi); break;
     }
 }
 
@@ -224,6 +225,12 @@ struct StageEvents {
     }
 };
 
+// whether depth d's shadow stage has a launch of its own (the first depth of a
+// batch stands for the batch)
+inline bool connect_launched(const tfg_pt* pt, uint32_t d) {
+    return pt->connect_batch == 0 || d <= pt->connect_batch;
+}
+
 inline void mark(hipEvent_t e, hipStream_t s) {
     if (e) (void)hipEventRecord(e, s);
 }
@@ -251,15 +258,20 @@ int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, hipStream_t* side, tfg_color
         if (nrc) hipLaunchKernelGGL(k_shade<true>, dim3(pt->shade_grid), dim3(256), 0, ts, T, q, d);
         else hipLaunchKernelGGL(k_shade<false>, dim3(pt->shade_grid), dim3(256), 0, ts, T, q, d);
         mark(pe ? pe->shd[t][d][1] : nullptr, ts);
-        hipStream_t cs = sides ? side[d % sides] : ts;
+        // the shadow stage of depth d, or of depths batch..max_depth-1 at once
+        const uint32_t batch = pt->connect_batch;
+        const bool batched = batch > 0 && d >= batch;
+        if (batched && d + 1 < pt->cfg.max_depth) continue;
+        const uint32_t d_lo = batched ? batch : d;
+        hipStream_t cs = sides ? side[d_lo % sides] : ts;
         if (sides) {
             TFG_HIP_CHECK(hipEventRecord(pt->ev_shaded[t][d], ts));
             TFG_HIP_CHECK(hipStreamWaitEvent(cs, pt->ev_shaded[t][d], 0));
         }
-        mark(pe ? pe->con[t][d][0] : nullptr, cs);
-        if (stats) launch_connect<true>(pt, cs, T, d);
-        else launch_connect<false>(pt, cs, T, d);
-        mark(pe ? pe->con[t][d][1] : nullptr, cs);
+        mark(pe ? pe->con[t][d_lo][0] : nullptr, cs);
+        if (stats) launch_connect<true>(pt, cs, T, d_lo, d);
+        else launch_connect<false>(pt, cs, T, d_lo, d);
+        mark(pe ? pe->con[t][d_lo][1] : nullptr, cs);
     }
     for (int k = 0; k < sides && k < (int)pt->cfg.max_depth; ++k) {
         TFG_HIP_CHECK(hipEventRecord(pt->ev_cjoin[t][k], side[k]));
@@ -652,14 +664,14 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
             for (uint32_t d = 0; d < pt->cfg.max_depth; ++d) {
                 out->ms_extend[d] += span(pe.ext[t][d][0], pe.ext[t][d][1]);
                 out->ms_shade[d] += span(pe.shd[t][d][0], pe.shd[t][d][1]);
-                out->ms_connect[d] += span(pe.con[t][d][0], pe.con[t][d][1]);
+                if (connect_launched(pt, d)) out->ms_connect[d] += span(pe.con[t][d][0], pe.con[t][d][1]);
             }
             out->ms_finalize += span(pe.fin[t][0], pe.fin[t][1]);
         }
         for (uint32_t d = 0; d < pt->cfg.max_depth; ++d) {
             out->at_extend[d] = span(pe.frame[0], pe.ext[0][d][0]);
             out->at_shade[d] = span(pe.frame[0], pe.shd[0][d][0]);
-            out->at_connect[d] = span(pe.frame[0], pe.con[0][d][0]);
+            if (connect_launched(pt, d)) out->at_connect[d] = span(pe.frame[0], pe.con[0][d][0]);
         }
         out->at_finalize = span(pe.frame[0], pe.fin[0][0]);
         out->ms_total = span(pe.frame[0], pe.frame[1]);
@@ -717,6 +729,7 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     pt->tiles = 1;
     pt->sides = 2;
     pt->overlap = true;
+    pt->connect_batch = 0;
     // 1. traversal launch shape (large multipliers give more workgroups than
     //    chunks: each then traces one chunk and retires)
     const int blocks[3] = {64, 128, 256};
@@ -740,6 +753,8 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     sweep(4, [&](int i) { pt->P.octet_below = octs[i]; });
     const int cpcs[3] = {0, 2, 3};
     sweep(3, [&](int i) { pt->connect_per_cu = cpcs[i]; });
+    const uint32_t batches[6] = {0, 2, 3, 4, 5, 6};
+    sweep(6, [&](int i) { pt->connect_batch = batches[i]; });
     drop_graph(pt);
     pt->P.mode = saved_mode;
     (void)hipEventDestroy(a);
@@ -767,6 +782,7 @@ int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out) {
     out->xcd_group = pt->P.xcd_group;
     out->octet_below = pt->P.octet_below;
     out->connect_per_cu = (uint32_t)pt->connect_per_cu;
+    out->connect_batch = pt->connect_batch;
     return TFG_OK;
 }
 
@@ -776,9 +792,10 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
     if ((b != 64 && b != 128 && b != 256) || sc->grid_mult == 0 || sc->grid_mult > 64 || sc->tiles == 0 ||
         sc->tiles > kMaxTiles || sc->side_streams == 0 || sc->side_streams > kMaxSide ||
         (sc->xcd_group != 0 && (sc->xcd_group % 256 != 0 || sc->xcd_group > (1u << 20))) ||
-        sc->connect_per_cu > 64) {
+        sc->connect_per_cu > 64 || sc->connect_batch > kMaxDepth) {
         set_error("tfg_pt_set_schedule: block 64/128/256, grid_mult 1..64, tiles 1..%u, side streams 1..%u, "
-                  "xcd_group 0 or a multiple of 256 up to 2^20, connect_per_cu 0..64", kMaxTiles, kMaxSide);
+                  "xcd_group 0 or a multiple of 256 up to 2^20, connect_per_cu 0..64, connect_batch 0..%u", kMaxTiles,
+                  kMaxSide, kMaxDepth);
         return TFG_E_INVALID;
     }
     if ((sc->use_graph != 0) != pt->run_graph) {
@@ -793,6 +810,7 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
     pt->P.xcd_group = sc->xcd_group;
     pt->P.octet_below = sc->octet_below;
     pt->connect_per_cu = (int)sc->connect_per_cu;
+    pt->connect_batch = sc->connect_batch;
     return TFG_OK;
 }
 

@@ -119,12 +119,12 @@ __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint3
     ChunkIter it((count + per_chunk - 1) / per_chunk, P.xcd_group / per_chunk, P.xcd_group != 0);
     if (!it.valid()) return;   // workgroups without a chunk leave before staging
     stage_top_nodes(P, top);
+    TraceStats st;   // accumulated over the chunks, flushed once per wave
     if (octet) {
         constexpr uint32_t kRays = kBlock / 8;
         const uint32_t j = threadIdx.x & 7u;
         for (; it.valid(); it.next()) {
             const uint32_t i = it.chunk * kRays + (threadIdx.x >> 3);
-            TraceStats st;
             if (i < count) {
                 const float4 a = rays[2 * (size_t)i];
                 const float4 b = rays[2 * (size_t)i + 1];
@@ -144,14 +144,13 @@ __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint3
                 }
                 if (j == 0) P.hits[i] = make_float4(t, u, v, __uint_as_float(prim));
             }
-            if (kStats) add_stats(P, 0, st);
         }
+        if (kStats) add_stats(P, 0, st);
         return;
     }
     for (; it.valid(); it.next()) {
         const uint32_t b0 = it.chunk * kBlock;
         const uint32_t i = b0 + threadIdx.x;
-        TraceStats st;
         if (i < count) {
             const float4 a = rays[2 * (size_t)i];
             const float4 b = rays[2 * (size_t)i + 1];
@@ -171,8 +170,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint3
             }
             P.hits[i] = make_float4(t, u, v, __uint_as_float(prim));
         }
-        if (kStats) add_stats(P, 0, st);
     }
+    if (kStats) add_stats(P, 0, st);
 }
 
 // ------------------------------------------------------------------ shade
@@ -188,6 +187,7 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
     const uint32_t count = P.counters[q];
     const uint32_t qo = q ^ 1u;
     const bool last = depth + 1 >= P.max_depth;
+    uint32_t facing = 0;   // shadow rays this thread's entries will trace (lights above the surface)
     for (uint32_t b = blockIdx.x * blockDim.x; b < count; b += gridDim.x * blockDim.x) {
         const uint32_t i = b + threadIdx.x;
         bool want_shadow = false, want_ray = false;
@@ -391,6 +391,11 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
         uint32_t ss, rs;
         block_append2<256>(want_shadow, &P.counters[CNT_SHQ + depth], ss, want_ray, &P.counters[qo], rs);
         if (want_shadow) {
+            for (uint32_t l = 0; l < P.n_lights; ++l) {
+                V3 dir;
+                float dist, c;
+                facing += light_facing(P.lights[l], xyz(e0), xyz(e1), dir, dist, c) ? 1u : 0u;
+            }
             float4* e = P.shadow + 4 * ((size_t)depth * P.depth_stride + ss);
             e[0] = e0;
             e[1] = e1;
@@ -404,26 +409,28 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
             P.qpath[qo][rs] = pid;
         }
     }
+    block_add<256>(facing, &P.gcounters[CNT_SHD + depth]);
 }
 
 // ---------------------------------------------------------------- connect
-// Shadow rays of the NEE requests made at one depth (escenario.h:144-184).
-// Each entry gets a group of 2^light_shift adjacent lanes, one per light (or
-// 8 lanes per light when the launch has few rays, see octet_below), so the
-// rays of one shading point are traced side by side rather than one after
-// another; the group's first lane then sums the unoccluded contributions in
-// light order (the order of the reference's loop).  The stage reads only the
-// shadow entries of its depth and writes only nee[depth][pid], so connect of
-// depth d can run on a second stream while extend/shade of depth d+1 run;
-// finalize adds the contributions in depth order.
+// Shadow rays of the NEE requests made at depths d_lo..d_hi
+// (escenario.h:144-184), one launch for the range (several depths when the
+// late, small depths are batched).  Each entry gets a group of
+// 2^light_shift adjacent lanes, one per light (or 8 lanes per light when the
+// launch has few rays, see octet_below), so the rays of one shading point are
+// traced side by side; the group's first lane then sums the unoccluded
+// contributions in light order (the order of the reference's loop).  The
+// stage reads only the shadow entries of its depths and writes only
+// nee[depth][pid], so it can run on a side stream while later depths are
+// extended and shaded; finalize adds the contributions in depth order.  The
+// shadow rays are counted by the shading stage (light_facing).
 template <int kBlock, bool kStats = false>
-__global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) {
+__global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t d_lo, uint32_t d_hi) {
     extern __shared__ __align__(16) unsigned char lds[];
     float4* top = reinterpret_cast<float4*>(lds);
     uint2* stack = reinterpret_cast<uint2*>(top + 5 * P.n_top);
-    const uint32_t count = P.counters[CNT_SHQ + depth];
-    const float4* entries = P.shadow + 4 * (size_t)depth * P.depth_stride;
-    float4* nee = P.nee + (size_t)depth * P.depth_stride;
+    uint32_t count = 0;   // entries of all depths in the range
+    for (uint32_t d = d_lo; d <= d_hi; ++d) count += P.counters[CNT_SHQ + d];
     const uint32_t shift = P.light_shift;
     const uint32_t group = 1u << shift;
     // few shadow rays: 8 lanes per ray (traverse_octet); needs <= 8 lights so
@@ -437,14 +444,20 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) 
     ChunkIter it((uint32_t)((work + kBlock - 1) / kBlock), P.xcd_group / rays_per_chunk, P.xcd_group != 0);
     if (!it.valid()) return;   // workgroups without a chunk leave before staging
     stage_top_nodes(P, top);
+    TraceStats st;            // accumulated over the chunks, flushed once per wave
     for (; it.valid(); it.next()) {
         const uint64_t b = (uint64_t)it.chunk * kBlock;
         const uint64_t w = b + threadIdx.x;
-        const uint32_t i = (uint32_t)(w >> sh);
+        uint32_t i = (uint32_t)(w >> sh);   // entry over the range, then within its depth
+        uint32_t d = d_lo;
+        for (; d < d_hi; ++d) {
+            const uint32_t c = P.counters[CNT_SHQ + d];
+            if (i < c) break;
+            i -= c;
+        }
+        const float4* entries = P.shadow + 4 * (size_t)d * P.depth_stride;
         const uint32_t l = (uint32_t)(w >> os) & (group - 1u);
         const uint32_t j = (uint32_t)w & ((1u << os) - 1u);
-        uint32_t traced = 0;
-        TraceStats st;
         V3 c = v3(0.f, 0.f, 0.f);
         float4 e0 = make_float4(0.f, 0.f, 0.f, 0.f), e3 = e0;
         if (w < work) {
@@ -460,7 +473,6 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) 
                 float dist;
                 if (light_term(P.lights[l], ps, n, xyz(e2), m, contrib, dir, dist)) {
                     const V3 o = ps + dir * kEps;
-                    traced = j == 0 ? 1u : 0u;
                     bool occ = false;
                     for (uint32_t k = 0; k < P.n_analytic && !occ; ++k) {   // escenario.h:255-266
                         const float ta = analytic_t(P.analytic[k], o, dir);
@@ -488,11 +500,10 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t depth) 
             const int src = lead + (int)(k << os);
             sum = sum + v3(__shfl(c.x, src), __shfl(c.y, src), __shfl(c.z, src));
         }
-        if (w < work && l == 0 && j == 0) nee[__float_as_uint(e0.w)] = f4(xyz(e3) * sum, 0.0f);
-        const uint32_t tot = wave_sum(traced);
-        if (lane == 0 && tot) atomicAdd(&P.gcounters[CNT_SHD + depth], tot);
-        if (kStats) add_stats(P, 2, st);
+        if (w < work && l == 0 && j == 0)
+            P.nee[(size_t)d * P.depth_stride + __float_as_uint(e0.w)] = f4(xyz(e3) * sum, 0.0f);
     }
+    if (kStats) add_stats(P, 2, st);
 }
 
 // --------------------------------------------------------------- finalize

@@ -226,16 +226,23 @@ __device__ __forceinline__ V3 shading_normal(const tfg_tri_shading& s, V3 ng, fl
     return n;
 }
 
+// Direction and distance to a point light, and whether it lies above the
+// surface (the only test that drops a light before its shadow ray).
+__device__ __forceinline__ bool light_facing(const tfg_point_light& L, V3 ps, V3 n, V3& dir, float& dist, float& c) {
+    V3 d = ld3(L.pos) - ps;
+    dist = sqrtf(dot(d, d));
+    dir = d * (1.0f / dist);
+    c = dot(n, dir);
+    return c > 0.0f;
+}
+
 // Point-light contribution at shadow point ps (escenario.h:144-184) for one
 // light, without the visibility term; returns false if the light is below
 // the surface.  wo in world space, for the GGX term.
 __device__ __forceinline__ bool light_term(const tfg_point_light& L, V3 ps, V3 n, V3 wo, const Mat& m, V3& contrib,
                                            V3& dir, float& dist) {
-    V3 d = ld3(L.pos) - ps;
-    dist = sqrtf(dot(d, d));
-    dir = d * (1.0f / dist);
-    const float c = dot(n, dir);
-    if (c <= 0.0f) return false;
+    float c;
+    if (!light_facing(L, ps, n, dir, dist, c)) return false;
     V3 f = m.kd * (1.0f / kPi);
     if (mat_glossy(m)) {
         const Frame fr = make_frame(n);

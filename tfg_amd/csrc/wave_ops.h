@@ -88,4 +88,20 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     return x;
 }
 
+// Adds the block's sum of v to *counter with one atomic.  Every thread of
+// the block must call it: it synchronises the block.
+template <int kBlock>
+__device__ __forceinline__ void block_add(uint32_t v, uint32_t* counter) {
+    __shared__ uint32_t partial[kBlock / 64];
+    const uint32_t s = wave_sum(v);
+    if (__lane_id() == 0) partial[threadIdx.x / 64u] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kBlock / 64; ++w) t += partial[w];
+        if (t) atomicAdd(counter, t);
+    }
+    __syncthreads();
+}
+
 }  // namespace tfg
