@@ -182,6 +182,204 @@ __device__ __forceinline__ void pack_vertex(float4* dst, V3 p, V3 dir_mapped, V3
     dst[3] = make_float4(m.kd.z, m.ks.x, m.ks.y, m.ks.z);
 }
 
+// One queue entry of the shading stage (render.h:62-224): the hit (h) of ray
+// (ro, rd) of path pid at this depth.  Updates the path's state; returns the
+// NEE request (want_shadow, shadow entry e0..e3) and the next ray (want_ray,
+// n0, n1).
+template <bool kNrc>
+__device__ __forceinline__ void shade_entry(const PtParams& P, uint32_t depth, bool last, uint32_t pid, float4 ro,
+                                            float4 rd, float4 h, bool& want_shadow, float4& e0, float4& e1,
+                                            float4& e2, float4& e3, bool& want_ray, float4& n0, float4& n1) {
+    const uint32_t prim = __float_as_uint(h.w);
+    const float4 thr4 = P.thr[pid];
+    V3 thr = xyz(thr4);
+    float pdf = thr4.w;
+    const float4 rad4 = P.rad[pid];
+    V3 rad = xyz(rad4);
+    float ior = rad4.w;
+    uint32_t s = P.rng[pid];
+    uint32_t fl = P.flags[pid];
+    const uint32_t fl_in = fl;
+    float4 foot = kNrc ? P.foot[pid] : make_float4(0.f, 0.f, 0.f, 0.f);
+    bool rad_dirty = false;
+    const V3 d = xyz(rd);
+    if (prim != kMiss) {
+        const float t = h.x;
+        const V3 p = xyz(ro) + d * t;
+        uint32_t mat_id;
+        V3 n;
+        if (prim & kAnalyticBit) {
+            const tfg_analytic q = P.analytic[prim & ~kAnalyticBit];
+            mat_id = q.material;
+            n = analytic_normal(q, p, d);
+        } else {
+            const tfg_tri_shading sh = P.shading[prim];
+            mat_id = sh.material;
+            n = shading_normal(sh, ld3(sh.ng), h.y, h.z, d);
+        }
+        const Mat m = load_mat(&P.mats[mat_id]);
+        const float cos_theta = fabsf(dot(n, -d));
+        const bool diffuse = mat_diffuse(m), glossy = mat_glossy(m), delta = mat_delta(m);
+        bool foot_ok = false;
+        if (kNrc) {   // render.h:86-102 path-footprint spread
+            const float dist_sq = t * t;
+            if (depth == 0) {
+                foot.x = dist_sq / (4.0f * kPi * fmaxf(cos_theta, 1e-4f));
+                foot.y = 0.0f;
+            } else {
+                const float spread = dist_sq / (fmaxf(pdf, 1e-4f) * fmaxf(cos_theta, 1e-4f));
+                if (!(fl & PF_HEAD)) foot.y += spread;
+                else foot.z += spread;
+            }
+            foot_ok = foot.y >= foot.x * 0.01f;
+        }
+        bool alive = true;
+        if (vmax(m.le) > 0.0f) {   // render.h:104-123
+            rad = rad + thr * m.le;
+            rad_dirty = true;
+            if (kNrc && (fl & PF_HEAD)) {
+                float4* tl = P.tail + (size_t)kVertF4 * pid;
+                Mat z = m;
+                z.kd = v3(0.f, 0.f, 0.f);
+                z.ks = v3(0.f, 0.f, 0.f);
+                pack_vertex(tl, p, -d, n, z);
+                P.tsuf[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+                fl |= PF_TAIL;
+            }
+            alive = false;
+        } else {
+            const V3 wo = -d;
+            if (diffuse || glossy) {   // NEE request (render.h:126-131)
+                want_shadow = true;
+                e0 = f4(p + n * kEps, __uint_as_float(pid));
+                e1 = f4(n, __uint_as_float(mat_id));
+                e2 = f4(wo, 0.0f);
+                e3 = f4(thr, 0.0f);
+            }
+            bool done = false;
+            const V3 dm = v3(wo.x * 0.5f + 0.5f, wo.y * 0.5f + 0.5f, wo.z * 0.5f + 0.5f);
+            const V3 nm = v3(n.x * 0.5f + 0.5f, n.y * 0.5f + 0.5f, n.z * 0.5f + 0.5f);
+            if (kNrc && (fl & PF_HEAD)) {   // render.h:134-161 suffix end
+                const bool stop = !(fl & PF_UNBIASED) && foot.z >= foot.w;
+                if (stop) {
+                    pack_vertex(P.tail + (size_t)kVertF4 * pid, p, dm, nm, m);
+                    P.tsuf[pid] = f4(thr * (m.kd + m.ks + m.kt), 0.0f);
+                    fl |= PF_TAIL;
+                    alive = false;
+                    done = true;
+                }
+            }
+            if (kNrc && !done && diffuse && !delta && foot_ok) {   // render.h:166-202
+                V3 refl = m.kd + m.ks + m.kt;
+                refl = v3(fmaxf(refl.x, 1e-3f), fmaxf(refl.y, 1e-3f), fmaxf(refl.z, 1e-3f));
+                if (!(fl & PF_TRAIN) && (P.mode & MODE_INFER)) {
+                    pack_vertex(P.head + (size_t)kVertF4 * pid, p, dm, nm, m);
+                    P.tthr[pid] = f4(thr * refl, 0.0f);
+                    fl |= PF_QUERY;
+                    alive = false;
+                    done = true;
+                } else if ((fl & PF_TRAIN) && !(fl & PF_HEAD)) {
+                    pack_vertex(P.head + (size_t)kVertF4 * pid, p, dm, nm, m);
+                    P.tthr[pid] = f4(thr * refl, 0.0f);
+                    P.pre[pid] = f4(rad, 0.0f);
+                    foot.w = foot.y;
+                    fl |= PF_HEAD | (depth << PF_HEAD_DEPTH_SHIFT);
+                }
+            }
+            if (!done) {   // calcularLuzIndirecta, render.h:264-335
+                const float mkd = vmax(m.kd), mks = vmax(m.ks), mkt = vmax(m.kt);
+                const float sum = mkd + mks + mkt;
+                V3 wi = v3(0.f, 0.f, 1.f);
+                if (!(sum > 0.0f)) {
+                    alive = false;
+                } else {
+                    const float pd = mkd / sum, ps = mks / sum, pt = 1.0f - pd - ps;
+                    const float r = pcg32_float(s);
+                    if (r < pd) {
+                        thr = cdiv(thr * m.kd, pd);
+                        const float r1 = pcg32_float(s);
+                        const float r2 = pcg32_float(s);
+                        const V3 wl = cosine_sample(r1, r2);
+                        if (wl.z <= kEps) {
+                            alive = false;
+                        } else {
+                            wi = to_world(make_frame(n), wl);
+                            pdf = fmaxf(fabsf(dot(wi, n)), 1e-4f) / kPi;
+                        }
+                    } else if (r < pd + ps) {
+                        if (glossy) {
+                            const float r1 = pcg32_float(s);
+                            const float r2 = pcg32_float(s);
+                            const Frame fr = make_frame(n);
+                            const V3 wol = to_local(fr, wo);
+                            const float alpha = ggx_alpha(m.rough);
+                            if (wol.z <= 0.0f) {
+                                alive = false;
+                            } else {
+                                const V3 hm = ggx_sample_vndf(wol, alpha, r1, r2);
+                                const float oh = dot(wol, hm);
+                                const V3 wil = hm * (2.0f * oh) - wol;
+                                if (wil.z <= 0.0f) {
+                                    alive = false;
+                                } else {
+                                    const float a2 = alpha * alpha;
+                                    thr = thr * schlick(m.ks, oh) * (ggx_G1(a2, wil.z) / ps);
+                                    wi = normalize(to_world(fr, wil));
+                                    pdf = ggx_D(a2, hm.z) * ggx_G1(a2, wol.z) / (4.0f * wol.z);
+                                }
+                            }
+                        } else {   // render.h:337-340
+                            wi = normalize(d - n * (2.0f * dot(d, n)));
+                            thr = cdiv(thr * m.ks, ps);
+                            pdf = 1e6f;
+                        }
+                    } else {   // render.h:342-376
+                        V3 N = n;
+                        float cosi = fminf(1.0f, fmaxf(-1.0f, dot(d, N)));
+                        float etat;
+                        if (cosi > 0.0f) {
+                            N = -N;
+                            etat = 1.0f;
+                        } else {
+                            cosi = -cosi;
+                            etat = m.ior;
+                        }
+                        const float eta = ior / etat;
+                        const float k = 1.0f - eta * eta * (1.0f - cosi * cosi);
+                        if (k < 0.0f) {
+                            wi = normalize(d - N * (2.0f * dot(d, N)));
+                        } else {
+                            ior = etat;
+                            wi = normalize(d * eta + N * (eta * cosi - sqrtf(k)));
+                        }
+                        thr = cdiv(thr * m.kt, pt);
+                        pdf = 1e6f;
+                    }
+                }
+                // Russian roulette, render.h:209-224 (after depth rr_depth)
+                if (alive && depth > P.rr_depth) {
+                    const float prob = fminf(vmax(thr), 0.95f);
+                    if (pcg32_float(s) > prob) alive = false;
+                    else thr = cdiv(thr, prob);
+                }
+                if (alive && !last) {
+                    want_ray = true;
+                    n0 = f4(p + wi * kEps, 0.0f);
+                    n1 = f4(wi, __int_as_float(0x7f800000));
+                }
+                P.thr[pid] = f4(thr, pdf);
+                P.rng[pid] = s;
+                rad_dirty = rad_dirty || ior != rad4.w;
+            }
+        }
+    }
+    if (rad_dirty) P.rad[pid] = f4(rad, ior);
+    if (kNrc) {
+        P.foot[pid] = foot;
+        if (fl != fl_in) P.flags[pid] = fl;
+    }
+}
+
 template <bool kNrc>
 __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t depth) {
     const uint32_t count = P.counters[q];
@@ -196,197 +394,8 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
         float4 n0, n1;                   // next ray
         if (i < count) {
             pid = P.qpath[q][i];
-            const float4 ro = P.rays[q][2 * (size_t)i];
-            const float4 rd = P.rays[q][2 * (size_t)i + 1];
-            const float4 h = P.hits[i];
-            const uint32_t prim = __float_as_uint(h.w);
-            const float4 thr4 = P.thr[pid];
-            V3 thr = xyz(thr4);
-            float pdf = thr4.w;
-            const float4 rad4 = P.rad[pid];
-            V3 rad = xyz(rad4);
-            float ior = rad4.w;
-            uint32_t s = P.rng[pid];
-            uint32_t fl = P.flags[pid];
-            const uint32_t fl_in = fl;
-            float4 foot = kNrc ? P.foot[pid] : make_float4(0.f, 0.f, 0.f, 0.f);
-            bool rad_dirty = false;
-            const V3 d = xyz(rd);
-            if (prim != kMiss) {
-                const float t = h.x;
-                const V3 p = xyz(ro) + d * t;
-                uint32_t mat_id;
-                V3 n;
-                if (prim & kAnalyticBit) {
-                    const tfg_analytic q = P.analytic[prim & ~kAnalyticBit];
-                    mat_id = q.material;
-                    n = analytic_normal(q, p, d);
-                } else {
-                    const tfg_tri_shading sh = P.shading[prim];
-                    mat_id = sh.material;
-                    n = shading_normal(sh, ld3(sh.ng), h.y, h.z, d);
-                }
-                const Mat m = load_mat(&P.mats[mat_id]);
-                const float cos_theta = fabsf(dot(n, -d));
-                const bool diffuse = mat_diffuse(m), glossy = mat_glossy(m), delta = mat_delta(m);
-                bool foot_ok = false;
-                if (kNrc) {   // render.h:86-102 path-footprint spread
-                    const float dist_sq = t * t;
-                    if (depth == 0) {
-                        foot.x = dist_sq / (4.0f * kPi * fmaxf(cos_theta, 1e-4f));
-                        foot.y = 0.0f;
-                    } else {
-                        const float spread = dist_sq / (fmaxf(pdf, 1e-4f) * fmaxf(cos_theta, 1e-4f));
-                        if (!(fl & PF_HEAD)) foot.y += spread;
-                        else foot.z += spread;
-                    }
-                    foot_ok = foot.y >= foot.x * 0.01f;
-                }
-                bool alive = true;
-                if (vmax(m.le) > 0.0f) {   // render.h:104-123
-                    rad = rad + thr * m.le;
-                    rad_dirty = true;
-                    if (kNrc && (fl & PF_HEAD)) {
-                        float4* tl = P.tail + (size_t)kVertF4 * pid;
-                        Mat z = m;
-                        z.kd = v3(0.f, 0.f, 0.f);
-                        z.ks = v3(0.f, 0.f, 0.f);
-                        pack_vertex(tl, p, -d, n, z);
-                        P.tsuf[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
-                        fl |= PF_TAIL;
-                    }
-                    alive = false;
-                } else {
-                    const V3 wo = -d;
-                    if (diffuse || glossy) {   // NEE request (render.h:126-131)
-                        want_shadow = true;
-                        e0 = f4(p + n * kEps, __uint_as_float(pid));
-                        e1 = f4(n, __uint_as_float(mat_id));
-                        e2 = f4(wo, 0.0f);
-                        e3 = f4(thr, 0.0f);
-                    }
-                    bool done = false;
-                    const V3 dm = v3(wo.x * 0.5f + 0.5f, wo.y * 0.5f + 0.5f, wo.z * 0.5f + 0.5f);
-                    const V3 nm = v3(n.x * 0.5f + 0.5f, n.y * 0.5f + 0.5f, n.z * 0.5f + 0.5f);
-                    if (kNrc && (fl & PF_HEAD)) {   // render.h:134-161 suffix end
-                        const bool stop = !(fl & PF_UNBIASED) && foot.z >= foot.w;
-                        if (stop) {
-                            pack_vertex(P.tail + (size_t)kVertF4 * pid, p, dm, nm, m);
-                            P.tsuf[pid] = f4(thr * (m.kd + m.ks + m.kt), 0.0f);
-                            fl |= PF_TAIL;
-                            alive = false;
-                            done = true;
-                        }
-                    }
-                    if (kNrc && !done && diffuse && !delta && foot_ok) {   // render.h:166-202
-                        V3 refl = m.kd + m.ks + m.kt;
-                        refl = v3(fmaxf(refl.x, 1e-3f), fmaxf(refl.y, 1e-3f), fmaxf(refl.z, 1e-3f));
-                        if (!(fl & PF_TRAIN) && (P.mode & MODE_INFER)) {
-                            pack_vertex(P.head + (size_t)kVertF4 * pid, p, dm, nm, m);
-                            P.tthr[pid] = f4(thr * refl, 0.0f);
-                            fl |= PF_QUERY;
-                            alive = false;
-                            done = true;
-                        } else if ((fl & PF_TRAIN) && !(fl & PF_HEAD)) {
-                            pack_vertex(P.head + (size_t)kVertF4 * pid, p, dm, nm, m);
-                            P.tthr[pid] = f4(thr * refl, 0.0f);
-                            P.pre[pid] = f4(rad, 0.0f);
-                            foot.w = foot.y;
-                            fl |= PF_HEAD | (depth << PF_HEAD_DEPTH_SHIFT);
-                        }
-                    }
-                    if (!done) {   // calcularLuzIndirecta, render.h:264-335
-                        const float mkd = vmax(m.kd), mks = vmax(m.ks), mkt = vmax(m.kt);
-                        const float sum = mkd + mks + mkt;
-                        V3 wi = v3(0.f, 0.f, 1.f);
-                        if (!(sum > 0.0f)) {
-                            alive = false;
-                        } else {
-                            const float pd = mkd / sum, ps = mks / sum, pt = 1.0f - pd - ps;
-                            const float r = pcg32_float(s);
-                            if (r < pd) {
-                                thr = cdiv(thr * m.kd, pd);
-                                const float r1 = pcg32_float(s);
-                                const float r2 = pcg32_float(s);
-                                const V3 wl = cosine_sample(r1, r2);
-                                if (wl.z <= kEps) {
-                                    alive = false;
-                                } else {
-                                    wi = to_world(make_frame(n), wl);
-                                    pdf = fmaxf(fabsf(dot(wi, n)), 1e-4f) / kPi;
-                                }
-                            } else if (r < pd + ps) {
-                                if (glossy) {
-                                    const float r1 = pcg32_float(s);
-                                    const float r2 = pcg32_float(s);
-                                    const Frame fr = make_frame(n);
-                                    const V3 wol = to_local(fr, wo);
-                                    const float alpha = ggx_alpha(m.rough);
-                                    if (wol.z <= 0.0f) {
-                                        alive = false;
-                                    } else {
-                                        const V3 hm = ggx_sample_vndf(wol, alpha, r1, r2);
-                                        const float oh = dot(wol, hm);
-                                        const V3 wil = hm * (2.0f * oh) - wol;
-                                        if (wil.z <= 0.0f) {
-                                            alive = false;
-                                        } else {
-                                            const float a2 = alpha * alpha;
-                                            thr = thr * schlick(m.ks, oh) * (ggx_G1(a2, wil.z) / ps);
-                                            wi = normalize(to_world(fr, wil));
-                                            pdf = ggx_D(a2, hm.z) * ggx_G1(a2, wol.z) / (4.0f * wol.z);
-                                        }
-                                    }
-                                } else {   // render.h:337-340
-                                    wi = normalize(d - n * (2.0f * dot(d, n)));
-                                    thr = cdiv(thr * m.ks, ps);
-                                    pdf = 1e6f;
-                                }
-                            } else {   // render.h:342-376
-                                V3 N = n;
-                                float cosi = fminf(1.0f, fmaxf(-1.0f, dot(d, N)));
-                                float etat;
-                                if (cosi > 0.0f) {
-                                    N = -N;
-                                    etat = 1.0f;
-                                } else {
-                                    cosi = -cosi;
-                                    etat = m.ior;
-                                }
-                                const float eta = ior / etat;
-                                const float k = 1.0f - eta * eta * (1.0f - cosi * cosi);
-                                if (k < 0.0f) {
-                                    wi = normalize(d - N * (2.0f * dot(d, N)));
-                                } else {
-                                    ior = etat;
-                                    wi = normalize(d * eta + N * (eta * cosi - sqrtf(k)));
-                                }
-                                thr = cdiv(thr * m.kt, pt);
-                                pdf = 1e6f;
-                            }
-                        }
-                        // Russian roulette, render.h:209-224 (after depth rr_depth)
-                        if (alive && depth > P.rr_depth) {
-                            const float prob = fminf(vmax(thr), 0.95f);
-                            if (pcg32_float(s) > prob) alive = false;
-                            else thr = cdiv(thr, prob);
-                        }
-                        if (alive && !last) {
-                            want_ray = true;
-                            n0 = f4(p + wi * kEps, 0.0f);
-                            n1 = f4(wi, __int_as_float(0x7f800000));
-                        }
-                        P.thr[pid] = f4(thr, pdf);
-                        P.rng[pid] = s;
-                        rad_dirty = rad_dirty || ior != rad4.w;
-                    }
-                }
-            }
-            if (rad_dirty) P.rad[pid] = f4(rad, ior);
-            if (kNrc) {
-                P.foot[pid] = foot;
-                if (fl != fl_in) P.flags[pid] = fl;
-            }
+            shade_entry<kNrc>(P, depth, last, pid, P.rays[q][2 * (size_t)i], P.rays[q][2 * (size_t)i + 1], P.hits[i],
+                              want_shadow, e0, e1, e2, e3, want_ray, n0, n1);
         }
         uint32_t ss, rs;
         block_append2<256>(want_shadow, &P.counters[CNT_SHQ + depth], ss, want_ray, &P.counters[qo], rs);
