@@ -469,12 +469,176 @@ def cpu_baseline_traversal(host, prim, bounce, shadow):
     }
 
 
+def bench_multiview(args, torch):
+    """16 seeded views (256x256 ... 1920x1080, scenes.views) of the 64-instance
+    scene sharing one online-trained radiance cache (BF16), measured two ways:
+      batched: all views on one canvas (tfg_pt_set_views), one chain of
+        launches per frame; per-frame latency p50/p95/p99 (HIP events on the
+        launch stream; a view's pixels are done when the frame is) and
+        aggregate Mrays/s = the canvas's rays / the frames' time;
+      per view: one tracer per view, all on one stream one after the other
+        (the shared cache's training steps keep view order); each view's frame
+        latency p50/p95/p99 and Mrays/s.
+    Then BF16 vs FP16 cache inference on the batched canvas: a fresh cache per
+    precision trained over the same frames, each view's image accumulated over
+    the same frames and scored by relMSE against a plain reference of that
+    view (FP64 accumulation)."""
+    from tfg_amd import experiments, nrc, pt, scenes
+
+    sc = scenes.instances()
+    ds = pt.DeviceScene(sc, torch)
+    views = scenes.views()
+    frames_v = [(cam.frame(W, H), W, H) for cam, W, H in views]
+    total = sum(W * H for _, W, H in views)
+    bases = np.cumsum([0] + [W * H for _, W, H in views])
+    MD, RR = 8, 3
+    lo = sc.verts.reshape(-1, 3).min(0) - 0.5
+    hi = sc.verts.reshape(-1, 3).max(0) + 0.5
+    bounds = dict(bounds_min=tuple(map(float, lo)), bounds_max=tuple(map(float, hi)))
+    train, both = experiments.pt_mode_train(), experiments.pt_mode_train() | experiments.pt_mode_infer()
+    stream = torch.cuda.current_stream()
+    warm, timed = 16, max(args.steps, 16)
+
+    def canvas(precision):
+        tr = pt.PathTracer(total, 1, MD, RR, mode=train, train_fraction=0.05, use_graph=False)
+        tr.set_scene(ds)
+        tr.set_views(frames_v)
+        cache = nrc.RadianceCache(precision=precision, **bounds)
+        tr.set_cache(cache)
+        return tr, cache
+
+    # batched: schedule tuned on plain frames of the canvas
+    tr, cache = canvas(0)
+    tr.set_mode(0)
+    tr.autotune()
+    sched = {k: v for k, v in tr.schedule().items() if k != "use_graph"}
+    _log(f"multiview: canvas of {total} pixels, schedule {sched}")
+    img = torch.zeros((total, 3), dtype=torch.float32, device="cuda")
+    tr.set_mode(train)
+    for f in range(warm):
+        tr.render(f, img, stream=stream.cuda_stream)
+    tr.set_mode(both)
+    for f in range(2):
+        tr.render(warm + f, img, stream=stream.cuda_stream)
+    tr.reset_totals()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(timed)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f, (a, b) in enumerate(ev):
+        a.record(stream)
+        tr.render(warm + 2 + f, img, stream=stream.cuda_stream)
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat_b = np.array([a.elapsed_time(b) for a, b in ev])
+    agg = float(sum(tr.ray_totals()) / wall / 1e6)
+    tr.close()
+    del cache
+    batched = {"mrays_per_s": round(agg, 1), "ms_per_frame": round(wall / timed * 1e3, 4),
+               **{f"p{q}_ms": round(float(np.percentile(lat_b, q)), 4) for q in (50, 95, 99)}}
+    _log(f"multiview batched: {batched}")
+
+    # per view: one tracer per view on one stream, the same schedule
+    cache = nrc.RadianceCache(precision=0, **bounds)
+    trs, imgs = [], []
+    for cam, W, H in frames_v:
+        t = pt.PathTracer(W, H, MD, RR, mode=train, train_fraction=0.05, use_graph=False)
+        t.set_scene(ds)
+        t.set_schedule(**sched)
+        t.set_camera(cam)
+        t.set_cache(cache)
+        trs.append(t)
+        imgs.append(torch.zeros((H * W, 3), dtype=torch.float32, device="cuda"))
+    for r in range(warm + 2):
+        if r == warm:
+            for t in trs:
+                t.set_mode(both)
+        for t, im in zip(trs, imgs):
+            t.render(r, im, stream=stream.cuda_stream)
+    for t in trs:
+        t.reset_totals()
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in views]
+          for _ in range(timed)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(timed):
+        for v, (t, im) in enumerate(zip(trs, imgs)):
+            ev[r][v][0].record(stream)
+            t.render(warm + 2 + r, im, stream=stream.cuda_stream)
+            ev[r][v][1].record(stream)
+    torch.cuda.synchronize()
+    wall_v = time.perf_counter() - t0
+    lat = np.array([[a.elapsed_time(b) for a, b in row] for row in ev])
+    rays = np.array([sum(t.ray_totals()) for t in trs], np.float64)
+    per_view = []
+    for v, (_, W, H) in enumerate(views):
+        per_view.append({"view": v, "width": W, "height": H,
+                         **{f"p{q}_ms": round(float(np.percentile(lat[:, v], q)), 4) for q in (50, 95, 99)},
+                         "mrays_per_s": round(float(rays[v] / timed / (np.median(lat[:, v]) * 1e-3) / 1e6), 1)})
+    sequential = {"mrays_per_s": round(float(rays.sum() / wall_v / 1e6), 1),
+                  "ms_per_round": round(wall_v / timed * 1e3, 4)}
+    _log(f"multiview per view: {sequential}")
+    for t in trs:
+        t.close()
+    del cache
+
+    # BF16 vs FP16 inference on the batched canvas
+    ref_spp, acc_frames = min(args.ref_spp, 512), 32
+    tp = pt.PathTracer(total, 1, MD, RR, mode=0, use_graph=False)
+    tp.set_scene(ds)
+    tp.set_schedule(**sched)
+    tp.set_views(frames_v)
+    acc = torch.zeros((total, 3), dtype=torch.float64, device="cuda")
+    for k in range(ref_spp):
+        tp.render(1_000_000 + k, img, acc)
+    torch.cuda.synchronize()
+    ref = (acc / ref_spp).cpu().numpy()
+    tp.close()
+    rel = {}
+    for name, prec in (("bf16", 0), ("fp16", 1)):
+        tr, cache = canvas(prec)
+        tr.set_schedule(**sched)
+        for f in range(warm):
+            tr.render(f, img)
+        tr.set_mode(both)
+        acc.zero_()
+        for f in range(acc_frames):
+            tr.render(warm + f, img, acc)
+        torch.cuda.synchronize()
+        mean = (acc / acc_frames).cpu().numpy()
+        rel[name] = [experiments.relmse(mean[bases[v]:bases[v + 1]], ref[bases[v]:bases[v + 1]])
+                     for v in range(len(views))]
+        tr.close()
+        del cache
+    for v in range(len(views)):
+        per_view[v]["relmse_bf16"] = rel["bf16"][v]
+        per_view[v]["relmse_fp16"] = rel["fp16"][v]
+    return {
+        "metric": "C5 batched multi-view: aggregate Mrays/s of 16 views on one canvas sharing one online-trained "
+                  "cache",
+        "value": batched["mrays_per_s"],
+        "unit": "Mrays/s",
+        "ms_per_step": batched["ms_per_frame"],
+        "batched": batched,
+        "per_view_tracers": sequential,
+        "views": per_view,
+        "relmse_mean": {k: float(np.mean(v)) for k, v in rel.items()},
+        "relmse_fp16_over_bf16_mean": float(np.mean(np.array(rel["fp16"]) / np.array(rel["bf16"]))),
+        "dtype": "fp32 tracing, bf16 cache (fp16 compared)",
+        "data": "synthetic (seeded scene and camera poses)",
+        "config": {"workload": "16 views (256x256 ... 1920x1080, scenes.views seeds) of the 64-instance scene "
+                               "(2.57M tris, 4 lights), depth 8, RR after 3, one shared cache, train+infer frames",
+                   "canvas_pixels": int(total), "schedule": sched, "frames": timed, "warmup_frames": warm,
+                   "relmse_reference_spp": ref_spp, "relmse_frames": acc_frames},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt1080", choices=["pt1080", "pt", "traversal", "copy", "nrc"])
+    ap.add_argument("--workload", default="pt1080", choices=["pt1080", "pt", "traversal", "copy", "nrc", "multiview"])
     ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the relMSE comparison")
     ap.add_argument("--budget", type=float, default=2.0, help="equal-time budget in seconds per method")
     ap.add_argument("--no-cpu", action="store_true")
@@ -495,6 +659,12 @@ def main():
                  "xcd_group": 4096}
         res = bench_nrc_kernels(torch, scenes.instances(), sched, reps=args.steps)
         print(json.dumps({"metric": "cache kernel throughput (MFMA)", **res}), flush=True)
+        return
+    if args.workload == "multiview":
+        res = bench_multiview(args, torch)
+        res.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+                    "scaling": "weak", "vs_baseline": None})
+        print(json.dumps(res), flush=True)
         return
     if args.workload == "copy":
         res = bench_copy(args, torch)

@@ -119,6 +119,22 @@ typedef struct tfg_pt_scene {
 int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out);
 int tfg_pt_set_scene(tfg_pt* pt, const tfg_pt_scene* scene);
 int tfg_pt_set_camera(tfg_pt* pt, const tfg_camera* cam);
+/* Several cameras traced in one frame (one chain of launches): the canvas of
+ * the config (width x height pixels) holds the views' images one after the
+ * other, row-major, view v at pixels base_v .. base_v + W_v*H_v - 1 with
+ * base_0 = 0 and base_{v+1} = base_v + W_v*H_v; the views' pixels must fill
+ * the canvas exactly.  Each view's pixels get the seeds and camera rays a
+ * tracer of that view's size with tfg_pt_set_camera would give them, so in a
+ * plain frame each view's image equals that tracer's, bit for bit.  Every
+ * path of the canvas shares the frame's stages, and in cache mode one
+ * radiance cache.  1..64 views of 1..65535 pixels each way.  Replaces the
+ * camera; tfg_pt_set_camera returns to one camera.  No reference
+ * counterpart (the reference renders one camera per process, main.cpp). */
+typedef struct tfg_pt_view {
+    uint32_t width, height;
+    tfg_camera camera;   /* the view's camera at its own resolution */
+} tfg_pt_view;
+int tfg_pt_set_views(tfg_pt* pt, const tfg_pt_view* views, uint32_t n_views);
 /* Renders frame `frame` (1 sample per pixel): writes d_image[W*H] and, if
  * d_accum is not NULL, adds the frame to d_accum[3*W*H] (FP64). */
 int tfg_pt_render(tfg_pt* pt, int32_t frame, tfg_color* d_image, double* d_accum, tfg_stream stream);

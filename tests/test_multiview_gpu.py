@@ -1,10 +1,14 @@
-"""Sixteen seeded views of the 64-instance scene (2,567,554 triangles, one
-CWBVH) at resolutions from 256x256 to 1920x1080, each traced by the plain
-wavefront path tracer and checked against the CPU oracle integrator.
+"""Seeded views of the 64-instance scene (2,567,554 triangles, one CWBVH) at
+resolutions from 256x256 to 1920x1080.
 
-Per view: the whole image is finite, and on a band of 4 rows across the
-middle of the image at least 97% of the pixels follow the oracle's path
-(same PCG32 streams, rng.h) to 1e-3 relative; over all views at least 98%.
+Sixteen views, each traced by the plain wavefront path tracer, are checked
+against the CPU oracle integrator: per view the whole image is finite, and on
+a band of 4 rows across the middle of the image at least 97% of the pixels
+follow the oracle's path (same PCG32 streams, rng.h) to 1e-3 relative; over
+all views at least 98%.
+
+Several views traced as one canvas (tfg_pt_set_views: one chain of launches
+for all of them) give each view the image its own tracer gives, bit for bit.
 """
 import dataclasses
 
@@ -17,10 +21,15 @@ from tfg_amd import pt, scenes
 pytestmark = pytest.mark.gpu
 
 
-def test_sixteen_views_match_oracle(torch_cuda):
-    torch = torch_cuda
+@pytest.fixture(scope="module")
+def scene64(torch_cuda):
     sc = scenes.instances()
-    ds = pt.DeviceScene(sc, torch)
+    return sc, pt.DeviceScene(sc, torch_cuda)
+
+
+def test_sixteen_views_match_oracle(torch_cuda, scene64):
+    torch = torch_cuda
+    sc, ds = scene64
     md, rr = 8, 3
     agree_all = []
     for k, (cam, W, H) in enumerate(scenes.views()):
@@ -47,3 +56,71 @@ def test_sixteen_views_match_oracle(torch_cuda):
         assert ok.mean() >= 0.97
         agree_all.append(ok.ravel())
     assert np.concatenate(agree_all).mean() >= 0.98
+
+
+def test_canvas_views_match_single_view_tracers(torch_cuda, scene64):
+    """Five views (256x256 ... 768x432) on one canvas against five single-view
+    tracers: equal images bit for bit and equal ray counts, plain frames, with
+    and without pixel tiles cutting across the views; then cache frames on the
+    canvas (one shared cache) stay finite and query the cache."""
+    from tfg_amd import _lib, experiments, nrc
+
+    torch = torch_cuda
+    sc, ds = scene64
+    md, rr = 8, 3
+    views = [(cam.frame(W, H), W, H) for cam, W, H in scenes.views()[:5]]
+    total = sum(W * H for _, W, H in views)
+    singles = []
+    for f in (3, 4):
+        imgs, rays = [], [0, 0]
+        for cam, W, H in views:
+            tr = pt.PathTracer(W, H, md, rr, mode=0, use_graph=False)
+            tr.set_scene(ds)
+            tr.set_camera(cam)
+            img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
+            tr.render(f, img)
+            torch.cuda.synchronize()
+            imgs.append(img.cpu().numpy())
+            e, s_ = tr.rays_last_frame()
+            rays[0] += e
+            rays[1] += s_
+            tr.close()
+        singles.append((np.concatenate(imgs), tuple(rays)))
+    for tiles, use_graph in ((1, False), (3, True)):
+        print(f"canvas tiles={tiles} use_graph={use_graph} pixels={total}", flush=True)
+        tr = pt.PathTracer(total, 1, md, rr, mode=0, use_graph=use_graph)
+        tr.set_scene(ds)
+        bases = tr.set_views(views)
+        assert bases[1] == views[0][1] * views[0][2]
+        tr.set_schedule(tiles=tiles)
+        img = torch.zeros((total, 3), dtype=torch.float32, device="cuda")
+        for f, (ref, rays) in zip((3, 4), singles):
+            tr.render(f, img)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(img.cpu().numpy(), ref)
+            assert tr.rays_last_frame() == rays
+        tr.close()
+    # cache frames: all views train and query one cache
+    tr = pt.PathTracer(total, 1, md, rr, mode=experiments.pt_mode_train(), use_graph=False)
+    tr.set_scene(ds)
+    tr.set_views(views)
+    lo = sc.verts.reshape(-1, 3).min(0) - 0.5
+    hi = sc.verts.reshape(-1, 3).max(0) + 0.5
+    cache = nrc.RadianceCache(bounds_min=tuple(map(float, lo)), bounds_max=tuple(map(float, hi)))
+    tr.set_cache(cache)
+    img = torch.zeros((total, 3), dtype=torch.float32, device="cuda")
+    for f in range(4):
+        tr.render(f, img)
+    tr.set_mode(experiments.pt_mode_train() | experiments.pt_mode_infer())
+    for f in range(4, 6):
+        tr.render(f, img)
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(img).all())
+    assert cache.stats()["steps"] > 0
+    _, qry = tr.read_cache_data()
+    assert qry.shape[0] > total // 10
+    # the views must fill the canvas exactly
+    for bad in (views[:4], views + views[:1], [(views[0][0], 70000, 1)]):
+        with pytest.raises(_lib.TfgError):
+            tr.set_views(bad)
+    tr.close()

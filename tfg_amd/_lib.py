@@ -116,6 +116,7 @@ _SIGS = {
     "tfg_pt_create": (C.c_int, [C.POINTER(PtConfig), C.POINTER(C.c_void_p)]),
     "tfg_pt_set_scene": (C.c_int, [C.c_void_p, C.POINTER(PtScene)]),
     "tfg_pt_set_camera": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tfg_pt_set_views": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "tfg_pt_render": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "tfg_pt_counters": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tfg_pt_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),

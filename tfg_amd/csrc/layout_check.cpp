@@ -5,6 +5,7 @@
 // RegistroEntrenamiento 232, SceneBounds 48, ImagenGPU 16, TinyBVHD_GPU 24).
 #include <cstddef>
 
+#include "../../include/tfg.h"
 #include "../../include/tfg_types.h"
 
 static_assert(sizeof(tfg_vec3d) == 24, "Vector3d");
@@ -30,3 +31,4 @@ static_assert(sizeof(tfg_ray) == 32 && sizeof(tfg_hit) == 16, "ray/hit");
 static_assert(sizeof(tfg_material) == 64 && sizeof(tfg_tri_shading) == 64, "material/shading");
 static_assert(sizeof(tfg_point_light) == 32 && sizeof(tfg_camera) == 64, "light/camera");
 static_assert(sizeof(tfg_analytic) == 48, "analytic");
+static_assert(sizeof(tfg_pt_view) == 72 && offsetof(tfg_pt_view, camera) == 8, "tfg_pt_view");

@@ -465,10 +465,14 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
     (void)u32p;
     int32_t* dframe = nullptr;
     tfg_camera* dcam = nullptr;
+    PtView* dviews = nullptr;
     if (rc == TFG_OK) rc = dalloc<int32_t>(pt, &dframe, 1);
     if (rc == TFG_OK) rc = dalloc<tfg_camera>(pt, &dcam, 1);
+    if (rc == TFG_OK) rc = dalloc<PtView>(pt, &dviews, kMaxViews);
     P.frame = dframe;
     P.cam = dcam;
+    P.views = dviews;
+    P.n_views = 0;
     P.gcounters = P.counters;
     if (rc == TFG_OK) {
         int dev = 0;
@@ -556,6 +560,40 @@ int tfg_pt_set_scene(tfg_pt* pt, const tfg_pt_scene* sc) {
 int tfg_pt_set_camera(tfg_pt* pt, const tfg_camera* cam) {
     if (!pt || !cam) { set_error("tfg_pt_set_camera: NULL argument"); return TFG_E_INVALID; }
     TFG_HIP_CHECK(hipMemcpy(const_cast<tfg_camera*>(pt->P.cam), cam, sizeof(*cam), hipMemcpyHostToDevice));
+    if (pt->P.n_views) drop_graph(pt);   // captured launches carry n_views
+    pt->P.n_views = 0;
+    pt->camera_set = true;
+    return TFG_OK;
+}
+
+int tfg_pt_set_views(tfg_pt* pt, const tfg_pt_view* views, uint32_t n_views) {
+    if (!pt || (!views && n_views)) { set_error("tfg_pt_set_views: NULL argument"); return TFG_E_INVALID; }
+    if (n_views == 0 || n_views > kMaxViews) {
+        set_error("tfg_pt_set_views: 1..%u views", kMaxViews);
+        return TFG_E_LIMIT;
+    }
+    PtView hv[kMaxViews] = {};
+    uint64_t total = 0;
+    for (uint32_t v = 0; v < n_views; ++v) {
+        const uint32_t w = views[v].width, h = views[v].height;
+        if (w == 0 || h == 0 || w > 65535 || h > 65535) {
+            set_error("tfg_pt_set_views: view %u is %ux%u (1..65535 each way)", v, w, h);
+            return TFG_E_INVALID;
+        }
+        hv[v].base = (uint32_t)total;
+        hv[v].width = w;
+        hv[v].height = h;
+        hv[v].cam = views[v].camera;
+        total += (uint64_t)w * h;
+    }
+    if (total != pt->P.n_paths) {
+        set_error("tfg_pt_set_views: the views hold %llu pixels, the canvas %u (tfg_pt_config width x height)",
+                  (unsigned long long)total, pt->P.n_paths);
+        return TFG_E_INVALID;
+    }
+    TFG_HIP_CHECK(hipMemcpy(const_cast<PtView*>(pt->P.views), hv, sizeof(PtView) * n_views, hipMemcpyHostToDevice));
+    drop_graph(pt);   // captured launches carry n_views
+    pt->P.n_views = n_views;
     pt->camera_set = true;
     return TFG_OK;
 }

@@ -33,6 +33,7 @@ enum : uint32_t {
     CNT_ALLOC = CNT_SHQ + kMaxDepth,
 };
 constexpr uint32_t kMaxTiles = 4;   // pixel tiles traced as independent stream chains
+constexpr uint32_t kMaxViews = 64;  // cameras sharing one canvas (tfg_pt_set_views)
 constexpr uint32_t kMaxSide = 4;    // side streams per tile for the shadow-ray stage
 
 // path flags
@@ -51,6 +52,13 @@ enum : uint32_t {
     MODE_TRAIN = 1u << 1,   // entrenar_red
 };
 
+// One view of a multi-view canvas: its pixels are canvas pixels
+// base .. base + width * height - 1, row-major.
+struct PtView {
+    uint32_t base, width, height, pad;
+    tfg_camera cam;
+};
+
 struct PtParams {
     // scene
     const float4* nodes;
@@ -67,6 +75,8 @@ struct PtParams {
     const tfg_analytic* analytic;
     uint32_t n_analytic;
     const tfg_camera* cam;
+    const PtView* views;     // n_views > 0: the canvas holds these views, cam unused
+    uint32_t n_views;
     const int32_t* frame;   // device frame index (seed), read by generate
     uint32_t width, height;
     uint32_t n_paths;        // paths of this tile (path ids 0..n_paths-1)

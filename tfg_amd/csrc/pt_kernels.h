@@ -55,15 +55,33 @@ __global__ __launch_bounds__(256) void k_generate(PtParams P) {
             P.counters[CNT_Q1] = 0u;
         }
     }
-    const tfg_camera cam = *P.cam;
     const int32_t frame = *P.frame;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n_paths; i += gridDim.x * blockDim.x) {
         const uint32_t px = P.pix_base + i;
-        const int x = (int)(px % P.width), y = (int)(px / P.width);
+        // the pixel's camera and its pixel within that camera's image (a
+        // view of a multi-view canvas is traced as a tracer of its own size
+        // would trace it: same seed, same ray)
+        tfg_camera cam;
+        uint32_t lp = px, w = P.width, h = P.height;
+        if (P.n_views == 0) {
+            cam = *P.cam;
+        } else {
+            uint32_t lo = 0, hi = P.n_views - 1;   // last view with base <= px
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (P.views[mid].base <= px) lo = mid;
+                else hi = mid - 1;
+            }
+            cam = P.views[lo].cam;
+            lp = px - P.views[lo].base;
+            w = P.views[lo].width;
+            h = P.views[lo].height;
+        }
+        const int x = (int)(lp % w), y = (int)(lp / w);
         uint32_t s = pixel_seed(x, y, frame);
         const float ox = pcg32_float(s);
         const float oy = pcg32_float(s);
-        const V3 d = camera_dir(cam, x, y, (int)P.width, (int)P.height, ox, oy);
+        const V3 d = camera_dir(cam, x, y, (int)w, (int)h, ox, oy);
         uint32_t fl = 0;
         // render.h:443: training pixel draw only when both flags are set
         if ((P.mode & MODE_TRAIN) && (P.mode & MODE_INFER)) {

@@ -37,6 +37,11 @@ class DeviceScene:
             self.desc.n_analytic = an.shape[0]
 
 
+# tfg_pt_view: width, height, tfg_camera (16 floats)
+VIEW_DTYPE = np.dtype([("width", np.uint32), ("height", np.uint32), ("camera", np.float32, 16)])
+assert VIEW_DTYPE.itemsize == 72
+
+
 class PathTracer:
     def __init__(self, width, height, max_depth, rr_depth, mode=0, train_fraction=0.05, use_graph=True):
         self.cfg = _lib.PtConfig(width, height, max_depth, rr_depth, mode, train_fraction, 1 if use_graph else 0)
@@ -52,6 +57,22 @@ class PathTracer:
         c = np.ascontiguousarray(cam_frame, np.float32)
         assert c.size == 16
         check(lib().tfg_pt_set_camera(self._h, c.ctypes.data), "tfg_pt_set_camera")
+
+    def set_views(self, views):
+        """views: (camera frame (16 floats, tfg_camera), width, height) per view;
+        their pixels, one view after the other, fill the tracer's canvas.
+        Returns the canvas offset of each view."""
+        rec = np.zeros(len(views), VIEW_DTYPE)
+        bases, b = [], 0
+        for k, (cam_frame, w, h) in enumerate(views):
+            c = np.ascontiguousarray(cam_frame, np.float32)
+            assert c.size == 16
+            rec[k]["width"], rec[k]["height"], rec[k]["camera"] = w, h, c
+            bases.append(b)
+            b += w * h
+        check(lib().tfg_pt_set_views(self._h, rec.ctypes.data if len(views) else None, len(views)),
+              "tfg_pt_set_views")
+        return bases
 
     def set_cache(self, cache):
         self._cache = cache
