@@ -48,10 +48,14 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE runs of this file's workloads, scaled by the same counters on a
 # 1 GiB copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Sources: the summaries in
 # profiles/ named in "traffic_source".
+# HBM bytes (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration) from the
+# committed rocprofv3 PMC summaries: per launch for the traversal microbench,
+# per frame (all launches of the stage) for the path tracer's stages, whose
+# launches per frame depend on the tuned schedule
 MEASURED_TRAFFIC = {
-    "k_trace_bounce": (358.1e6, "profiles/k_trace_bounce_displaced_sphere_1M.md"),
-    "k_extend": (1515.0e6 / 8, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
-    "k_connect": (2451.4e6 / 8, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
+    "k_trace_bounce": (366.0e6, "profiles/k_trace_bounce_displaced_sphere_1M.md"),
+    "k_extend": (1443.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
+    "k_connect": (2302.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
 }
 
 
@@ -244,7 +248,8 @@ def bench_pt(args, torch):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": MEASURED_TRAFFIC["k_" + dom][0],
+            "traffic": MEASURED_TRAFFIC["k_" + dom][0] / n_launch,
+            "traffic_per_frame": MEASURED_TRAFFIC["k_" + dom][0],
             "traffic_source": MEASURED_TRAFFIC["k_" + dom][1],
             "algorithmic_bytes": int(dom_bytes),
             "algorithmic_bytes_per_launch": int(dom_bytes / n_launch),
@@ -351,6 +356,14 @@ def bench_nrc_kernels(torch, sc, schedule, reps=20):
         return a.elapsed_time(b) / reps
 
     ms_inf = timed(lambda: cache.infer_queries(q_ptr, c_ptr + 4 * 4, img.data_ptr(), stream=sh))
+    # the configuration's query count (~2M per frame): the frame's queries tiled
+    # to 2^21 (their pixel ids stay inside the image)
+    n_big = 1 << 21
+    big = np.ascontiguousarray(np.resize(qry, (n_big, qry.shape[1])))
+    d_big = torch.from_numpy(big).cuda()
+    d_cnt = torch.tensor([n_big], dtype=torch.int32, device="cuda")
+    ms_big = timed(lambda: cache.infer_queries(d_big.data_ptr(), d_cnt.data_ptr(), img.data_ptr(), stream=sh))
+    big_tf = n_big * NRC_INFER_FLOP / (ms_big * 1e-3) / 1e12
     steps = int(cache.cfg.steps_per_frame)
     batch = int(cache.cfg.batch)
     ms_train = timed(lambda: cache.train_ptr(r_ptr, c_ptr + 4 * 3, steps=steps, stream=sh))
@@ -361,6 +374,9 @@ def bench_nrc_kernels(torch, sc, schedule, reps=20):
         "inference": {"kernel": "k_nrc_infer (+ composite)", "queries": int(n_q), "ms": ms_inf,
                       "flop_per_query": NRC_INFER_FLOP, "achieved_tflops": inf_tf,
                       "mfma_frac": inf_tf / MFMA_BF16_PEAK_TFLOPS},
+        "inference_2m": {"kernel": "k_nrc_infer (+ composite)", "queries": n_big, "ms": ms_big,
+                         "achieved_tflops": big_tf, "mfma_frac": big_tf / MFMA_BF16_PEAK_TFLOPS,
+                         "note": "the frame's queries tiled to 2^21"},
         "training": {"kernels": "k_nrc_train + k_nrc_grad_reduce + k_nrc_adam", "steps": steps, "batch": batch,
                      "records_available": int(n_rec), "ms": ms_train, "flop_per_sample": NRC_TRAIN_FLOP,
                      "achieved_tflops": tr_tf, "mfma_frac": tr_tf / MFMA_BF16_PEAK_TFLOPS},
