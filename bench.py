@@ -184,8 +184,12 @@ def bench_pt(args, torch):
 
     t0 = time.time()
     sc = scenes.instances()
+    t_scene = time.time() - t0
+    t0 = time.time()
     ds = pt.DeviceScene(sc, torch)
-    _log(f"scene {sc.n_tris} tris, {ds.host.num_nodes} nodes, depth {ds.host.max_depth} ({time.time() - t0:.1f}s)")
+    t_build = time.time() - t0
+    _log(f"scene {sc.n_tris} tris, {ds.host.num_nodes} nodes, depth {ds.host.max_depth} "
+         f"(scene {t_scene:.1f}s, CWBVH build + upload {t_build:.1f}s)")
     W, H, MD, RR = 1920, 1080, 8, 3
     img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
     accum = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
@@ -216,6 +220,21 @@ def bench_pt(args, torch):
     rays_frame = (ext + shd) / args.steps
     value = rays_frame / (ms_frame * 1e-3) / 1e6
     prof = tracer.profile_frame(1000, img)
+    # GPU refit of the tree to its vertices (the animated-geometry path; with
+    # the build's own vertices the tree is unchanged), HIP events on the stream
+    from tfg_amd import _lib
+
+    refitter = _lib.Refitter(ds.host)
+    d_verts = torch.from_numpy(np.ascontiguousarray(sc.verts, np.float32).reshape(-1, 9)).cuda()
+    refitter.refit(ds.bvh, d_verts, stream=stream.cuda_stream)
+    ra, rb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ra.record(stream)
+    for _ in range(5):
+        refitter.refit(ds.bvh, d_verts, stream=stream.cuda_stream)
+    rb.record(stream)
+    torch.cuda.synchronize()
+    refit_ms = ra.elapsed_time(rb) / 5
+    refitter.close()
     ms_ext = sum(prof["ms_extend"])
     n_ext = sum(prof["ext_rays"])
     ext_bytes = 48 * n_ext + 80 * prof["ext_nodes"] + 48 * prof["ext_tris"]
@@ -270,6 +289,8 @@ def bench_pt(args, torch):
                              "shade": [round(x, 4) for x in prof["ms_shade"]],
                              "connect": [round(x, 4) for x in prof["ms_connect"]]},
             "launch": shape,
+            "cwbvh_build_s": round(t_build, 2),
+            "cwbvh_refit_ms": round(refit_ms, 3),
             "seeds": {"scene": sc.seed, "frames": [1000, 1000 + args.steps - 1]},
         },
         "wall_s": wall,

@@ -69,6 +69,24 @@ void tfg_cwbvh_host_free(tfg_cwbvh_host* h);
  * (tinybvh_wrapper.cpp:59-81).  Free with tfg_cwbvh_gpu_free. */
 int tfg_cwbvh_upload(const tfg_cwbvh_host* h, tfg_cwbvh_gpu* out);
 int tfg_cwbvh_gpu_free(tfg_cwbvh_gpu* g);
+/* Copies an uploaded tree back to host memory (20 floats per node, 16 per
+ * triangle record); synchronises the device. */
+int tfg_cwbvh_gpu_download(const tfg_cwbvh_gpu* g, float* nodes, float* tris);
+/* Refit of an uploaded tree to moved vertices (animated geometry whose
+ * triangles keep their identity, e.g. rigidly moved instances): the tree is
+ * kept, the triangle records are rewritten from the new vertices and every
+ * node's quantisation frame and child bounds are recomputed bottom-up on
+ * the GPU, one launch per level.  Same box and quantisation arithmetic as
+ * the builder: with unchanged vertices the uploaded tree is reproduced bit
+ * for bit.  The refit object is made once from the host tree (its level
+ * ranges plus a device box per node); d_verts holds 9 finite floats per
+ * triangle in the build's input order.  Asynchronous on `stream`.  No
+ * reference counterpart (the reference rebuilds, tinybvh_wrapper.cpp:53). */
+typedef struct tfg_cwbvh_refitter tfg_cwbvh_refitter;
+int tfg_cwbvh_refit_create(const tfg_cwbvh_host* h, tfg_cwbvh_refitter** out);
+int tfg_cwbvh_refit(tfg_cwbvh_refitter* r, tfg_cwbvh_gpu* bvh, const float* d_verts, uint32_t n_tris,
+                    tfg_stream stream);
+int tfg_cwbvh_refit_destroy(tfg_cwbvh_refitter* r);
 
 /* ---- traversal ----------------------------------------------------------
  * Device function traverse_cwbvh (traverse_cwbvh.cuh:51-257) as batched

@@ -105,6 +105,10 @@ _SIGS = {
     "tfg_cwbvh_host_free": (None, [C.c_void_p]),
     "tfg_cwbvh_upload": (C.c_int, [C.c_void_p, C.POINTER(CwbvhGpu)]),
     "tfg_cwbvh_gpu_free": (C.c_int, [C.POINTER(CwbvhGpu)]),
+    "tfg_cwbvh_gpu_download": (C.c_int, [C.POINTER(CwbvhGpu), C.c_void_p, C.c_void_p]),
+    "tfg_cwbvh_refit_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "tfg_cwbvh_refit": (C.c_int, [C.c_void_p, C.POINTER(CwbvhGpu), C.c_void_p, C.c_uint32, C.c_void_p]),
+    "tfg_cwbvh_refit_destroy": (C.c_int, [C.c_void_p]),
     "tfg_trace_closest": (
         C.c_int,
         [C.POINTER(CwbvhGpu), C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
@@ -269,9 +273,43 @@ class DeviceBvh:
             "tfg_trace_any",
         )
 
+    def download(self):
+        """(nodes (n, 20), tris (m, 16)) float32 copies of the device tree."""
+        nodes = np.zeros((self.desc.num_nodos, 20), np.float32)
+        tris = np.zeros((self.desc.num_tris, 16), np.float32)
+        check(lib().tfg_cwbvh_gpu_download(C.byref(self.desc), nodes.ctypes.data, tris.ctypes.data),
+              "tfg_cwbvh_gpu_download")
+        return nodes, tris
+
     def close(self):
         if self.desc.d_nodos:
             check(lib().tfg_cwbvh_gpu_free(C.byref(self.desc)), "tfg_cwbvh_gpu_free")
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Refitter:
+    """GPU refit of a device tree to moved vertices (tfg_cwbvh_refit): made
+    once from the host tree the device tree was uploaded from."""
+
+    def __init__(self, host: CwbvhHost):
+        self._h = C.c_void_p()
+        self.num_tris = host.num_tris
+        check(lib().tfg_cwbvh_refit_create(host._h, C.byref(self._h)), "tfg_cwbvh_refit_create")
+
+    def refit(self, dev: DeviceBvh, verts, stream=None):
+        """verts: device tensor of 9 float32 per triangle, the build's input order."""
+        assert verts.numel() == 9 * self.num_tris
+        check(lib().tfg_cwbvh_refit(self._h, C.byref(dev.desc), ptr(verts), self.num_tris, stream), "tfg_cwbvh_refit")
+
+    def close(self):
+        if self._h:
+            lib().tfg_cwbvh_refit_destroy(self._h)
+            self._h = C.c_void_p()
 
     def __del__(self):
         try:

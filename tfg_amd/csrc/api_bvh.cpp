@@ -47,10 +47,6 @@ void register_tree_depth(const void* d_nodes, uint32_t depth, uint32_t format = 
 
 }  // namespace tfg
 
-struct tfg_cwbvh_host {
-    tfg::CwbvhHost b;
-};
-
 extern "C" {
 
 const char* tfg_last_error(void) { return tfg::g_error.c_str(); }
@@ -144,6 +140,17 @@ int tfg_cwbvh_upload(const tfg_cwbvh_host* h, tfg_cwbvh_gpu* out) {
     out->num_nodos = h->b.num_nodes;
     out->num_tris = h->b.num_tris;
     tfg::register_tree_depth(dn, h->b.max_depth, h->b.format);
+    return TFG_OK;
+}
+
+int tfg_cwbvh_gpu_download(const tfg_cwbvh_gpu* g, float* nodes, float* tris) {
+    if (!g || !nodes || !tris || !g->d_nodos || !g->d_tris) {
+        tfg::set_error("tfg_cwbvh_gpu_download: NULL argument");
+        return TFG_E_INVALID;
+    }
+    TFG_HIP_CHECK(hipDeviceSynchronize());
+    TFG_HIP_CHECK(hipMemcpy(nodes, g->d_nodos, (size_t)g->num_nodos * 80, hipMemcpyDeviceToHost));
+    TFG_HIP_CHECK(hipMemcpy(tris, g->d_tris, (size_t)g->num_tris * 64, hipMemcpyDeviceToHost));
     return TFG_OK;
 }
 

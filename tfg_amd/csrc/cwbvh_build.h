@@ -48,3 +48,8 @@ bool build_cwbvh(const float* verts, uint32_t n_tris, uint32_t format, CwbvhHost
                  const BuildOptions& opt = BuildOptions());
 
 }  // namespace tfg
+
+// The C ABI's opaque host tree (tfg_cwbvh_build / tfg_cwbvh_refit_create).
+struct tfg_cwbvh_host {
+    tfg::CwbvhHost b;
+};
