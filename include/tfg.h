@@ -179,9 +179,10 @@ typedef struct tfg_pt_profile {
 } tfg_pt_profile;
 int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_profile* out);
 /* Times candidate schedules (tfg_pt_schedule) on the current scene and keeps
- * the fastest: first the traversal launch shape, then tiles, side streams and
- * overlap, then the XCD super-chunk size, octet_below, connect_per_cu and
- * connect_batch, all timed on direct launches (the tuner captures no graph).
+ * the fastest: first the traversal launch shape together with top_nodes,
+ * then tiles, side streams and overlap, then the XCD super-chunk size,
+ * octet_below, connect_per_cu and connect_batch, all timed on direct
+ * launches (the tuner captures no graph).
  * Drops any captured graph. */
 int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream);
 int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_grid, uint32_t* shade_grid);
@@ -211,6 +212,10 @@ int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_g
  *       launch; k > 0 traces the shadow rays of depths k..max_depth-1 in one
  *       launch after the last shading stage (late depths have few rays, and
  *       one launch of them replaces a chain of small launches);
+ *   top_nodes (0..73): leading tree nodes (breadth-first) each traversal
+ *       workgroup copies to LDS before tracing; more saves global fetches of
+ *       the upper levels but costs every workgroup its staging, which
+ *       matters for grids of many short-lived workgroups (default 9);
  *   use_graph: replay captured hipGraphs (one per tile, plus one for the
  *       cache work) instead of launching the kernels directly; read-only,
  *       fixed by tfg_pt_config.use_graph (set_schedule must pass the same
@@ -222,6 +227,7 @@ typedef struct tfg_pt_schedule {
     uint32_t octet_below;
     uint32_t connect_per_cu;
     uint32_t connect_batch;
+    uint32_t top_nodes;
 } tfg_pt_schedule;
 int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out);
 int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* schedule);

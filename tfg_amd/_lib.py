@@ -64,7 +64,7 @@ class PtSchedule(C.Structure):
     _fields_ = [("trace_block", C.c_uint32), ("grid_mult", C.c_uint32), ("tiles", C.c_uint32),
                 ("side_streams", C.c_uint32), ("overlap", C.c_int32), ("use_graph", C.c_int32),
                 ("xcd_group", C.c_uint32), ("octet_below", C.c_uint32), ("connect_per_cu", C.c_uint32),
-                ("connect_batch", C.c_uint32)]
+                ("connect_batch", C.c_uint32), ("top_nodes", C.c_uint32)]
 
 
 class NrcConfig(C.Structure):

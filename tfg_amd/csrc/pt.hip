@@ -26,6 +26,8 @@ struct tfg_pt {
     int trace_grid = 0;
     int connect_per_cu = 0;   // shadow-stage workgroups per CU (0: the closest-hit grid)
     uint32_t connect_batch = 0;   // depths >= this (if > 0) share one shadow-stage launch
+    uint32_t top_nodes = 9;       // leading tree nodes each traversal workgroup stages in LDS
+    uint32_t scene_nodes = 0;     // nodes of the current scene's tree
     int shade_grid = 0;
     int gen_grid = 0;
     // pixel tiles: each runs its own chain of stages on its own streams
@@ -93,7 +95,7 @@ void drop_graph(tfg_pt* pt) {
     pt->graphs_ready = false;
 }
 
-constexpr uint32_t kTopNodes = 73;   // root + 8 + 64 nodes staged in LDS
+constexpr uint32_t kTopNodes = 73;   // at most root + 8 + 64 nodes staged in LDS
 
 size_t trace_lds(const tfg_pt* pt, int block) {
     return (size_t)pt->P.n_top * 80 + (size_t)pt->P.stack_depth * block * 8;
@@ -510,7 +512,7 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
         tfg_pt_destroy(pt);
         return rc;
     }
-    set_trace_shape(pt, 128, 1);
+    set_trace_shape(pt, 64, 16);   // measured default (top_nodes 9): many short-lived workgroups
     pt->run_graph = cfg->use_graph != 0;
     pt->shade_grid = pt->n_cu * 8;
     pt->gen_grid = pt->n_cu * 8;
@@ -542,7 +544,8 @@ int tfg_pt_set_scene(tfg_pt* pt, const tfg_pt_scene* sc) {
     P.tris = static_cast<const float4*>(sc->bvh.d_tris);
     const uint32_t depth = known_tree_depth(sc->bvh.d_nodos);
     P.stack_depth = depth ? (int)(depth > 1 ? depth - 1 : 1) : TFG_MAX_TREE_DEPTH - 1;
-    P.n_top = sc->bvh.num_nodos < kTopNodes ? sc->bvh.num_nodos : kTopNodes;
+    pt->scene_nodes = sc->bvh.num_nodos;
+    P.n_top = pt->scene_nodes < pt->top_nodes ? pt->scene_nodes : pt->top_nodes;
     P.shading = sc->d_shading;
     P.mats = sc->d_materials;
     P.lights = sc->d_lights;
@@ -744,7 +747,7 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     // a candidate's time is the faster of two interleaved passes of 2 untimed
     // and 8 timed frames (direct launches; the tuner captures no graph).
     auto sweep = [&](int n, auto&& apply) {
-        float best_ms[16];
+        float best_ms[64];
         for (int i = 0; i < n; ++i) best_ms[i] = 1e30f;
         for (int pass = 0; pass < 2 && rc == TFG_OK; ++pass) {
             for (int i = 0; i < n && rc == TFG_OK; ++i) {
@@ -768,11 +771,17 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     pt->sides = 2;
     pt->overlap = true;
     pt->connect_batch = 0;
-    // 1. traversal launch shape (large multipliers give more workgroups than
-    //    chunks: each then traces one chunk and retires)
+    // 1. traversal launch shape and LDS-staged top nodes together (large
+    //    multipliers give more workgroups than chunks: each then traces one
+    //    chunk and retires, which pays only with cheap workgroup start-up)
     const int blocks[3] = {64, 128, 256};
     const int mults[4] = {1, 4, 16, 64};
-    sweep(12, [&](int i) { set_trace_shape(pt, blocks[i / 4], mults[i % 4]); });
+    const uint32_t tops[4] = {0, 1, 9, 73};
+    sweep(48, [&](int i) {
+        pt->top_nodes = tops[i / 12];
+        pt->P.n_top = pt->scene_nodes < pt->top_nodes ? pt->scene_nodes : pt->top_nodes;
+        set_trace_shape(pt, blocks[(i % 12) / 4], mults[i % 4]);
+    });
     // 2. tiles, side streams and overlap
     struct Cand {
         int tiles, sides;
@@ -821,6 +830,7 @@ int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out) {
     out->octet_below = pt->P.octet_below;
     out->connect_per_cu = (uint32_t)pt->connect_per_cu;
     out->connect_batch = pt->connect_batch;
+    out->top_nodes = pt->top_nodes;
     return TFG_OK;
 }
 
@@ -830,10 +840,10 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
     if ((b != 64 && b != 128 && b != 256) || sc->grid_mult == 0 || sc->grid_mult > 64 || sc->tiles == 0 ||
         sc->tiles > kMaxTiles || sc->side_streams == 0 || sc->side_streams > kMaxSide ||
         (sc->xcd_group != 0 && (sc->xcd_group % 256 != 0 || sc->xcd_group > (1u << 20))) ||
-        sc->connect_per_cu > 64 || sc->connect_batch > kMaxDepth) {
+        sc->connect_per_cu > 64 || sc->connect_batch > kMaxDepth || sc->top_nodes > kTopNodes) {
         set_error("tfg_pt_set_schedule: block 64/128/256, grid_mult 1..64, tiles 1..%u, side streams 1..%u, "
-                  "xcd_group 0 or a multiple of 256 up to 2^20, connect_per_cu 0..64, connect_batch 0..%u", kMaxTiles,
-                  kMaxSide, kMaxDepth);
+                  "xcd_group 0 or a multiple of 256 up to 2^20, connect_per_cu 0..64, connect_batch 0..%u, "
+                  "top_nodes 0..%u", kMaxTiles, kMaxSide, kMaxDepth, kTopNodes);
         return TFG_E_INVALID;
     }
     if ((sc->use_graph != 0) != pt->run_graph) {
@@ -841,6 +851,8 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
         return TFG_E_STATE;
     }
     drop_graph(pt);
+    pt->top_nodes = sc->top_nodes;
+    pt->P.n_top = pt->scene_nodes < pt->top_nodes ? pt->scene_nodes : pt->top_nodes;
     set_trace_shape(pt, (int)b, (int)sc->grid_mult);
     pt->tiles = (int)sc->tiles;
     pt->sides = (int)sc->side_streams;
