@@ -439,6 +439,51 @@ def reference_scene(mesh_level: int = 3, seed: int = 0) -> Scene:
                  analytic=analytic_records(items))
 
 
+# main.cpp:769-799: the mesh models of the reference's "concurso" scene, as
+# (file, scale, rotation X/Y/Z in degrees, translation, diffuse, specular).
+# columna_suelo.glb and cabeza.glb are named there but absent from the
+# reference's tree; their loads fail and add nothing, as they do there.
+REFERENCE_MODELS = (
+    ("concurso/columna2.glb", 0.5, (0.0, -90.0, 0.0), (-1.7, -3.3, -6.5), (0.75, 0.65, 0.55), (0.01,) * 3),
+    ("concurso/columna1.glb", 0.5, (0.0, -90.0, 0.0), (1.7, -3.1, -2.5), (0.75, 0.65, 0.55), (0.01,) * 3),
+    ("concurso/columna3.glb", 0.5, (0.0, 180.0, 0.0), (1.7, -3.4, -6.5), (0.75, 0.65, 0.55), (0.01,) * 3),
+    ("concurso/columna_suelo.glb", 0.5, (0.0, -90.0, 0.0), (-1.7, -3.1, -2.5), (0.75, 0.65, 0.55), (0.01,) * 3),
+    ("concurso/suelo2.glb", 0.5, (0.0, 0.0, 0.0), (-4.0, -4.8, -5.2), (0.1, 0.2, 0.4), (0.0,) * 3),
+    ("concurso/suelo_grieta.glb", 1.0, (0.0, 0.0, 0.0), (0.0, -3.3, -5.2), (0.73, 0.73, 0.73), (0.0,) * 3),
+    ("concurso/esfera_rota.glb", 0.5, (30.0, 110.0, 0.0), (1.0, -2.8, -2.0), (0.85, 0.4, 0.3), (0.0,) * 3),
+    ("concurso/cabeza.glb", 0.15, (0.0, 80.0, 30.0), (0.0, -1.4, -2.3), (0.73, 0.73, 0.73), (0.0,) * 3),
+)
+
+
+def reference_models(model_dir: str, seed: int = 0) -> Scene:
+    """The reference's mesh scene (main.cpp:745-830): the base scene's
+    analytic planes, spheres and light (main.cpp:164-345) plus the glTF
+    models of REFERENCE_MODELS loaded from `model_dir` (the reference's
+    modelos/ folder) with tfg_amd.gltf and placed as cargarModelo places
+    them.  Missing model files are skipped."""
+    import os
+
+    from . import gltf
+
+    base = reference_scene(seed=seed)
+    mats = [m for m in base.materials[:-1]]   # the base scene's icosphere material is not used
+    verts, norms, mids = [], [], []
+    for rel, scale, rot, trans, kd, ks in REFERENCE_MODELS:
+        path = os.path.join(model_dir, rel)
+        if not os.path.exists(path):
+            continue
+        v, n = gltf.load(path)
+        v, n = gltf.place(v, n, scale, rot[0], rot[1], rot[2], trans)
+        mats.append(material(kd=kd, ks=ks))
+        verts.append(v)
+        norms.append(n)
+        mids.append(np.full(v.shape[0], len(mats) - 1, np.uint32))
+    if not verts:
+        raise FileNotFoundError(f"no reference model under {model_dir}")
+    return Scene("reference_models", np.concatenate(verts), np.concatenate(norms), np.concatenate(mids),
+                 np.stack(mats), base.lights, base.camera, seed, analytic=base.analytic)
+
+
 def random_soup(n: int, seed: int, extent: float = 1.0) -> np.ndarray:
     """n random small triangles in [-extent, extent]^3 (tests)."""
     rng = np.random.default_rng(seed)
