@@ -675,7 +675,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt1080", choices=["pt1080", "pt", "traversal", "copy", "nrc", "multiview"])
+    ap.add_argument("--workload", default="pt1080", choices=["pt1080", "pt", "traversal", "copy", "nrc", "multiview", "sweep"])
     ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the relMSE comparison")
     ap.add_argument("--budget", type=float, default=2.0, help="equal-time budget in seconds per method")
     ap.add_argument("--no-cpu", action="store_true")
@@ -702,6 +702,16 @@ def main():
         res.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
                     "scaling": "weak", "vs_baseline": None})
         print(json.dumps(res), flush=True)
+        return
+    if args.workload == "sweep":
+        from tfg_amd import experiments, scenes
+
+        t0 = time.time()
+        res = experiments.grid_search(torch, scenes.instances(), 1920, 1080, 8, 3, frames=100, score_from=10)
+        print(json.dumps({"metric": "radiance-cache hyper-parameter sweep (mean training loss, frames 10-99)",
+                          "best": res[0], "results": res, "grid": experiments.SWEEP_GRID,
+                          "config": {"workload": "64-instance scene, 1920x1080, training-only cache frames",
+                                     "frames": 100, "seconds": round(time.time() - t0, 1)}}), flush=True)
         return
     if args.workload == "copy":
         res = bench_copy(args, torch)

@@ -128,3 +128,60 @@ def equal_time(torch, scene, W, H, max_depth, rr_depth, ref_spp, budget_s, warmu
                 "cache": st},
         "relmse_ratio_plain_over_nrc": rp / rn if rn > 0 else None,
     }
+
+
+# gridsearch.h:54-61: the reference's search space for the optimiser settings
+# (its network width/depth, hash-grid size and one-blob bins have no
+# counterpart: this cache's network and encoding are fixed)
+SWEEP_GRID = {
+    "lr": (1e-2, 5e-3, 1e-3),
+    "ema_decay": (0.95, 0.99, 0.999),
+    "steps_per_frame": (1, 4),
+    "precision": (0, 1),
+}
+
+
+def grid_search(torch, scene, W, H, max_depth, rr_depth, grid=None, frames=100, score_from=10, schedule=None,
+                frame0=0):
+    """Hyper-parameter sweep of the radiance cache, restating
+    gridsearch.h:39-314: for every combination of `grid` (default
+    SWEEP_GRID), a fresh cache trained by `frames` training-only frames
+    (every pixel a training path, gridsearch.h:178-186); the score is the mean
+    of the frames' training losses from frame `score_from` on
+    (gridsearch.h:196-205), and the lowest mean wins (gridsearch.h:214-217;
+    non-finite or zero scores never win).  Every configuration sees the same
+    frame seeds.  Returns the results sorted by score, best first."""
+    import itertools
+
+    grid = grid or SWEEP_GRID
+    dscene = pt.DeviceScene(scene, torch)
+    lo = scene.verts.reshape(-1, 3).min(0) - 0.5
+    hi = scene.verts.reshape(-1, 3).max(0) + 0.5
+    keys = sorted(grid)
+    tr = pt.PathTracer(W, H, max_depth, rr_depth, mode=pt_mode_train(), use_graph=_graph(schedule))
+    tr.set_scene(dscene)
+    _apply(tr, schedule)
+    tr.set_camera(scene.camera.frame(W, H))
+    img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
+    results = []
+    for combo in itertools.product(*(grid[k] for k in keys)):
+        hp = dict(zip(keys, combo))
+        cache = nrc.RadianceCache(bounds_min=tuple(map(float, lo)), bounds_max=tuple(map(float, hi)), **hp)
+        tr.set_cache(cache)
+        losses = []
+        t0 = _sync_time(torch)
+        for f in range(frames):
+            tr.render(frame0 + f, img)
+            st = cache.stats()   # synchronises: the frame's last loss
+            if f >= score_from:
+                losses.append(st["last_loss"])
+        t = _sync_time(torch) - t0
+        score = float(np.mean(losses)) if losses else float("inf")
+        if not np.isfinite(score) or score <= 1e-7:
+            score = float("inf")
+        results.append({**hp, "score": score, "final_loss": losses[-1] if losses else None, "seconds": t,
+                        "steps": cache.stats()["steps"]})
+        tr.set_cache(None)
+        del cache
+    tr.close()
+    return sorted(results, key=lambda r: r["score"])
