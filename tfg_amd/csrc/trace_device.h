@@ -35,7 +35,13 @@ __device__ __forceinline__ float safe_rcp(float d) {
     return 1.0f / d;
 }
 
-__device__ __forceinline__ float sel3(const float* v, int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
+// Component k of (x, y, z) as two selects on values (no branches, no
+// indexed array: the compares are per ray and leave the traversal loop).
+__device__ __forceinline__ float sel3(float x, float y, float z, int k) {
+    const float r = k == 1 ? y : x;
+    return k == 2 ? z : r;
+}
+__device__ __forceinline__ float sel3(const float* v, int k) { return sel3(v[0], v[1], v[2], k); }
 
 __device__ __forceinline__ RayPre make_ray(float ox, float oy, float oz, float dx, float dy, float dz, float tmin,
                                            float tmax) {
@@ -61,19 +67,27 @@ __device__ __forceinline__ RayPre make_ray(float ox, float oy, float oz, float d
     return p;
 }
 
-// Watertight test.  Vertices arrive as float4 (w ignored).
-__device__ __forceinline__ bool watertight(const float4 a, const float4 b, const float4 c, const RayPre& r,
+// Watertight test.  Vertices arrive as float4 (w ignored).  Accepts the hit
+// when t = T / det lies in (tmin, tcur]; the division is done only for
+// candidates that pass a division-free screen of T against tmin * det and
+// tcur * det (widened by 2^-20 relative, far beyond the 2^-24 rounding of
+// either side, so the screen never rejects a hit the exact test accepts;
+// |det| < 2^-100 always takes the exact path), and u = V / det, v = W / det
+// only for accepted hits and when kUV.  The accepted t, u, v are the same
+// divisions as always, so results do not depend on the screen.
+template <bool kUV>
+__device__ __forceinline__ bool watertight(const float4 a, const float4 b, const float4 c, const RayPre& r, float tcur,
                                            float& t, float& u, float& v) {
-    const float A[3] = {a.x - r.O[0], a.y - r.O[1], a.z - r.O[2]};
-    const float B[3] = {b.x - r.O[0], b.y - r.O[1], b.z - r.O[2]};
-    const float C[3] = {c.x - r.O[0], c.y - r.O[1], c.z - r.O[2]};
-    const float Akz = sel3(A, r.kz), Bkz = sel3(B, r.kz), Ckz = sel3(C, r.kz);
-    const float Ax = sel3(A, r.kx) - r.Sx * Akz;
-    const float Ay = sel3(A, r.ky) - r.Sy * Akz;
-    const float Bx = sel3(B, r.kx) - r.Sx * Bkz;
-    const float By = sel3(B, r.ky) - r.Sy * Bkz;
-    const float Cx = sel3(C, r.kx) - r.Sx * Ckz;
-    const float Cy = sel3(C, r.ky) - r.Sy * Ckz;
+    const float ax = a.x - r.O[0], ay = a.y - r.O[1], az = a.z - r.O[2];
+    const float bx = b.x - r.O[0], by = b.y - r.O[1], bz = b.z - r.O[2];
+    const float cx = c.x - r.O[0], cy = c.y - r.O[1], cz = c.z - r.O[2];
+    const float Akz = sel3(ax, ay, az, r.kz), Bkz = sel3(bx, by, bz, r.kz), Ckz = sel3(cx, cy, cz, r.kz);
+    const float Ax = sel3(ax, ay, az, r.kx) - r.Sx * Akz;
+    const float Ay = sel3(ax, ay, az, r.ky) - r.Sy * Akz;
+    const float Bx = sel3(bx, by, bz, r.kx) - r.Sx * Bkz;
+    const float By = sel3(bx, by, bz, r.ky) - r.Sy * Bkz;
+    const float Cx = sel3(cx, cy, cz, r.kx) - r.Sx * Ckz;
+    const float Cy = sel3(cx, cy, cz, r.ky) - r.Sy * Ckz;
     float U = Cx * By - Cy * Bx;
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
@@ -89,9 +103,18 @@ __device__ __forceinline__ bool watertight(const float4 a, const float4 b, const
     const float Bz = r.Sz * Bkz;
     const float Cz = r.Sz * Ckz;
     const float T = U * Az + V * Bz + W * Cz;
+    const float ad = fabsf(det);
+    if (ad >= 0x1p-100f) {
+        const float Ts = det < 0.0f ? -T : T;
+        if (Ts <= r.tmin * ad * (1.0f - 0x1p-20f)) return false;
+        if (Ts > tcur * ad * (1.0f + 0x1p-20f)) return false;
+    }
     t = T / det;
-    u = V / det;
-    v = W / det;
+    if (!(t > r.tmin && t <= tcur)) return false;
+    if (kUV) {
+        u = V / det;
+        v = W / det;
+    }
     return true;
 }
 
@@ -122,13 +145,12 @@ __device__ __forceinline__ bool affine(const float4* __restrict__ tri, const Ray
 
 __device__ __forceinline__ float ubyte_f(uint32_t w, int j) { return (float)((w >> (8 * j)) & 0xFFu); }
 
-// One child box test, written like the oracle's node_hitmask loop body.
-__device__ __forceinline__ uint32_t child_bits(uint32_t meta, float qlx, float qly, float qlz, float qhx, float qhy,
-                                               float qhz, float idirx, float idiry, float idirz, float origx,
+// One child box test, the oracle's node_hitmask loop body: lo* / hi* are the
+// child's near and far quantised bounds per axis (the low or high bound by
+// the sign of the ray direction on that axis).
+__device__ __forceinline__ uint32_t child_bits(uint32_t meta, float lox, float loy, float loz, float hix, float hiy,
+                                               float hiz, float idirx, float idiry, float idirz, float origx,
                                                float origy, float origz, const RayPre& r, float tcur) {
-    const float lox = r.rD[0] < 0.0f ? qhx : qlx, hix = r.rD[0] < 0.0f ? qlx : qhx;
-    const float loy = r.rD[1] < 0.0f ? qhy : qly, hiy = r.rD[1] < 0.0f ? qly : qhy;
-    const float loz = r.rD[2] < 0.0f ? qhz : qlz, hiz = r.rD[2] < 0.0f ? qlz : qhz;
     const float tminx = fmaf(lox, idirx, origx), tmaxx = fmaf(hix, idirx, origx);
     const float tminy = fmaf(loy, idiry, origy), tmaxy = fmaf(hiy, idiry, origy);
     const float tminz = fmaf(loz, idirz, origz), tmaxz = fmaf(hiz, idirz, origz);
@@ -152,21 +174,35 @@ __device__ __forceinline__ uint32_t node_hitmask(const float4 n0, const float4 n
     const float origy = (n0.y - r.O[1]) * r.rD[1];
     const float origz = (n0.z - r.O[2]) * r.rD[2];
     const uint32_t m[2] = {__float_as_uint(n1.z), __float_as_uint(n1.w)};
-    const uint32_t lx[2] = {__float_as_uint(n2.x), __float_as_uint(n2.y)};
-    const uint32_t ly[2] = {__float_as_uint(n2.z), __float_as_uint(n2.w)};
-    const uint32_t lz[2] = {__float_as_uint(n3.x), __float_as_uint(n3.y)};
-    const uint32_t hx[2] = {__float_as_uint(n3.z), __float_as_uint(n3.w)};
-    const uint32_t hy[2] = {__float_as_uint(n4.x), __float_as_uint(n4.y)};
-    const uint32_t hz[2] = {__float_as_uint(n4.z), __float_as_uint(n4.w)};
+    // near / far bound words per axis, chosen once per node by the direction signs
+    const bool nx = r.rD[0] < 0.0f, ny = r.rD[1] < 0.0f, nz = r.rD[2] < 0.0f;
+    const uint32_t qlx[2] = {__float_as_uint(n2.x), __float_as_uint(n2.y)};
+    const uint32_t qly[2] = {__float_as_uint(n2.z), __float_as_uint(n2.w)};
+    const uint32_t qlz[2] = {__float_as_uint(n3.x), __float_as_uint(n3.y)};
+    const uint32_t qhx[2] = {__float_as_uint(n3.z), __float_as_uint(n3.w)};
+    const uint32_t qhy[2] = {__float_as_uint(n4.x), __float_as_uint(n4.y)};
+    const uint32_t qhz[2] = {__float_as_uint(n4.z), __float_as_uint(n4.w)};
     uint32_t hitmask = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
+        const uint32_t lx = nx ? qhx[h] : qlx[h], hx = nx ? qlx[h] : qhx[h];
+        const uint32_t ly = ny ? qhy[h] : qly[h], hy = ny ? qly[h] : qhy[h];
+        const uint32_t lz = nz ? qhz[h] : qlz[h], hz = nz ? qlz[h] : qhz[h];
+        // child_bits' bit index and triangle bits for the word's 4 children at
+        // once: inner children (meta bits 3 and 4 set) get the octant xor
+        const uint32_t mw = m[h];
+        const uint32_t inner = (mw & (mw << 1)) & 0x10101010u;
+        const uint32_t idx4 = (mw ^ ((r.octinv * 0x01010101u) & ((inner >> 4) * 0x1Fu))) & 0x1F1F1F1Fu;
+        const uint32_t cnt4 = (mw >> 5) & 0x07070707u;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const uint32_t meta = (m[h] >> (8 * j)) & 0xFFu;
-            hitmask |= child_bits(meta, ubyte_f(lx[h], j), ubyte_f(ly[h], j), ubyte_f(lz[h], j), ubyte_f(hx[h], j),
-                                  ubyte_f(hy[h], j), ubyte_f(hz[h], j), idirx, idiry, idirz, origx, origy, origz, r,
-                                  tcur);
+            const float tminx = fmaf(ubyte_f(lx, j), idirx, origx), tmaxx = fmaf(ubyte_f(hx, j), idirx, origx);
+            const float tminy = fmaf(ubyte_f(ly, j), idiry, origy), tmaxy = fmaf(ubyte_f(hy, j), idiry, origy);
+            const float tminz = fmaf(ubyte_f(lz, j), idirz, origz), tmaxz = fmaf(ubyte_f(hz, j), idirz, origz);
+            const float tnear = fmaxf(fmaxf(fmaxf(tminx, tminy), tminz), r.tmin);
+            const float tfar = fminf(fminf(fminf(tmaxx, tmaxy), tmaxz), tcur);
+            const uint32_t cnt = tnear <= tfar * kBoxSlack ? (cnt4 >> (8 * j)) & 7u : 0u;
+            hitmask |= cnt << ((idx4 >> (8 * j)) & 0x1Fu);
         }
     }
     return hitmask;
@@ -206,8 +242,8 @@ __device__ __forceinline__ bool test_triangles(const float4* __restrict__ tris, 
                 if (k == 1 && !two) break;
                 if (stats) stats->tris++;
                 float t, u, v;
-                const bool ok = (k == 0 ? watertight(a0, a1, a2, r, t, u, v) : watertight(c0, c1, c2, r, t, u, v)) &&
-                                t > r.tmin && t <= tcur;
+                const bool ok = k == 0 ? watertight<!kAnyHit>(a0, a1, a2, r, tcur, t, u, v)
+                                       : watertight<!kAnyHit>(c0, c1, c2, r, tcur, t, u, v);
                 if (!ok) continue;
                 const uint32_t prim = __float_as_uint((k == 0 ? T0 : T1)[3].w);
                 if (kAnyHit) {
@@ -399,7 +435,7 @@ __device__ __forceinline__ bool traverse_octet(const float4* __restrict__ nodes,
                 const int b = highest_bit(m);
                 const float4* T = tris + (size_t)(tg_x + 4u * (uint32_t)b);
                 if (stats) stats->tris++;
-                ok = watertight(T[0], T[1], T[2], r, t, u, v) && t > r.tmin && t <= tcur;
+                ok = watertight<!kAnyHit>(T[0], T[1], T[2], r, tcur, t, u, v);
                 if (ok) prim = __float_as_uint(T[3].w);
             }
             // drop the (up to) 8 highest pending bits
@@ -452,9 +488,10 @@ __device__ __forceinline__ bool traverse_octet(const float4* __restrict__ nodes,
         const uint32_t hx = h ? __float_as_uint(n3.w) : __float_as_uint(n3.z);
         const uint32_t hy = h ? __float_as_uint(n4.y) : __float_as_uint(n4.x);
         const uint32_t hz = h ? __float_as_uint(n4.w) : __float_as_uint(n4.z);
-        const uint32_t bits = child_bits(meta, ubyte_f(lx, jj), ubyte_f(ly, jj), ubyte_f(lz, jj), ubyte_f(hx, jj),
-                                         ubyte_f(hy, jj), ubyte_f(hz, jj), idirx, idiry, idirz, origx, origy, origz,
-                                         r, tcur);
+        const bool nx = r.rD[0] < 0.0f, ny = r.rD[1] < 0.0f, nz = r.rD[2] < 0.0f;   // near / far by sign
+        const uint32_t bits = child_bits(meta, ubyte_f(nx ? hx : lx, jj), ubyte_f(ny ? hy : ly, jj),
+                                         ubyte_f(nz ? hz : lz, jj), ubyte_f(nx ? lx : hx, jj), ubyte_f(ny ? ly : hy, jj),
+                                         ubyte_f(nz ? lz : hz, jj), idirx, idiry, idirz, origx, origy, origz, r, tcur);
         const uint32_t hm = octet_or(bits);
         ng_x = __float_as_uint(n1.x);
         ng_y = (hm & 0xFF000000u) | (__float_as_uint(n0.w) >> 24);
