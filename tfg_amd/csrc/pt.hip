@@ -796,8 +796,8 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     // 3. XCD super-chunks, octet threshold, shadow-stage grid
     const uint32_t groups[4] = {0, 1024, 4096, 16384};
     sweep(4, [&](int i) { pt->P.xcd_group = groups[i]; });
-    const uint32_t octs[4] = {0, 1u << 16, 1u << 18, 1u << 20};
-    sweep(4, [&](int i) { pt->P.octet_below = octs[i]; });
+    const uint32_t octs[6] = {0, 1u << 16, 1u << 17, 1u << 18, 1u << 19, 1u << 20};
+    sweep(6, [&](int i) { pt->P.octet_below = octs[i]; });
     const int cpcs[3] = {0, 2, 3};
     sweep(3, [&](int i) { pt->connect_per_cu = cpcs[i]; });
     const uint32_t batches[6] = {0, 2, 3, 4, 5, 6};
