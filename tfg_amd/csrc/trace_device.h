@@ -91,12 +91,13 @@ __device__ __forceinline__ bool watertight(const float4 a, const float4 b, const
     float U = Cx * By - Cy * Bx;
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
-    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    // bitwise (not short-circuit) conditions: one branch each instead of a chain
+    if ((U == 0.0f) | (V == 0.0f) | (W == 0.0f)) {
         U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
         V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
         W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
     }
-    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    if (((U < 0.0f) | (V < 0.0f) | (W < 0.0f)) & ((U > 0.0f) | (V > 0.0f) | (W > 0.0f))) return false;
     const float det = U + V + W;
     if (det == 0.0f) return false;
     const float Az = r.Sz * Akz;
