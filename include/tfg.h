@@ -176,13 +176,16 @@ typedef struct tfg_pt_profile {
     uint64_t ext_nodes, ext_tris, shadow_nodes, shadow_tris;
     /* start of each of tile 0's launches, ms after the frame's start */
     float at_extend[32], at_shade[32], at_connect[32], at_finalize;
+    /* the tail kernel (schedule tail_from): duration summed over the tiles,
+     * start of tile 0's launch; 0 without one */
+    float ms_tail, at_tail;
 } tfg_pt_profile;
 int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_profile* out);
 /* Times candidate schedules (tfg_pt_schedule) on the current scene and keeps
  * the fastest: first the traversal launch shape together with top_nodes,
  * then tiles, side streams and overlap, then the XCD super-chunk size,
- * octet_below, connect_per_cu and connect_batch, all timed on direct
- * launches (the tuner captures no graph).
+ * octet_below, connect_per_cu, connect_batch and tail_from, all timed on
+ * plain frames with direct launches (the tuner captures no graph).
  * Drops any captured graph. */
 int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream);
 int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_grid, uint32_t* shade_grid);
@@ -216,6 +219,12 @@ int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_g
  *       workgroup copies to LDS before tracing; more saves global fetches of
  *       the upper levels but costs every workgroup its staging, which
  *       matters for grids of many short-lived workgroups (default 9);
+ *   tail_from (0..32): 0 traces every depth with the per-depth stages;
+ *       k > 0 traces depths k..max_depth-1 with one launch of a tail kernel
+ *       in which each remaining path is carried to its end by 8 lanes
+ *       (closest hit, shading, shadow rays one lane per light): late depths
+ *       hold few paths (in cache frames mostly training paths), and the
+ *       per-depth chain of 3 launches per depth is bound by latency there;
  *   use_graph: replay captured hipGraphs (one per tile, plus one for the
  *       cache work) instead of launching the kernels directly; read-only,
  *       fixed by tfg_pt_config.use_graph (set_schedule must pass the same
@@ -228,6 +237,7 @@ typedef struct tfg_pt_schedule {
     uint32_t connect_per_cu;
     uint32_t connect_batch;
     uint32_t top_nodes;
+    uint32_t tail_from;
 } tfg_pt_schedule;
 int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out);
 int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* schedule);

@@ -119,9 +119,9 @@ def test_pt_schedules_match_serial(torch_cuda, use_graph):
     """Shadow rays on 1-3 side streams (overlapping the following depths), the
     frame split into 2 or 4 pixel tiles on their own streams, any XCD
     super-chunk size, 8-lane (octet) closest-hit traversal for all or some
-    launches, the shadow stages of late depths batched into one launch and
-    any number of LDS-staged top nodes give the same image bit for bit as one
-    serial chain, for
+    launches, the shadow stages of late depths batched into one launch, any
+    number of LDS-staged top nodes and the last depths run by the one-launch
+    tail kernel give the same image bit for bit as one serial chain, for
     plain and cache-capture frames, with the same ray counts and the same
     captured records and queries."""
     torch = torch_cuda
@@ -129,22 +129,23 @@ def test_pt_schedules_match_serial(torch_cuda, use_graph):
     W, H = 96, 64
     ds = pt.DeviceScene(sc, torch)
     big = 1 << 30
-    schedules = [(1, False, 1, 0, 0, 0, 73), (1, True, 1, 4096, 0, 3, 9), (1, True, 2, 256, big, 0, 0),
-                 (2, True, 4, 1024, 2000, 1, 1), (4, False, 1, 0, big, 5, 9), (4, True, 3, 16384, 0, 7, 73)]
+    schedules = [(1, False, 1, 0, 0, 0, 73, 0), (1, True, 1, 4096, 0, 3, 9, 6), (1, True, 2, 256, big, 0, 0, 2),
+                 (2, True, 4, 1024, 2000, 1, 1, 4), (4, False, 1, 0, big, 5, 9, 0), (4, True, 3, 16384, 0, 7, 73, 1),
+                 (1, True, 2, 0, 0, 2, 9, 3), (1, False, 1, 0, 0, 0, 9, 8)]
     for mode in (0, 2):
         imgs, counts, caps = [], [], []
-        for tiles, overlap, sides, xcd, octet, batch, top in schedules:
+        for tiles, overlap, sides, xcd, octet, batch, top, tail in schedules:
             print(f"mode={mode} tiles={tiles} overlap={overlap} sides={sides} xcd_group={xcd} octet_below={octet} "
-                  f"connect_batch={batch} top_nodes={top}", flush=True)
+                  f"connect_batch={batch} top_nodes={top} tail_from={tail}", flush=True)
             tr = pt.PathTracer(W, H, 8, 3, mode=mode, use_graph=use_graph)
             tr.set_scene(ds)
             tr.set_camera(sc.camera.frame(W, H))
             tr.set_schedule(overlap=overlap, tiles=tiles, side_streams=sides, xcd_group=xcd, octet_below=octet,
-                            connect_batch=batch, top_nodes=top)
+                            connect_batch=batch, top_nodes=top, tail_from=tail)
             shape = tr.launch_shape()
             assert shape["overlap"] == overlap and shape["tiles"] == tiles and shape["side_streams"] == sides
             assert shape["xcd_group"] == xcd and shape["octet_below"] == octet and shape["connect_batch"] == batch
-            assert shape["top_nodes"] == top
+            assert shape["top_nodes"] == top and shape["tail_from"] == tail
             assert shape["use_graph"] == use_graph
             img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
             out = []
@@ -180,7 +181,7 @@ def test_pt_rejects_bad_config(torch_cuda):
     img = torch_cuda.zeros((64, 3), device="cuda")
     with pytest.raises(_lib.TfgError):
         t.render(0, img)   # no scene yet
-    for bad in ({"connect_batch": 33}, {"top_nodes": 74}, {"connect_per_cu": 65}, {"tiles": 5}, {"trace_block": 96},
+    for bad in ({"connect_batch": 33}, {"top_nodes": 74}, {"tail_from": 33}, {"connect_per_cu": 65}, {"tiles": 5}, {"trace_block": 96},
                 {"xcd_group": 300}, {"use_graph": not t.schedule()["use_graph"]}):
         with pytest.raises(_lib.TfgError):
             t.set_schedule(**bad)

@@ -57,14 +57,15 @@ class PtProfile(C.Structure):
                 ("ext_rays", C.c_uint32 * 32), ("shadow_rays", C.c_uint32 * 32),
                 ("ext_nodes", C.c_uint64), ("ext_tris", C.c_uint64), ("shadow_nodes", C.c_uint64),
                 ("shadow_tris", C.c_uint64), ("at_extend", C.c_float * 32), ("at_shade", C.c_float * 32),
-                ("at_connect", C.c_float * 32), ("at_finalize", C.c_float)]
+                ("at_connect", C.c_float * 32), ("at_finalize", C.c_float), ("ms_tail", C.c_float),
+                ("at_tail", C.c_float)]
 
 
 class PtSchedule(C.Structure):
     _fields_ = [("trace_block", C.c_uint32), ("grid_mult", C.c_uint32), ("tiles", C.c_uint32),
                 ("side_streams", C.c_uint32), ("overlap", C.c_int32), ("use_graph", C.c_int32),
                 ("xcd_group", C.c_uint32), ("octet_below", C.c_uint32), ("connect_per_cu", C.c_uint32),
-                ("connect_batch", C.c_uint32), ("top_nodes", C.c_uint32)]
+                ("connect_batch", C.c_uint32), ("top_nodes", C.c_uint32), ("tail_from", C.c_uint32)]
 
 
 class NrcConfig(C.Structure):

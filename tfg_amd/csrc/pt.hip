@@ -27,6 +27,7 @@ struct tfg_pt {
     int connect_per_cu = 0;   // shadow-stage workgroups per CU (0: the closest-hit grid)
     uint32_t connect_batch = 0;   // depths >= this (if > 0) share one shadow-stage launch
     uint32_t top_nodes = 9;       // leading tree nodes each traversal workgroup stages in LDS
+    uint32_t tail_from = 0;       // depths >= this (if > 0) run in the one-launch tail kernel
     uint32_t scene_nodes = 0;     // nodes of the current scene's tree
     int shade_grid = 0;
     int gen_grid = 0;
@@ -129,6 +130,22 @@ void launch_extend(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t q, uin
     }
 }
 
+template <int B, bool kStats>
+void launch_tail_b(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t q, uint32_t d0) {
+    const size_t lds = trace_lds(pt, B) + (size_t)pt->P.stack_depth * (B / 8) * 8;
+    if (pt->P.mode != 0) hipLaunchKernelGGL((k_tail<B, true, kStats>), dim3(pt->trace_grid), dim3(B), lds, s, T, q, d0);
+    else hipLaunchKernelGGL((k_tail<B, false, kStats>), dim3(pt->trace_grid), dim3(B), lds, s, T, q, d0);
+}
+
+template <bool kStats>
+void launch_tail(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t q, uint32_t d0) {
+    switch (pt->trace_block) {
+        case 64: launch_tail_b<64, kStats>(pt, s, T, q, d0); break;
+        case 256: launch_tail_b<256, kStats>(pt, s, T, q, d0); break;
+        default: launch_tail_b<128, kStats>(pt, s, T, q, d0); break;
+    }
+}
+
 template <bool kStats>
 void launch_connect(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t d_lo, uint32_t d_hi) {
     // a bounded persistent grid leaves CU slots to the closest-hit chain
@@ -188,7 +205,7 @@ struct StageEvents {
     hipEvent_t frame[2] = {};
     hipEvent_t gen[kMaxTiles][2] = {}, fin[kMaxTiles][2] = {};
     hipEvent_t ext[kMaxTiles][kMaxDepth][2] = {}, shd[kMaxTiles][kMaxDepth][2] = {},
-               con[kMaxTiles][kMaxDepth][2] = {};
+               con[kMaxTiles][kMaxDepth][2] = {}, tail[kMaxTiles][2] = {};
     int create(uint32_t tiles, uint32_t depth) {
         hipError_t e = hipSuccess;
         auto mk = [&](hipEvent_t* p) {
@@ -199,6 +216,7 @@ struct StageEvents {
             for (uint32_t t = 0; t < tiles; ++t) {
                 mk(&gen[t][k]);
                 mk(&fin[t][k]);
+                mk(&tail[t][k]);
                 for (uint32_t d = 0; d < depth; ++d) {
                     mk(&ext[t][d][k]);
                     mk(&shd[t][d][k]);
@@ -217,6 +235,7 @@ struct StageEvents {
             for (uint32_t t = 0; t < kMaxTiles; ++t) {
                 rm(gen[t][k]);
                 rm(fin[t][k]);
+                rm(tail[t][k]);
                 for (uint32_t d = 0; d < kMaxDepth; ++d) {
                     rm(ext[t][d][k]);
                     rm(shd[t][d][k]);
@@ -227,10 +246,15 @@ struct StageEvents {
     }
 };
 
+// depths traced by the per-depth stages; the tail kernel runs the rest
+inline uint32_t staged_depths(const tfg_pt* pt) {
+    return pt->tail_from > 0 && pt->tail_from < pt->cfg.max_depth ? pt->tail_from : pt->cfg.max_depth;
+}
+
 // whether depth d's shadow stage has a launch of its own (the first depth of a
-// batch stands for the batch)
+// batch stands for the batch; depths of the tail kernel have none)
 inline bool connect_launched(const tfg_pt* pt, uint32_t d) {
-    return pt->connect_batch == 0 || d <= pt->connect_batch;
+    return d < staged_depths(pt) && (pt->connect_batch == 0 || d <= pt->connect_batch);
 }
 
 inline void mark(hipEvent_t e, hipStream_t s) {
@@ -250,7 +274,8 @@ int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, hipStream_t* side, tfg_color
     mark(pe ? pe->gen[t][0] : nullptr, ts);
     hipLaunchKernelGGL(k_generate, dim3(pt->gen_grid), dim3(256), 0, ts, T);
     mark(pe ? pe->gen[t][1] : nullptr, ts);
-    for (uint32_t d = 0; d < pt->cfg.max_depth; ++d) {
+    const uint32_t staged_d = staged_depths(pt);
+    for (uint32_t d = 0; d < staged_d; ++d) {
         const uint32_t q = d & 1u;
         mark(pe ? pe->ext[t][d][0] : nullptr, ts);
         if (stats) launch_extend<true>(pt, ts, T, q, d);
@@ -263,7 +288,7 @@ int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, hipStream_t* side, tfg_color
         // the shadow stage of depth d, or of depths batch..max_depth-1 at once
         const uint32_t batch = pt->connect_batch;
         const bool batched = batch > 0 && d >= batch;
-        if (batched && d + 1 < pt->cfg.max_depth) continue;
+        if (batched && d + 1 < staged_d) continue;
         const uint32_t d_lo = batched ? batch : d;
         hipStream_t cs = sides ? side[d_lo % sides] : ts;
         if (sides) {
@@ -275,7 +300,13 @@ int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, hipStream_t* side, tfg_color
         else launch_connect<false>(pt, cs, T, d_lo, d);
         mark(pe ? pe->con[t][d_lo][1] : nullptr, cs);
     }
-    for (int k = 0; k < sides && k < (int)pt->cfg.max_depth; ++k) {
+    if (staged_d < pt->cfg.max_depth) {   // the remaining depths in one launch
+        mark(pe ? pe->tail[t][0] : nullptr, ts);
+        if (stats) launch_tail<true>(pt, ts, T, staged_d & 1u, staged_d);
+        else launch_tail<false>(pt, ts, T, staged_d & 1u, staged_d);
+        mark(pe ? pe->tail[t][1] : nullptr, ts);
+    }
+    for (int k = 0; k < sides && k < (int)staged_d; ++k) {
         TFG_HIP_CHECK(hipEventRecord(pt->ev_cjoin[t][k], side[k]));
         TFG_HIP_CHECK(hipStreamWaitEvent(ts, pt->ev_cjoin[t][k], 0));
     }
@@ -702,19 +733,21 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
         // per stage: the sum of its launch durations over the tiles
         for (int t = 0; t < pt->tiles; ++t) {
             out->ms_generate += span(pe.gen[t][0], pe.gen[t][1]);
-            for (uint32_t d = 0; d < pt->cfg.max_depth; ++d) {
+            for (uint32_t d = 0; d < staged_depths(pt); ++d) {
                 out->ms_extend[d] += span(pe.ext[t][d][0], pe.ext[t][d][1]);
                 out->ms_shade[d] += span(pe.shd[t][d][0], pe.shd[t][d][1]);
                 if (connect_launched(pt, d)) out->ms_connect[d] += span(pe.con[t][d][0], pe.con[t][d][1]);
             }
             out->ms_finalize += span(pe.fin[t][0], pe.fin[t][1]);
+            if (staged_depths(pt) < pt->cfg.max_depth) out->ms_tail += span(pe.tail[t][0], pe.tail[t][1]);
         }
-        for (uint32_t d = 0; d < pt->cfg.max_depth; ++d) {
+        for (uint32_t d = 0; d < staged_depths(pt); ++d) {
             out->at_extend[d] = span(pe.frame[0], pe.ext[0][d][0]);
             out->at_shade[d] = span(pe.frame[0], pe.shd[0][d][0]);
             if (connect_launched(pt, d)) out->at_connect[d] = span(pe.frame[0], pe.con[0][d][0]);
         }
         out->at_finalize = span(pe.frame[0], pe.fin[0][0]);
+        if (staged_depths(pt) < pt->cfg.max_depth) out->at_tail = span(pe.frame[0], pe.tail[0][0]);
         out->ms_total = span(pe.frame[0], pe.frame[1]);
         uint32_t c[CNT_ALLOC];
         TFG_HIP_CHECK(hipMemcpy(c, pt->P.counters, sizeof(c), hipMemcpyDeviceToHost));
@@ -771,6 +804,7 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     pt->sides = 2;
     pt->overlap = true;
     pt->connect_batch = 0;
+    pt->tail_from = 0;
     // 1. traversal launch shape and LDS-staged top nodes together (large
     //    multipliers give more workgroups than chunks: each then traces one
     //    chunk and retires, which pays only with cheap workgroup start-up)
@@ -802,6 +836,8 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     sweep(3, [&](int i) { pt->connect_per_cu = cpcs[i]; });
     const uint32_t batches[6] = {0, 2, 3, 4, 5, 6};
     sweep(6, [&](int i) { pt->connect_batch = batches[i]; });
+    const uint32_t tails[6] = {0, 2, 3, 4, 5, 6};
+    sweep(6, [&](int i) { pt->tail_from = tails[i]; });
     drop_graph(pt);
     pt->P.mode = saved_mode;
     (void)hipEventDestroy(a);
@@ -831,6 +867,7 @@ int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out) {
     out->connect_per_cu = (uint32_t)pt->connect_per_cu;
     out->connect_batch = pt->connect_batch;
     out->top_nodes = pt->top_nodes;
+    out->tail_from = pt->tail_from;
     return TFG_OK;
 }
 
@@ -840,10 +877,11 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
     if ((b != 64 && b != 128 && b != 256) || sc->grid_mult == 0 || sc->grid_mult > 64 || sc->tiles == 0 ||
         sc->tiles > kMaxTiles || sc->side_streams == 0 || sc->side_streams > kMaxSide ||
         (sc->xcd_group != 0 && (sc->xcd_group % 256 != 0 || sc->xcd_group > (1u << 20))) ||
-        sc->connect_per_cu > 64 || sc->connect_batch > kMaxDepth || sc->top_nodes > kTopNodes) {
+        sc->connect_per_cu > 64 || sc->connect_batch > kMaxDepth || sc->top_nodes > kTopNodes ||
+        sc->tail_from > kMaxDepth) {
         set_error("tfg_pt_set_schedule: block 64/128/256, grid_mult 1..64, tiles 1..%u, side streams 1..%u, "
-                  "xcd_group 0 or a multiple of 256 up to 2^20, connect_per_cu 0..64, connect_batch 0..%u, "
-                  "top_nodes 0..%u", kMaxTiles, kMaxSide, kMaxDepth, kTopNodes);
+                  "xcd_group 0 or a multiple of 256 up to 2^20, connect_per_cu 0..64, connect_batch and "
+                  "tail_from 0..%u, top_nodes 0..%u", kMaxTiles, kMaxSide, kMaxDepth, kTopNodes);
         return TFG_E_INVALID;
     }
     if ((sc->use_graph != 0) != pt->run_graph) {
@@ -861,6 +899,7 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
     pt->P.octet_below = sc->octet_below;
     pt->connect_per_cu = (int)sc->connect_per_cu;
     pt->connect_batch = sc->connect_batch;
+    pt->tail_from = sc->tail_from;
     return TFG_OK;
 }
 

@@ -533,6 +533,121 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t d_lo, u
     if (kStats) add_stats(P, 2, st);
 }
 
+// ------------------------------------------------------------------- tail
+// The frame's last depths in one launch: from depth d0 on, each path still
+// in queue q is traced to its end by one aligned octet of lanes.  Per depth:
+// the closest hit (traverse_octet), the shading step (shade_entry, the code
+// of the shading stage), and the NEE of the request it makes, one lane per
+// light (8 lights per round) with the unoccluded terms summed in light order
+// by shuffles, as the shadow stage does; nee[depth][pid] and the mask are
+// written as the shadow and shading stages write them.  Results are those
+// of the per-depth stages bit for bit.  Late depths hold few paths (training
+// paths past their head vertex, paths that survive Russian roulette), and
+// one launch replaces a chain of 3 small launches per depth, each of which
+// lasts as long as its slowest ray.
+// Dynamic LDS: [n_top nodes][stack_depth x kBlock uint2 (per-lane shadow
+// rays)][stack_depth x kBlock/8 uint2 (per-octet closest hit)].
+template <int kBlock, bool kNrc, bool kStats = false>
+__global__ __launch_bounds__(kBlock) void k_tail(PtParams P, uint32_t q, uint32_t d0) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    constexpr uint32_t kOct = kBlock / 8;
+    __shared__ uint32_t n_ext[kMaxDepth], n_shd[kMaxDepth];
+    float4* top = reinterpret_cast<float4*>(lds);
+    uint2* lane_stack = reinterpret_cast<uint2*>(top + 5 * P.n_top);
+    uint2* oct_stack = lane_stack + (size_t)P.stack_depth * kBlock;
+    const uint32_t count = P.counters[q];
+    const uint32_t chunks = (count + kOct - 1) / kOct;
+    if (blockIdx.x >= chunks) return;   // workgroups without paths leave before staging
+    for (uint32_t k = threadIdx.x; k < kMaxDepth; k += kBlock) n_ext[k] = n_shd[k] = 0u;
+    stage_top_nodes(P, top);   // synchronises the block
+    const uint32_t j = threadIdx.x & 7u;
+    const int base = (int)(__lane_id() & ~7u);
+    TraceStats st_e, st_s;
+    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const uint32_t i = c * kOct + (threadIdx.x >> 3);   // uniform over the octet
+        if (i >= count) continue;
+        const uint32_t pid = P.qpath[q][i];
+        float4 ro = P.rays[q][2 * (size_t)i];
+        float4 rd = P.rays[q][2 * (size_t)i + 1];
+        for (uint32_t d = d0; d < P.max_depth; ++d) {
+            if (j == 0) atomicAdd(&n_ext[d], 1u);
+            const RayPre r = make_ray(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, ro.w, rd.w);
+            float t, u, v;
+            uint32_t prim;
+            traverse_octet<false>(P.nodes, P.tris, top, P.n_top, r, oct_stack + (threadIdx.x >> 3), kOct,
+                                  P.stack_depth, t, u, v, prim, kStats ? &st_e : nullptr);
+            for (uint32_t k = 0; k < P.n_analytic; ++k) {   // escenario.h:323-334
+                const float ta = analytic_t(P.analytic[k], v3(ro.x, ro.y, ro.z), v3(rd.x, rd.y, rd.z));
+                if (ta > ro.w && ta < t) {
+                    t = ta;
+                    u = 0.0f;
+                    v = 0.0f;
+                    prim = kAnalyticBit | k;
+                }
+            }
+            bool want_shadow = false, want_ray = false;
+            float4 e0, e1, e2, e3, n0, n1;
+            shade_entry<kNrc>(P, d, d + 1 >= P.max_depth, pid, ro, rd, make_float4(t, u, v, __uint_as_float(prim)),
+                              want_shadow, e0, e1, e2, e3, want_ray, n0, n1);
+            if (want_shadow) {   // the shadow stage for this entry (escenario.h:144-184)
+                const V3 ps = xyz(e0);
+                const V3 n = xyz(e1);
+                const Mat m = load_mat(&P.mats[__float_as_uint(e1.w) & 0x00FFFFFFu]);
+                V3 sum = v3(0.f, 0.f, 0.f);
+                uint32_t facing = 0;
+                for (uint32_t l0 = 0; l0 < P.n_lights; l0 += 8) {
+                    const uint32_t l = l0 + j;
+                    V3 ch = v3(0.f, 0.f, 0.f);
+                    if (l < P.n_lights) {
+                        V3 contrib, dir;
+                        float dist;
+                        if (light_term(P.lights[l], ps, n, xyz(e2), m, contrib, dir, dist)) {
+                            ++facing;
+                            const V3 o = ps + dir * kEps;
+                            bool occ = false;
+                            for (uint32_t k = 0; k < P.n_analytic && !occ; ++k) {   // escenario.h:255-266
+                                const float ta = analytic_t(P.analytic[k], o, dir);
+                                occ = ta > 0.0f && ta < dist;
+                            }
+                            if (!occ) {
+                                const RayPre rs = make_ray(o.x, o.y, o.z, dir.x, dir.y, dir.z, 0.0f, dist);
+                                float ts, us, vs;
+                                uint32_t prs;
+                                occ = traverse<0, true>(P.nodes, P.tris, top, P.n_top, rs, lane_stack + threadIdx.x,
+                                                        kBlock, P.stack_depth, ts, us, vs, prs,
+                                                        kStats ? &st_s : nullptr);
+                            }
+                            if (!occ) ch = contrib;
+                        }
+                    }
+                    const uint32_t nk = P.n_lights - l0 < 8u ? P.n_lights - l0 : 8u;
+                    for (uint32_t k = 0; k < nk; ++k) {
+                        const int src = base + (int)k;
+                        sum = sum + v3(__shfl(ch.x, src), __shfl(ch.y, src), __shfl(ch.z, src));
+                    }
+                }
+                if (facing) atomicAdd(&n_shd[d], facing);
+                if (j == 0) {
+                    P.nee[(size_t)d * P.depth_stride + pid] = f4(xyz(e3) * sum, 0.0f);
+                    P.neemask[pid] |= 1u << d;
+                }
+            }
+            if (!want_ray) break;
+            ro = n0;
+            rd = n1;
+        }
+    }
+    if (kStats) {
+        add_stats(P, 0, st_e);
+        add_stats(P, 2, st_s);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < kMaxDepth; k += kBlock) {
+        if (n_ext[k]) atomicAdd(&P.gcounters[CNT_EXT + k], n_ext[k]);
+        if (n_shd[k]) atomicAdd(&P.gcounters[CNT_SHD + k], n_shd[k]);
+    }
+}
+
 // --------------------------------------------------------------- finalize
 // Path radiance = emitted radiance + the NEE contributions of every depth,
 // added in depth order.  Plain mode: pixel = path radiance.  Cache modes:

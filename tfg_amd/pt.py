@@ -125,7 +125,8 @@ class PathTracer:
             "shadow_rays": list(pr.shadow_rays[:d]), "ext_nodes": pr.ext_nodes, "ext_tris": pr.ext_tris,
             "shadow_nodes": pr.shadow_nodes, "shadow_tris": pr.shadow_tris,
             "at_extend": list(pr.at_extend[:d]), "at_shade": list(pr.at_shade[:d]),
-            "at_connect": list(pr.at_connect[:d]), "at_finalize": pr.at_finalize,
+            "at_connect": list(pr.at_connect[:d]), "at_finalize": pr.at_finalize, "ms_tail": pr.ms_tail,
+            "at_tail": pr.at_tail,
         }
 
     def reset_totals(self):
@@ -141,11 +142,11 @@ class PathTracer:
         return {"trace_block": sc.trace_block, "grid_mult": sc.grid_mult, "tiles": sc.tiles,
                 "side_streams": sc.side_streams, "overlap": bool(sc.overlap), "use_graph": bool(sc.use_graph),
                 "xcd_group": sc.xcd_group, "octet_below": sc.octet_below, "connect_per_cu": sc.connect_per_cu,
-                "connect_batch": sc.connect_batch, "top_nodes": sc.top_nodes}
+                "connect_batch": sc.connect_batch, "top_nodes": sc.top_nodes, "tail_from": sc.tail_from}
 
     def set_schedule(self, **kw):
         """Change any of trace_block, grid_mult, tiles, side_streams, overlap, xcd_group,
-        octet_below, connect_per_cu, connect_batch, top_nodes (use_graph is fixed when the tracer is created)."""
+        octet_below, connect_per_cu, connect_batch, top_nodes, tail_from (use_graph is fixed when the tracer is created)."""
         cur = self.schedule()
         unknown = set(kw) - set(cur)
         if unknown:
@@ -153,7 +154,8 @@ class PathTracer:
         cur.update(kw)
         sc = _lib.PtSchedule(cur["trace_block"], cur["grid_mult"], cur["tiles"], cur["side_streams"],
                              1 if cur["overlap"] else 0, 1 if cur["use_graph"] else 0, cur["xcd_group"],
-                             cur["octet_below"], cur["connect_per_cu"], cur["connect_batch"], cur["top_nodes"])
+                             cur["octet_below"], cur["connect_per_cu"], cur["connect_batch"], cur["top_nodes"],
+                             cur["tail_from"])
         check(lib().tfg_pt_set_schedule(self._h, C.byref(sc)), "tfg_pt_set_schedule")
 
     def launch_shape(self):
