@@ -305,7 +305,8 @@ def bench_pt1080(args, torch):
     1920x1080 (value = its Mrays/s), then the equal-time comparison of the
     cache-accelerated tracer (BF16 MLP, 48 warmup frames counted in its time)
     against the plain one, both measured by relMSE against a plain reference
-    accumulated in FP64."""
+    accumulated in FP64; the pair of runs is repeated --repeats times with
+    distinct frame seeds and the ratio pooled over them."""
     from tfg_amd import experiments, scenes
 
     res = bench_pt(args, torch)
@@ -313,7 +314,7 @@ def bench_pt1080(args, torch):
     W, H, MD, RR = 1920, 1080, 8, 3
     t0 = time.time()
     cmp = experiments.equal_time(torch, sc, W, H, MD, RR, ref_spp=args.ref_spp, budget_s=args.budget,
-                                 warmup=48, precision=0, schedule=res["config"]["launch"])
+                                 warmup=48, precision=0, schedule=res["config"]["launch"], repeats=args.repeats)
     _log(f"equal-time comparison: {json.dumps(cmp)} ({time.time() - t0:.1f}s)")
     res["relmse"] = cmp
     t0 = time.time()
@@ -324,6 +325,7 @@ def bench_pt1080(args, torch):
                                  "NRC-BF16 vs plain against a plain FP64-accumulated reference")
     res["config"]["relmse_reference_spp"] = args.ref_spp
     res["config"]["equal_time_budget_s"] = args.budget
+    res["config"]["equal_time_repeats"] = args.repeats
     res["config"]["cache_frame_tail_from"] = experiments.CACHE_TAIL_FROM
     return res
 
@@ -708,6 +710,8 @@ def main():
     ap.add_argument("--workload", default="pt1080", choices=["pt1080", "pt", "traversal", "copy", "nrc", "multiview", "sweep"])
     ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the relMSE comparison")
     ap.add_argument("--budget", type=float, default=2.0, help="equal-time budget in seconds per method")
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="equal-time runs per method (distinct frame seeds, fresh cache); the ratio is pooled")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

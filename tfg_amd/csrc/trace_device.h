@@ -285,6 +285,29 @@ __device__ __forceinline__ bool test_triangles(const float4* __restrict__ tris, 
     return false;
 }
 
+// Node ni from the LDS copy of the top nodes or from global memory, as two
+// separate paths so that each compiles to its own load kind (LDS reads,
+// global loads) rather than generic-address loads that wait on both.
+__device__ __forceinline__ void load_node(const float4* __restrict__ nodes, const float4* __restrict__ top_nodes,
+                                          uint32_t n_top, uint32_t ni, float4& n0, float4& n1, float4& n2,
+                                          float4& n3, float4& n4) {
+    if (ni < n_top) {
+        const float4* n = top_nodes + 5 * ni;
+        n0 = n[0];
+        n1 = n[1];
+        n2 = n[2];
+        n3 = n[3];
+        n4 = n[4];
+    } else {
+        const float4* n = nodes + 5 * (size_t)ni;
+        n0 = n[0];
+        n1 = n[1];
+        n2 = n[2];
+        n3 = n[3];
+        n4 = n[4];
+    }
+}
+
 // Per-lane LDS stack: entry k of this lane lives at base[k * stride]; it
 // holds at most stack_depth entries (tree depth - 1 suffices: one pending
 // node group per level).  Nodes with index < n_top are read from the LDS
@@ -334,12 +357,7 @@ __device__ __forceinline__ bool traverse(const float4* __restrict__ nodes, const
             const uint32_t slot = ((uint32_t)bit - 24u) ^ r.octinv;
             const uint32_t rel = (uint32_t)__popc(imask & ~(0xFFFFFFFFu << slot));
             const uint32_t ni = ng_x + rel;
-            const float4* n = ni < n_top ? top_nodes + 5 * ni : nodes + 5 * (size_t)ni;
-            n0 = n[0];
-            n1 = n[1];
-            n2 = n[2];
-            n3 = n[3];
-            n4 = n[4];
+            load_node(nodes, top_nodes, n_top, ni, n0, n1, n2, n3, n4);
         }
         if (test_triangles<kFormat, kAnyHit>(tris, tg_x, tg_y, r, tcur, bu, bv, best_prim, found, stats))
             return true;
@@ -416,12 +434,7 @@ __device__ __forceinline__ bool traverse_octet(const float4* __restrict__ nodes,
             const uint32_t slot = ((uint32_t)bit - 24u) ^ r.octinv;
             const uint32_t rel = (uint32_t)__popc(imask & ~(0xFFFFFFFFu << slot));
             const uint32_t ni = ng_x + rel;
-            const float4* n = ni < n_top ? top_nodes + 5 * ni : nodes + 5 * (size_t)ni;
-            n0 = n[0];
-            n1 = n[1];
-            n2 = n[2];
-            n3 = n[3];
-            n4 = n[4];
+            load_node(nodes, top_nodes, n_top, ni, n0, n1, n2, n3, n4);
         }
         // triangles of the previous node, up to 8 per round: lane j takes
         // the (j+1)-th highest pending bit
