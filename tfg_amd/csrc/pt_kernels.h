@@ -242,10 +242,10 @@ __device__ __forceinline__ void shade_entry(const PtParams& P, uint32_t depth, b
         if (kNrc) {   // render.h:86-102 path-footprint spread
             const float dist_sq = t * t;
             if (depth == 0) {
-                foot.x = dist_sq / (4.0f * kPi * fmaxf(cos_theta, 1e-4f));
+                foot.x = fdiv(dist_sq, 4.0f * kPi * fmaxf(cos_theta, 1e-4f));
                 foot.y = 0.0f;
             } else {
-                const float spread = dist_sq / (fmaxf(pdf, 1e-4f) * fmaxf(cos_theta, 1e-4f));
+                const float spread = fdiv(dist_sq, fmaxf(pdf, 1e-4f) * fmaxf(cos_theta, 1e-4f));
                 if (!(fl & PF_HEAD)) foot.y += spread;
                 else foot.z += spread;
             }
@@ -311,7 +311,8 @@ __device__ __forceinline__ void shade_entry(const PtParams& P, uint32_t depth, b
                 if (!(sum > 0.0f)) {
                     alive = false;
                 } else {
-                    const float pd = mkd / sum, ps = mks / sum, pt = 1.0f - pd - ps;
+                    const float isum = frcp(sum);
+                    const float pd = mkd * isum, ps = mks * isum, pt = 1.0f - pd - ps;
                     const float r = pcg32_float(s);
                     if (r < pd) {
                         thr = cdiv(thr * m.kd, pd);
@@ -322,7 +323,7 @@ __device__ __forceinline__ void shade_entry(const PtParams& P, uint32_t depth, b
                             alive = false;
                         } else {
                             wi = to_world(make_frame(n), wl);
-                            pdf = fmaxf(fabsf(dot(wi, n)), 1e-4f) / kPi;
+                            pdf = fmaxf(fabsf(dot(wi, n)), 1e-4f) * (1.0f / kPi);
                         }
                     } else if (r < pd + ps) {
                         if (glossy) {
@@ -341,9 +342,9 @@ __device__ __forceinline__ void shade_entry(const PtParams& P, uint32_t depth, b
                                     alive = false;
                                 } else {
                                     const float a2 = alpha * alpha;
-                                    thr = thr * schlick(m.ks, oh) * (ggx_G1(a2, wil.z) / ps);
+                                    thr = thr * schlick(m.ks, oh) * fdiv(ggx_G1(a2, wil.z), ps);
                                     wi = normalize(to_world(fr, wil));
-                                    pdf = ggx_D(a2, hm.z) * ggx_G1(a2, wol.z) / (4.0f * wol.z);
+                                    pdf = fdiv(ggx_D(a2, hm.z) * ggx_G1(a2, wol.z), 4.0f * wol.z);
                                 }
                             }
                         } else {   // render.h:337-340
@@ -362,13 +363,13 @@ __device__ __forceinline__ void shade_entry(const PtParams& P, uint32_t depth, b
                             cosi = -cosi;
                             etat = m.ior;
                         }
-                        const float eta = ior / etat;
+                        const float eta = fdiv(ior, etat);
                         const float k = 1.0f - eta * eta * (1.0f - cosi * cosi);
                         if (k < 0.0f) {
                             wi = normalize(d - N * (2.0f * dot(d, N)));
                         } else {
                             ior = etat;
-                            wi = normalize(d * eta + N * (eta * cosi - sqrtf(k)));
+                            wi = normalize(d * eta + N * (eta * cosi - fsqrt(k)));
                         }
                         thr = cdiv(thr * m.kt, pt);
                         pdf = 1e6f;

@@ -50,16 +50,31 @@ __device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y 
 __device__ __forceinline__ V3 cross(V3 a, V3 b) {
     return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
+// Shading arithmetic uses the hardware reciprocal, square root and sine
+// (v_rcp_f32, v_sqrt_f32, v_sin_f32 / v_cos_f32 on revolutions, about 1 ulp
+// or 2^-20 absolute) instead of the correctly rounded sequences the build
+// gives "/" and sqrtf (8-10 instructions each).  Shading values are compared
+// with the oracle within a tolerance; the ray set-up and the triangle test of
+// the traversal, whose hits must equal the oracle's bit for bit, keep the
+// correctly rounded forms.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * frcp(b); }
+// sin and cos of 2 pi r for r in [0, 1)
+__device__ __forceinline__ void sincos_rev(float r, float& s, float& c) {
+    s = __builtin_amdgcn_sinf(r);
+    c = __builtin_amdgcn_cosf(r);
+}
 __device__ __forceinline__ V3 normalize(V3 a) {
-    const float l = sqrtf(dot(a, a));
-    return l > 0.0f ? a * (1.0f / l) : a;
+    const float l = fsqrt(dot(a, a));
+    return l > 0.0f ? a * frcp(l) : a;
 }
 __device__ __forceinline__ float vmax(V3 a) { return fmaxf(fmaxf(a.x, a.y), a.z); }
 // Colour divided by a scalar with the reference's rule that a zero divisor
 // gives black (color.h:42-48): a lobe drawn with probability 0 (a uniform
 // draw that rounds to 1.0 picks transmission on a material without it)
 // zeroes the throughput instead of producing 0 * inf.
-__device__ __forceinline__ V3 cdiv(V3 a, float s) { return s == 0.0f ? v3(0.0f, 0.0f, 0.0f) : a * (1.0f / s); }
+__device__ __forceinline__ V3 cdiv(V3 a, float s) { return s == 0.0f ? v3(0.0f, 0.0f, 0.0f) : a * frcp(s); }
 __device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
 
 // ------------------------------------------------------------ camera
@@ -89,11 +104,10 @@ __device__ __forceinline__ V3 to_local(const Frame& f, V3 w) { return v3(dot(w, 
 
 // render.h:379-393: phi = 2 pi r1, theta = acos(sqrt(1 - r2)).
 __device__ __forceinline__ V3 cosine_sample(float r1, float r2) {
-    const float phi = 2.0f * kPi * r1;
-    const float ct = sqrtf(fmaxf(0.0f, 1.0f - r2));
-    const float st = sqrtf(fmaxf(0.0f, r2));
+    const float ct = fsqrt(fmaxf(0.0f, 1.0f - r2));
+    const float st = fsqrt(fmaxf(0.0f, r2));
     float sp, cp;
-    sincosf(phi, &sp, &cp);
+    sincos_rev(r1, sp, cp);   // phi = 2 pi r1
     return v3(st * cp, st * sp, ct);
 }
 
@@ -101,11 +115,11 @@ __device__ __forceinline__ V3 cosine_sample(float r1, float r2) {
 __device__ __forceinline__ float ggx_alpha(float roughness) { return fmaxf(roughness * roughness, 1e-3f); }
 __device__ __forceinline__ float ggx_D(float a2, float nh) {
     const float d = nh * nh * (a2 - 1.0f) + 1.0f;
-    return a2 / (kPi * d * d);
+    return fdiv(a2, kPi * d * d);
 }
 __device__ __forceinline__ float ggx_G1(float a2, float nv) {
     nv = fmaxf(nv, 0.0f);
-    return 2.0f * nv / (nv + sqrtf(a2 + (1.0f - a2) * nv * nv));
+    return fdiv(2.0f * nv, nv + fsqrt(a2 + (1.0f - a2) * nv * nv));
 }
 __device__ __forceinline__ V3 schlick(V3 f0, float c) {
     const float m = 1.0f - fminf(fmaxf(c, 0.0f), 1.0f);
@@ -120,22 +134,21 @@ __device__ __forceinline__ V3 ggx_eval(V3 ks, float alpha, V3 wo, V3 wi) {
     const float D = ggx_D(a2, h.z);
     const float G = ggx_G1(a2, wo.z) * ggx_G1(a2, wi.z);
     const V3 F = schlick(ks, dot(wi, h));
-    return F * (D * G / (4.0f * wo.z * wi.z));
+    return F * fdiv(D * G, 4.0f * wo.z * wi.z);
 }
 // VNDF sample of a microfacet normal (Heitz 2018, listing 1).
 __device__ __forceinline__ V3 ggx_sample_vndf(V3 wo, float alpha, float u1, float u2) {
     const V3 vh = normalize(v3(alpha * wo.x, alpha * wo.y, wo.z));
     const float lensq = vh.x * vh.x + vh.y * vh.y;
-    const V3 t1 = lensq > 0.0f ? v3(-vh.y, vh.x, 0.0f) * (1.0f / sqrtf(lensq)) : v3(1.0f, 0.0f, 0.0f);
+    const V3 t1 = lensq > 0.0f ? v3(-vh.y, vh.x, 0.0f) * frcp(fsqrt(lensq)) : v3(1.0f, 0.0f, 0.0f);
     const V3 t2 = cross(vh, t1);
-    const float r = sqrtf(u1);
-    const float phi = 2.0f * kPi * u2;
+    const float r = fsqrt(u1);
     float sp, cp;
-    sincosf(phi, &sp, &cp);
+    sincos_rev(u2, sp, cp);   // phi = 2 pi u2
     const float p1 = r * cp;
     const float s = 0.5f * (1.0f + vh.z);
-    const float p2 = (1.0f - s) * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1)) + s * (r * sp);
-    const V3 nh = t1 * p1 + t2 * p2 + vh * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
+    const float p2 = (1.0f - s) * fsqrt(fmaxf(0.0f, 1.0f - p1 * p1)) + s * (r * sp);
+    const V3 nh = t1 * p1 + t2 * p2 + vh * fsqrt(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
     return normalize(v3(alpha * nh.x, alpha * nh.y, fmaxf(0.0f, nh.z)));
 }
 
@@ -218,8 +231,8 @@ __device__ __forceinline__ V3 shading_normal(const tfg_tri_shading& s, V3 ng, fl
     if (s.flags & 1u) {
         const float b0 = 1.0f - u - v;
         const V3 n = ld3(s.n0) * b0 + ld3(s.n1) * u + ld3(s.n2) * v;
-        const float l = sqrtf(dot(n, n));
-        if (l > 1e-6f) return n * (1.0f / l);
+        const float l = fsqrt(dot(n, n));
+        if (l > 1e-6f) return n * frcp(l);
     }
     V3 n = normalize(ng);
     if (dot(n, dir) > 0.0f) n = -n;
@@ -230,8 +243,8 @@ __device__ __forceinline__ V3 shading_normal(const tfg_tri_shading& s, V3 ng, fl
 // surface (the only test that drops a light before its shadow ray).
 __device__ __forceinline__ bool light_facing(const tfg_point_light& L, V3 ps, V3 n, V3& dir, float& dist, float& c) {
     V3 d = ld3(L.pos) - ps;
-    dist = sqrtf(dot(d, d));
-    dir = d * (1.0f / dist);
+    dist = fsqrt(dot(d, d));
+    dir = d * frcp(dist);
     c = dot(n, dir);
     return c > 0.0f;
 }
@@ -243,13 +256,13 @@ __device__ __forceinline__ bool light_term(const tfg_point_light& L, V3 ps, V3 n
                                            V3& dir, float& dist) {
     float c;
     if (!light_facing(L, ps, n, dir, dist, c)) return false;
-    V3 f = m.kd * (1.0f / kPi);
+    V3 f = m.kd * (1.0f / kPi);   // constant
     if (mat_glossy(m)) {
         const Frame fr = make_frame(n);
         f = f + ggx_eval(m.ks, ggx_alpha(m.rough), to_local(fr, wo), to_local(fr, dir));
     }
     const float dc = fmaxf(dist, 1.0f);
-    contrib = ld3(L.intensity) * (1.0f / (dc * dc)) * f * c;
+    contrib = ld3(L.intensity) * frcp(dc * dc) * f * c;
     return true;
 }
 
