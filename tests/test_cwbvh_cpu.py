@@ -82,6 +82,36 @@ def test_traversal_equals_brute_force(name):
     np.testing.assert_array_equal(a["prim"], b["prim"])
 
 
+def test_edge_case_rays_equal_brute_force():
+    """Rays through shared vertices and edges, axis-aligned rays with +0 and
+    -0 direction components, tmax equal to the hit distance and one ulp
+    below, rays starting on the surface, rays leaving a closed cube through
+    its corners and edges, edge-on rays and zero-area triangles
+    (tests/edge_cases.py): the oracle traversal equals brute force over all
+    triangles bit for bit, any-hit equals closest t < tmax, and the
+    watertight guarantees hold (no ray leaks through a shared edge or
+    vertex, the exact t at the tmax boundary, no degenerate triangle hit)."""
+    import edge_cases as E
+
+    v = E.scene()
+    h = _lib.CwbvhHost(v, 0)
+    rc, _ = oracle.validate(h.nodes, h.tris, 0, v)
+    assert rc == 0
+    fam = E.families()
+    hits = {}
+    for name, rays in fam.items():
+        a = oracle.cwbvh_trace(h.nodes, h.tris, 0, rays)
+        b = oracle.brute_trace(v, rays)
+        for f in ("t", "u", "v"):
+            np.testing.assert_array_equal(a[f].view(np.uint32), b[f].view(np.uint32), err_msg=name)
+        np.testing.assert_array_equal(a["prim"], b["prim"], err_msg=name)
+        occ = oracle.cwbvh_occluded(h.nodes, h.tris, 0, rays)
+        np.testing.assert_array_equal(occ, ((a["prim"] != 0xFFFFFFFF) & (a["t"] < rays[:, 7])).astype(np.uint32),
+                                      err_msg=name)
+        hits[name] = a
+    E.check_properties(fam, hits)
+
+
 def test_any_hit_matches_closest():
     v = MESHES["sphere"]()
     h = _lib.CwbvhHost(v, 0)
