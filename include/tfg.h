@@ -184,8 +184,9 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
 /* Times candidate schedules (tfg_pt_schedule) on the current scene and keeps
  * the fastest: first the traversal launch shape together with top_nodes,
  * then tiles, side streams and overlap, then the XCD super-chunk size,
- * octet_below, connect_per_cu, connect_batch and tail_from, all timed on
- * plain frames with direct launches (the tuner captures no graph).
+ * octet_below, connect_per_cu, connect_batch and tail_from, then a second
+ * round over all of them but the launch shape (top_nodes included), all
+ * timed on plain frames with direct launches (the tuner captures no graph).
  * Drops any captured graph. */
 int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream);
 int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_grid, uint32_t* shade_grid);
@@ -225,6 +226,15 @@ int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_g
  *       (closest hit, shading, shadow rays one lane per light): late depths
  *       hold few paths (in cache frames mostly training paths), and the
  *       per-depth chain of 3 launches per depth is bound by latency there;
+ *   defer_training (0/1, direct launches only): a training frame's
+ *       optimiser steps run at the start of the next tfg_pt_render, on a
+ *       stream of their own beside that frame's tracing, on a copy of the
+ *       records they draw (made at the end of their frame); the next cache
+ *       inference waits for them, so every image is the one the steps at the
+ *       end of their own frame give.  The last frame's steps wait for the next
+ *       render or tfg_pt_flush_training.  Only for a cache attached to
+ *       this tracer alone: tracers sharing a cache would train it from
+ *       several streams at once.  The tuner leaves it unchanged;
  *   use_graph: replay captured hipGraphs (one per tile, plus one for the
  *       cache work) instead of launching the kernels directly; read-only,
  *       fixed by tfg_pt_config.use_graph (set_schedule must pass the same
@@ -238,6 +248,7 @@ typedef struct tfg_pt_schedule {
     uint32_t connect_batch;
     uint32_t top_nodes;
     uint32_t tail_from;
+    uint32_t defer_training;
 } tfg_pt_schedule;
 int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out);
 int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* schedule);
@@ -324,8 +335,15 @@ int tfg_nrc_infer_datos(tfg_nrc* nrc, const tfg_datos_mlp* d_in, uint32_t n, tfg
  * configured batch drawn (NrcKernel.cu:10-16) from d_in[0..n) with target =
  * DatosMLP.color; no-op with *loss = 0 when n < 100; synchronous. */
 int tfg_nrc_train_datos(tfg_nrc* nrc, const tfg_datos_mlp* d_in, uint32_t n, float* loss, tfg_stream stream);
-/* Attaches the cache to a path tracer created with a nonzero mode. */
+/* Attaches the cache to a path tracer created with a nonzero mode (pending
+ * deferred steps of the previous cache are enqueued first). */
 int tfg_pt_set_nrc(tfg_pt* pt, tfg_nrc* nrc);
+/* With schedule defer_training: enqueues, ordered after the work on
+ * `stream`, the optimiser steps of the last training frame, which otherwise
+ * run at the start of the next tfg_pt_render; no-op when none is pending.
+ * Call it before reading or saving the cache's weights or statistics after
+ * the last frame.  tfg_pt_destroy drops steps still pending. */
+int tfg_pt_flush_training(tfg_pt* pt, tfg_stream stream);
 /* Switches the frame mode (0 = plain, TFG_PT_INFER | TFG_PT_TRAIN); a
  * tracer created in plain mode cannot switch to a cache mode. */
 int tfg_pt_set_mode(tfg_pt* pt, uint32_t mode);

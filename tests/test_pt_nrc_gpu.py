@@ -118,3 +118,58 @@ def test_cache_frame_trains_and_composites(torch_cuda):
     print("means: cache", a.mean(), "prefix only", prefix.mean(), "plain", plain.mean(), flush=True)
     assert a.mean() > prefix.mean()
     assert abs(a.mean() - plain.mean()) < 0.35 * plain.mean()
+
+
+def test_deferred_training_matches_end_of_frame_training(torch_cuda):
+    """Schedule defer_training: each training frame's optimiser steps run at
+    the start of the next frame, beside its tracing, on a copy of the records
+    they draw, and the next inference waits for them.  On a 16x16 image every
+    record and query append comes from one workgroup, so record order and
+    the drawn batches are reproducible: every frame's image, and the weights
+    and statistics after tfg_pt_flush_training, equal those of training at
+    the end of each frame bit for bit; before the flush the last frame's
+    steps are still pending."""
+    torch = torch_cuda
+    sc = scenes.cornell(level=3)
+    w, h, steps = 16, 16, 3
+    ds = pt.DeviceScene(sc, torch)
+    runs = []
+    for defer in (False, True):
+        print(f"defer_training={defer}", flush=True)
+        tr = pt.PathTracer(w, h, MD, RR, mode=2, train_fraction=0.5, use_graph=False)
+        tr.set_scene(ds)
+        tr.set_camera(sc.camera.frame(w, h))
+        tr.set_schedule(tiles=1, defer_training=defer)
+        assert tr.schedule()["defer_training"] == defer
+        cache = nrc.RadianceCache(batch=256, steps_per_frame=steps, min_records=8)
+        tr.set_cache(cache)
+        img = torch.zeros((w * h, 3), device="cuda")
+        acc = torch.zeros((w * h, 3), dtype=torch.float64, device="cuda")
+        frames = []
+        for f in range(24):
+            if f == 8:
+                tr.set_mode(3)                  # warmup over: 50% training paths, the rest query
+            tr.render(f, img, acc)
+            torch.cuda.synchronize()
+            frames.append(img.cpu().numpy().copy())
+        pending = cache.stats()
+        tr.flush_training()
+        torch.cuda.synchronize()
+        runs.append((frames, acc.cpu().numpy(), cache.weights(), cache.stats(), pending))
+        tr.close()
+    (fa, aa, wa, sa, pa), (fb, ab, wb, sb, pb) = runs
+    for k, (a, b) in enumerate(zip(fa, fb)):
+        np.testing.assert_array_equal(a, b, err_msg=f"frame {k}")
+    np.testing.assert_array_equal(aa, ab)
+    np.testing.assert_array_equal(wa[0], wb[0])
+    np.testing.assert_array_equal(wa[1], wb[1])
+    # the loss is a diagnostic summed with float atomics in any order: equal to
+    # a few ulps; every other statistic exactly
+    nl = lambda st: {k: v for k, v in st.items() if k != "last_loss"}
+    assert nl(sa) == nl(sb) and sa["steps"] > 0
+    assert abs(sa["last_loss"] - sb["last_loss"]) <= 1e-5 * abs(sa["last_loss"])
+    assert nl(pa) == nl(sa)                           # serial: nothing pending
+    assert pb["seed_counter"] + steps == sb["seed_counter"]
+    with pytest.raises(Exception):
+        g = pt.PathTracer(w, h, MD, RR, mode=2, use_graph=True)
+        g.set_schedule(defer_training=True)       # needs direct launches

@@ -65,7 +65,8 @@ class PtSchedule(C.Structure):
     _fields_ = [("trace_block", C.c_uint32), ("grid_mult", C.c_uint32), ("tiles", C.c_uint32),
                 ("side_streams", C.c_uint32), ("overlap", C.c_int32), ("use_graph", C.c_int32),
                 ("xcd_group", C.c_uint32), ("octet_below", C.c_uint32), ("connect_per_cu", C.c_uint32),
-                ("connect_batch", C.c_uint32), ("top_nodes", C.c_uint32), ("tail_from", C.c_uint32)]
+                ("connect_batch", C.c_uint32), ("top_nodes", C.c_uint32), ("tail_from", C.c_uint32),
+                ("defer_training", C.c_uint32)]
 
 
 class NrcConfig(C.Structure):
@@ -137,6 +138,7 @@ _SIGS = {
     "tfg_pt_read_cache_data": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
                                          C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "tfg_pt_set_mode": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "tfg_pt_flush_training": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tfg_pt_nrc_buffers": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                      C.POINTER(C.c_void_p)]),
     "tfg_nrc_default_config": (C.c_int, [C.POINTER(NrcConfig)]),

@@ -8,6 +8,7 @@
 // graph-capturable sequence per frame.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -186,7 +187,8 @@ __global__ __launch_bounds__(256) void k_nrc_eval(const float4* __restrict__ ver
 
 // ------------------------------------------------------------- training
 struct TrainArgs {
-    const float4* records;
+    const float4* records;   // all records, or (gathered) this step's batch in sample order
+    int gathered;
     const uint32_t* n_rec;
     uint32_t min_records;
     uint32_t* state;
@@ -227,7 +229,7 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
     const int m = 16 * w + ml;   // sample within the block
     const uint32_t i = blockIdx.x * 64 + m;
     // generarIndicesAleatorios (NrcKernel.cu:10-16): seed = step counter
-    const uint32_t idx = pcg32_step(A.state[kStSeed] ^ i) % n_rec;
+    const uint32_t idx = A.gathered ? i : pcg32_step(A.state[kStSeed] ^ i) % n_rec;
     const float4* R = A.records + (size_t)kRecF4 * idx;
 
     V x[1][2];
@@ -424,12 +426,34 @@ __global__ __launch_bounds__(256) void k_nrc_publish(const float* __restrict__ m
     put16<T>(fop_infer, fop_pos(p), ema[p]);
 }
 
+// The records the next `steps` optimiser steps will draw, copied in sample
+// order: step k (seed = the seed counter + k, as each step advances it by
+// one) takes records[pcg32(seed ^ i) % n] for sample i, the draw
+// k_nrc_train makes itself; plus the record count.  A copy lets the steps
+// run after the tracer has started appending the next frame's records.
+__global__ void k_nrc_gather(const float4* __restrict__ records, const uint32_t* __restrict__ n_rec, uint32_t min_records,
+                             const uint32_t* __restrict__ state, uint32_t batch, uint32_t steps, float4* __restrict__ out,
+                             uint32_t* __restrict__ out_n) {
+    const uint32_t n = *n_rec;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *out_n = n;
+    if (n <= min_records) return;   // the steps will skip for lack of data
+    const uint32_t seed0 = state[kStSeed];
+    const uint64_t total = (uint64_t)batch * steps * kRecF4;
+    for (uint64_t w = blockIdx.x * blockDim.x + threadIdx.x; w < total; w += gridDim.x * blockDim.x) {
+        const uint32_t j = (uint32_t)(w / kRecF4), c = (uint32_t)(w - (uint64_t)j * kRecF4);
+        const uint32_t k = j / batch, i = j - k * batch;
+        const uint32_t idx = pcg32_step((seed0 + k) ^ i) % n;
+        out[w] = records[(size_t)kRecF4 * idx + c];
+    }
+}
+
 template <typename T>
 int enqueue_train_steps(tfg_nrc* nc, const float4* records, const uint32_t* n_rec, uint32_t steps, hipStream_t s,
-                        uint32_t min_records) {
+                        uint32_t min_records, bool gathered = false) {
     const uint32_t nblk = nc->cfg.batch / 64;
     TrainArgs A;
     A.records = records;
+    A.gathered = gathered ? 1 : 0;
     A.n_rec = n_rec;
     A.min_records = min_records;
     A.state = nc->state;
@@ -445,6 +469,7 @@ int enqueue_train_steps(tfg_nrc* nc, const float4* records, const uint32_t* n_re
     const Hyper h{nc->cfg.lr, nc->cfg.beta1, nc->cfg.beta2, nc->cfg.eps, nc->cfg.ema_decay, 1.0f / nc->cfg.loss_scale};
     const int pblocks = (kNrcParams + 255) / 256;
     for (uint32_t k = 0; k < steps; ++k) {
+        if (gathered) A.records = records + (size_t)k * nc->cfg.batch * kRecF4;
         hipLaunchKernelGGL(k_nrc_step_begin, dim3(1), dim3(64), 0, s, nc->state, nc->loss_accum);
         hipLaunchKernelGGL(k_nrc_train<T>, dim3(nblk), dim3(256), kTrainLds, s, A);
         hipLaunchKernelGGL(k_nrc_grad_reduce, dim3((kNrcParams + kReduceParams - 1) / kReduceParams), dim3(256), 0,
@@ -514,6 +539,27 @@ int dm_reserve(tfg_nrc* nc, size_t n) {
     if (!nc->dm_count) TFG_HIP_CHECK(hipMalloc((void**)&nc->dm_count, sizeof(uint32_t)));
     nc->dm_cap = n;
     return TFG_OK;
+}
+
+size_t nrc_gather_records(const tfg_nrc* nc) {
+    return (size_t)nc->cfg.batch * nc->cfg.steps_per_frame;
+}
+
+int nrc_gather_enqueue(tfg_nrc* nc, const tfg_nrc_frame& f, float4* out, uint32_t* out_n, hipStream_t s) {
+    const size_t total = nrc_gather_records(nc) * kRecF4;
+    const uint32_t grid = (uint32_t)std::min<size_t>((total + 255) / 256, (size_t)nc->n_cu * 16);
+    hipLaunchKernelGGL(k_nrc_gather, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, f.records, f.counters + CNT_RECORDS,
+                       nc->cfg.min_records, nc->state, nc->cfg.batch, nc->cfg.steps_per_frame, out, out_n);
+    TFG_HIP_CHECK(hipGetLastError());
+    return TFG_OK;
+}
+
+int nrc_train_gathered_enqueue(tfg_nrc* nc, const float4* gathered, const uint32_t* n_rec, hipStream_t s) {
+    if (nc->cfg.steps_per_frame == 0) return TFG_OK;
+    return nc->cfg.precision == TFG_NRC_FP16
+               ? enqueue_train_steps<_Float16>(nc, gathered, n_rec, nc->cfg.steps_per_frame, s, nc->cfg.min_records,
+                                               true)
+               : enqueue_train_steps<__bf16>(nc, gathered, n_rec, nc->cfg.steps_per_frame, s, nc->cfg.min_records, true);
 }
 
 int nrc_frame_enqueue(tfg_nrc* nc, const tfg_nrc_frame& f, tfg_color* image, hipStream_t s) {

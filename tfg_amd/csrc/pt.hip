@@ -66,6 +66,19 @@ struct tfg_pt {
     tfg_nrc* nrc = nullptr;
     tfg_nrc_frame nrc_frame{};
     bool allow_no_cache = false;   // drop-in export frames: queries leave the frame uncomposited
+    // Deferred training (schedule defer_training, direct launches only): a
+    // frame's optimiser steps run at the start of the next frame on cstream,
+    // beside its tracing, on a copy of the records they draw; the next cache
+    // inference waits for them, so every inference sees the weights it would
+    // see with the steps at the end of their own frame.
+    bool defer_train = false;
+    bool train_pending = false;   // a copy waits for its steps
+    bool wait_trained = false;    // the next cache inference waits for ev_trained
+    float4* train_buf = nullptr;
+    uint32_t* train_nrec = nullptr;
+    size_t train_cap = 0;         // float4 entries of train_buf
+    hipStream_t cstream = nullptr;
+    hipEvent_t ev_gathered = nullptr, ev_trained = nullptr;
 };
 
 namespace tfg {
@@ -318,13 +331,53 @@ int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, hipStream_t* side, tfg_color
     return TFG_OK;
 }
 
+int ensure_train_buf(tfg_pt* pt) {
+    const size_t need = nrc_gather_records(pt->nrc) * kRecF4;
+    if (need <= pt->train_cap && pt->train_nrec) return TFG_OK;
+    if (pt->train_buf) (void)hipFree(pt->train_buf);
+    pt->train_buf = nullptr;
+    pt->train_cap = 0;
+    TFG_HIP_CHECK(hipMalloc((void**)&pt->train_buf, (need ? need : 1) * sizeof(float4)));
+    if (!pt->train_nrec) TFG_HIP_CHECK(hipMalloc((void**)&pt->train_nrec, sizeof(uint32_t)));
+    pt->train_cap = need;
+    return TFG_OK;
+}
+
+// Enqueues the pending optimiser steps on q, after their copy of the records
+// is complete; the next cache inference waits for them.
+int run_pending_training(tfg_pt* pt, hipStream_t q) {
+    if (!pt->train_pending || !pt->nrc) return TFG_OK;
+    TFG_HIP_CHECK(hipStreamWaitEvent(q, pt->ev_gathered, 0));
+    const int rc = nrc_train_gathered_enqueue(pt->nrc, pt->train_buf, pt->train_nrec, q);
+    if (rc != TFG_OK) return rc;
+    TFG_HIP_CHECK(hipEventRecord(pt->ev_trained, q));
+    pt->train_pending = false;
+    pt->wait_trained = true;
+    return TFG_OK;
+}
+
 // After every tile: cache inference (composite) and training, then the
-// accumulation of the composited image (cache modes only).
-int enqueue_post(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum) {
+// accumulation of the composited image (cache modes only).  With deferred
+// training (direct launches) the training is replaced by the copy of the
+// records its steps will draw.
+int enqueue_post(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum, bool direct) {
     if (pt->P.mode == 0) return TFG_OK;
     if (pt->nrc) {
-        const int rc = nrc_frame_enqueue(pt->nrc, pt->nrc_frame, image, s);
+        if (pt->wait_trained) {
+            TFG_HIP_CHECK(hipStreamWaitEvent(s, pt->ev_trained, 0));
+            pt->wait_trained = false;
+        }
+        const bool defer = direct && pt->defer_train && (pt->P.mode & MODE_TRAIN) && nrc_gather_records(pt->nrc) > 0;
+        tfg_nrc_frame f = pt->nrc_frame;
+        if (defer) f.mode &= ~MODE_TRAIN;
+        int rc = nrc_frame_enqueue(pt->nrc, f, image, s);
+        if (rc == TFG_OK && defer) rc = ensure_train_buf(pt);
+        if (rc == TFG_OK && defer) rc = nrc_gather_enqueue(pt->nrc, pt->nrc_frame, pt->train_buf, pt->train_nrec, s);
         if (rc != TFG_OK) return rc;
+        if (defer) {
+            TFG_HIP_CHECK(hipEventRecord(pt->ev_gathered, s));
+            pt->train_pending = true;
+        }
     }
     if (accum) hipLaunchKernelGGL(k_accumulate, dim3(pt->gen_grid), dim3(256), 0, s, image, accum, pt->P.n_paths);
     TFG_HIP_CHECK(hipGetLastError());
@@ -356,11 +409,12 @@ int enqueue_frame(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum, St
                   bool stats = false) {
     make_tiles(pt);
     mark(pe ? pe->frame[0] : nullptr, s);
-    int rc = fork_tiles(pt, s);
+    int rc = run_pending_training(pt, pt->cstream);   // the previous frame's steps, beside this frame's tracing
+    if (rc == TFG_OK) rc = fork_tiles(pt, s);
     for (int t = 0; t < pt->tiles && rc == TFG_OK; ++t)
         rc = enqueue_tile(pt, t, pt->tstream[t], pt->tside[t], image, accum, pe, stats);
     if (rc == TFG_OK) rc = join_tiles(pt, s);
-    if (rc == TFG_OK) rc = enqueue_post(pt, s, image, accum);
+    if (rc == TFG_OK) rc = enqueue_post(pt, s, image, accum, true);
     mark(pe ? pe->frame[1] : nullptr, s);
     return rc;
 }
@@ -386,7 +440,7 @@ int capture_graphs(tfg_pt* pt, tfg_color* image, double* accum) {
     if (pt->P.mode != 0) {
         hipStream_t cs = pt->cap_post;
         TFG_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_post(pt, cs, image, accum);
+        const int rc = enqueue_post(pt, cs, image, accum, false);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(cs, &g);
         if (rc != TFG_OK || e != hipSuccess) {
@@ -408,6 +462,28 @@ int launch_graphs(tfg_pt* pt, hipStream_t s) {
     if (rc == TFG_OK) rc = join_tiles(pt, s);
     if (rc == TFG_OK && pt->exec_post) TFG_HIP_CHECK(hipGraphLaunch(pt->exec_post, s));
     return rc;
+}
+
+// The HIP runtime maps streams onto a small pool of hardware queues (4 per
+// process by default) that it creates as streams appear.  Measured on the
+// 1080p workload, a tracer whose streams are the first ones the process
+// creates runs the overlapped schedules 15-35% slower (2 side streams: 3.60
+// ms per frame instead of 2.70) than any tracer created after some streams
+// already existed (torch streams, or a tracer created and destroyed without
+// rendering).  So, once per device and process, the pool is primed by
+// creating and destroying as many streams as a tracer uses before the first
+// tracer creates its own.
+void prime_hw_queues() {
+    static bool primed[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || primed[dev]) return;
+    primed[dev] = true;
+    constexpr int kPrime = 2 + kMaxTiles * (1 + kMaxSide);
+    hipStream_t q[kPrime] = {};
+    for (int i = 0; i < kPrime; ++i)
+        if (hipStreamCreateWithFlags(&q[i], hipStreamNonBlocking) != hipSuccess) q[i] = nullptr;
+    for (int i = 0; i < kPrime; ++i)
+        if (q[i]) (void)hipStreamDestroy(q[i]);
 }
 
 }  // namespace
@@ -512,10 +588,14 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
         hipDeviceProp_t prop;
         hipError_t e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+        if (e == hipSuccess) prime_hw_queues();
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&pt->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_in, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_out, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_start, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&pt->cstream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_gathered, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&pt->ev_trained, hipEventDisableTiming);
         if (e == hipSuccess && cfg->use_graph) e = hipStreamCreateWithFlags(&pt->cap_post, hipStreamNonBlocking);
         for (uint32_t t = 0; t < kMaxTiles && e == hipSuccess && cfg->use_graph; ++t) {
             e = hipStreamCreateWithFlags(&pt->cap_tile[t], hipStreamNonBlocking);
@@ -816,28 +896,38 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
         pt->P.n_top = pt->scene_nodes < pt->top_nodes ? pt->scene_nodes : pt->top_nodes;
         set_trace_shape(pt, blocks[(i % 12) / 4], mults[i % 4]);
     });
-    // 2. tiles, side streams and overlap
+    // 2. tiles, side streams and overlap; 3. XCD super-chunks, octet
+    // threshold, shadow-stage grid and batching, tail kernel; then a second
+    // round over every setting but the launch shape, from the first round's
+    // result (the first round sweeps the top nodes and some settings before
+    // the ones that interact with them are tuned)
     struct Cand {
         int tiles, sides;
         bool overlap;
     };
     const Cand cands[] = {{1, 1, false}, {1, 1, true}, {1, 2, true}, {1, 4, true}, {2, 2, true}, {2, 4, true}};
-    sweep(6, [&](int i) {
-        pt->tiles = cands[i].tiles;
-        pt->sides = cands[i].sides;
-        pt->overlap = cands[i].overlap;
-    });
-    // 3. XCD super-chunks, octet threshold, shadow-stage grid
     const uint32_t groups[4] = {0, 1024, 4096, 16384};
-    sweep(4, [&](int i) { pt->P.xcd_group = groups[i]; });
     const uint32_t octs[6] = {0, 1u << 16, 1u << 17, 1u << 18, 1u << 19, 1u << 20};
-    sweep(6, [&](int i) { pt->P.octet_below = octs[i]; });
     const int cpcs[3] = {0, 2, 3};
-    sweep(3, [&](int i) { pt->connect_per_cu = cpcs[i]; });
     const uint32_t batches[6] = {0, 2, 3, 4, 5, 6};
-    sweep(6, [&](int i) { pt->connect_batch = batches[i]; });
     const uint32_t tails[6] = {0, 2, 3, 4, 5, 6};
-    sweep(6, [&](int i) { pt->tail_from = tails[i]; });
+    for (int round = 0; round < 2 && rc == TFG_OK; ++round) {
+        if (round > 0)
+            sweep(4, [&](int i) {
+                pt->top_nodes = tops[i];
+                pt->P.n_top = pt->scene_nodes < pt->top_nodes ? pt->scene_nodes : pt->top_nodes;
+            });
+        sweep(6, [&](int i) {
+            pt->tiles = cands[i].tiles;
+            pt->sides = cands[i].sides;
+            pt->overlap = cands[i].overlap;
+        });
+        sweep(4, [&](int i) { pt->P.xcd_group = groups[i]; });
+        sweep(6, [&](int i) { pt->P.octet_below = octs[i]; });
+        sweep(3, [&](int i) { pt->connect_per_cu = cpcs[i]; });
+        sweep(6, [&](int i) { pt->connect_batch = batches[i]; });
+        sweep(6, [&](int i) { pt->tail_from = tails[i]; });
+    }
     drop_graph(pt);
     pt->P.mode = saved_mode;
     (void)hipEventDestroy(a);
@@ -868,6 +958,7 @@ int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out) {
     out->connect_batch = pt->connect_batch;
     out->top_nodes = pt->top_nodes;
     out->tail_from = pt->tail_from;
+    out->defer_training = pt->defer_train ? 1 : 0;
     return TFG_OK;
 }
 
@@ -878,16 +969,25 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
         sc->tiles > kMaxTiles || sc->side_streams == 0 || sc->side_streams > kMaxSide ||
         (sc->xcd_group != 0 && (sc->xcd_group % 256 != 0 || sc->xcd_group > (1u << 20))) ||
         sc->connect_per_cu > 64 || sc->connect_batch > kMaxDepth || sc->top_nodes > kTopNodes ||
-        sc->tail_from > kMaxDepth) {
+        sc->tail_from > kMaxDepth || sc->defer_training > 1) {
         set_error("tfg_pt_set_schedule: block 64/128/256, grid_mult 1..64, tiles 1..%u, side streams 1..%u, "
                   "xcd_group 0 or a multiple of 256 up to 2^20, connect_per_cu 0..64, connect_batch and "
-                  "tail_from 0..%u, top_nodes 0..%u", kMaxTiles, kMaxSide, kMaxDepth, kTopNodes);
+                  "tail_from 0..%u, top_nodes 0..%u, defer_training 0/1", kMaxTiles, kMaxSide, kMaxDepth, kTopNodes);
         return TFG_E_INVALID;
     }
     if ((sc->use_graph != 0) != pt->run_graph) {
         set_error("tfg_pt_set_schedule: graph replay is fixed when the tracer is created");
         return TFG_E_STATE;
     }
+    if (sc->defer_training && pt->run_graph) {
+        set_error("tfg_pt_set_schedule: defer_training needs direct launches (tfg_pt_config.use_graph = 0)");
+        return TFG_E_STATE;
+    }
+    if (!sc->defer_training) {
+        const int rc = run_pending_training(pt, pt->stream);
+        if (rc != TFG_OK) return rc;
+    }
+    pt->defer_train = sc->defer_training != 0;
     drop_graph(pt);
     pt->top_nodes = sc->top_nodes;
     pt->P.n_top = pt->scene_nodes < pt->top_nodes ? pt->scene_nodes : pt->top_nodes;
@@ -906,6 +1006,10 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
 int tfg_pt_set_nrc(tfg_pt* pt, tfg_nrc* nrc) {
     if (!pt) { set_error("tfg_pt_set_nrc: NULL"); return TFG_E_INVALID; }
     if (nrc && pt->cfg.mode == 0) { set_error("tfg_pt_set_nrc: tracer was created in plain mode"); return TFG_E_STATE; }
+    if (nrc != pt->nrc) {   // the previous cache's pending steps first
+        const int rc = run_pending_training(pt, pt->stream);
+        if (rc != TFG_OK) return rc;
+    }
     pt->nrc = nrc;
     pt->nrc_frame.records = pt->P.records;
     pt->nrc_frame.queries = pt->P.queries;
@@ -913,6 +1017,18 @@ int tfg_pt_set_nrc(tfg_pt* pt, tfg_nrc* nrc) {
     pt->nrc_frame.n_paths = pt->P.n_paths;
     pt->nrc_frame.mode = pt->P.mode;
     drop_graph(pt);
+    return TFG_OK;
+}
+
+int tfg_pt_flush_training(tfg_pt* pt, tfg_stream stream) {
+    if (!pt) { set_error("tfg_pt_flush_training: NULL"); return TFG_E_INVALID; }
+    if (!pt->train_pending) return TFG_OK;
+    TFG_HIP_CHECK(hipEventRecord(pt->ev_in, stream));
+    TFG_HIP_CHECK(hipStreamWaitEvent(pt->stream, pt->ev_in, 0));
+    const int rc = run_pending_training(pt, pt->stream);
+    if (rc != TFG_OK) return rc;
+    TFG_HIP_CHECK(hipEventRecord(pt->ev_out, pt->stream));
+    TFG_HIP_CHECK(hipStreamWaitEvent(stream, pt->ev_out, 0));
     return TFG_OK;
 }
 
@@ -954,6 +1070,7 @@ int tfg_pt_read_cache_data(tfg_pt* pt, float* records, uint32_t max_records, flo
 int tfg_pt_destroy(tfg_pt* pt) {
     if (!pt) return TFG_OK;
     if (pt->stream) (void)hipStreamSynchronize(pt->stream);
+    if (pt->cstream) (void)hipStreamSynchronize(pt->cstream);   // pending steps not yet launched are dropped
     for (uint32_t t = 0; t < kMaxTiles; ++t) {
         if (pt->tstream[t]) (void)hipStreamSynchronize(pt->tstream[t]);
         for (hipStream_t q : pt->tside[t])
@@ -964,6 +1081,10 @@ int tfg_pt_destroy(tfg_pt* pt) {
     if (pt->ev_in) (void)hipEventDestroy(pt->ev_in);
     if (pt->ev_out) (void)hipEventDestroy(pt->ev_out);
     if (pt->ev_start) (void)hipEventDestroy(pt->ev_start);
+    if (pt->ev_gathered) (void)hipEventDestroy(pt->ev_gathered);
+    if (pt->ev_trained) (void)hipEventDestroy(pt->ev_trained);
+    if (pt->train_buf) (void)hipFree(pt->train_buf);
+    if (pt->train_nrec) (void)hipFree(pt->train_nrec);
     for (uint32_t t = 0; t < kMaxTiles; ++t) {
         for (hipEvent_t e : pt->ev_cjoin[t])
             if (e) (void)hipEventDestroy(e);
@@ -979,6 +1100,7 @@ int tfg_pt_destroy(tfg_pt* pt) {
     }
     if (pt->cap_post) (void)hipStreamDestroy(pt->cap_post);
     if (pt->stream) (void)hipStreamDestroy(pt->stream);
+    if (pt->cstream) (void)hipStreamDestroy(pt->cstream);
     delete pt;
     return TFG_OK;
 }

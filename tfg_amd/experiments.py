@@ -25,6 +25,26 @@ def _sync_time(torch):
     return time.perf_counter()
 
 
+def _run_budget(torch, render, budget_s, first):
+    """render(first), render(first + 1), ... until budget_s of wall time have
+    passed, with one frame in flight: after submitting frame n the host waits
+    for frame n - 1 and checks the clock, so the GPU never idles between
+    frames while the host enqueues the next one.  The time returned runs to
+    the end of the last submitted frame.  Returns (frames, seconds)."""
+    evs = [torch.cuda.Event(), torch.cuda.Event()]
+    t0 = _sync_time(torch)
+    n = 0
+    while True:
+        render(first + n)
+        evs[n % 2].record()
+        if n > 0:
+            evs[(n - 1) % 2].synchronize()
+        n += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    return n, _sync_time(torch) - t0
+
+
 def _graph(schedule):
     return bool(schedule.get("use_graph", True)) if schedule else True
 
@@ -56,19 +76,12 @@ def plain_run(torch, dscene, W, H, max_depth, rr_depth, budget_s, frame0=0, sche
     img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
     acc = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
     tr.render(frame0, img)          # capture the graph outside the timed region
-    t0 = _sync_time(torch)
-    n = 0
-    while True:
-        tr.render(frame0 + 1 + n, img, acc)
-        n += 1
-        t = _sync_time(torch)
-        if t - t0 >= budget_s:
-            break
-    return (acc / n).cpu().numpy(), n, t - t0
+    n, secs = _run_budget(torch, lambda f: tr.render(f, img, acc), budget_s, frame0 + 1)
+    return (acc / n).cpu().numpy(), n, secs
 
 
 def nrc_run(torch, dscene, W, H, max_depth, rr_depth, budget_s, warmup=48, precision=0, train_fraction=0.05,
-            frame0=0, cache_cfg=None, count_warmup=True, schedule=None, cache_tail_from=None):
+            frame0=0, cache_cfg=None, count_warmup=True, schedule=None, cache_tail_from=None, defer_training=True):
     """Cache-accelerated PT: `warmup` training-only frames, then train+infer
     frames until the budget (wall time including warmup when
     count_warmup) is spent.  Returns (mean image, frames, seconds, cache stats).
@@ -76,11 +89,16 @@ def nrc_run(torch, dscene, W, H, max_depth, rr_depth, budget_s, warmup=48, preci
     Train+infer frames end most paths at their first cache query, so depths
     >= 2 hold little more than the training paths; they run with the
     schedule's other settings but ``tail_from=cache_tail_from`` (default
-    CACHE_TAIL_FROM; negative keeps the schedule's value)."""
+    CACHE_TAIL_FROM; negative keeps the schedule's value).  With direct
+    launches and defer_training, each frame's optimiser steps run beside the
+    next frame's tracing (schedule defer_training: same images and weights);
+    the last frame's steps are flushed after the clock stops."""
     tr = pt.PathTracer(W, H, max_depth, rr_depth, mode=pt_mode_train(), train_fraction=train_fraction,
                        use_graph=_graph(schedule))
     tr.set_scene(dscene)
     _apply(tr, schedule)
+    if defer_training and not _graph(schedule):
+        tr.set_schedule(defer_training=True)
     tr.set_camera(dscene.scene.camera.frame(W, H))
     sc = dscene.scene
     lo = sc.verts.reshape(-1, 3).min(0) - 0.5
@@ -100,15 +118,11 @@ def nrc_run(torch, dscene, W, H, max_depth, rr_depth, budget_s, warmup=48, preci
     cache_tail_from = CACHE_TAIL_FROM if cache_tail_from is None else cache_tail_from
     if cache_tail_from >= 0:
         tr.set_schedule(tail_from=cache_tail_from)
-    start = t0 if count_warmup else t_w
-    n = 0
-    while True:
-        tr.render(frame0 + warmup + n, img, acc)
-        n += 1
-        t = _sync_time(torch)
-        if t - start >= budget_s and n >= 1:
-            break
-    return (acc / n).cpu().numpy(), n, t - start, cache.stats(), t_w - t0
+    spent = t_w - t0 if count_warmup else 0.0
+    n, secs = _run_budget(torch, lambda f: tr.render(f, img, acc), budget_s - spent, frame0 + warmup)
+    tr.flush_training()   # the last frame's steps, after the clock
+    torch.cuda.synchronize()
+    return (acc / n).cpu().numpy(), n, secs + spent, cache.stats(), t_w - t0
 
 
 # Train+infer frames end most paths at their first cache query, so depths >= 2

@@ -81,6 +81,11 @@ class PathTracer:
     def set_mode(self, mode: int):
         check(lib().tfg_pt_set_mode(self._h, mode), "tfg_pt_set_mode")
 
+    def flush_training(self, stream=None):
+        """With schedule defer_training: enqueue the last training frame's
+        optimiser steps now (they otherwise run at the next render)."""
+        check(lib().tfg_pt_flush_training(self._h, stream), "tfg_pt_flush_training")
+
     def cache_buffers(self):
         r, q, c = C.c_void_p(), C.c_void_p(), C.c_void_p()
         check(lib().tfg_pt_nrc_buffers(self._h, C.byref(r), C.byref(q), C.byref(c)), "tfg_pt_nrc_buffers")
@@ -142,11 +147,12 @@ class PathTracer:
         return {"trace_block": sc.trace_block, "grid_mult": sc.grid_mult, "tiles": sc.tiles,
                 "side_streams": sc.side_streams, "overlap": bool(sc.overlap), "use_graph": bool(sc.use_graph),
                 "xcd_group": sc.xcd_group, "octet_below": sc.octet_below, "connect_per_cu": sc.connect_per_cu,
-                "connect_batch": sc.connect_batch, "top_nodes": sc.top_nodes, "tail_from": sc.tail_from}
+                "connect_batch": sc.connect_batch, "top_nodes": sc.top_nodes, "tail_from": sc.tail_from,
+                "defer_training": bool(sc.defer_training)}
 
     def set_schedule(self, **kw):
         """Change any of trace_block, grid_mult, tiles, side_streams, overlap, xcd_group,
-        octet_below, connect_per_cu, connect_batch, top_nodes, tail_from (use_graph is fixed when the tracer is created)."""
+        octet_below, connect_per_cu, connect_batch, top_nodes, tail_from, defer_training (use_graph is fixed when the tracer is created)."""
         cur = self.schedule()
         unknown = set(kw) - set(cur)
         if unknown:
@@ -155,7 +161,7 @@ class PathTracer:
         sc = _lib.PtSchedule(cur["trace_block"], cur["grid_mult"], cur["tiles"], cur["side_streams"],
                              1 if cur["overlap"] else 0, 1 if cur["use_graph"] else 0, cur["xcd_group"],
                              cur["octet_below"], cur["connect_per_cu"], cur["connect_batch"], cur["top_nodes"],
-                             cur["tail_from"])
+                             cur["tail_from"], 1 if cur["defer_training"] else 0)
         check(lib().tfg_pt_set_schedule(self._h, C.byref(sc)), "tfg_pt_set_schedule")
 
     def launch_shape(self):
