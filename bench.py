@@ -544,7 +544,9 @@ def bench_multiview(args, torch):
         (the shared cache's training steps keep view order); each view's frame
         latency p50/p95/p99 and Mrays/s.
     Each tracer's train+infer frames run with the faster of tail_from 0 and
-    experiments.CACHE_TAIL_FROM (_pick_cache_tail, before the timed frames).
+    experiments.CACHE_TAIL_FROM (_pick_cache_tail, before the timed frames);
+    the canvas tracers, each the only user of its cache, defer their
+    training to the next frame (schedule defer_training).
     Then BF16 vs FP16 cache inference on the batched canvas: a fresh cache per
     precision trained over the same frames, each view's image accumulated over
     the same frames and scored by relMSE against a plain reference of that
@@ -584,6 +586,7 @@ def bench_multiview(args, torch):
     for f in range(warm):
         tr.render(f, img, stream=stream.cuda_stream)
     tr.set_mode(both)
+    tr.set_schedule(defer_training=True)   # one tracer owns this cache
     tail_b = _pick_cache_tail(torch, lambda f: tr.render(f, img, stream=stream.cuda_stream), tr, 500_000)
     for f in range(2):
         tr.render(warm + f, img, stream=stream.cuda_stream)
@@ -665,7 +668,7 @@ def bench_multiview(args, torch):
     rel = {}
     for name, prec in (("bf16", 0), ("fp16", 1)):
         tr, cache = canvas(prec)
-        tr.set_schedule(**sched)
+        tr.set_schedule(**{**sched, "defer_training": True})
         for f in range(warm):
             tr.render(f, img)
         tr.set_mode(both)
