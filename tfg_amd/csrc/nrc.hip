@@ -105,8 +105,14 @@ __device__ __forceinline__ void load_ops(typename MfmaOps<T>::vec* dst, const ty
 // ------------------------------------------------------------- inference
 // Queries (kQueryF4 float4: vertex + (throughput, path id)) -> cache
 // prediction, sanitised, composited into the image (RenderKernel.cu:49-66).
+// Workgroups of 8 waves share one LDS copy of the weights (42 KB: 3
+// workgroups per CU); a wave infers 16 queries at a time (G = 1: 68 VGPRs,
+// 6 waves per SIMD), which measured faster than 2 or 4 groups of 16 per wave
+// with their fewer waves: 505 vs 417 / 376 TFLOP/s on a 1080p frame's queries.
+constexpr int kInferBlock = 512;
+constexpr int kInferGroups = 1;
 template <typename T, int G>
-__global__ __launch_bounds__(256) void k_nrc_infer(const float4* __restrict__ queries, const uint32_t* __restrict__ count,
+__global__ __launch_bounds__(kInferBlock) void k_nrc_infer(const float4* __restrict__ queries, const uint32_t* __restrict__ count,
                                                    tfg_color* __restrict__ image,
                                                    const typename MfmaOps<T>::vec* __restrict__ fop, NrcBounds b) {
     typedef typename MfmaOps<T>::vec V;
@@ -115,7 +121,8 @@ __global__ __launch_bounds__(256) void k_nrc_infer(const float4* __restrict__ qu
     __syncthreads();
     const uint32_t n = *count;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, ml = lane & 15;
-    for (uint32_t base = (blockIdx.x * 4 + w) * 16u * G; base < n; base += gridDim.x * 4 * 16u * G) {
+    constexpr uint32_t kWaves = kInferBlock / 64;
+    for (uint32_t base = (blockIdx.x * kWaves + w) * 16u * G; base < n; base += gridDim.x * kWaves * 16u * G) {
         V x[G][2];
 #pragma unroll
         for (int q = 0; q < G; ++q) {
@@ -485,7 +492,7 @@ int enqueue_train_steps(tfg_nrc* nc, const float4* records, const uint32_t* n_re
 template <typename T>
 int enqueue_infer(tfg_nrc* nc, const float4* queries, const uint32_t* count, tfg_color* image, hipStream_t s) {
     typedef typename MfmaOps<T>::vec V;
-    hipLaunchKernelGGL((k_nrc_infer<T, 4>), dim3(nc->n_cu * 4), dim3(256), 0, s, queries, count, image,
+    hipLaunchKernelGGL((k_nrc_infer<T, kInferGroups>), dim3(nc->n_cu * 4), dim3(kInferBlock), 0, s, queries, count, image,
                        static_cast<const V*>(nc->fop_infer), nc->bounds);
     TFG_HIP_CHECK(hipGetLastError());
     return TFG_OK;
