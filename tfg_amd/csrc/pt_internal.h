@@ -19,8 +19,6 @@ constexpr uint32_t kMaxLights = 64;   // one wave64 lane per light of a shadow e
 // counts; each pixel tile has its own block for its queue lengths.  Both use
 // this layout; a block uses only its own entries.
 enum : uint32_t {
-    CNT_Q0 = 0,           // tile: ray queue 0 length
-    CNT_Q1 = 1,           // tile: ray queue 1 length
     CNT_RESERVED = 2,
     CNT_RECORDS = 3,      // frame: training records written this frame
     CNT_QUERIES = 4,      // frame: cache queries written this frame
@@ -30,7 +28,11 @@ enum : uint32_t {
     CNT_CUM = 80,         // frame: 2 x uint64, rays of all previous frames
     CNT_STAT = 84,        // frame: 4 x uint64, extend nodes, extend tris, shadow nodes, shadow tris
     CNT_SHQ = 96,         // tile, + depth: shadow queue length of that depth
-    CNT_ALLOC = CNT_SHQ + kMaxDepth,
+    // tile, + depth: length of the ray queue traced at that depth (the ray
+    // buffers alternate, the counts do not: a launch that appends to the next
+    // depth's queue never shares its counter with a launch that reads one)
+    CNT_QD = CNT_SHQ + kMaxDepth,
+    CNT_ALLOC = 192,      // >= CNT_QD + kMaxDepth + 1
 };
 constexpr uint32_t kMaxTiles = 4;   // pixel tiles traced as independent stream chains
 constexpr uint32_t kMaxViews = 64;  // cameras sharing one canvas (tfg_pt_set_views)

@@ -50,10 +50,7 @@ __global__ void k_set_frame(int32_t* frame, int32_t value) { *frame = value; }
 __global__ __launch_bounds__(256) void k_generate(PtParams P) {
     if (blockIdx.x == 0) {
         for (uint32_t k = threadIdx.x; k < kMaxDepth; k += blockDim.x) P.counters[CNT_SHQ + k] = 0u;
-        if (threadIdx.x == 0) {
-            P.counters[CNT_Q0] = P.n_paths;
-            P.counters[CNT_Q1] = 0u;
-        }
+        for (uint32_t k = threadIdx.x; k <= kMaxDepth; k += blockDim.x) P.counters[CNT_QD + k] = k ? 0u : P.n_paths;
     }
     const int32_t frame = *P.frame;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n_paths; i += gridDim.x * blockDim.x) {
@@ -125,11 +122,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint3
     extern __shared__ __align__(16) unsigned char lds[];
     float4* top = reinterpret_cast<float4*>(lds);
     uint2* stack = reinterpret_cast<uint2*>(top + 5 * P.n_top);
-    const uint32_t count = P.counters[q];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (count) atomicAdd(&P.gcounters[CNT_EXT + depth], count);
-        P.counters[q ^ 1u] = 0;
-    }
+    const uint32_t count = P.counters[CNT_QD + depth];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && count) atomicAdd(&P.gcounters[CNT_EXT + depth], count);
     const float4* rays = P.rays[q];
     // few rays: 8 lanes per ray (traverse_octet), kBlock / 8 rays per chunk
     const bool octet = count <= P.octet_below;
@@ -401,7 +395,7 @@ __device__ __forceinline__ void shade_entry(const PtParams& P, uint32_t depth, b
 
 template <bool kNrc>
 __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t depth) {
-    const uint32_t count = P.counters[q];
+    const uint32_t count = P.counters[CNT_QD + depth];
     const uint32_t qo = q ^ 1u;
     const bool last = depth + 1 >= P.max_depth;
     uint32_t facing = 0;   // shadow rays this thread's entries will trace (lights above the surface)
@@ -417,7 +411,8 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
                               want_shadow, e0, e1, e2, e3, want_ray, n0, n1);
         }
         uint32_t ss, rs;
-        block_append2<256>(want_shadow, &P.counters[CNT_SHQ + depth], ss, want_ray, &P.counters[qo], rs);
+        block_append2<256>(want_shadow, &P.counters[CNT_SHQ + depth], ss, want_ray, &P.counters[CNT_QD + depth + 1],
+                           rs);
         if (want_shadow) {
             for (uint32_t l = 0; l < P.n_lights; ++l) {
                 V3 dir;
@@ -556,7 +551,7 @@ __global__ __launch_bounds__(kBlock) void k_tail(PtParams P, uint32_t q, uint32_
     float4* top = reinterpret_cast<float4*>(lds);
     uint2* lane_stack = reinterpret_cast<uint2*>(top + 5 * P.n_top);
     uint2* oct_stack = lane_stack + (size_t)P.stack_depth * kBlock;
-    const uint32_t count = P.counters[q];
+    const uint32_t count = P.counters[CNT_QD + d0];
     const uint32_t chunks = (count + kOct - 1) / kOct;
     if (blockIdx.x >= chunks) return;   // workgroups without paths leave before staging
     for (uint32_t k = threadIdx.x; k < kMaxDepth; k += kBlock) n_ext[k] = n_shd[k] = 0u;
