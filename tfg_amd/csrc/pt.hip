@@ -1,6 +1,7 @@
 // pt.hip - host side of the wavefront path tracer (tfg_pt_* C ABI).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -474,10 +475,9 @@ int launch_graphs(tfg_pt* pt, hipStream_t s) {
 // creating and destroying as many streams as a tracer uses before the first
 // tracer creates its own.
 void prime_hw_queues() {
-    static bool primed[64] = {};
+    static std::atomic<bool> primed[64];
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || primed[dev]) return;
-    primed[dev] = true;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || primed[dev].exchange(true)) return;
     constexpr int kPrime = 2 + kMaxTiles * (1 + kMaxSide);
     hipStream_t q[kPrime] = {};
     for (int i = 0; i < kPrime; ++i)
