@@ -210,7 +210,12 @@ struct TrainArgs {
     float* loss_accum;
 };
 
-constexpr size_t kTrainLds = (size_t)(2 * kFopTiles + kBopTiles) * 64 * kOpBytes16 + 2 * 64 * 64 * 2;
+// Row stride (16-bit elements) of the transposed dy / x tiles in LDS: 72,
+// not 64, so that the 4 lane groups writing rows 4 apart hit different banks
+// (a 128-byte stride put them on one bank); rows stay 16-byte aligned for the
+// operand reads.
+constexpr int kTS = 72;
+constexpr size_t kTrainLds = (size_t)(2 * kFopTiles + kBopTiles) * 64 * kOpBytes16 + 2 * 64 * kTS * 2;
 
 // One block = 64 training samples (4 waves x 16).  Tail inference with the
 // inference (EMA) weights, targets (RenderKernel.cu:92-137), head forward
@@ -224,8 +229,8 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
     V* s_fop = reinterpret_cast<V*>(smem);
     V* s_inf = s_fop + kFopTiles * 64;
     V* s_bop = s_inf + kFopTiles * 64;
-    T* s_dyT = reinterpret_cast<T*>(s_bop + kBopTiles * 64);   // [64 neurons][64 samples]
-    T* s_xT = s_dyT + 64 * 64;                                 // [64 inputs][64 samples]
+    T* s_dyT = reinterpret_cast<T*>(s_bop + kBopTiles * 64);   // [64 neurons][kTS: 64 samples]
+    T* s_xT = s_dyT + 64 * kTS;                                // [64 inputs][kTS: 64 samples]
     const uint32_t n_rec = *A.n_rec;
     if (n_rec <= A.min_records) return;
     load_ops<T>(s_fop, static_cast<const V*>(A.fop_train), kFopTiles);
@@ -273,18 +278,18 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
 
     float* part = A.partial + (size_t)blockIdx.x * kNrcParams;
     // ---- layer 5 (16 x 64) ----
-    for (int k = threadIdx.x; k < 16 * 64; k += 256) s_dyT[k] = (T)0.0f;   // rows 4..15 stay zero
+    for (int k = threadIdx.x; k < 16 * kTS; k += 256) s_dyT[k] = (T)0.0f;   // rows 4..15 stay zero
     __syncthreads();
-    for (int ii = 0; ii < 4; ++ii) s_dyT[(4 * g + ii) * 64 + m] = (T)dy5[ii];
+    for (int ii = 0; ii < 4; ++ii) s_dyT[(4 * g + ii) * kTS + m] = (T)dy5[ii];
     for (int c = 0; c < 2; ++c)
-        for (int j = 0; j < 8; ++j) s_xT[nrc_perm(c, g, j) * 64 + m] = xs[5][c][j];
+        for (int j = 0; j < 8; ++j) s_xT[nrc_perm(c, g, j) * kTS + m] = xs[5][c][j];
     __syncthreads();
     {
         const int u = w;   // k-tile of this wave
         floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
         for (int c = 0; c < 2; ++c) {
-            const V a = *reinterpret_cast<const V*>(&s_dyT[ml * 64 + 32 * c + 8 * g]);
-            const V bb = *reinterpret_cast<const V*>(&s_xT[(16 * u + ml) * 64 + 32 * c + 8 * g]);
+            const V a = *reinterpret_cast<const V*>(&s_dyT[ml * kTS + 32 * c + 8 * g]);
+            const V bb = *reinterpret_cast<const V*>(&s_xT[(16 * u + ml) * kTS + 32 * c + 8 * g]);
             acc = M::mfma(a, bb, acc);
         }
         for (int ii = 0; ii < 4; ++ii) part[nrc_layer_offset(5) + (4 * g + ii) * 64 + 16 * u + ml] = acc[ii];
@@ -304,15 +309,15 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
 #pragma unroll
     for (int l = 4; l >= 0; --l) {
         for (int t = 0; t < 4; ++t)
-            for (int ii = 0; ii < 4; ++ii) s_dyT[(16 * t + 4 * g + ii) * 64 + m] = (T)dy[t][ii];
+            for (int ii = 0; ii < 4; ++ii) s_dyT[(16 * t + 4 * g + ii) * kTS + m] = (T)dy[t][ii];
         for (int c = 0; c < 2; ++c)
-            for (int j = 0; j < 8; ++j) s_xT[nrc_perm(c, g, j) * 64 + m] = xs[l][c][j];
+            for (int j = 0; j < 8; ++j) s_xT[nrc_perm(c, g, j) * kTS + m] = xs[l][c][j];
         __syncthreads();
         for (int u = 0; u < 4; ++u) {
             floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
             for (int c = 0; c < 2; ++c) {
-                const V a = *reinterpret_cast<const V*>(&s_dyT[(16 * w + ml) * 64 + 32 * c + 8 * g]);
-                const V bb = *reinterpret_cast<const V*>(&s_xT[(16 * u + ml) * 64 + 32 * c + 8 * g]);
+                const V a = *reinterpret_cast<const V*>(&s_dyT[(16 * w + ml) * kTS + 32 * c + 8 * g]);
+                const V bb = *reinterpret_cast<const V*>(&s_xT[(16 * u + ml) * kTS + 32 * c + 8 * g]);
                 acc = M::mfma(a, bb, acc);
             }
             for (int ii = 0; ii < 4; ++ii) part[nrc_layer_offset(l) + (16 * w + 4 * g + ii) * 64 + 16 * u + ml] = acc[ii];
