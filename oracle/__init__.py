@@ -25,6 +25,12 @@ class TfgRay(C.Structure):
 _lib = None
 
 _SIGS = {
+    "orc_cosine_local": (None, [C.c_double, C.c_double, C.c_void_p]),
+    "orc_ggx_D": (C.c_double, [C.c_double, C.c_double]),
+    "orc_ggx_G1": (C.c_double, [C.c_double, C.c_double]),
+    "orc_ggx_eval": (None, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "orc_ggx_vndf": (None, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_void_p]),
+    "orc_analytic_t": (C.c_float, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "orc_pcg32": (C.c_uint32, [C.POINTER(C.c_uint32)]),
     "orc_pcg32_float": (C.c_float, [C.POINTER(C.c_uint32)]),
     "orc_seed": (C.c_uint32, [C.c_int, C.c_int, C.c_int]),

@@ -743,6 +743,13 @@ D3 ggx_vndf_d(D3 wo, double alpha, double u1, double u2) {   // Heitz 2018
     return dnorm(d3(alpha * nh.x, alpha * nh.y, std::max(0.0, nh.z)));
 }
 
+// render.h:383-390 cosine-weighted hemisphere draw about +z: phi = 2 pi r1,
+// theta = acos(sqrt(1 - r2)), so cos(theta) = sqrt(1 - r2) has pdf 2 cos.
+D3 cosine_local(double r1, double r2) {
+    const double phi = 2.0 * M_PI * r1, th = std::acos(std::sqrt(1.0 - r2));
+    return {std::sin(th) * std::cos(phi), std::sin(th) * std::sin(phi), std::cos(th)};
+}
+
 // primitiva.h:137-187: vectors in FP64, the scalars the reference keeps in
 // float rounded to float.  Returns t > 1e-6 or -1.
 float o_analytic_t(const tfg_analytic& q, D3 o, D3 d) {
@@ -888,8 +895,7 @@ Col o_path(const orc_scene* sc, int W, int H, int x, int y, int frame, int max_d
         if (r < pd) {
             camino = cdiv(camino * m.kd, (float)pd);
             const double r1 = pcg32_float(s), r2 = pcg32_float(s);
-            const double phi = 2.0 * M_PI * r1, th = std::acos(std::sqrt(1.0 - r2));
-            const D3 wl = {std::sin(th) * std::cos(phi), std::sin(th) * std::sin(phi), std::cos(th)};
+            const D3 wl = cosine_local(r1, r2);
             if (wl.z <= kEpsF) break;
             wi = o_world(oframe(n), wl);
         } else if (r < pd + ps) {
@@ -935,6 +941,25 @@ Col o_path(const orc_scene* sc, int W, int H, int x, int y, int frame, int max_d
 }  // namespace
 
 extern "C" {
+
+// Shading functions of the integrator, exported for the closed-form tests.
+void orc_cosine_local(double r1, double r2, double out[3]) {
+    const D3 w = cosine_local(r1, r2);
+    out[0] = w.x; out[1] = w.y; out[2] = w.z;
+}
+double orc_ggx_D(double alpha, double nh) { return ggx_D_d(alpha * alpha, nh); }
+double orc_ggx_G1(double alpha, double nv) { return ggx_G1_d(alpha * alpha, nv); }
+void orc_ggx_eval(const float ks[3], double alpha, const double wo[3], const double wi[3], float out[3]) {
+    const Col f = ggx_eval_d({ks[0], ks[1], ks[2]}, alpha, d3(wo[0], wo[1], wo[2]), d3(wi[0], wi[1], wi[2]));
+    out[0] = f.r; out[1] = f.g; out[2] = f.b;
+}
+void orc_ggx_vndf(const double wo[3], double alpha, double u1, double u2, double out[3]) {
+    const D3 h = ggx_vndf_d(d3(wo[0], wo[1], wo[2]), alpha, u1, u2);
+    out[0] = h.x; out[1] = h.y; out[2] = h.z;
+}
+float orc_analytic_t(const tfg_analytic* q, const double o[3], const double d[3]) {
+    return o_analytic_t(*q, d3(o[0], o[1], o[2]), d3(d[0], d[1], d[2]));
+}
 
 void orc_render_rows(const orc_scene* sc, int width, int height, int y0, int y1, int frame0, int spp, int max_depth,
                      int rr_depth, double* accum, uint64_t* rays_out, int nthreads) {
@@ -1088,8 +1113,7 @@ void o_path_nrc(const orc_scene* sc, int W, int H, int x, int y, int frame, int 
         if (r < pd) {
             camino = cdiv(camino * m.kd, (float)pd);
             const double r1 = pcg32_float(s), r2 = pcg32_float(s);
-            const double phi = 2.0 * M_PI * r1, th = std::acos(std::sqrt(1.0 - r2));
-            const D3 wl = {std::sin(th) * std::cos(phi), std::sin(th) * std::sin(phi), std::cos(th)};
+            const D3 wl = cosine_local(r1, r2);
             if (wl.z <= kEpsF) dead = true;
             else {
                 wi = o_world(oframe(n), wl);

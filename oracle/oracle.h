@@ -102,6 +102,18 @@ void orc_render_nrc(const orc_scene* sc, int width, int height, int frame, int m
 void orc_render_pixel(const orc_scene* sc, int width, int height, int x, int y, int frame, int max_depth,
                       int rr_depth, float* out_rgb);
 
+/* Shading functions of the integrator above, for the closed-form tests:
+ * cosine-weighted direction about +z (render.h:383-390); GGX D, Smith G1
+ * and the microfacet BRDF times Schlick Fresnel in the local frame (n =
+ * +z); the visible-normal draw (Heitz 2018); analytic sphere / plane /
+ * triangle distance (primitiva.h:137-187, -1 for a miss). */
+void orc_cosine_local(double r1, double r2, double out[3]);
+double orc_ggx_D(double alpha, double nh);
+double orc_ggx_G1(double alpha, double nv);
+void orc_ggx_eval(const float ks[3], double alpha, const double wo[3], const double wi[3], float out[3]);
+void orc_ggx_vndf(const double wo[3], double alpha, double u1, double u2, double out[3]);
+float orc_analytic_t(const tfg_analytic* q, const double o[3], const double d[3]);
+
 #ifdef __cplusplus
 }
 #endif
