@@ -18,6 +18,17 @@
 //                     decoded boxes contain their triangles.
 //   BVH2 + MT         equals FP64-vector brute force MT; agrees with the
 //                     watertight path except on edge/tie rays.
+//   shading           cosine draw (exact map, moments 2/3 and 1/2); GGX D
+//                     normalised, Smith G1 = its tan form, visible normals
+//                     integrate to one, the visible-normal draw matches its
+//                     density's moments, E[G1(wi)] = the BRDF's albedo,
+//                     reciprocity; analytic sphere / plane distances at
+//                     known points (tests/test_oracle_shading_cpu.py).
+//   integrator        scenes with closed-form pixels
+//                     (tests/closed_form_scenes.py): furnace rho * Le,
+//                     mixed lobes kd / pd * Le or ks / ps * Le, point light
+//                     kd / pi * I * cos / max(d, 1)^2 with a shadow, glass
+//                     lens decided by Snell's law in angle form.
 #include "oracle.h"
 
 #include <algorithm>

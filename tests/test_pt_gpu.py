@@ -11,6 +11,7 @@ differ by less than 4.5 standard errors, and the image means by < 1%.
 import numpy as np
 import pytest
 
+import closed_form_scenes
 import oracle
 from tfg_amd import pt, scenes
 
@@ -207,3 +208,21 @@ def test_unit_lobe_draw_stays_finite(torch_cuda):
         g = img[y * W + x].cpu().numpy().astype(np.float64)
         c = ref.pixel(x, y, frame, md, rr).astype(np.float64)
         np.testing.assert_allclose(g, c, rtol=1e-3, atol=1e-3)
+
+
+@pytest.mark.parametrize("make", closed_form_scenes.CASES, ids=lambda f: f.__name__)
+def test_pt_closed_form_scenes(torch_cuda, make):
+    """The HIP path tracer's pixels equal the closed forms of
+    tests/closed_form_scenes.py: a furnace (rho * Le), mixed diffuse and
+    mirror lobes (kd / pd * Le or ks / ps * Le), a point light over a
+    Lambertian plane with a shadow, a glass sphere lensing an emissive
+    quad.  FP32 shading with hardware reciprocals and square roots: within
+    2e-5 relative where the oracle is exact."""
+    torch = torch_cuda
+    case = make()
+    print(f"case={case.name} {case.W}x{case.H} frame={case.frame}", flush=True)
+    ds, tracer, img = _setup(torch, case.scene, case.W, case.H, case.max_depth, case.rr_depth)
+    tracer.render(case.frame, img)
+    torch.cuda.synchronize()
+    case.check(img.cpu().numpy().astype(np.float64), rtol=2e-5)
+    tracer.close()
