@@ -15,7 +15,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# TFG_ORACLE_LIB names another build of oracle.cpp (the planted-mistake
+# tests load deliberately broken copies through it)
+LIB_PATH = os.environ.get("TFG_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 
 class TfgRay(C.Structure):
