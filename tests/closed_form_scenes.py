@@ -231,3 +231,51 @@ def glass_lens() -> Case:
 
 
 CASES = (furnace, mixed_lobes, point_light, glass_lens)
+
+
+# ------------------------------------------------------ cache capture
+
+R1, R2 = np.float32([0.6, 0.5, 0.4]), np.float32([0.7, 0.8, 0.9])
+
+
+def two_planes() -> Case:
+    """A diffuse floor (y = -1, albedo R1) and ceiling (y = 1, albedo R2), no
+    light: cache-mode vertices alternate floor, ceiling, floor ...  The
+    primary vertex never takes the head (its footprint spread is zero,
+    render.h:86-102), so the head / query is the first ceiling vertex: on
+    y = 1 with normal code (0.5, 0, 0.5), kd R2, a downward outgoing
+    direction and normaliser throughput x reflectance = R1 * R2 exactly.  A
+    tail at vertex k carries the suffix throughput r(0) r(1) ... r(k) of the
+    alternating albedos and lies on the plane of k's parity.  Nothing is
+    lit: pixels and radiance suffixes are zero."""
+    mats = [scenes.material(kd=R1), scenes.material(kd=R2)]
+    cam = scenes.Camera((0.0, 0.0, 0.0), (0.0, -1.0, -0.5), (0.0, 1.0, 0.0))
+    sc = _scene("two_planes", FAR_TRI, [0], mats, [(PL, (0, 1, 0), (0, 0, 0), -1.0, (0, 0, 0), 0),
+                                                   (PL, (0, -1, 0), (0, 0, 0), -1.0, (0, 0, 0), 1)], [], cam)
+    return Case("two_planes", sc, 48, 32, 4, 8, 20)
+
+
+def check_heads(head, norm, atol=1e-6):
+    """head: (n, 16) packed vertices; norm: (n, 3) normalisers."""
+    n = head.shape[0]
+    np.testing.assert_allclose(head[:, 1], 1.0, atol=atol)                      # on the ceiling
+    np.testing.assert_array_equal(head[:, 8:11], np.broadcast_to([0.5, 0.0, 0.5], (n, 3)))
+    np.testing.assert_array_equal(head[:, [7, 11, 12]], np.broadcast_to(R2, (n, 3)))
+    assert not head[:, [3, 13, 14, 15]].any()
+    dm = 2.0 * head[:, 4:7].astype(np.float64) - 1.0
+    np.testing.assert_allclose(np.linalg.norm(dm, axis=1), 1.0, atol=10 * atol)
+    assert (dm[:, 1] < 0).all()
+    np.testing.assert_array_equal(norm, np.broadcast_to(R1 * R2, (n, 3)))
+
+
+def check_tails(tail, tsuf, max_depth, atol=1e-6):
+    """tail: (n, 16) packed tail vertices; tsuf: (n, 3) suffix throughputs."""
+    chain, c = [], np.float32([1, 1, 1])
+    for k in range(max_depth):
+        c = c * (R1 if k % 2 == 0 else R2)
+        chain.append(c.copy())
+    on_floor = np.isclose(tail[:, 1], -1.0, atol=atol)
+    assert (on_floor | np.isclose(tail[:, 1], 1.0, atol=atol)).all()
+    for row, ts, fl in zip(tail, tsuf, on_floor):
+        np.testing.assert_array_equal(row[[7, 11, 12]], R1 if fl else R2)
+        assert any((k % 2 == 0) == fl and np.array_equal(chain[k], ts) for k in range(2, max_depth)), (ts, fl)

@@ -203,54 +203,21 @@ def test_integrator_closed_form_scenes(make):
 
 
 def test_cache_capture_between_two_planes():
-    """Cache-mode paths (render.h:86-202) between a diffuse floor (y = -1,
-    albedo r1) and ceiling (y = 1, albedo r2) with no light: vertices
-    alternate floor, ceiling, floor ...  The primary vertex never takes the
-    head (its footprint spread is zero, render.h:86-102), so a training
-    path's head is the first ceiling vertex: on y = 1 with normal code
-    (0.5, 0, 0.5), kd r2, an outgoing direction pointing down, and the
-    normaliser throughput x reflectance = r1 * r2 exactly.  A tail at vertex
-    k carries the suffix throughput r(0) r(1) ... r(k) of alternating
-    albedos and lies on the plane of k's parity.  Nothing is lit: pixel and
-    radiance suffix are zero.  Inference paths stop at the same vertex as a
-    query with the same normaliser."""
-    r1, r2 = np.float32([0.6, 0.5, 0.4]), np.float32([0.7, 0.8, 0.9])
-    mats = [scenes.material(kd=r1), scenes.material(kd=r2)]
-    cam = scenes.Camera((0.0, 0.0, 0.0), (0.0, -1.0, -0.5), (0.0, 1.0, 0.0))
-    sc = closed_form_scenes._scene("two_planes", closed_form_scenes.FAR_TRI, [0], mats,
-                                   [(PL, (0, 1, 0), (0, 0, 0), -1.0, (0, 0, 0), 0),
-                                    (PL, (0, -1, 0), (0, 0, 0), -1.0, (0, 0, 0), 1)], [], cam)
-    W, H, md = 48, 32, 8
-    host = _lib.CwbvhHost(sc.verts, _lib.TFG_TRI_WATERTIGHT)
-    ren = oracle.Renderer(sc, host.nodes, host.tris, W, H)
-    chain, c = [], np.float32([1, 1, 1])
-    for k in range(md):
-        c = c * (r1 if k % 2 == 0 else r2)
-        chain.append(c.copy())
+    """The closed forms of closed_form_scenes.two_planes on the oracle's
+    cache-mode paths: training heads and tails (mode 2), queries (mode 1)."""
+    case = closed_form_scenes.two_planes()
+    host = _lib.CwbvhHost(case.scene.verts, _lib.TFG_TRI_WATERTIGHT)
+    ren = oracle.Renderer(case.scene, host.nodes, host.tris, case.W, case.H)
     for mode in (2, 1):
-        out = ren.render_nrc(4, md, 20, mode)
+        out = ren.render_nrc(case.frame, case.max_depth, case.rr_depth, mode)
         fl = out["flags"]
         assert not out["pixel"].any() and not out["lsuf"].any()
-        hd = out["head"]
         sel = (fl & 2) != 0 if mode == 2 else (fl & 4) != 0
         assert sel.mean() > 0.95 and (fl & 1 == 1).all() == (mode == 2)
-        np.testing.assert_allclose(hd[sel, 1], 1.0, atol=1e-6)                        # on the ceiling
-        np.testing.assert_array_equal(hd[sel][:, 8:11], np.broadcast_to([0.5, 0.0, 0.5], (sel.sum(), 3)))
-        np.testing.assert_array_equal(hd[sel][:, [7, 11, 12]], np.broadcast_to(r2, (sel.sum(), 3)))
-        assert not hd[sel][:, [3, 13, 14, 15]].any()
-        dm = 2.0 * hd[sel][:, 4:7] - 1.0
-        np.testing.assert_allclose(np.linalg.norm(dm, axis=1), 1.0, atol=1e-6)
-        assert (dm[:, 1] < 0).all()
-        np.testing.assert_array_equal(out["norm"][sel], np.broadcast_to(r1 * r2, (sel.sum(), 3)))
-        if mode == 1:
-            assert not (fl & 8).any()
-            continue
+        closed_form_scenes.check_heads(out["head"][sel], out["norm"][sel])
         tl = (fl & 8) != 0
-        assert tl.sum() > 0.5 * W * H and not (tl & ~sel).any()
-        tv = out["tail"][tl]
-        on_floor = np.isclose(tv[:, 1], -1.0, atol=1e-6)
-        assert (on_floor | np.isclose(tv[:, 1], 1.0, atol=1e-6)).all()
-        for row, ts, fl_ in zip(tv, out["tsuf"][tl], on_floor):
-            np.testing.assert_array_equal(row[[7, 11, 12]], r1 if fl_ else r2)
-            ks = [k for k in range(2, md) if (k % 2 == 0) == fl_ and np.array_equal(chain[k], ts)]
-            assert ks, (ts, fl_)
+        if mode == 1:
+            assert not tl.any()
+            continue
+        assert tl.sum() > 0.5 * case.W * case.H and not (tl & ~sel).any()
+        closed_form_scenes.check_tails(out["tail"][tl], out["tsuf"][tl], case.max_depth)

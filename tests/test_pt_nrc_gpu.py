@@ -173,3 +173,44 @@ def test_deferred_training_matches_end_of_frame_training(torch_cuda):
     with pytest.raises(Exception):
         g = pt.PathTracer(w, h, MD, RR, mode=2, use_graph=True)
         g.set_schedule(defer_training=True)       # needs direct launches
+
+
+def test_cache_capture_closed_forms(torch_cuda):
+    """The HIP path's records and queries between two diffuse planes meet
+    the closed forms of closed_form_scenes.two_planes: training heads and
+    tails in mode 2, and in mode 3 (zero cache) the queries of the paths
+    that do not train."""
+    import closed_form_scenes as CF
+
+    torch = torch_cuda
+    case = CF.two_planes()
+    ds = pt.DeviceScene(case.scene, torch)
+    for mode in (2, 3):
+        print(f"case=two_planes mode={mode}", flush=True)
+        tr = pt.PathTracer(case.W, case.H, case.max_depth, case.rr_depth, mode=mode, train_fraction=0.05,
+                           use_graph=True)
+        tr.set_scene(ds)
+        tr.set_camera(case.scene.camera.frame(case.W, case.H))
+        if mode == 3:
+            cache = nrc.RadianceCache(batch=256)
+            cache.set_weights(np.zeros(21504, np.float32))   # predictions 0
+            tr.set_cache(cache)
+        img = torch.zeros((case.W * case.H, 3), device="cuda")
+        tr.render(case.frame, img)
+        torch.cuda.synchronize()
+        rec, q = tr.read_cache_data()
+        assert not img.cpu().numpy().any()
+        r = rec.reshape(-1, 11, 4)
+        if mode == 2:
+            assert r.shape[0] > 0.95 * case.W * case.H
+            assert not r[:, 8, :3].any()
+            CF.check_heads(r[:, 0:4].reshape(-1, 16), r[:, 10, :3], atol=1e-5)
+            has_tail = r[:, 9, :3].any(axis=1)
+            assert has_tail.mean() > 0.5
+            CF.check_tails(r[has_tail, 4:8].reshape(-1, 16), r[has_tail, 9, :3], case.max_depth, atol=1e-5)
+        else:
+            qq = q.reshape(-1, 5, 4)
+            trains = np.isin(qq[:, 4, 3].view(np.uint32), r[:, 9, 3].view(np.uint32))
+            assert (~trains).sum() > 0.8 * case.W * case.H
+            CF.check_heads(qq[~trains, 0:4].reshape(-1, 16), qq[~trains, 4, :3], atol=1e-5)
+        tr.close()
