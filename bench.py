@@ -53,9 +53,9 @@ HBM_PEAK_GBS = 8000.0
 # per frame (all launches of the stage) for the path tracer's stages, whose
 # launches per frame depend on the tuned schedule
 MEASURED_TRAFFIC = {
-    "k_trace_bounce": (365.9e6, "profiles/k_trace_bounce_displaced_sphere_1M.md"),
-    "k_extend": (1941.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
-    "k_connect": (1915.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
+    "k_trace_bounce": (366.2e6, "profiles/k_trace_bounce_displaced_sphere_1M.md"),
+    "k_extend": (1882.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
+    "k_connect": (1801.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
 }
 
 
