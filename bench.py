@@ -278,7 +278,7 @@ def bench_pt(args, torch):
         },
         "config": {
             "workload": "plain FP32 wavefront PT: 64 seeded instances (32 displaced spheres + 32 boxes, "
-                        f"{sc.n_tris} tris) + ground, 4 point lights, Lambertian+GGX; 1920x1080, 1 spp/frame, "
+                        f"{sc.n_tris} tris) in a closed 12-triangle room, 4 point lights, Lambertian+GGX; 1920x1080, 1 spp/frame, "
                         "max depth 8, RR after depth 3; frame schedule (graph replay or direct launches, streams) chosen by the tuner, see launch",
             "tris": sc.n_tris,
             "cwbvh_nodes": ds.host.num_nodes,
@@ -320,8 +320,8 @@ def bench_pt1080(args, torch):
     t0 = time.time()
     res["nrc_kernels"] = bench_nrc_kernels(torch, sc, res["config"]["launch"])
     _log(f"cache kernels: {json.dumps(res['nrc_kernels'])} ({time.time() - t0:.1f}s)")
-    res["config"]["workload"] = ("1080p path tracer (64 instances, 2.57M tris, 4 lights, Lambertian+GGX, depth 8, "
-                                 "RR after 3): plain FP32 wavefront PT Mrays/s; equal-time relMSE of "
+    res["config"]["workload"] = ("1080p path tracer (64 instances, 2.57M tris, in a closed room, 4 lights, "
+                                 "Lambertian+GGX, depth 8, RR after 3): plain FP32 wavefront PT Mrays/s; equal-time relMSE of "
                                  "NRC-BF16 vs plain against a plain FP64-accumulated reference")
     res["config"]["relmse_reference_spp"] = args.ref_spp
     res["config"]["equal_time_budget_s"] = args.budget
@@ -508,6 +508,62 @@ def cpu_baseline_traversal(host, prim, bounce, shadow):
         "single_thread": {"value": total1 / dt1 / 1e6, "unit": "Mrays/s",
                           "sample": f"3 x 2^16 rays drawn from the same sets on 1 thread, {dt1:.2f} s"},
     }
+
+
+def run_record(torch):
+    """The machine the run measured (BASELINE.md asks for it with every
+    measurement): host CPU model, logical CPUs and the cores this process
+    may use, GPU model, compute units and memory, and the device-to-device
+    copy bandwidth measured now (1 GiB read + 1 GiB written per copy, HIP
+    events, best of 5 after 2 warmups)."""
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    prop = torch.cuda.get_device_properties(0)
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    best = None
+    for _ in range(5):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(src)
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b)
+        best = ms if best is None else min(best, ms)
+    del src, dst
+    torch.cuda.empty_cache()
+    return {
+        "host_cpu": cpu,
+        "host_logical_cpus": os.cpu_count(),
+        "host_cores_usable": _host_cores(),
+        "gpu": torch.cuda.get_device_name(0),
+        "gpu_compute_units": prop.multi_processor_count,
+        "gpu_memory_gib": round(prop.total_memory / 2 ** 30, 1),
+        "gpu_arch": getattr(prop, "gcnArchName", ""),
+        "copy_gbs_measured": round(2 * 4 * n / (best * 1e-3) / 1e9, 1),
+        "torch": torch.__version__,
+        "hip": getattr(torch.version, "hip", None),
+    }
+
+
+def _children():
+    """Child processes of this process still alive (names), for the run record."""
+    try:
+        import psutil
+
+        return [c.name() for c in psutil.Process().children(recursive=True)]
+    except Exception as e:   # psutil missing or a process gone while listing
+        return [f"unknown ({type(e).__name__})"]
 
 
 def _pick_cache_tail(torch, render, tr, frame0, frames=3):
@@ -755,6 +811,8 @@ def main():
         res.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True})
         print(json.dumps(res), flush=True)
         return
+    record = run_record(torch)
+    _log(f"run record: {json.dumps(record)}")
     if args.workload in ("pt", "pt1080"):
         res = bench_pt1080(args, torch) if args.workload == "pt1080" else bench_pt(args, torch)
         cpu_in = res.pop("_cpu_inputs")
@@ -786,6 +844,8 @@ def main():
         out["relmse"] = res["relmse"]
     if "nrc_kernels" in res:
         out["nrc_kernels"] = res["nrc_kernels"]
+    record["child_processes_at_end"] = _children()
+    out["run"] = record
     print(json.dumps(out), flush=True)
 
 

@@ -322,7 +322,11 @@ int tfg_nrc_eval(tfg_nrc* nrc, const float* d_verts, uint32_t n, float* d_out, f
 int tfg_nrc_infer_queries(tfg_nrc* nrc, const float* d_queries, const uint32_t* d_count, tfg_color* d_image,
                           tfg_stream stream);
 /* `steps` optimiser steps on training records (11 float4 each) whose count
- * is read from device memory at *d_n_records. */
+ * is read from device memory at *d_n_records.  Every step forms its targets
+ * (L_suffix + T_suffix * cache(tail)) / normaliser, RenderKernel.cu:92-166,
+ * with the inference (EMA) weights as they were when the call was enqueued,
+ * as the reference computes a frame's targets once before its training step
+ * (main.cpp:1047-1056); the call stands for one frame's steps. */
 int tfg_nrc_train(tfg_nrc* nrc, const float* d_records, const uint32_t* d_n_records, uint32_t steps,
                   tfg_stream stream);
 /* Gradient (unscaled, FP32) of the last optimiser step, to host. */

@@ -11,7 +11,11 @@ Tolerances:
     relative RMS error <= 5e-2 (BF16) / 1e-2 (FP16);
   * weight gradient of one training step: cosine similarity >= 0.999 and
     relative L2 error <= 5e-2 against the emulating oracle on the same
-    batch (same PCG32 batch indices);
+    batch (same PCG32 batch indices).  The records carry a tail vertex and
+    a suffix throughput, so the targets (L_suf + T_suf * cache(tail)) /
+    normaliser (RenderKernel.cu:92-166) depend on the kernel's fused tail
+    inference; master and EMA weights differ, so only the EMA tail
+    prediction of the oracle matches;
   * Adam/EMA update: master weights within 1e-5 relative of the oracle's
     Adam step applied to the kernel's own gradient.
 """
@@ -33,14 +37,42 @@ def _verts(n, seed):
     return v
 
 
-def _records(n, seed):
+def _records(n, seed, tail=False):
+    """Training records (11 float4: head, tail, L_suf, T_suf, normaliser).
+    tail=True: every 8th record has no tail (zero vertex and T_suf, as the
+    tracer writes a suffix that left the scene); the others carry a tail
+    vertex and a suffix throughput large enough that the tail term
+    dominates the target."""
     rng = np.random.default_rng(seed)
     r = np.zeros((n, 11, 4), np.float32)
     r[:, 0:4, :] = _verts(n, seed + 1).reshape(n, 4, 4)
     r[:, 8, :3] = rng.uniform(0.0, 2.0, (n, 3))
     r[:, 8, 3] = 1.0
     r[:, 10, :3] = rng.uniform(0.2, 1.0, (n, 3))
+    if tail:
+        r[:, 8, :3] *= 0.1
+        r[:, 4:8, :] = _verts(n, seed + 2).reshape(n, 4, 4)
+        r[:, 9, :3] = rng.uniform(4.0, 40.0, (n, 3))
+        r[::8, 4:8, :] = 0.0
+        r[::8, 9, :3] = 0.0
     return r.reshape(n, 44)
+
+
+def _oracle_grad(w, e_tail, rec, seed, batch, prec):
+    """Oracle loss and gradient of the step drawing batch `seed` from rec,
+    with master weights w and tail predictions from the weights e_tail."""
+    n_rec = rec.shape[0]
+    idx = M.batch_indices(seed, batch, n_rec)
+    r = rec.reshape(n_rec, 11, 4)[idx]
+    x = M.encode(r[:, 0:4].reshape(batch, 16), BMIN, BMAX)
+    xt = M.encode(r[:, 4:8].reshape(batch, 16), BMIN, BMAX)
+    pred_tail = M.forward(e_tail, xt, emulate=prec)[0][5][:, :3]
+    tgt = M.targets(rec[idx], pred_tail)
+    return M.loss_and_grad(w, x, tgt, emulate=prec)
+
+
+def _cosine(a, b):
+    return float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
 
 
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
@@ -80,31 +112,78 @@ def test_gradient_and_adam_match_oracle(torch_cuda, prec):
     torch = torch_cuda
     batch = 1024
     c = nrc.RadianceCache(precision=1 if prec == "fp16" else 0, seed=5, batch=batch)
+    w_init, _ = c.weights()
+    e_init = nrc.RadianceCache(seed=6, batch=batch).weights()[1]   # an independent network as the EMA
+    c.set_weights(w_init, e_init)
     w0, e0 = c.weights()
     n_rec = 5000
-    rec = _records(n_rec, 7)
+    rec = _records(n_rec, 7, tail=True)
     d_rec = torch.from_numpy(rec).cuda()
     d_n = torch.tensor([n_rec], dtype=torch.int32, device="cuda")
     c.train(d_rec, d_n, steps=1)
     g = c.last_gradient().astype(np.float64)
     st = c.stats()
     assert st["steps"] == 1 and st["seed_counter"] == 1
-    idx = M.batch_indices(0, batch, n_rec)
-    x = M.encode(rec.reshape(n_rec, 11, 4)[idx, 0:4].reshape(batch, 16), BMIN, BMAX)
-    tgt = M.targets(rec[idx], np.zeros((batch, 3)))
-    loss, g_ref = M.loss_and_grad(w0, x, tgt, emulate=prec)
-    cos = g @ g_ref / (np.linalg.norm(g) * np.linalg.norm(g_ref))
+    loss, g_ref = _oracle_grad(w0, e0, rec, 0, batch, prec)
+    cos = _cosine(g, g_ref)
     rel = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
-    print(f"{prec}: loss gpu {st['last_loss']:.6f} oracle {loss:.6f}; grad cos {cos:.6f} rel {rel:.3e}", flush=True)
+    # the test tells the tail inference apart: targets without the tail term,
+    # or with the master weights' tail prediction, give other gradients
+    _, g_notail = _oracle_grad(w0, np.zeros_like(e0), rec, 0, batch, prec)
+    _, g_master = _oracle_grad(w0, w0, rec, 0, batch, prec)
+    print(f"{prec}: loss gpu {st['last_loss']:.6f} oracle {loss:.6f}; grad cos {cos:.6f} rel {rel:.3e}; "
+          f"cos without tail {_cosine(g, g_notail):.4f}, with master tail {_cosine(g, g_master):.4f}", flush=True)
     assert cos >= 0.999
     assert rel <= 5e-2
     assert abs(st["last_loss"] - loss) <= 1e-2 * loss
+    assert _cosine(g, g_notail) < 0.99 and _cosine(g, g_master) < 0.99
     cfg = c.cfg
     w1, _, _, e1 = M.adam_step(w0.astype(np.float64), 0, 0, e0.astype(np.float64), g, 1, cfg.lr, cfg.beta1,
                                cfg.beta2, cfg.eps, cfg.ema_decay)
     m1, me1 = c.weights()
     np.testing.assert_allclose(m1, w1, rtol=1e-5, atol=1e-7)
     np.testing.assert_allclose(me1, e1, rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.parametrize("prec", ["bf16"])
+def test_steps_of_one_call_share_frame_start_targets(torch_cuda, prec):
+    """All steps of one tfg_nrc_train call form their targets with the EMA
+    weights of the call's start, as the reference computes a frame's targets
+    once before its step (main.cpp:1047-1056).  Two identical caches: A runs
+    2 steps in one call, B runs 1; B's weights after its step are A's before
+    A's second step (the reduction is deterministic).  A's second gradient
+    matches the oracle with B's master weights and the initial EMA as the
+    tail network, and not with B's EMA (ema_decay 0: EMA = master after a
+    step, far from the initial EMA; a learning rate of 0.05 moves the master
+    weights far enough for the two target networks to give clearly different
+    gradients)."""
+    torch = torch_cuda
+    batch = 1024
+    kw = dict(precision=0, seed=5, batch=batch, ema_decay=0.0, lr=0.05)
+    a, b = nrc.RadianceCache(**kw), nrc.RadianceCache(**kw)
+    w_init = a.weights()[0]
+    e_init = nrc.RadianceCache(seed=6, batch=batch).weights()[1]
+    a.set_weights(w_init, e_init)
+    b.set_weights(w_init, e_init)
+    n_rec = 5000
+    rec = _records(n_rec, 11, tail=True)
+    d_rec = torch.from_numpy(rec).cuda()
+    d_n = torch.tensor([n_rec], dtype=torch.int32, device="cuda")
+    a.train(d_rec, d_n, steps=2)
+    b.train(d_rec, d_n, steps=1)
+    g2 = a.last_gradient().astype(np.float64)
+    w1, e1 = b.weights()
+    assert a.stats()["steps"] == 2 and b.stats()["steps"] == 1
+    np.testing.assert_array_equal(e1, w1)          # ema_decay 0
+    _, g_ref = _oracle_grad(w1, e_init, rec, 1, batch, prec)
+    _, g_wrong = _oracle_grad(w1, e1, rec, 1, batch, prec)
+    cos, cos_wrong = _cosine(g2, g_ref), _cosine(g2, g_wrong)
+    rel = np.linalg.norm(g2 - g_ref) / np.linalg.norm(g_ref)
+    rel_wrong = np.linalg.norm(g2 - g_wrong) / np.linalg.norm(g_wrong)
+    print(f"second step: frame-start EMA targets cos {cos:.6f} rel {rel:.2e}; updated EMA cos {cos_wrong:.4f} "
+          f"rel {rel_wrong:.2e}", flush=True)
+    assert cos >= 0.999 and rel <= 5e-2
+    assert cos_wrong < 0.99 and rel_wrong > 0.5
 
 
 def test_training_fits_a_smooth_function(torch_cuda):

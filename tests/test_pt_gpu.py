@@ -92,6 +92,58 @@ def test_pt_converges_to_oracle(torch_cuda):
     assert abs(mg.mean() - mc.mean()) / mc.mean() < 0.01
 
 
+def test_c1_cornell_256_16spp(torch_cuda):
+    """C1 at its configuration: the 12-quad room with the level-4 icosphere
+    (5,144 triangles) and one point light, 256 x 256, 16 spp, max depth 5
+    (Russian roulette after depth 10, the reference's default, so never),
+    frame seed = base seed + sample.  Per path: in each of the 16 frames
+    >= 98% of the 65,536 paths agree with the oracle within 1e-3 relative.
+    Converged image: 16 spp on the GPU against 16 spp of the oracle with
+    independent seeds, per 16 x 16 block mean within 5 standard errors, and
+    the image means within 1%."""
+    torch = torch_cuda
+    sc = scenes.cornell(level=4)
+    assert sc.n_tris == 24 + 5120
+    W, H, md, rr, spp, base = 256, 256, 5, 10, 16, 500
+    ds, tracer, img = _setup(torch, sc, W, H, md, rr, use_graph=False)
+    ref = oracle.Renderer(sc, ds.host.nodes, ds.host.tris, W, H)
+    s1 = np.zeros((H * W, 3))
+    s2 = np.zeros((H * W, 3))
+    agree = []
+    for k in range(spp):
+        tracer.render(base + k, img)
+        torch.cuda.synchronize()
+        g = img.cpu().numpy().astype(np.float64)
+        c, _ = ref.render(base + k, 1, md, rr)
+        agree.append(float(np.all(np.abs(g - c) <= 1e-3 * (1.0 + np.abs(c)), axis=1).mean()))
+        s1 += g
+        s2 += g * g
+    print("C1 per-frame agreement min", min(agree), "mean", np.mean(agree), flush=True)
+    assert min(agree) >= 0.98
+    c1 = np.zeros((H * W, 3))
+    c2 = np.zeros((H * W, 3))
+    for k in range(spp):
+        c, _ = ref.render(base + 100_000 + k, 1, md, rr)
+        c1 += c
+        c2 += c * c
+
+    def block_stats(a1, a2, b=16):
+        m = a1 / spp
+        var = np.maximum(a2 / spp - m * m, 0.0) / (spp - 1)
+        mb = m.reshape(H // b, b, W // b, b, 3).mean((1, 3))
+        vb = var.reshape(H // b, b, W // b, b, 3).sum((1, 3)) / float(b * b) ** 2
+        return mb, vb
+
+    mg, vg = block_stats(s1, s2)
+    mc, vc = block_stats(c1, c2)
+    z = np.abs(mg - mc) / np.sqrt(vg + vc + 1e-12)
+    rel = abs(mg.mean() - mc.mean()) / mc.mean()
+    print(f"C1 16 spp: max block z {z.max():.2f}, image mean {mg.mean():.5f} vs {mc.mean():.5f} ({rel:.2e})",
+          flush=True)
+    assert z.max() < 5.0
+    assert rel < 0.01
+
+
 def test_pt_graph_matches_eager_and_replays(torch_cuda):
     torch = torch_cuda
     sc = scenes.cornell(level=3)
@@ -171,6 +223,40 @@ def test_pt_schedules_match_serial(torch_cuda, use_graph):
         np.testing.assert_array_equal(imgs[1][0], imgs[1][2])
 
 
+def test_autotune_schedule_in_range_and_image_unchanged(torch_cuda):
+    """tfg_pt_autotune on a 64 x 64 frame: every field of the schedule it
+    keeps lies in its documented range (include/tfg.h, tfg_pt_schedule), and
+    the tuned tracer renders the same image, bit for bit, as an untuned one
+    on the same frame seeds."""
+    torch = torch_cuda
+    sc = scenes.instances(n_inst=8, sphere_res=(32, 16), box_res=4, seed=5)
+    W, H = 64, 64
+    ds = pt.DeviceScene(sc, torch)
+    tuned = pt.PathTracer(W, H, 8, 3, mode=0, use_graph=False)
+    plain = pt.PathTracer(W, H, 8, 3, mode=0, use_graph=False)
+    for t in (tuned, plain):
+        t.set_scene(ds)
+        t.set_camera(sc.camera.frame(W, H))
+    shape = tuned.autotune()
+    print("tuned", shape, flush=True)
+    assert shape["trace_block"] in (64, 128, 256) and 1 <= shape["grid_mult"] <= 64
+    assert 1 <= shape["tiles"] <= 4 and 1 <= shape["side_streams"] <= 4
+    assert shape["xcd_group"] == 0 or (shape["xcd_group"] % 256 == 0 and shape["xcd_group"] <= 1 << 20)
+    assert shape["connect_per_cu"] <= 64 and shape["connect_batch"] <= 32 and shape["tail_from"] <= 32
+    assert shape["top_nodes"] <= 73 and not shape["defer_training"] and not shape["use_graph"]
+    assert shape["trace_grid"] > 0 and shape["shade_grid"] > 0
+    a = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
+    b = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
+    for f in (11, 12):
+        tuned.render(f, a)
+        plain.render(f, b)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+        assert tuned.rays_last_frame() == plain.rays_last_frame()
+    tuned.close()
+    plain.close()
+
+
 def test_pt_rejects_bad_config(torch_cuda):
     from tfg_amd import _lib
 
@@ -193,9 +279,9 @@ def test_unit_lobe_draw_stays_finite(torch_cuda):
     material with no transmission (probability 0); the reference's colour
     division by zero gives black (color.h:42-48), so the pixel stays finite
     and equal to the oracle's.  The listed (frame, pixel) pairs of the
-    1080p 64-instance workload each contain such a draw."""
+    1080p 64-instance workload in its open variant each contain such a draw."""
     torch = torch_cuda
-    sc = scenes.instances()
+    sc = scenes.instances(enclosed=False)
     W, H, md, rr = 1920, 1080, 8, 3
     ds, tracer, img = _setup(torch, sc, W, H, md, rr)
     ref = oracle.Renderer(sc, ds.host.nodes, ds.host.tris, W, H)

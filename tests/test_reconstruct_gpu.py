@@ -1,0 +1,181 @@
+"""Reconstruction mode (renderizarModoReconstruccion, main.cpp:454-668):
+frames with cache inference and no training (usar_red = true,
+entrenar_red = false) from a weight file, against the oracle.
+
+  * queries of a mode-1 frame vs the oracle's cache branch (orc_render_nrc,
+    mode 1): >= 95% of the paths agree (vertex records within 1e-3 relative,
+    throughput x reflectance within 1e-3), the rest diverge on an edge;
+  * composited pixels: the pixel equals the oracle's prefix colour plus
+    throughput x the FP64 oracle network (rounding its operands like the
+    kernel) evaluated on the GPU's own query vertex, for >= 99% of the
+    queries within 2e-2 relative (BF16 operands);
+  * the driver (experiments.reconstruct) loads the TFGNRC01 file, and its
+    FP64 accumulation and 8-bit resolve equal those of the frames rendered
+    one by one;
+  * the launchKernelRenderTiny drop-in with flags (1, 0) writes queries and
+    throughputs only where a query is made and leaves the training counter
+    and records untouched.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp_oracle as M
+from tfg_amd import _lib, experiments, nrc, pt, scenes
+
+pytestmark = pytest.mark.gpu
+
+W, H, MD, RR = 64, 48, 5, 2
+
+
+def _bounds(sc):
+    v = sc.verts.reshape(-1, 3)
+    return v.min(0) - 0.5, v.max(0) + 0.5
+
+
+def _trained_model(torch, ds, path):
+    """A cache trained by 12 training-only frames of the scene, saved to path."""
+    sc = ds.scene
+    lo, hi = _bounds(sc)
+    tr = pt.PathTracer(W, H, MD, RR, mode=2, use_graph=False)
+    tr.set_scene(ds)
+    tr.set_camera(sc.camera.frame(W, H))
+    cache = nrc.RadianceCache(batch=1024, steps_per_frame=2, min_records=64, bounds_min=tuple(map(float, lo)),
+                              bounds_max=tuple(map(float, hi)))
+    tr.set_cache(cache)
+    img = torch.zeros((W * H, 3), device="cuda")
+    for f in range(12):
+        tr.render(f, img)
+    torch.cuda.synchronize()
+    assert cache.stats()["steps"] == 24
+    cache.save(str(path))
+    m, e = cache.weights()
+    tr.close()
+    cache.close()
+    return m, e
+
+
+def test_reconstruction_frames_match_oracle(torch_cuda, tmp_path):
+    torch = torch_cuda
+    sc = scenes.cornell(level=3)
+    ds = pt.DeviceScene(sc, torch)
+    path = tmp_path / "cache.tfgnrc"
+    _, ema = _trained_model(torch, ds, path)
+    lo, hi = _bounds(sc)
+
+    # one inference-only frame, queries and composite
+    tr = pt.PathTracer(W, H, MD, RR, mode=1, use_graph=False)
+    tr.set_scene(ds)
+    tr.set_camera(sc.camera.frame(W, H))
+    cache = nrc.RadianceCache(bounds_min=tuple(map(float, lo)), bounds_max=tuple(map(float, hi)))
+    cache.load(str(path))
+    tr.set_cache(cache)
+    frame = 40
+    img = torch.zeros((W * H, 3), device="cuda")
+    tr.render(frame, img)
+    torch.cuda.synchronize()
+    rec, q = tr.read_cache_data()
+    assert rec.shape[0] == 0                        # no training paths without entrenar_red
+    assert cache.stats()["steps"] == 0
+    ref = oracle.Renderer(sc, ds.host.nodes, ds.host.tris, W, H).render_nrc(frame, MD, RR, 1)
+    assert ((ref["flags"] & 1) == 0).all()          # render.h:443: no training draws
+    need = np.nonzero(ref["flags"] & 4)[0]
+    print(f"mode-1 frame: {q.shape[0]} queries, oracle {need.size}", flush=True)
+    assert abs(q.shape[0] - need.size) <= 0.02 * need.size and need.size > 0.5 * W * H
+    qq = q.reshape(-1, 5, 4)
+    pid = qq[:, 4, 3].view(np.uint32)
+    assert np.unique(pid).size == pid.size
+    v_ok = np.all(np.abs(qq[:, 0:4].reshape(-1, 16) - ref["head"][pid]) <= 1e-4 + 1e-3 * np.abs(ref["head"][pid]),
+                  axis=1)
+    t_ok = np.all(np.abs(qq[:, 4, :3] - ref["norm"][pid]) <= 1e-5 + 1e-3 * np.abs(ref["norm"][pid]), axis=1)
+    print("query agreement", (v_ok & t_ok).mean(), flush=True)
+    assert (v_ok & t_ok).mean() >= 0.95
+    # composite: prefix + throughput x network(query vertex), the network the
+    # FP64 oracle with the kernel's BF16 operand rounding
+    pred = M.sanitize(M.forward(ema, M.encode(qq[:, 0:4].reshape(-1, 16), lo, hi), emulate="bf16")[0][5][:, :3])
+    g = img.cpu().numpy().astype(np.float64)
+    ok = v_ok & t_ok
+    expect = ref["pixel"][pid[ok]] + qq[ok, 4, :3].astype(np.float64) * pred[ok]
+    got = g[pid[ok]]
+    px_ok = np.all(np.abs(got - expect) <= 1e-3 + 2e-2 * np.abs(expect), axis=1)
+    print("composited pixels within tolerance", px_ok.mean(), "mean", got.mean(), expect.mean(), flush=True)
+    assert px_ok.mean() >= 0.99
+    assert (np.abs(qq[ok, 4, :3] * pred[ok]).sum(1) > 0).mean() > 0.5   # the cache adds light
+    # pixels without a query are the plain path's colour
+    noq = np.setdiff1d(np.arange(W * H), pid)
+    np.testing.assert_allclose(g[noq], ref["pixel"][noq], rtol=1e-3, atol=1e-4)
+    tr.close()
+    cache.close()
+
+    # the driver: FP64 accumulation of spp frames and the 8-bit resolve
+    spp = 4
+    png = tmp_path / "recon.png"
+    mean, rgb8 = experiments.reconstruct(torch, ds, W, H, MD, RR, str(path), spp, frame0=frame, png_path=str(png),
+                                         bounds=(lo, hi))
+    tr = pt.PathTracer(W, H, MD, RR, mode=1, use_graph=True)   # same frames replayed from graphs
+    tr.set_scene(ds)
+    tr.set_camera(sc.camera.frame(W, H))
+    cache = nrc.RadianceCache(bounds_min=tuple(map(float, lo)), bounds_max=tuple(map(float, hi)))
+    cache.load(str(path))
+    tr.set_cache(cache)
+    acc = np.zeros((W * H, 3))
+    for k in range(spp):
+        tr.render(frame + k, img)
+        torch.cuda.synchronize()
+        acc += img.cpu().numpy().astype(np.float64)
+    np.testing.assert_allclose(mean, acc / spp, rtol=1e-12, atol=1e-12)
+    from oracle import image_oracle
+    from tfg_amd import image
+
+    np.testing.assert_array_equal(rgb8, image_oracle.resolve(acc, spp, True))
+    w, h, back = image.read_png_rgb8(png.read_bytes())
+    assert (w, h) == (W, H)
+    np.testing.assert_array_equal(back, rgb8)
+    tr.close()
+    cache.close()
+
+
+def test_dropin_inference_only_flags(torch_cuda):
+    """launchKernelRenderTiny with usar_red = true, entrenar_red = false (the
+    reconstruction frame, main.cpp:566-574): buffer_inference and
+    buffer_throughput are written where a query is made (throughput 0
+    elsewhere, render.h:452-458), the training counter and records stay as
+    they were, and the pixels are the prefix colours of the oracle."""
+    import test_dropin_gpu as D
+
+    torch = torch_cuda
+    sc = scenes.reference_scene()
+    W2, H2 = 96, 64
+    an, mesh = D._prims_from_scene(sc)
+    host = _lib.CwbvhHost(sc.verts)
+    dbvh = host.upload()
+    luz = np.zeros(1, D.LUZ)
+    luz["pos"], luz["I"] = sc.lights[0, 0:3], sc.lights[0, 4:7]
+    cam = torch.zeros(112, dtype=torch.uint8, device="cuda")
+    _lib.check(_lib.lib().tfg_launch_inicializar_camara(cam.data_ptr(), W2, H2, None))
+    d = {"cam": cam, "an": D._dev(torch, an), "n_an": an.shape[0], "luz": D._dev(torch, luz),
+         "mesh": D._dev(torch, mesh), "n_mesh": mesh.shape[0]}
+    img = torch.zeros((H2 * W2, 3), device="cuda")
+    inf = torch.zeros(W2 * H2 * 112, dtype=torch.uint8, device="cuda")
+    thr = torch.full((H2 * W2, 3), 7.0, device="cuda")
+    recs = torch.full((W2 * H2 * 232,), 0xA5, dtype=torch.uint8, device="cuda")
+    counter = torch.tensor([12345], dtype=torch.int32, device="cuda")
+    _lib.check(_lib.lib().tfg_dropin_configure(8, 5))
+    D._call(torch, d, dbvh, W2, H2, 6, True, False, img, inf, thr, recs, counter)
+    assert counter.item() == 12345
+    assert (recs.cpu().numpy() == 0xA5).all()
+    o = oracle.Renderer(sc, host.nodes, host.tris, W2, H2).render_nrc(6, 8, 5, 1)
+    need = (o["flags"] & 4) != 0
+    t = thr.cpu().numpy()
+    has = np.any(t != 0, axis=1)
+    print("queries", int(has.sum()), "oracle", int(need.sum()), "agree", (has == need).mean(), flush=True)
+    assert (has == need).mean() >= 0.95 and need.sum() > 0
+    assert (t[~has] == 0).all()
+    dd = np.frombuffer(inf.cpu().numpy().tobytes(), D.DATOS)
+    sel = np.nonzero(has & need)[0]
+    pos_ok = np.all(np.abs(dd["pos"][sel] - o["head"][sel, 0:3]) <= 1e-3, axis=1)
+    thr_ok = np.all(np.abs(t[sel] - o["norm"][sel]) <= 1e-5 + 1e-3 * np.abs(o["norm"][sel]), axis=1)
+    assert pos_ok.mean() >= 0.95 and thr_ok.mean() >= 0.95
+    g = img.cpu().numpy()
+    px_ok = np.all(np.abs(g - o["pixel"]) <= 1e-3 * (1 + np.abs(o["pixel"])), axis=1)
+    assert px_ok.mean() >= 0.95

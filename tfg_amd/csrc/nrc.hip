@@ -37,6 +37,10 @@ struct tfg_nrc {
     void* fop_train = nullptr;
     void* bop_train = nullptr;
     void* fop_infer = nullptr;
+    // the inference (EMA) operands as they were when a call's optimiser
+    // steps were enqueued: every step of the call forms its targets with
+    // them (the reference computes a frame's targets once, before its step)
+    void* fop_tail = nullptr;
     // device scalars: [0] step count t, [1] seed counter, [2] flags of the
     // current step, [3] skipped (non-finite) steps, [4] no-data steps,
     // [5] loss of the last step (float bits), [6] steps taken
@@ -205,7 +209,7 @@ struct TrainArgs {
     NrcBounds b;
     const void* fop_train;
     const void* bop_train;
-    const void* fop_infer;
+    const void* fop_tail;
     float* partial;
     float* loss_accum;
 };
@@ -218,7 +222,9 @@ constexpr int kTS = 72;
 constexpr size_t kTrainLds = (size_t)(2 * kFopTiles + kBopTiles) * 64 * kOpBytes16 + 2 * 64 * kTS * 2;
 
 // One block = 64 training samples (4 waves x 16).  Tail inference with the
-// inference (EMA) weights, targets (RenderKernel.cu:92-137), head forward
+// inference (EMA) weights of the call's start (kernelPrepararInferenciaTail
+// + ColorMLP::inference, RenderKernel.cu:139-166, main.cpp:1047-1050),
+// targets (RenderKernel.cu:92-137), head forward
 // with the training weights, RelativeL2Luminance gradient, backward through
 // all layers and this block's weight gradient (summed over its samples).
 template <typename T>
@@ -234,7 +240,7 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
     const uint32_t n_rec = *A.n_rec;
     if (n_rec <= A.min_records) return;
     load_ops<T>(s_fop, static_cast<const V*>(A.fop_train), kFopTiles);
-    load_ops<T>(s_inf, static_cast<const V*>(A.fop_infer), kFopTiles);
+    load_ops<T>(s_inf, static_cast<const V*>(A.fop_tail), kFopTiles);
     load_ops<T>(s_bop, static_cast<const V*>(A.bop_train), kBopTiles);
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, ml = lane & 15;
@@ -405,6 +411,11 @@ __global__ __launch_bounds__(256) void k_nrc_adam(float* __restrict__ master, fl
     put16<T>(fop_infer, fop_pos(p), e);
 }
 
+// fop_tail = fop_infer (16-byte entries), once per call, before its steps
+__global__ __launch_bounds__(256) void k_nrc_snapshot(const uint4* __restrict__ src, uint4* __restrict__ dst, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
 __global__ void k_nrc_step_begin(uint32_t* state, float* loss_accum) {
     if (threadIdx.x == 0) {
         state[kStFlags] = 0u;
@@ -475,11 +486,15 @@ int enqueue_train_steps(tfg_nrc* nc, const float4* records, const uint32_t* n_re
     A.b = nc->bounds;
     A.fop_train = nc->fop_train;
     A.bop_train = nc->bop_train;
-    A.fop_infer = nc->fop_infer;
+    A.fop_tail = nc->fop_tail;
     A.partial = nc->partial;
     A.loss_accum = nc->loss_accum;
     const Hyper h{nc->cfg.lr, nc->cfg.beta1, nc->cfg.beta2, nc->cfg.eps, nc->cfg.ema_decay, 1.0f / nc->cfg.loss_scale};
     const int pblocks = (kNrcParams + 255) / 256;
+    const uint32_t op_entries = (uint32_t)kFopTiles * 64;
+    if (steps > 0)
+        hipLaunchKernelGGL(k_nrc_snapshot, dim3((op_entries + 255) / 256), dim3(256), 0, s,
+                           static_cast<const uint4*>(nc->fop_infer), static_cast<uint4*>(nc->fop_tail), op_entries);
     for (uint32_t k = 0; k < steps; ++k) {
         if (gathered) A.records = records + (size_t)k * nc->cfg.batch * kRecF4;
         hipLaunchKernelGGL(k_nrc_step_begin, dim3(1), dim3(64), 0, s, nc->state, nc->loss_accum);
@@ -645,6 +660,7 @@ int tfg_nrc_create(const tfg_nrc_config* cfg, tfg_nrc** out) {
     if (e == hipSuccess) e = hipMalloc(&nc->fop_train, (size_t)kFopTiles * 64 * kOpBytes16);
     if (e == hipSuccess) e = hipMalloc(&nc->bop_train, (size_t)kBopTiles * 64 * kOpBytes16);
     if (e == hipSuccess) e = hipMalloc(&nc->fop_infer, (size_t)kFopTiles * 64 * kOpBytes16);
+    if (e == hipSuccess) e = hipMalloc(&nc->fop_tail, (size_t)kFopTiles * 64 * kOpBytes16);
     if (e == hipSuccess) e = hipMalloc((void**)&nc->state, 16 * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&nc->loss_accum, sizeof(float));
     int dev = 0;
@@ -808,7 +824,7 @@ int tfg_nrc_destroy(tfg_nrc* nc) {
     if (!nc) return TFG_OK;
     (void)hipDeviceSynchronize();
     void* bufs[] = {nc->master, nc->adam_m, nc->adam_v, nc->ema, nc->grad, nc->partial, nc->fop_train,
-                    nc->bop_train, nc->fop_infer, nc->state, nc->loss_accum, nc->dm_buf, nc->dm_out, nc->dm_count};
+                    nc->bop_train, nc->fop_infer, nc->fop_tail, nc->state, nc->loss_accum, nc->dm_buf, nc->dm_out, nc->dm_count};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete nc;

@@ -180,6 +180,51 @@ def equal_time(torch, scene, W, H, max_depth, rr_depth, ref_spp, budget_s, warmu
     }
 
 
+def reconstruct(torch, dscene, W, H, max_depth, rr_depth, model_path, spp, frame0=0, png_path=None, tonemap=True,
+                precision=None, schedule=None, bounds=None):
+    """Reconstruction mode (renderizarModoReconstruccion, main.cpp:454-668):
+    a trained cache is loaded from a weight file (TFGNRC01, RadianceCache.save)
+    and `spp` frames are rendered with cache inference and no training
+    (usar_red = true, entrenar_red = false, main.cpp:566-574): every path ends
+    at its first cacheable vertex, whose cached radiance is composited into
+    the pixel (main.cpp:596-597), and the frames (seeds frame0 + sample, as
+    main.cpp:568 passes the sample index) are accumulated in FP64 on the GPU.
+    The mean goes through the ACES filmic curve (main.cpp:640) into an 8-bit
+    PNG at png_path (optional).
+
+    bounds: the cache's scene bounds (min, max); default: the scene's vertex
+    box widened by 0.5, as the training drivers here choose them.  Returns
+    (mean image (H*W, 3) float64, rgb8 bytes (H*W, 3) uint8)."""
+    from . import image
+
+    tr = pt.PathTracer(W, H, max_depth, rr_depth, mode=pt_mode_infer(), use_graph=_graph(schedule))
+    tr.set_scene(dscene)
+    _apply(tr, schedule)
+    tr.set_camera(dscene.scene.camera.frame(W, H))
+    if bounds is None:
+        v = dscene.scene.verts.reshape(-1, 3)
+        bounds = (v.min(0) - 0.5, v.max(0) + 0.5)
+    kw = dict(bounds_min=tuple(map(float, bounds[0])), bounds_max=tuple(map(float, bounds[1])))
+    if precision is not None:
+        kw["precision"] = precision
+    cache = nrc.RadianceCache(**kw)
+    cache.load(model_path)
+    tr.set_cache(cache)
+    img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
+    acc = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
+    for k in range(spp):
+        tr.render(frame0 + k, img, acc)
+    rgb8 = image.resolve(torch, acc, spp, tonemap=tonemap)
+    torch.cuda.synchronize()
+    out = rgb8.cpu().numpy()
+    if png_path:
+        image.write_png(png_path, out, W, H)
+    mean = (acc / max(spp, 1)).cpu().numpy()
+    tr.close()
+    cache.close()
+    return mean, out
+
+
 # gridsearch.h:54-61: the reference's search space for the optimiser settings
 # (its network width/depth, hash-grid size and one-blob bins have no
 # counterpart: this cache's network and encoding are fixed)

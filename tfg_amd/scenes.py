@@ -10,7 +10,8 @@ are generated instead, so every run is reproducible from its seed:
 * ``displaced_sphere`` - UV sphere displaced along its normals by seeded
                        Perlin noise.
 * ``instances``      - seeded displaced spheres and boxes flattened into one
-                       triangle list, a ground quad and several point lights.
+                       triangle list inside a closed room (or on a ground
+                       quad under a black sky), several point lights.
 
 Every scene is returned as a :class:`Scene` of plain numpy arrays: triangle
 vertices (N, 9), per-corner shading normals (N, 9; all-zero rows mean "use
@@ -229,7 +230,13 @@ def displaced_uv_sphere(nu: int, nv: int, radius: float, amplitude: float, freq:
     tris.append(np.stack([a[-1], e[-1], b[-1]], -1).reshape(-1, 3))     # bottom cap
     F = np.concatenate(tris)
     P3 = pos.reshape(-1, 3)
+    # outward orientation: flip winding where the face normal points inward,
+    # then accumulate the (now outward) face normals into vertex normals
+    ctr = np.asarray(center)
     fn = np.cross(P3[F[:, 1]] - P3[F[:, 0]], P3[F[:, 2]] - P3[F[:, 0]])
+    inward = np.einsum("ij,ij->i", fn, P3[F].mean(1) - ctr) < 0
+    F[inward] = F[inward][:, [0, 2, 1]]
+    fn[inward] = -fn[inward]
     vn = np.zeros_like(P3)
     for k in range(3):
         np.add.at(vn, F[:, k], fn)
@@ -242,10 +249,6 @@ def displaced_uv_sphere(nu: int, nv: int, radius: float, amplitude: float, freq:
     vn[-1, :] = vn[-1, :].sum(0)
     vn = vn.reshape(-1, 3)
     vn /= np.maximum(np.linalg.norm(vn, axis=1, keepdims=True), 1e-30)
-    # outward orientation: flip winding where the face normal points inward
-    ctr = np.asarray(center)
-    inward = np.einsum("ij,ij->i", fn, P3[F].mean(1) - ctr) < 0
-    F[inward] = F[inward][:, [0, 2, 1]]
     verts = P3[F].reshape(-1, 9).astype(np.float32)
     norms = vn[F].reshape(-1, 9).astype(np.float32)
     return verts, norms
@@ -322,11 +325,27 @@ def displaced_sphere(nu: int = 1024, nv: int = 512, seed: int = 1) -> Scene:
     return Scene("displaced_sphere", v, n, np.zeros(v.shape[0], np.uint32), mats, lights, cam, seed)
 
 
-def instances(n_inst: int = 64, sphere_res=(256, 128), box_res: int = 35, seed: int = 3) -> Scene:
-    """`n_inst` seeded instances (half displaced spheres, half boxes) on a ground quad,
-    4 point lights, Lambertian albedo U[0.2,0.8] and GGX roughness U[0.1,0.8].
+# Room around the instance grid (enclosed=True): x, y, z extents.  The
+# instances span x in [-10.4, 10.4], z in [-23.5, -2.7], y in [-1, 1]; the
+# lights sit at y 6..10 inside it and the camera at z = 8.
+ROOM = ((-14.0, 14.0), (-1.0, 12.0), (-28.0, 12.0))
 
-    At the defaults: 32 x 65,536 + 32 x 14,700 + 2 = 2,567,554 triangles."""
+
+def instances(n_inst: int = 64, sphere_res=(256, 128), box_res: int = 35, seed: int = 3,
+              enclosed: bool = True) -> Scene:
+    """`n_inst` seeded instances (half displaced spheres, half boxes), 4 point
+    lights, Lambertian albedo U[0.2,0.8] and GGX roughness U[0.1,0.8].
+
+    enclosed=True (the default) puts them in a closed room, as the
+    reference's scenes are closed rooms (main.cpp:164-345: floor, ceiling,
+    back and side walls): floor, ceiling, back wall, a wall behind the
+    camera and two coloured side walls, 12 triangles, all Lambertian, so
+    every path ends on a surface and indirect light reaches every pixel.
+    enclosed=False gives the open variant: a large ground quad under a
+    black sky (most second-bounce rays escape).
+
+    At the defaults: 32 x 65,536 + 32 x 14,700 + 12 = 2,567,564 triangles
+    (open: + 2)."""
     rng = np.random.default_rng(seed)
     verts, norms, mids, mats = [], [], [], []
     side = int(math.ceil(math.sqrt(n_inst)))
@@ -351,17 +370,37 @@ def instances(n_inst: int = 64, sphere_res=(256, 128), box_res: int = 35, seed: 
         verts.append(v)
         norms.append(n)
         mids.append(np.full(v.shape[0], i, np.uint32))
-    ground = quad((-40.0, -1.0, 10.0), (40.0, -1.0, 10.0), (40.0, -1.0, -60.0), (-40.0, -1.0, -60.0))
-    verts.append(ground)
-    norms.append(np.zeros_like(ground))
-    mids.append(np.full(2, n_inst, np.uint32))
-    mats.append(material(kd=(0.6, 0.6, 0.6), ks=(0.04, 0.04, 0.04), roughness=0.6))
+    if enclosed:
+        (X0, X1), (Y0, Y1), (Z0, Z1) = ROOM
+        walls = [
+            (quad((X0, Y0, Z1), (X1, Y0, Z1), (X1, Y0, Z0), (X0, Y0, Z0)), 0),   # floor
+            (quad((X0, Y1, Z0), (X1, Y1, Z0), (X1, Y1, Z1), (X0, Y1, Z1)), 1),   # ceiling
+            (quad((X0, Y0, Z0), (X1, Y0, Z0), (X1, Y1, Z0), (X0, Y1, Z0)), 1),   # back
+            (quad((X1, Y0, Z1), (X0, Y0, Z1), (X0, Y1, Z1), (X1, Y1, Z1)), 1),   # behind the camera
+            (quad((X0, Y0, Z1), (X0, Y0, Z0), (X0, Y1, Z0), (X0, Y1, Z1)), 2),   # left
+            (quad((X1, Y0, Z0), (X1, Y0, Z1), (X1, Y1, Z1), (X1, Y1, Z0)), 3),   # right
+        ]
+        for q, k in walls:
+            verts.append(q)
+            norms.append(np.zeros_like(q))
+            mids.append(np.full(2, n_inst + k, np.uint32))
+        mats.append(material(kd=(0.6, 0.6, 0.6), ks=(0.04, 0.04, 0.04), roughness=0.6))   # floor
+        mats.append(material(kd=(0.7, 0.7, 0.7)))     # ceiling, back and front walls
+        mats.append(material(kd=(0.7, 0.25, 0.2)))    # left wall
+        mats.append(material(kd=(0.2, 0.45, 0.7)))    # right wall
+    else:
+        ground = quad((-40.0, -1.0, 10.0), (40.0, -1.0, 10.0), (40.0, -1.0, -60.0), (-40.0, -1.0, -60.0))
+        verts.append(ground)
+        norms.append(np.zeros_like(ground))
+        mids.append(np.full(2, n_inst, np.uint32))
+        mats.append(material(kd=(0.6, 0.6, 0.6), ks=(0.04, 0.04, 0.04), roughness=0.6))
     lights = np.zeros((4, 8), np.float32)
     for k in range(4):
         lights[k, 0:3] = (rng.uniform(-10, 10), rng.uniform(6, 10), rng.uniform(-22, 2))
         lights[k, 4:7] = rng.uniform(150.0, 300.0, 3)
     cam = Camera(pos=(0.0, 4.0, 8.0), front=(0.0, -0.35, -1.0), up=(0.0, 1.0, 0.0), fov_deg=60.0)
-    return Scene("instances", np.concatenate(verts).astype(np.float32), np.concatenate(norms).astype(np.float32),
+    return Scene("instances" if enclosed else "instances_open", np.concatenate(verts).astype(np.float32),
+                 np.concatenate(norms).astype(np.float32),
                  np.concatenate(mids), np.stack(mats), lights, cam, seed)
 
 
