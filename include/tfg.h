@@ -339,6 +339,19 @@ int tfg_pt_read_cache_data(tfg_pt* pt, float* records, uint32_t max_records, flo
 /* ColorMLP::inference (mlp.cpp:127-170) on DatosMLP records: sanitised
  * predictions (NrcKernel.cu:99-126) into d_out[n]; synchronous. */
 int tfg_nrc_infer_datos(tfg_nrc* nrc, const tfg_datos_mlp* d_in, uint32_t n, tfg_color* d_out, tfg_stream stream);
+/* tfg_nrc_infer_datos without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The converted inputs and raw outputs live in one static device array of
+ * the library (4.75 MiB on every GPU it runs on, held until the process
+ * ends), so at most TFG_NRC_NOALLOC_MAX_QUERIES queries per call (more:
+ * TFG_E_LIMIT).  Same kernels and results as tfg_nrc_infer_datos, but
+ * asynchronous on `stream`.  Every call on a GPU shares that GPU's array:
+ * calls ordered on one stream are safe, calls that overlap on one GPU (other
+ * streams, other threads) race. */
+#define TFG_NRC_NOALLOC_MAX_QUERIES 65536
+int tfg_nrc_infer_datos_noalloc(tfg_nrc* nrc, const tfg_datos_mlp* d_in, uint32_t n, tfg_color* d_out,
+                                tfg_stream stream);
 /* ColorMLP::train_step (mlp.cpp:95-125): one optimiser step on the
  * configured batch drawn (NrcKernel.cu:10-16) from d_in[0..n) with target =
  * DatosMLP.color; no-op with *loss = 0 when n < 100; synchronous. */

@@ -155,6 +155,7 @@ _SIGS = {
     "tfg_nrc_last_gradient": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tfg_nrc_infer_datos": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "tfg_nrc_train_datos": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.c_void_p]),
+    "tfg_nrc_infer_datos_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "tfg_launch_inicializar_camara": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "tfg_launch_composite": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "tfg_launch_calcular_targets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),

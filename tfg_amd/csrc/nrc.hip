@@ -21,7 +21,19 @@
 #include "pt_internal.h"
 #include "shade_device.h"
 
+#include <mutex>
+
 using namespace tfg;
+
+// Scratch memory of tfg_nrc_infer_datos_noalloc: per query one vertex record
+// (kVertF4 float4) followed by the 3 raw network outputs of all queries.
+// Every GPU the library runs on holds its own copy from module load until
+// the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (scratch_ptr below).
+constexpr uint32_t kNoAllocMaxQueries = TFG_NRC_NOALLOC_MAX_QUERIES;
+constexpr size_t kScratchVertF4 = (size_t)kNoAllocMaxQueries * kVertF4;
+constexpr size_t kScratchF4 = kScratchVertF4 + (3 * (size_t)kNoAllocMaxQueries + 3) / 4;
+static __device__ __align__(256) float4 scratch[kScratchF4];
 
 struct tfg_nrc {
     tfg_nrc_config cfg{};
@@ -608,6 +620,31 @@ int nrc_frame_enqueue(tfg_nrc* nc, const tfg_nrc_frame& f, tfg_color* image, hip
 
 }  // namespace tfg
 
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch[64] = {};
+
+// Device address of the current device's scratch array, looked up once per
+// device (with that device current) and kept.
+int scratch_ptr(float4** out) {
+    int dev = -1;
+    TFG_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) {
+        set_error("tfg_nrc_infer_datos_noalloc: device %d out of range", dev);
+        return TFG_E_LIMIT;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        g_scratch[dev] = p;
+    }
+    *out = static_cast<float4*>(g_scratch[dev]);
+    return TFG_OK;
+}
+}  // namespace
+
 // ===================================================================== C ABI
 extern "C" {
 
@@ -792,6 +829,40 @@ int tfg_nrc_infer_datos(tfg_nrc* nc, const tfg_datos_mlp* d_in, uint32_t n, tfg_
     hipLaunchKernelGGL(k_to_color, dim3(1024), dim3(256), 0, stream, nc->dm_out, n, d_out);
     TFG_HIP_CHECK(hipGetLastError());
     TFG_HIP_CHECK(hipStreamSynchronize(stream));
+    return TFG_OK;
+}
+
+
+// tfg_nrc_infer_datos without allocation: the converted inputs and raw
+// outputs live in the library's scratch array instead of buffers grown with
+// hipMalloc.  Same kernels, same results; asynchronous on `stream`.
+int tfg_nrc_infer_datos_noalloc(tfg_nrc* nc, const tfg_datos_mlp* d_in, uint32_t n, tfg_color* d_out,
+                                tfg_stream stream) {
+    if (!nc || !d_in || !d_out) { set_error("tfg_nrc_infer_datos_noalloc: NULL argument"); return TFG_E_INVALID; }
+    if (n > kNoAllocMaxQueries) {
+        set_error("tfg_nrc_infer_datos_noalloc: %u queries exceed the scratch capacity of %u per call", n,
+                  kNoAllocMaxQueries);
+        return TFG_E_LIMIT;
+    }
+    if (n == 0) return TFG_OK;
+    float4* base = nullptr;
+    const int rc = scratch_ptr(&base);
+    if (rc != TFG_OK) return rc;
+    float4* verts = base;
+    float* raw = reinterpret_cast<float*>(base + kScratchVertF4);
+    // every entry read below is written first: nothing needs clearing
+    const uint32_t grid = std::min<uint32_t>((n + 255) / 256, 1024u);
+    hipLaunchKernelGGL(k_from_datos, dim3(grid), dim3(256), 0, stream, d_in, n, verts, 0);
+    const void* fop = nc->fop_infer;
+    const dim3 egrid((n + 63) / 64);
+    if (nc->cfg.precision == TFG_NRC_FP16)
+        hipLaunchKernelGGL(k_nrc_eval<_Float16>, egrid, dim3(256), 0, stream, verts, n, raw, nullptr,
+                           static_cast<const f16x8*>(fop), nc->bounds);
+    else
+        hipLaunchKernelGGL(k_nrc_eval<__bf16>, egrid, dim3(256), 0, stream, verts, n, raw, nullptr,
+                           static_cast<const bf16x8*>(fop), nc->bounds);
+    hipLaunchKernelGGL(k_to_color, dim3(grid), dim3(256), 0, stream, raw, n, d_out);
+    TFG_HIP_CHECK(hipGetLastError());
     return TFG_OK;
 }
 
