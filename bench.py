@@ -44,18 +44,16 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrays/s + equal-time relMSE, NRC vs plain PT, 1080p, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
 
-# Bytes per launch between L2 and memory, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE runs of this file's workloads, scaled by the same counters on a
-# 1 GiB copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Sources: the summaries in
-# profiles/ named in "traffic_source".
-# HBM bytes (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration) from the
-# committed rocprofv3 PMC summaries: per launch for the traversal microbench,
-# per frame (all launches of the stage) for the path tracer's stages, whose
-# launches per frame depend on the tuned schedule
+# HBM bytes between L2 and memory (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# runs of this file's workloads, scaled by the same counters on a 1 GiB copy:
+# FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), from the committed summaries in
+# profiles/ named in "traffic_source": per launch for the traversal
+# microbench; per frame (the 8 launches of the stage in the roofline frames,
+# overlap off, saved schedule) for the path tracer's stages.
 MEASURED_TRAFFIC = {
     "k_trace_bounce": (366.2e6, "profiles/k_trace_bounce_displaced_sphere_1M.md"),
-    "k_extend": (1882.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
-    "k_connect": (1801.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
+    "k_extend": (1871.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
+    "k_connect": (3104.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
 }
 
 
@@ -175,11 +173,48 @@ def bench_traversal(args, torch):
     return res
 
 
+# The schedule tfg_pt_autotune picked for the 1080p workload below on MI355X
+# (plain frames, direct launches).  The default run uses it as saved, so the
+# timed frames, the roofline frames and every counter pass of a profile run
+# describe the same launches (the tuner's pick between near-equal candidates
+# varies from run to run); --tune re-tunes instead.
+TUNED_SCHEDULE_1080P = {"trace_block": 64, "grid_mult": 4, "tiles": 1, "side_streams": 2, "overlap": True,
+                        "xcd_group": 0, "octet_below": 131072, "connect_per_cu": 0, "connect_batch": 0,
+                        "top_nodes": 1, "tail_from": 0}
+
+
+def _stage_roofline(prof, frames, stage, n_launch):
+    """Algorithmic bytes and summed launch time of one traversal stage over
+    `frames` profiled frames (tfg_pt_profile_frame dicts)."""
+    if stage == "extend":
+        rays = sum(sum(p["ext_rays"]) for p in prof)
+        nodes = sum(p["ext_nodes"] for p in prof)
+        tris = sum(p["ext_tris"] for p in prof)
+        ms = sum(sum(p["ms_extend"]) for p in prof)
+    else:
+        rays = sum(sum(p["shadow_rays"]) for p in prof)
+        nodes = sum(p["shadow_nodes"] for p in prof)
+        tris = sum(p["shadow_tris"] for p in prof)
+        ms = sum(sum(p["ms_connect"]) for p in prof)
+    byt = 48 * rays + 80 * nodes + 48 * tris
+    return {"rays_per_frame": rays / frames, "bytes_per_frame": byt / frames, "ms_per_frame": ms / frames,
+            "bytes_per_launch": byt / frames / n_launch, "avg_launch_ms": ms / frames / n_launch,
+            "gbs": byt / (ms * 1e-3) / 1e9, "nodes_per_ray": nodes / rays, "tris_per_ray": tris / rays}
+
+
 def bench_pt(args, torch):
-    """Plain FP32 wavefront path tracer, 64-instance scene, 1920x1080, 1 spp per
-    frame, max depth 8, Russian roulette after depth 3;
-    frames with the tuned schedule (launch shape, side streams, graph replay or
-    direct launches).  One step = one frame."""
+    """Plain FP32 wavefront path tracer, 64-instance scene in its closed room,
+    1920x1080, 1 spp per frame, max depth 8, Russian roulette after depth 3;
+    frames with the saved tuned schedule (or a fresh tune with --tune).  One
+    step = one frame.
+
+    Roofline: the traversal stages are then timed on `steps` further frames
+    of the same schedule with the shadow stage on the main stream (overlap
+    off), so that every launch has the GPU to itself and its HIP-event
+    duration is the kernel's own; the dominant stage is the one with the
+    larger total.  Algorithmic bytes per launch = (48 B per ray + 80 B per
+    node decoded + 48 B per triangle tested) / launches, the node and
+    triangle counts from the counting pass of the same frames."""
     from tfg_amd import pt, scenes
 
     t0 = time.time()
@@ -197,9 +232,14 @@ def bench_pt(args, torch):
     tracer = pt.PathTracer(W, H, MD, RR, mode=0, use_graph=False)
     tracer.set_scene(ds)
     tracer.set_camera(sc.camera.frame(W, H))
-    tracer.autotune()
+    if args.tune:
+        tracer.autotune()
+        tuned = "tfg_pt_autotune in this run"
+    else:
+        tracer.set_schedule(**TUNED_SCHEDULE_1080P)
+        tuned = "saved (bench.TUNED_SCHEDULE_1080P, an earlier tfg_pt_autotune pick on MI355X)"
     shape = tracer.launch_shape()
-    _log(f"tuned schedule {shape}")
+    _log(f"schedule ({tuned}) {shape}")
     for f in range(args.warmup):
         tracer.render(f, img, accum)
     torch.cuda.synchronize()
@@ -219,7 +259,14 @@ def bench_pt(args, torch):
     ext, shd = tracer.ray_totals()
     rays_frame = (ext + shd) / args.steps
     value = rays_frame / (ms_frame * 1e-3) / 1e6
+    _log(f"timed {args.steps} frames: {ms_frame:.4f} ms per frame, {value:.1f} Mrays/s")
+    # one profiled frame in the timed schedule (stages overlapped as timed)
     prof = tracer.profile_frame(1000, img)
+    # roofline frames: the same schedule serialised (overlap off)
+    tracer.set_schedule(overlap=False)
+    n_roof = max(3, args.steps)
+    rprof = [tracer.profile_frame(1000 + k, img) for k in range(n_roof)]
+    tracer.set_schedule(overlap=shape["overlap"])
     # GPU refit of the tree to its vertices (the animated-geometry path; with
     # the build's own vertices the tree is unchanged), HIP events on the stream
     from tfg_amd import _lib
@@ -235,60 +282,77 @@ def bench_pt(args, torch):
     torch.cuda.synchronize()
     refit_ms = ra.elapsed_time(rb) / 5
     refitter.close()
-    ms_ext = sum(prof["ms_extend"])
-    n_ext = sum(prof["ext_rays"])
-    ext_bytes = 48 * n_ext + 80 * prof["ext_nodes"] + 48 * prof["ext_tris"]
-    ms_con = sum(prof["ms_connect"])
-    n_shd = sum(prof["shadow_rays"])
-    con_bytes = 48 * n_shd + 80 * prof["shadow_nodes"] + 48 * prof["shadow_tris"]
-    # the roofline object describes whichever traversal stage takes longer
-    stages = {
-        "extend": ("k_extend (closest-hit stage, all depths of one frame)", ext_bytes, ms_ext),
-        "connect": ("k_connect (shadow any-hit stage, all depths of one frame)", con_bytes, ms_con),
-    }
-    dom = max(stages, key=lambda k: stages[k][2])
-    name, dom_bytes, dom_ms = stages[dom]
-    # launches per frame: one per depth and tile, except that connect_batch
+    # launches per frame and tile: one per depth, except that connect_batch
     # k > 0 traces the shadow rays of depths k..MD-1 in one launch
     cb = shape.get("connect_batch", 0)
-    per_tile = MD if dom == "extend" or cb == 0 or cb >= MD else cb + 1
-    n_launch = per_tile * shape.get("tiles", 1)
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    launches = {"extend": MD * shape.get("tiles", 1),
+                "connect": (MD if cb == 0 or cb >= MD else cb + 1) * shape.get("tiles", 1)}
+    serial = {k: _stage_roofline(rprof, n_roof, k, launches[k]) for k in launches}
+    in_sched = {k: _stage_roofline([prof], 1, k, launches[k]) for k in launches}
+    dom = max(serial, key=lambda k: serial[k]["ms_per_frame"])
+    d = serial[dom]
+    names = {"extend": "k_extend (closest-hit stage)", "connect": "k_connect (shadow any-hit stage)"}
+    traffic = MEASURED_TRAFFIC.get("k_" + dom)
+    tr_launch = traffic[0] / launches[dom] if traffic else None
+    ser_ms = {"generate": float(np.mean([p["ms_generate"] for p in rprof])),
+              "extend": serial["extend"]["ms_per_frame"],
+              "shade": float(np.mean([sum(p["ms_shade"]) for p in rprof])),
+              "connect": serial["connect"]["ms_per_frame"],
+              "finalize": float(np.mean([p["ms_finalize"] for p in rprof])),
+              "frame": float(np.mean([p["ms_total"] for p in rprof]))}
+    algo_frame = serial["extend"]["bytes_per_frame"] + serial["connect"]["bytes_per_frame"]
     res = {
         "value": value,
         "ms_per_step": ms_frame,
-        "per_kernel_ms": {"generate": prof["ms_generate"], "extend": ms_ext, "shade": sum(prof["ms_shade"]),
-                          "connect": ms_con, "finalize": prof["ms_finalize"], "profiled_frame": prof["ms_total"]},
-        "mrays_per_kernel": {"extend": n_ext / (ms_ext * 1e-3) / 1e6, "connect": n_shd / (ms_con * 1e-3) / 1e6},
+        "per_kernel_ms": {"generate": prof["ms_generate"], "extend": sum(prof["ms_extend"]),
+                          "shade": sum(prof["ms_shade"]), "connect": sum(prof["ms_connect"]),
+                          "finalize": prof["ms_finalize"], "profiled_frame": prof["ms_total"],
+                          "note": "one frame of the timed schedule: stages on side streams overlap, so their "
+                                  "sums can exceed the frame"},
+        "per_kernel_ms_serial": ser_ms,
+        "mrays_per_kernel": {k: serial[k]["rays_per_frame"] / (serial[k]["ms_per_frame"] * 1e-3) / 1e6
+                             for k in serial},
         "roofline": {
-            "kernel": name,
+            "kernel": names[dom],
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": d["gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": MEASURED_TRAFFIC["k_" + dom][0] / n_launch,
-            "traffic_per_frame": MEASURED_TRAFFIC["k_" + dom][0],
-            "traffic_source": MEASURED_TRAFFIC["k_" + dom][1],
-            "algorithmic_bytes": int(dom_bytes),
-            "algorithmic_bytes_per_launch": int(dom_bytes / n_launch),
-            "launches_per_frame": n_launch,
-            "avg_launch_ms": dom_ms / n_launch,
-            "stage_gbs": {k: v[1] / (v[2] * 1e-3) / 1e9 for k, v in stages.items()},
+            "frac": d["gbs"] / HBM_PEAK_GBS,
+            "traffic": tr_launch,
+            "traffic_per_frame": traffic[0] if traffic else None,
+            "traffic_source": traffic[1] if traffic else None,
+            "achieved_measured_traffic": tr_launch / (d["avg_launch_ms"] * 1e-3) / 1e9 if traffic else None,
+            "frac_measured_traffic": tr_launch / (d["avg_launch_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
+            "algorithmic_bytes_per_launch": int(d["bytes_per_launch"]),
+            "launches_per_frame": launches[dom],
+            "avg_launch_ms": d["avg_launch_ms"],
+            "timing": f"HIP events around each launch on its stream, {n_roof} frames of the timed schedule with "
+                      "overlap off (each launch alone on the GPU)",
+            "nodes_per_ray": d["nodes_per_ray"], "tris_per_ray": d["tris_per_ray"],
+            "stages_serial": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                              for k, v in serial.items()},
+            "stages_in_timed_schedule": {k: {"avg_launch_ms": round(v["avg_launch_ms"], 4), "gbs": round(v["gbs"], 1)}
+                                         for k, v in in_sched.items()},
+            "frame_level": {"algorithmic_bytes_per_frame": int(algo_frame),
+                            "gbs_over_timed_frame": algo_frame / (ms_frame * 1e-3) / 1e9,
+                            "frac": algo_frame / (ms_frame * 1e-3) / 1e9 / HBM_PEAK_GBS},
         },
         "config": {
             "workload": "plain FP32 wavefront PT: 64 seeded instances (32 displaced spheres + 32 boxes, "
-                        f"{sc.n_tris} tris) in a closed 12-triangle room, 4 point lights, Lambertian+GGX; 1920x1080, 1 spp/frame, "
-                        "max depth 8, RR after depth 3; frame schedule (graph replay or direct launches, streams) chosen by the tuner, see launch",
+                        f"{sc.n_tris} tris) in a closed 12-triangle room, 4 point lights, Lambertian+GGX; 1920x1080, "
+                        "1 spp/frame, max depth 8, RR after depth 3; direct launches, schedule in launch",
             "tris": sc.n_tris,
             "cwbvh_nodes": ds.host.num_nodes,
             "rays_per_frame": rays_frame,
             "ext_rays_per_depth": prof["ext_rays"],
             "shadow_rays_per_depth": prof["shadow_rays"],
-            "ms_per_depth": {"extend": [round(x, 4) for x in prof["ms_extend"]],
-                             "shade": [round(x, 4) for x in prof["ms_shade"]],
-                             "connect": [round(x, 4) for x in prof["ms_connect"]]},
+            "ms_per_depth_serial": {
+                "extend": [round(float(np.mean([p["ms_extend"][i] for p in rprof])), 4) for i in range(MD)],
+                "shade": [round(float(np.mean([p["ms_shade"][i] for p in rprof])), 4) for i in range(MD)],
+                "connect": [round(float(np.mean([p["ms_connect"][i] for p in rprof])), 4) for i in range(MD)]},
             "launch": shape,
+            "schedule_source": tuned,
             "cwbvh_build_s": round(t_build, 2),
             "cwbvh_refit_ms": round(refit_ms, 3),
             "seeds": {"scene": sc.seed, "frames": [1000, 1000 + args.steps - 1]},
@@ -772,6 +836,8 @@ def main():
     ap.add_argument("--repeats", type=int, default=5,
                     help="equal-time runs per method (distinct frame seeds, fresh cache); the ratio is pooled")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="store_true",
+                    help="re-tune the path tracer's schedule (tfg_pt_autotune) instead of the saved one")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -837,6 +903,7 @@ def main():
         "config": res["config"],
         "per_kernel_ms": res["per_kernel_ms"],
         "mrays_per_kernel": res["mrays_per_kernel"],
+        "per_kernel_ms_serial": res.get("per_kernel_ms_serial"),
         "roofline": res["roofline"],
         "cpu_baseline": cpu,
     }
