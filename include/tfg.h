@@ -123,14 +123,26 @@ typedef struct tfg_pt_config {
     uint32_t use_graph;      /* 1: capture the frame into a hipGraph and replay it */
 } tfg_pt_config;
 
+/* Scene of a tracer.  bvh's triangle records are watertight
+ * (tri_format TFG_TRI_WATERTIGHT) or tinybvh's affine unit-triangle records
+ * (TFG_TRI_AFFINE, the layout the reference uploads, tinybvh_wrapper.cpp:3,
+ * traverse_cwbvh.cuh:187-209, primIdx in T3.w).  For an affine tree d_verts
+ * holds the triangles' vertices, 9 floats per primIdx (the build's input
+ * order): the tracer keeps a watertight copy of the records, made from them
+ * on the device at set_scene (call set_scene again after changing them).  A
+ * tree not built by this library is walked once on the host to find its
+ * depth (at most TFG_MAX_TREE_DEPTH levels). */
 typedef struct tfg_pt_scene {
-    tfg_cwbvh_gpu bvh;                  /* watertight triangle records */
+    tfg_cwbvh_gpu bvh;
     const tfg_tri_shading* d_shading;   /* per primIdx */
     const tfg_material* d_materials;
     const tfg_point_light* d_lights;
     uint32_t n_lights;                  /* <= 64 */
     const tfg_analytic* d_analytic;     /* may be NULL */
     uint32_t n_analytic;                /* <= 64 */
+    uint32_t tri_format;                /* TFG_TRI_WATERTIGHT (0) or TFG_TRI_AFFINE */
+    const float* d_verts;               /* TFG_TRI_AFFINE: 9 floats per primIdx; else unused */
+    uint32_t n_verts;                   /* triangles in d_verts (primIdx < n_verts) */
 } tfg_pt_scene;
 
 #define TFG_PT_NCOUNTERS 72
@@ -399,8 +411,13 @@ int tfg_launch_guardar_salida(const float* d_out, tfg_color* d_col, int n, tfg_s
  * queries (buffer_inference / buffer_throughput, throughput 0 where no
  * query, render.h:452-458) and the training records
  * (buffer_training[atomicAdd(counter_train, 1) % max_cap_train],
- * render.h:461-464) in the reference layouts.  The CWBVH must come from this
- * library (tfg_cwbvh_upload / the TinyBVH shim).  Converted scene data are
+ * render.h:461-464) in the reference layouts.  The CWBVH may come from this
+ * library (tfg_cwbvh_upload / the TinyBVH shim, either triangle format) or
+ * be any tree in tinybvh's layout (80 B nodes, 64 B affine triangle records
+ * with primIdx in T3.w, tinybvh_wrapper.cpp:53-75): a tree this library did
+ * not upload is taken as tinybvh's, and its triangles are traced with the
+ * watertight test on the vertices of the per-primIdx Primitiva copy
+ * (primitivas_bvh, RenderKernel.h:55-61).  Converted scene data are
  * cached per scene buffer; call tfg_dropin_invalidate() after rewriting a
  * scene buffer in place.  At most 64 analytic primitives and 64 lights. */
 int tfg_launch_render_tiny(const tfg_camara* d_camara, const tfg_primitiva* d_prims, int n_prims,

@@ -15,8 +15,19 @@
 
 class ColorMLP {
 public:
+    // config_override stands for the reference's tcnn::json argument
+    // (mlp.cpp:16-73: HashGrid + OneBlob + Identity by default, any tcnn
+    // model by JSON).  This shim does not read it: the network is always
+    // the library's (12-frequency position encoding + one-blob direction,
+    // normal and roughness + albedos, 5 x 64 ReLU, BF16 operands, RGB out;
+    // RelativeL2Luminance, EMA 0.99 over Adam 1e-2), and a non-null
+    // configuration is reported on stderr once.  n_in must be 15 and n_out
+    // 3 (DatosMLP's 15 inputs, Color's 3 outputs); any other pair leaves the
+    // object without a network, whose calls then do nothing.
     ColorMLP(uint32_t n_in, uint32_t n_out, uint32_t batch, const void* config_override = nullptr) {
-        (void)config_override;
+        if (config_override)
+            std::fprintf(stderr, "ColorMLP: the tiny-cuda-nn configuration is not used; the library's fixed network "
+                                 "is trained instead\n");
         if (n_in != 15 || n_out != 3) return;   // DatosMLP carries 15 inputs, Color 3 outputs
         tfg_nrc_default_config(&cfg_);
         cfg_.batch = (batch + 63) / 64 * 64;

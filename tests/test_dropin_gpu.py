@@ -115,3 +115,123 @@ def test_render_tiny_dropin(torch_cuda):
         _lib.check(_lib.lib().tfg_launch_render_tiny(
             None, None, 0, None, 0, None, 0, W, H, 1, 0, dbvh.desc, d["mesh"].data_ptr(), d["n_mesh"],
             _lib.ImagenGpu(W, H, img.data_ptr()), None, None, None, 0, None, thr.data_ptr(), 0, 0, None))
+
+
+def _foreign_desc(torch, host):
+    """The host tree's node and record arrays in device buffers this library
+    never saw (a tree from another builder, as tinybvh's upload makes one)."""
+    nodes = torch.from_numpy(np.ascontiguousarray(host.nodes)).cuda()
+    tris = torch.from_numpy(np.ascontiguousarray(host.tris)).cuda()
+    return _lib.CwbvhGpu(nodes.data_ptr(), tris.data_ptr(), host.num_nodes, host.num_tris), (nodes, tris)
+
+
+@pytest.mark.parametrize("source", ["uploaded_affine", "foreign_affine"])
+def test_render_tiny_affine_tree_matches_watertight(torch_cuda, source):
+    """A CWBVH in tinybvh's layout (80 B nodes, affine unit-triangle records
+    with primIdx in T3.w, tinybvh_wrapper.cpp:3,53-75): built here in
+    TFG_TRI_AFFINE and uploaded by tfg_cwbvh_upload, or the same arrays in
+    device buffers this library never saw (then taken as tinybvh's layout,
+    its depth found by walking the nodes).  The drop-in traces its triangles
+    with the watertight test on the vertices of the Primitiva copy, so the
+    image, the cache queries and throughputs and the record count equal
+    those of the watertight-format tree of the same scene, bit for bit."""
+    torch = torch_cuda
+    sc = scenes.reference_scene()
+    W, H = 96, 64
+    an, mesh = _prims_from_scene(sc)
+    hw = _lib.CwbvhHost(sc.verts, _lib.TFG_TRI_WATERTIGHT)
+    ha = _lib.CwbvhHost(sc.verts, _lib.TFG_TRI_AFFINE)
+    np.testing.assert_array_equal(hw.nodes, ha.nodes)      # same tree, other triangle records
+    assert not np.array_equal(hw.tris, ha.tris)
+    dw = hw.upload()
+    if source == "uploaded_affine":
+        da = ha.upload()
+        keep = None
+    else:
+        desc_a, keep = _foreign_desc(torch, ha)
+        da = type("ForeignTree", (), {"desc": desc_a})()
+    luz = np.zeros(1, LUZ)
+    luz["pos"], luz["I"] = sc.lights[0, 0:3], sc.lights[0, 4:7]
+    cam = torch.zeros(112, dtype=torch.uint8, device="cuda")
+    _lib.check(_lib.lib().tfg_launch_inicializar_camara(cam.data_ptr(), W, H, None))
+    d = {"cam": cam, "an": _dev(torch, an), "n_an": an.shape[0], "luz": _dev(torch, luz), "mesh": _dev(torch, mesh),
+         "n_mesh": mesh.shape[0]}
+    _lib.check(_lib.lib().tfg_dropin_configure(8, 5))
+    out = {}
+    for name, tree in (("watertight", dw), ("affine", da)):
+        res = []
+        for frame, usar, entrenar in ((4, False, False), (5, True, True)):
+            print(f"tree={name} source={source} frame={frame} flags=({int(usar)},{int(entrenar)})", flush=True)
+            img = torch.zeros((H * W, 3), device="cuda")
+            inf = torch.zeros(W * H * 112, dtype=torch.uint8, device="cuda")
+            thr = torch.full((H * W, 3), 7.0, device="cuda")
+            recs = torch.zeros(W * H * 232, dtype=torch.uint8, device="cuda")
+            counter = torch.zeros(1, dtype=torch.int32, device="cuda")
+            _call(torch, d, tree, W, H, frame, usar, entrenar, img, inf, thr, recs, counter)
+            res.append((img.cpu().numpy(), thr.cpu().numpy(), inf.cpu().numpy(), counter.item()))
+        out[name] = res
+    for (iw, tw, fw, cw), (ia, ta, fa, ca) in zip(out["watertight"], out["affine"]):
+        np.testing.assert_array_equal(iw, ia)
+        np.testing.assert_array_equal(tw, ta)
+        np.testing.assert_array_equal(fw, fa)
+        assert cw == ca
+    assert out["affine"][0][0].sum() > 0 and out["affine"][1][3] > 0
+    del keep
+
+
+def test_tracer_affine_scene_and_malformed_foreign_tree(torch_cuda):
+    """tfg_pt_set_scene with an affine tree and its vertices renders the
+    watertight tree's image bit for bit (plain and cache-capture frames);
+    an affine tree without vertices, a format that contradicts the
+    uploaded tree's, and a foreign tree whose child index leaves the node
+    array are refused on the host before any launch."""
+    from tfg_amd import pt
+
+    torch = torch_cuda
+    sc = scenes.instances(n_inst=8, sphere_res=(32, 16), box_res=4, seed=5)
+    W, H = 96, 64
+    ds = pt.DeviceScene(sc, torch)
+    ha = _lib.CwbvhHost(sc.verts, _lib.TFG_TRI_AFFINE)
+    da = ha.upload()
+    verts = torch.from_numpy(np.ascontiguousarray(sc.verts, np.float32)).cuda()
+    aff = _lib.PtScene()
+    C.pointer(aff)[0] = ds.desc
+    aff.bvh = da.desc
+    aff.tri_format = _lib.TFG_TRI_AFFINE
+    aff.d_verts = verts.data_ptr()
+    aff.n_verts = sc.n_tris
+    for mode in (0, 2):
+        imgs = []
+        for desc in (ds.desc, aff):
+            tr = pt.PathTracer(W, H, 8, 3, mode=mode, use_graph=False)
+            _lib.check(_lib.lib().tfg_pt_set_scene(tr._h, C.byref(desc)), "tfg_pt_set_scene")
+            tr.set_camera(sc.camera.frame(W, H))
+            img = torch.zeros((H * W, 3), device="cuda")
+            tr.render(9, img)
+            torch.cuda.synchronize()
+            imgs.append((img.cpu().numpy(), tr.rays_last_frame()))
+            tr.close()
+        np.testing.assert_array_equal(imgs[0][0], imgs[1][0])
+        assert imgs[0][1] == imgs[1][1]
+    tr = pt.PathTracer(W, H, 8, 3, mode=0, use_graph=False)
+    bad = _lib.PtScene()
+    C.pointer(bad)[0] = aff
+    bad.d_verts = None
+    assert _lib.lib().tfg_pt_set_scene(tr._h, C.byref(bad)) == -1
+    bad = _lib.PtScene()
+    C.pointer(bad)[0] = ds.desc
+    bad.tri_format = _lib.TFG_TRI_AFFINE          # the uploaded tree is watertight
+    bad.d_verts = verts.data_ptr()
+    bad.n_verts = sc.n_tris
+    assert _lib.lib().tfg_pt_set_scene(tr._h, C.byref(bad)) == -1
+    nodes = ha.nodes.copy()
+    nodes.view(np.uint32)[0, 4] = ha.num_nodes + 5   # root's first child beyond the array
+    foreign, keep = _foreign_desc(torch, type("H", (), {"nodes": nodes, "tris": ha.tris, "num_nodes": ha.num_nodes,
+                                                        "num_tris": ha.num_tris})())
+    bad = _lib.PtScene()
+    C.pointer(bad)[0] = aff
+    bad.bvh = foreign
+    rc = _lib.lib().tfg_pt_set_scene(tr._h, C.byref(bad))
+    assert rc == -1 and b"malformed" in _lib.lib().tfg_last_error()
+    tr.close()
+    del keep

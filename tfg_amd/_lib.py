@@ -48,7 +48,8 @@ class PtConfig(C.Structure):
 
 class PtScene(C.Structure):
     _fields_ = [("bvh", CwbvhGpu), ("d_shading", C.c_void_p), ("d_materials", C.c_void_p), ("d_lights", C.c_void_p),
-                ("n_lights", C.c_uint32), ("d_analytic", C.c_void_p), ("n_analytic", C.c_uint32)]
+                ("n_lights", C.c_uint32), ("d_analytic", C.c_void_p), ("n_analytic", C.c_uint32),
+                ("tri_format", C.c_uint32), ("d_verts", C.c_void_p), ("n_verts", C.c_uint32)]
 
 
 class PtProfile(C.Structure):

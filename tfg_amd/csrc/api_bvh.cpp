@@ -4,6 +4,8 @@
 #include <mutex>
 #include <new>
 #include <unordered_map>
+#include <vector>
+#include <algorithm>
 
 #include "cwbvh_build.h"
 #include "host_common.h"
@@ -43,6 +45,42 @@ void register_tree_depth(const void* d_nodes, uint32_t depth, uint32_t format = 
     std::lock_guard<std::mutex> lk(g_depth_mu);
     if (depth == 0) g_depth.erase(d_nodes);
     else g_depth[d_nodes] = {depth, format};
+}
+
+// Depth of a tree this library did not build (root = 1), from a walk over a
+// host copy of its nodes: the children of node i are nodes n1.x + rank, one
+// per set bit of imask (byte 3 of n0.w), the node layout the traversal
+// decodes (traverse_cwbvh.cuh:73-99).  0 when a child index leaves the node
+// array, a node is reached twice or the tree is deeper than 64 levels.
+uint32_t walk_tree_depth(const void* d_nodes, uint32_t num_nodes, int* rc) {
+    *rc = TFG_OK;
+    if (!d_nodes || num_nodes == 0) return 0;
+    std::vector<uint32_t> words((size_t)num_nodes * 20);
+    const hipError_t e = hipMemcpy(words.data(), d_nodes, words.size() * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        *rc = hip_fail(e, "hipMemcpy(CWBVH nodes)");
+        return 0;
+    }
+    std::vector<uint8_t> seen(num_nodes, 0);
+    std::vector<std::pair<uint32_t, uint32_t>> todo{{0u, 1u}};
+    uint32_t depth = 0;
+    seen[0] = 1;
+    while (!todo.empty()) {
+        const auto [i, d] = todo.back();
+        todo.pop_back();
+        if (d > 64) return 0;
+        depth = std::max(depth, d);
+        const uint32_t* n = &words[(size_t)i * 20];
+        const uint32_t imask = n[3] >> 24, base = n[4];
+        for (uint32_t slot = 0; slot < 8; ++slot) {
+            if (!(imask & (1u << slot))) continue;
+            const uint64_t c = (uint64_t)base + (uint32_t)__builtin_popcount(imask & ((1u << slot) - 1u));
+            if (c >= num_nodes || seen[c]) return 0;
+            seen[c] = 1;
+            todo.push_back({(uint32_t)c, d + 1});
+        }
+    }
+    return depth;
 }
 
 }  // namespace tfg

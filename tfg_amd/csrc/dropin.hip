@@ -46,6 +46,8 @@ struct DropIn {
     tfg_material* mats = nullptr;   // n_mesh + n_prims entries
     tfg_point_light* lights = nullptr;
     tfg_analytic* analytic = nullptr;
+    float* verts = nullptr;          // affine (tinybvh-layout) trees: 9 floats per primIdx
+    size_t cap_verts = 0;
     tfg_pt_scene scene{};
 };
 DropIn g_drop;
@@ -63,8 +65,10 @@ __device__ __forceinline__ tfg_material mat_of(const tfg_primitiva& p) {
 
 // Mesh Primitiva (indexed by primIdx, tinybvh_wrapper.cpp:77-79) -> shading
 // record + material; smooth normals when the vertex normals differ
-// (escenario.h:126-134).
-__global__ void k_conv_mesh(const tfg_primitiva* __restrict__ prims, uint32_t n, tfg_tri_shading* sh, tfg_material* mats) {
+// (escenario.h:126-134).  verts (nullable): the FP32 vertices, rounded from
+// the FP64 ones as tinybvh_wrapper.cpp:38-40 rounds them for the build.
+__global__ void k_conv_mesh(const tfg_primitiva* __restrict__ prims, uint32_t n, tfg_tri_shading* sh, tfg_material* mats,
+                            float* verts) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const tfg_primitiva p = prims[i];
         const auto& t = p.g.triangulo;
@@ -84,6 +88,12 @@ __global__ void k_conv_mesh(const tfg_primitiva* __restrict__ prims, uint32_t n,
         s.ng[2] = (float)(e1x * e2y - e1y * e2x);
         sh[i] = s;
         mats[i] = mat_of(p);
+        if (verts) {
+            float* v = verts + 9 * (size_t)i;
+            v[0] = (float)t.v0.x; v[1] = (float)t.v0.y; v[2] = (float)t.v0.z;
+            v[3] = (float)t.v1.x; v[4] = (float)t.v1.y; v[5] = (float)t.v1.z;
+            v[6] = (float)t.v2.x; v[7] = (float)t.v2.y; v[8] = (float)t.v2.z;
+        }
     }
 }
 
@@ -214,9 +224,22 @@ int convert_scene(DropIn& g, const tfg_primitiva* prims, int n_prims, const tfg_
         TFG_HIP_CHECK(hipMalloc(&g.mats, sizeof(tfg_material) * (need + kMaxAnalytic)));
         g.cap_mesh = need;
     }
+    // a tree this library did not upload is taken as tinybvh's layout (affine
+    // records, tinybvh_wrapper.cpp:3): the tracer traces it with the
+    // watertight test on the Primitiva copy's vertices
+    const bool affine = known_tree_format(bvh.d_nodos) != TFG_TRI_WATERTIGHT;
+    if (affine && need > g.cap_verts) {
+        if (g.verts) (void)hipFree(g.verts);
+        g.verts = nullptr;
+        g.cap_verts = 0;
+        TFG_HIP_CHECK(hipMalloc(&g.verts, 9 * sizeof(float) * need));
+        g.cap_verts = need;
+    }
     if (!g.lights) TFG_HIP_CHECK(hipMalloc(&g.lights, sizeof(tfg_point_light) * 64));
     if (!g.analytic) TFG_HIP_CHECK(hipMalloc(&g.analytic, sizeof(tfg_analytic) * kMaxAnalytic));
-    if (n_mesh) hipLaunchKernelGGL(k_conv_mesh, dim3(1024), dim3(256), 0, s, mesh, (uint32_t)n_mesh, g.shading, g.mats);
+    if (n_mesh)
+        hipLaunchKernelGGL(k_conv_mesh, dim3(1024), dim3(256), 0, s, mesh, (uint32_t)n_mesh, g.shading, g.mats,
+                           affine ? g.verts : nullptr);
     if (n_prims) hipLaunchKernelGGL(k_conv_prims, dim3(1), dim3(64), 0, s, prims, (uint32_t)n_prims, (uint32_t)n_mesh,
                                     g.analytic, g.mats);
     if (n_luces) hipLaunchKernelGGL(k_conv_lights, dim3(1), dim3(64), 0, s, luces, (uint32_t)n_luces, g.lights);
@@ -228,6 +251,9 @@ int convert_scene(DropIn& g, const tfg_primitiva* prims, int n_prims, const tfg_
     g.scene.n_lights = (uint32_t)n_luces;
     g.scene.d_analytic = g.analytic;
     g.scene.n_analytic = (uint32_t)n_prims;
+    g.scene.tri_format = affine ? TFG_TRI_AFFINE : TFG_TRI_WATERTIGHT;
+    g.scene.d_verts = affine ? g.verts : nullptr;
+    g.scene.n_verts = affine ? (uint32_t)n_mesh : 0u;
     const int rc = tfg_pt_set_scene(g.pt, &g.scene);
     if (rc != TFG_OK) return rc;
     g.k_bvh = bvh.d_nodos;
@@ -292,10 +318,6 @@ int tfg_launch_render_tiny(const tfg_camara* d_camara, const tfg_primitiva* d_pr
     }
     if (!bvh.d_nodos || n_prims_bvh == 0 || !d_prims_bvh) {
         set_error("tfg_launch_render_tiny: a CWBVH mesh is required (TinyBVH::construirBVH)");
-        return TFG_E_INVALID;
-    }
-    if (known_tree_format(bvh.d_nodos) != TFG_TRI_WATERTIGHT) {
-        set_error("tfg_launch_render_tiny: CWBVH not built by this library in the watertight format");
         return TFG_E_INVALID;
     }
     DropIn& g = g_drop;

@@ -1,6 +1,7 @@
 // pt.hip - host side of the wavefront path tracer (tfg_pt_* C ABI).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstring>
 #include <new>
@@ -12,6 +13,8 @@
 
 namespace tfg {
 uint32_t known_tree_depth(const void* d_nodes);
+uint32_t known_tree_format(const void* d_nodes);
+uint32_t walk_tree_depth(const void* d_nodes, uint32_t num_nodes, int* rc);
 }
 
 struct tfg_pt {
@@ -30,6 +33,9 @@ struct tfg_pt {
     uint32_t top_nodes = 9;       // leading tree nodes each traversal workgroup stages in LDS
     uint32_t tail_from = 0;       // depths >= this (if > 0) run in the one-launch tail kernel
     uint32_t scene_nodes = 0;     // nodes of the current scene's tree
+    // watertight copy of an affine (tinybvh-layout) scene's triangle records
+    float4* wt_tris = nullptr;
+    size_t wt_cap = 0;            // records
     int shade_grid = 0;
     int gen_grid = 0;
     // pixel tiles: each runs its own chain of stages on its own streams
@@ -169,6 +175,30 @@ void launch_connect(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t d_lo,
         case 64: hipLaunchKernelGGL((k_connect<64, kStats>), g, dim3(64), lds, s, T, d_lo, d_hi); break;
         case 256: hipLaunchKernelGGL((k_connect<256, kStats>), g, dim3(256), lds, s, T, d_lo, d_hi); break;
         default: hipLaunchKernelGGL((k_connect<128, kStats>), g, dim3(128), lds, s, T, d_lo, d_hi); break;
+    }
+}
+
+// Watertight triangle records (v0, v1, v2, primIdx in T3.w) for the affine
+// records of a tinybvh-layout tree: record k keeps its position (the nodes
+// address it as triBase + 4k) and its primIdx (T3.w), and takes that
+// triangle's vertices from verts (9 floats per primIdx).  A primIdx beyond
+// the vertex array gives a zero-area record, which no ray hits.
+__global__ __launch_bounds__(256) void k_watertight_records(const float4* __restrict__ aff, uint32_t n_rec,
+                                                            const float* __restrict__ verts, uint32_t n_verts,
+                                                            float4* __restrict__ out) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_rec; k += gridDim.x * blockDim.x) {
+        const float pw = aff[4 * (size_t)k + 3].w;
+        const uint32_t prim = __float_as_uint(pw);
+        float4 v[3] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f),
+                       make_float4(0.f, 0.f, 0.f, 0.f)};
+        if (prim < n_verts) {
+            const float* p = verts + 9 * (size_t)prim;
+            for (int j = 0; j < 3; ++j) v[j] = make_float4(p[3 * j], p[3 * j + 1], p[3 * j + 2], 0.0f);
+        }
+        out[4 * (size_t)k] = v[0];
+        out[4 * (size_t)k + 1] = v[1];
+        out[4 * (size_t)k + 2] = v[2];
+        out[4 * (size_t)k + 3] = make_float4(0.f, 0.f, 0.f, pw);
     }
 }
 
@@ -646,14 +676,56 @@ int tfg_pt_set_scene(tfg_pt* pt, const tfg_pt_scene* sc) {
         set_error("tfg_pt_set_scene: at most %u analytic primitives", kMaxAnalytic);
         return TFG_E_LIMIT;
     }
-    if (known_tree_depth(sc->bvh.d_nodos) > TFG_MAX_TREE_DEPTH) {
-        set_error("tfg_pt_set_scene: tree too deep");
+    if (sc->tri_format != TFG_TRI_WATERTIGHT && sc->tri_format != TFG_TRI_AFFINE) {
+        set_error("tfg_pt_set_scene: unknown triangle format %u", sc->tri_format);
+        return TFG_E_INVALID;
+    }
+    const uint32_t known_fmt = known_tree_format(sc->bvh.d_nodos);
+    if (known_fmt != 0xFFFFFFFFu && known_fmt != sc->tri_format) {
+        set_error("tfg_pt_set_scene: the tree's triangle records are in format %u, the scene says %u", known_fmt,
+                  sc->tri_format);
+        return TFG_E_INVALID;
+    }
+    if (sc->tri_format == TFG_TRI_AFFINE && (!sc->d_verts || sc->n_verts == 0)) {
+        set_error("tfg_pt_set_scene: an affine tree needs its vertices (d_verts, n_verts)");
+        return TFG_E_INVALID;
+    }
+    uint32_t depth = known_tree_depth(sc->bvh.d_nodos);
+    if (depth == 0) {   // a tree this library did not build: walk it once
+        int rc = TFG_OK;
+        depth = walk_tree_depth(sc->bvh.d_nodos, sc->bvh.num_nodos, &rc);
+        if (rc != TFG_OK) return rc;
+        if (depth == 0) {
+            set_error("tfg_pt_set_scene: malformed CWBVH (child index outside the %u nodes, or a node reached twice)",
+                      sc->bvh.num_nodos);
+            return TFG_E_INVALID;
+        }
+    }
+    if (depth > TFG_MAX_TREE_DEPTH) {
+        set_error("tfg_pt_set_scene: tree depth %u exceeds %d", depth, TFG_MAX_TREE_DEPTH);
         return TFG_E_LIMIT;
     }
     PtParams& P = pt->P;
     P.nodes = static_cast<const float4*>(sc->bvh.d_nodos);
     P.tris = static_cast<const float4*>(sc->bvh.d_tris);
-    const uint32_t depth = known_tree_depth(sc->bvh.d_nodos);
+    if (sc->tri_format == TFG_TRI_AFFINE) {
+        const size_t n_rec = sc->bvh.num_tris;
+        if (n_rec > pt->wt_cap) {
+            if (pt->wt_tris) (void)hipFree(pt->wt_tris);
+            pt->wt_tris = nullptr;
+            pt->wt_cap = 0;
+            TFG_HIP_CHECK(hipMalloc((void**)&pt->wt_tris, 4 * n_rec * sizeof(float4)));
+            pt->wt_cap = n_rec;
+        }
+        TFG_HIP_CHECK(hipDeviceSynchronize());   // the caller's writes of the vertices
+        const uint32_t grid = (uint32_t)std::min<size_t>((n_rec + 255) / 256, (size_t)pt->n_cu * 16);
+        hipLaunchKernelGGL(k_watertight_records, dim3(grid ? grid : 1), dim3(256), 0, pt->stream,
+                           static_cast<const float4*>(sc->bvh.d_tris), (uint32_t)n_rec, sc->d_verts, sc->n_verts,
+                           pt->wt_tris);
+        TFG_HIP_CHECK(hipGetLastError());
+        TFG_HIP_CHECK(hipStreamSynchronize(pt->stream));
+        P.tris = pt->wt_tris;
+    }
     P.stack_depth = depth ? (int)(depth > 1 ? depth - 1 : 1) : TFG_MAX_TREE_DEPTH - 1;
     pt->scene_nodes = sc->bvh.num_nodos;
     P.n_top = pt->scene_nodes < pt->top_nodes ? pt->scene_nodes : pt->top_nodes;
@@ -1085,6 +1157,7 @@ int tfg_pt_destroy(tfg_pt* pt) {
     if (pt->ev_trained) (void)hipEventDestroy(pt->ev_trained);
     if (pt->train_buf) (void)hipFree(pt->train_buf);
     if (pt->train_nrec) (void)hipFree(pt->train_nrec);
+    if (pt->wt_tris) (void)hipFree(pt->wt_tris);
     for (uint32_t t = 0; t < kMaxTiles; ++t) {
         for (hipEvent_t e : pt->ev_cjoin[t])
             if (e) (void)hipEventDestroy(e);
