@@ -70,7 +70,8 @@ struct tfg_nrc {
 namespace tfg {
 namespace {
 
-constexpr int kStLt = 0, kStSeed = 1, kStFlags = 2, kStSkipNan = 3, kStSkipData = 4, kStLoss = 5, kStSteps = 6;
+constexpr int kStLt = 0, kStSeed = 1, kStFlags = 2, kStSkipNan = 3, kStSkipData = 4, kStLoss = 5, kStSteps = 6,
+              kStDone = 7;
 constexpr size_t kOpBytes16 = 16;   // one operand tile entry: 8 x 16-bit per lane
 
 struct Hyper {
@@ -112,10 +113,24 @@ __host__ __device__ inline size_t bop_pos(int p) {   // layers 1..5
     return ((size_t)tile * 64 + lane) * 8 + j;
 }
 
-template <typename T>
-__device__ __forceinline__ void load_ops(typename MfmaOps<T>::vec* dst, const typename MfmaOps<T>::vec* src,
-                                         int tiles) {
-    for (int i = threadIdx.x; i < tiles * 64; i += blockDim.x) dst[i] = src[i];
+// Copies `kTiles` operand tiles (64 x 16 B each) into LDS with every load of
+// a thread in flight before its first store: one memory latency per copy
+// instead of one per 16 B entry.
+template <typename T, int kTiles, int kThreads>
+__device__ __forceinline__ void load_ops(typename MfmaOps<T>::vec* dst, const typename MfmaOps<T>::vec* src) {
+    typedef typename MfmaOps<T>::vec V;
+    constexpr int kN = kTiles * 64, kPer = (kN + kThreads - 1) / kThreads;
+    V tmp[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = (int)threadIdx.x + k * kThreads;
+        if (i < kN) tmp[k] = src[i];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = (int)threadIdx.x + k * kThreads;
+        if (i < kN) dst[i] = tmp[k];
+    }
 }
 
 // ------------------------------------------------------------- inference
@@ -133,7 +148,7 @@ __global__ __launch_bounds__(kInferBlock) void k_nrc_infer(const float4* __restr
                                                    const typename MfmaOps<T>::vec* __restrict__ fop, NrcBounds b) {
     typedef typename MfmaOps<T>::vec V;
     __shared__ V W[kFopTiles * 64];
-    load_ops<T>(W, fop, kFopTiles);
+    load_ops<T, kFopTiles, kInferBlock>(W, fop);
     __syncthreads();
     const uint32_t n = *count;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, ml = lane & 15;
@@ -181,7 +196,7 @@ __global__ __launch_bounds__(256) void k_nrc_eval(const float4* __restrict__ ver
     typedef MfmaOps<T> M;
     typedef typename M::vec V;
     __shared__ V W[kFopTiles * 64];
-    load_ops<T>(W, fop, kFopTiles);
+    load_ops<T, kFopTiles, 256>(W, fop);
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, ml = lane & 15;
     for (uint32_t base = (blockIdx.x * 4 + w) * 16u; base < n; base += gridDim.x * 64u) {
@@ -251,35 +266,50 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
     T* s_xT = s_dyT + 64 * kTS;                                // [64 inputs][kTS: 64 samples]
     const uint32_t n_rec = *A.n_rec;
     if (n_rec <= A.min_records) return;
-    load_ops<T>(s_fop, static_cast<const V*>(A.fop_train), kFopTiles);
-    load_ops<T>(s_inf, static_cast<const V*>(A.fop_tail), kFopTiles);
-    load_ops<T>(s_bop, static_cast<const V*>(A.bop_train), kBopTiles);
-    __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, ml = lane & 15;
     const int m = 16 * w + ml;   // sample within the block
     const uint32_t i = blockIdx.x * 64 + m;
     // generarIndicesAleatorios (NrcKernel.cu:10-16): seed = step counter
     const uint32_t idx = A.gathered ? i : pcg32_step(A.state[kStSeed] ^ i) % n_rec;
     const float4* R = A.records + (size_t)kRecF4 * idx;
+    float4 rec[kRecF4];   // the sample's record, in flight while the weights are staged
+#pragma unroll
+    for (int k = 0; k < kRecF4; ++k) rec[k] = R[k];
+    load_ops<T, kFopTiles, 256>(s_fop, static_cast<const V*>(A.fop_train));
+    load_ops<T, kFopTiles, 256>(s_inf, static_cast<const V*>(A.fop_tail));
+    load_ops<T, kBopTiles, 256>(s_bop, static_cast<const V*>(A.bop_train));
+    __syncthreads();
 
-    V x[1][2];
-    floatx4 out[1];
-    encode<T>(R[4], R[5], R[6], R[7], A.b, g, x[0][0], x[0][1]);
-    mlp_forward<T, 1>(s_inf, lane, x, out);
+    // tail inference (EMA weights) and head forward (training weights, layer
+    // inputs kept for the backward pass), one layer of each at a time: the
+    // two chains are independent and fill each other's latencies
+    V xt[1][2], x[1][2];
+    encode<T>(rec[4], rec[5], rec[6], rec[7], A.b, g, xt[0][0], xt[0][1]);
+    encode<T>(rec[0], rec[1], rec[2], rec[3], A.b, g, x[0][0], x[0][1]);
+    V xs[kNrcLayers][2];
+#pragma unroll
+    for (int l = 0; l < kNrcHidden; ++l) {
+        xs[l][0] = x[0][0];
+        xs[l][1] = x[0][1];
+        mlp_layer<T, 1>(s_inf, lane, l, xt);
+        mlp_layer<T, 1>(s_fop, lane, l, x);
+    }
+    xs[5][0] = x[0][0];
+    xs[5][1] = x[0][1];
+    floatx4 out_t[1], out[1];
+    mlp_output<T, 1>(s_inf, lane, xt, out_t);
+    mlp_output<T, 1>(s_fop, lane, x, out);
     float tgt[3] = {0.f, 0.f, 0.f};
     if (g == 0) {
-        const float4 ls = R[8], ts = R[9], nm = R[10];
+        const float4 ls = rec[8], ts = rec[9], nm = rec[10];
         const float lsv[3] = {ls.x, ls.y, ls.z}, tsv[3] = {ts.x, ts.y, ts.z}, nmv[3] = {nm.x, nm.y, nm.z};
         for (int c = 0; c < 3; ++c) {
-            float t = (lsv[c] + tsv[c] * sanitize_out(out[0][c])) / fmaxf(nmv[c], 1e-8f);
+            float t = (lsv[c] + tsv[c] * sanitize_out(out_t[0][c])) / fmaxf(nmv[c], 1e-8f);
             t = fminf(t, 100000.0f);
             if (isnan(t) || isinf(t)) t = 0.0f;
             tgt[c] = t;
         }
     }
-    encode<T>(R[0], R[1], R[2], R[3], A.b, g, x[0][0], x[0][1]);
-    V xs[kNrcLayers][2];
-    mlp_forward_keep<T>(s_fop, lane, x, out, xs);
     floatx4 dy5 = floatx4{0.f, 0.f, 0.f, 0.f};
     float loss = 0.0f;
     if (g == 0) {   // RelativeL2Luminance: (p - t)^2 / (lum(p)^2 + 0.01), lum held constant
@@ -295,12 +325,20 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
     if (lane == 0) atomicAdd(A.loss_accum, loss * A.loss_norm);
 
     float* part = A.partial + (size_t)blockIdx.x * kNrcParams;
-    // ---- layer 5 (16 x 64) ----
-    for (int k = threadIdx.x; k < 16 * kTS; k += 256) s_dyT[k] = (T)0.0f;   // rows 4..15 stay zero
-    __syncthreads();
+    // ---- layer 5 (16 x 64): every row of the 16 is written (rows 3..15 zero) ----
     for (int ii = 0; ii < 4; ++ii) s_dyT[(4 * g + ii) * kTS + m] = (T)dy5[ii];
     for (int c = 0; c < 2; ++c)
         for (int j = 0; j < 8; ++j) s_xT[nrc_perm(c, g, j) * kTS + m] = xs[5][c][j];
+    floatx4 dy[4];
+    {   // input gradient of layer 5 (registers only), before the barrier
+        V bdy;
+        for (int j = 0; j < 4; ++j) { bdy[j] = (T)dy5[j]; bdy[4 + j] = (T)0.0f; }
+        for (int u = 0; u < 4; ++u) {
+            dy[u] = M::mfma(s_bop[(bop_tile_base(5) + u) * 64 + lane], bdy, floatx4{0.f, 0.f, 0.f, 0.f});
+            for (int ii = 0; ii < 4; ++ii)
+                if (!((float)xs[5][u >> 1][(u & 1) * 4 + ii] > 0.0f)) dy[u][ii] = 0.0f;
+        }
+    }
     __syncthreads();
     {
         const int u = w;   // k-tile of this wave
@@ -312,16 +350,6 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
         }
         for (int ii = 0; ii < 4; ++ii) part[nrc_layer_offset(5) + (4 * g + ii) * 64 + 16 * u + ml] = acc[ii];
     }
-    floatx4 dy[4];
-    {
-        V bdy;
-        for (int j = 0; j < 4; ++j) { bdy[j] = (T)dy5[j]; bdy[4 + j] = (T)0.0f; }
-        for (int u = 0; u < 4; ++u) {
-            dy[u] = M::mfma(s_bop[(bop_tile_base(5) + u) * 64 + lane], bdy, floatx4{0.f, 0.f, 0.f, 0.f});
-            for (int ii = 0; ii < 4; ++ii)
-                if (!((float)xs[5][u >> 1][(u & 1) * 4 + ii] > 0.0f)) dy[u][ii] = 0.0f;
-        }
-    }
     __syncthreads();
     // ---- layers 4..0 (unrolled: the saved operands stay in registers) ----
 #pragma unroll
@@ -330,17 +358,8 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
             for (int ii = 0; ii < 4; ++ii) s_dyT[(16 * t + 4 * g + ii) * kTS + m] = (T)dy[t][ii];
         for (int c = 0; c < 2; ++c)
             for (int j = 0; j < 8; ++j) s_xT[nrc_perm(c, g, j) * kTS + m] = xs[l][c][j];
-        __syncthreads();
-        for (int u = 0; u < 4; ++u) {
-            floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
-            for (int c = 0; c < 2; ++c) {
-                const V a = *reinterpret_cast<const V*>(&s_dyT[(16 * w + ml) * kTS + 32 * c + 8 * g]);
-                const V bb = *reinterpret_cast<const V*>(&s_xT[(16 * u + ml) * kTS + 32 * c + 8 * g]);
-                acc = M::mfma(a, bb, acc);
-            }
-            for (int ii = 0; ii < 4; ++ii) part[nrc_layer_offset(l) + (16 * w + 4 * g + ii) * 64 + 16 * u + ml] = acc[ii];
-        }
-        if (l > 0) {
+        floatx4 ndy[4];
+        if (l > 0) {   // input gradient of layer l (registers only), before the barrier
             V b0, b1;
             for (int ii = 0; ii < 4; ++ii) {
                 b0[ii] = (T)dy[0][ii];
@@ -354,9 +373,21 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
                 acc = M::mfma(s_bop[(bop_tile_base(l) + 2 * u + 1) * 64 + lane], b1, acc);
                 for (int ii = 0; ii < 4; ++ii)
                     if (!((float)xs[l][u >> 1][(u & 1) * 4 + ii] > 0.0f)) acc[ii] = 0.0f;
-                dy[u] = acc;
+                ndy[u] = acc;
             }
         }
+        __syncthreads();
+        for (int u = 0; u < 4; ++u) {
+            floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+            for (int c = 0; c < 2; ++c) {
+                const V a = *reinterpret_cast<const V*>(&s_dyT[(16 * w + ml) * kTS + 32 * c + 8 * g]);
+                const V bb = *reinterpret_cast<const V*>(&s_xT[(16 * u + ml) * kTS + 32 * c + 8 * g]);
+                acc = M::mfma(a, bb, acc);
+            }
+            for (int ii = 0; ii < 4; ++ii) part[nrc_layer_offset(l) + (16 * w + 4 * g + ii) * 64 + 16 * u + ml] = acc[ii];
+        }
+        if (l > 0)
+            for (int u = 0; u < 4; ++u) dy[u] = ndy[u];
         __syncthreads();
     }
 }
@@ -398,57 +429,59 @@ __device__ __forceinline__ void put16(void* base, size_t pos, float v) { static_
 
 // Adam (Kingma & Ba 2015) on the FP32 master weights, EMA of the weights for
 // inference (mlp.cpp:61-69: EMA(0.99) over Adam(lr 1e-2)), then the 16-bit
-// operand layouts.  Skipped entirely when the step was flagged.
+// operand layouts; no parameter changes when the step was flagged.  The
+// last block to finish (a done counter) closes the step: it advances the
+// batch seed (mlp.cpp:102 step_counter++ on every call), counts the step as
+// taken or skipped, keeps its loss, and clears the flags and the loss sum
+// for the next step, so a step is three launches (train, reduce, Adam).
 template <typename T>
 __global__ __launch_bounds__(256) void k_nrc_adam(float* __restrict__ master, float* __restrict__ am,
                                                   float* __restrict__ av, float* __restrict__ ema,
-                                                  const float* __restrict__ grad, const uint32_t* state, Hyper h,
-                                                  void* fop_train, void* bop_train, void* fop_infer) {
+                                                  const float* __restrict__ grad, uint32_t* state, float* loss_accum,
+                                                  Hyper h, void* fop_train, void* bop_train, void* fop_infer) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kNrcParams || state[kStFlags] != 0u) return;
-    const float t = (float)(state[kStLt] + 1u);
-    const float g = grad[p];
-    const float m = h.beta1 * am[p] + (1.0f - h.beta1) * g;
-    const float v = h.beta2 * av[p] + (1.0f - h.beta2) * g * g;
-    const float mh = m / (1.0f - powf(h.beta1, t));
-    const float vh = v / (1.0f - powf(h.beta2, t));
-    const float w = master[p] - h.lr * mh / (sqrtf(vh) + h.eps);
-    const float e = h.ema_decay * ema[p] + (1.0f - h.ema_decay) * w;
-    am[p] = m;
-    av[p] = v;
-    master[p] = w;
-    ema[p] = e;
-    put16<T>(fop_train, fop_pos(p), w);
-    if (param_layer(p) >= 1) put16<T>(bop_train, bop_pos(p), w);
-    put16<T>(fop_infer, fop_pos(p), e);
+    const uint32_t flags = state[kStFlags];
+    if (p < kNrcParams && flags == 0u) {
+        const float t = (float)(state[kStLt] + 1u);
+        const float g = grad[p];
+        const float m = h.beta1 * am[p] + (1.0f - h.beta1) * g;
+        const float v = h.beta2 * av[p] + (1.0f - h.beta2) * g * g;
+        const float mh = m / (1.0f - powf(h.beta1, t));
+        const float vh = v / (1.0f - powf(h.beta2, t));
+        const float w = master[p] - h.lr * mh / (sqrtf(vh) + h.eps);
+        const float e = h.ema_decay * ema[p] + (1.0f - h.ema_decay) * w;
+        am[p] = m;
+        av[p] = v;
+        master[p] = w;
+        ema[p] = e;
+        put16<T>(fop_train, fop_pos(p), w);
+        if (param_layer(p) >= 1) put16<T>(bop_train, bop_pos(p), w);
+        put16<T>(fop_infer, fop_pos(p), e);
+    }
+    __syncthreads();   // every read of the state in this block is done
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&state[kStDone], 1u) == gridDim.x - 1u) {
+            state[kStSeed] += 1u;
+            if (flags == 0u) {
+                state[kStLt] += 1u;
+                state[kStSteps] += 1u;
+                state[kStLoss] = __float_as_uint(*loss_accum);
+            } else if (flags & 1u) {
+                state[kStSkipNan] += 1u;
+            } else {
+                state[kStSkipData] += 1u;
+            }
+            state[kStFlags] = 0u;
+            *loss_accum = 0.0f;
+            state[kStDone] = 0u;
+        }
+    }
 }
 
 // fop_tail = fop_infer (16-byte entries), once per call, before its steps
 __global__ __launch_bounds__(256) void k_nrc_snapshot(const uint4* __restrict__ src, uint4* __restrict__ dst, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
-}
-
-__global__ void k_nrc_step_begin(uint32_t* state, float* loss_accum) {
-    if (threadIdx.x == 0) {
-        state[kStFlags] = 0u;
-        *loss_accum = 0.0f;
-    }
-}
-
-__global__ void k_nrc_step_end(uint32_t* state, const float* loss_accum) {
-    if (threadIdx.x == 0) {
-        const uint32_t f = state[kStFlags];
-        state[kStSeed] += 1u;   // mlp.cpp:102 step_counter++ on every call
-        if (f == 0u) {
-            state[kStLt] += 1u;
-            state[kStSteps] += 1u;
-            state[kStLoss] = __float_as_uint(*loss_accum);
-        } else if (f & 1u) {
-            state[kStSkipNan] += 1u;
-        } else {
-            state[kStSkipData] += 1u;
-        }
-    }
 }
 
 template <typename T>
@@ -509,13 +542,11 @@ int enqueue_train_steps(tfg_nrc* nc, const float4* records, const uint32_t* n_re
                            static_cast<const uint4*>(nc->fop_infer), static_cast<uint4*>(nc->fop_tail), op_entries);
     for (uint32_t k = 0; k < steps; ++k) {
         if (gathered) A.records = records + (size_t)k * nc->cfg.batch * kRecF4;
-        hipLaunchKernelGGL(k_nrc_step_begin, dim3(1), dim3(64), 0, s, nc->state, nc->loss_accum);
         hipLaunchKernelGGL(k_nrc_train<T>, dim3(nblk), dim3(256), kTrainLds, s, A);
         hipLaunchKernelGGL(k_nrc_grad_reduce, dim3((kNrcParams + kReduceParams - 1) / kReduceParams), dim3(256), 0,
                            s, nc->partial, nblk, nc->grad, nc->state, n_rec, min_records, h.inv_loss_scale);
         hipLaunchKernelGGL(k_nrc_adam<T>, dim3(pblocks), dim3(256), 0, s, nc->master, nc->adam_m, nc->adam_v, nc->ema,
-                           nc->grad, nc->state, h, nc->fop_train, nc->bop_train, nc->fop_infer);
-        hipLaunchKernelGGL(k_nrc_step_end, dim3(1), dim3(64), 0, s, nc->state, nc->loss_accum);
+                           nc->grad, nc->state, nc->loss_accum, h, nc->fop_train, nc->bop_train, nc->fop_infer);
     }
     TFG_HIP_CHECK(hipGetLastError());
     return TFG_OK;
@@ -744,6 +775,7 @@ int tfg_nrc_set_weights(tfg_nrc* nc, const float* master, const float* ema) {
     TFG_HIP_CHECK(hipMemset(nc->bop_train, 0, (size_t)kBopTiles * 64 * kOpBytes16));
     TFG_HIP_CHECK(hipMemset(nc->fop_infer, 0, (size_t)kFopTiles * 64 * kOpBytes16));
     TFG_HIP_CHECK(hipMemset(nc->state, 0, 16 * sizeof(uint32_t)));
+    TFG_HIP_CHECK(hipMemset(nc->loss_accum, 0, sizeof(float)));   // the steps' loss sum starts cleared
     const int pblocks = (kNrcParams + 255) / 256;
     if (nc->cfg.precision == TFG_NRC_FP16)
         hipLaunchKernelGGL(k_nrc_publish<_Float16>, dim3(pblocks), dim3(256), 0, nullptr, nc->master, nc->ema,
