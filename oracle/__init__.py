@@ -62,6 +62,8 @@ _SIGS = {
                               C.c_void_p, C.c_int]),
     "orc_render_pixel": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                 C.c_void_p]),
+    "orc_render_pixels_perturbed": (None, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_uint32, C.c_int, C.c_int,
+                                           C.c_int, C.c_double, C.c_uint32, C.c_void_p, C.c_int]),
 }
 
 
@@ -212,6 +214,16 @@ class Renderer:
                              train_frac, _p(out), nthreads)
         return {"pixel": out[:, 0:3], "flags": out[:, 3].view(np.uint32), "head": out[:, 4:20],
                 "tail": out[:, 20:36], "lsuf": out[:, 36:39], "tsuf": out[:, 39:42], "norm": out[:, 42:45]}
+
+    def pixels_perturbed(self, xy, frame, max_depth, rr_depth, delta, seed, nthreads=os.cpu_count() or 1):
+        """Paths of pixels xy ((n, 2) x, y) with every ray displaced by `delta`
+        (relative) before its closest-hit query: a conditioning probe."""
+        xy = np.ascontiguousarray(xy, np.int32).reshape(-1, 2)
+        out = np.zeros((xy.shape[0], 3), np.float32)
+        if xy.shape[0]:
+            lib().orc_render_pixels_perturbed(C.byref(self.desc), self.width, self.height, _p(xy), xy.shape[0], frame,
+                                              max_depth, rr_depth, delta, seed, _p(out), nthreads)
+        return out
 
     def pixel(self, x, y, frame, max_depth, rr_depth):
         out = np.zeros(3, np.float32)

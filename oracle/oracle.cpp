@@ -18,6 +18,11 @@
 //                     decoded boxes contain their triangles.
 //   BVH2 + MT         equals FP64-vector brute force MT; agrees with the
 //                     watertight path except on edge/tie rays.
+//   conditioning      orc_render_pixels_perturbed at delta 0 equals the
+//   probe             integrator bit for bit; well-conditioned pixels of a
+//                     closed-form scene stay within tolerance under it; a
+//                     planted systematic error is not classed as
+//                     ill-conditioned (tests/test_oracle_shading_cpu.py).
 //   shading           cosine draw (exact map, moments 2/3 and 1/2); GGX D
 //                     normalised, Smith G1 = its tan form, visible normals
 //                     integrate to one, the visible-normal draw matches its
@@ -870,8 +875,29 @@ Col o_direct(const orc_scene* sc, D3 ps, D3 n, D3 wo, const OMat& m, uint64_t& n
     return L;
 }
 
+// Conditioning probe (test infrastructure): with delta > 0, before every
+// closest-hit query the ray origin moves by delta * (1 + |o|) and the
+// direction by delta (then renormalised), each along a vector of U[-1, 1]
+// components from a SplitMix64 stream (Steele, Lea & Flood 2014) that is
+// separate from the path's PCG32 draws.  A path whose result changes under
+// such a displacement is ill-conditioned: rounding at the level of FP32
+// arithmetic can change its discrete decisions (triangle, occlusion,
+// Russian roulette).
+struct Perturb {
+    double delta = 0.0;
+    uint64_t state = 0;
+};
+double splitmix_unit(uint64_t& s) {   // U[-1, 1)
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
 Col o_path(const orc_scene* sc, int W, int H, int x, int y, int frame, int max_depth, int rr_depth,
-           uint64_t& next, uint64_t& nshadow) {
+           uint64_t& next, uint64_t& nshadow, Perturb* pert = nullptr) {
     uint32_t s = seed_of(x, y, frame);
     const float ox = pcg32_float(s);
     const float oy = pcg32_float(s);
@@ -886,6 +912,13 @@ Col o_path(const orc_scene* sc, int W, int H, int x, int y, int frame, int max_d
     Col color = {0, 0, 0}, camino = {1, 1, 1};
     double ior = 1.0;
     for (int depth = 0; depth < max_depth; ++depth) {
+        if (pert && pert->delta > 0.0) {
+            const double so = pert->delta * (1.0 + std::sqrt(ddot(o, o)));
+            const double ux = splitmix_unit(pert->state), uy = splitmix_unit(pert->state), uz = splitmix_unit(pert->state);
+            o = o + d3(ux, uy, uz) * so;
+            const double vx = splitmix_unit(pert->state), vy = splitmix_unit(pert->state), vz = splitmix_unit(pert->state);
+            d = dnorm(d + d3(vx, vy, vz) * pert->delta);
+        }
         tfg_hit h;
         ++next;
         if (!o_closest(sc, o, d, h)) break;   // black background, render.h:227-239
@@ -1007,6 +1040,23 @@ void orc_render_pixel(const orc_scene* sc, int width, int height, int x, int y, 
     out_rgb[0] = c.r;
     out_rgb[1] = c.g;
     out_rgb[2] = c.b;
+}
+
+// The paths of pixels xy[2i], xy[2i + 1] of one frame under the conditioning
+// probe of magnitude delta (stream seed: pseed and i).
+void orc_render_pixels_perturbed(const orc_scene* sc, int width, int height, const int32_t* xy, uint32_t n, int frame,
+                                 int max_depth, int rr_depth, double delta, uint32_t pseed, float* out_rgb,
+                                 int nthreads) {
+    parallel_for(n, nthreads, [&](uint32_t i) {
+        uint64_t e = 0, sh = 0;
+        Perturb p;
+        p.delta = delta;
+        p.state = ((uint64_t)pseed << 32) ^ (uint64_t)i;
+        const Col c = o_path(sc, width, height, xy[2 * i], xy[2 * i + 1], frame, max_depth, rr_depth, e, sh, &p);
+        out_rgb[3 * (size_t)i] = c.r;
+        out_rgb[3 * (size_t)i + 1] = c.g;
+        out_rgb[3 * (size_t)i + 2] = c.b;
+    });
 }
 
 }  // extern "C"

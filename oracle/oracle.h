@@ -101,6 +101,11 @@ void orc_render_nrc(const orc_scene* sc, int width, int height, int frame, int m
 /* One path of pixel (x, y) at `frame`; out_rgb[3]. */
 void orc_render_pixel(const orc_scene* sc, int width, int height, int x, int y, int frame, int max_depth,
                       int rr_depth, float* out_rgb);
+// Conditioning probe: the paths of n pixels (xy: x, y pairs) of one frame,
+// each ray displaced by delta before its closest-hit query (test use).
+void orc_render_pixels_perturbed(const orc_scene* sc, int width, int height, const int32_t* xy, uint32_t n, int frame,
+                                 int max_depth, int rr_depth, double delta, uint32_t pseed, float* out_rgb,
+                                 int nthreads);
 
 /* Shading functions of the integrator above, for the closed-form tests:
  * cosine-weighted direction about +z (render.h:383-390); GGX D, Smith G1

@@ -8,6 +8,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
+import conditioning
 import oracle
 from tfg_amd import _lib, scenes
 
@@ -90,10 +91,14 @@ def test_render_tiny_dropin(torch_cuda):
     g = img.cpu().numpy()
     assert (thr.cpu().numpy() == 0).all()            # render.h:457
     assert counter.item() == 0
-    ref = oracle.Renderer(sc, host.nodes, host.tris, W, H).render(4, 1, 8, 5)[0]
-    ok = np.all(np.abs(g - ref) <= 1e-3 * (1 + np.abs(ref)), axis=1)
-    print("plain agreement", ok.mean(), flush=True)
+    R = oracle.Renderer(sc, host.nodes, host.tris, W, H)
+    ref = R.render(4, 1, 8, 5)[0]
+    ok = conditioning.agree(g, ref)
+    n_bad, n_expl, base = conditioning.explain(R, g, ref, 4, 8, 5)
+    print(f"plain agreement {ok.mean()}; {n_bad} disagreeing paths, {n_expl} ill-conditioned in the oracle "
+          f"(base rate {base:.4f})", flush=True)
     assert ok.mean() >= 0.95
+    assert n_expl >= 0.9 * n_bad
     # cache frame: 5% training paths, the rest queries
     _call(torch, d, dbvh, W, H, 5, True, True, img, inf, thr, recs, counter)
     n_rec = counter.item()

@@ -1,11 +1,12 @@
-"""Seeded views of the 64-instance scene (2,567,554 triangles, one CWBVH) at
+"""Seeded views of the 64-instance scene (2,567,564 triangles, one CWBVH) at
 resolutions from 256x256 to 1920x1080.
 
 Sixteen views, each traced by the plain wavefront path tracer, are checked
 against the CPU oracle integrator: per view the whole image is finite, and on
 a band of 4 rows across the middle of the image at least 97% of the pixels
 follow the oracle's path (same PCG32 streams, rng.h) to 1e-3 relative; over
-all views at least 98%.
+all views at least 98%, and at least 90% of the paths that disagree are
+ill-conditioned in the oracle (tests/conditioning.py).
 
 Several views traced as one canvas (tfg_pt_set_views: one chain of launches
 for all of them) give each view the image its own tracer gives, bit for bit.
@@ -15,6 +16,7 @@ import dataclasses
 import numpy as np
 import pytest
 
+import conditioning
 import oracle
 from tfg_amd import pt, scenes
 
@@ -31,7 +33,7 @@ def test_sixteen_views_match_oracle(torch_cuda, scene64):
     torch = torch_cuda
     sc, ds = scene64
     md, rr = 8, 3
-    agree_all = []
+    agree_all, n_bad, n_expl = [], 0, 0
     for k, (cam, W, H) in enumerate(scenes.views()):
         print(f"view={k} {W}x{H} pos={np.round(cam.pos, 2).tolist()}", flush=True)
         tracer = pt.PathTracer(W, H, md, rr, mode=0, use_graph=True)
@@ -51,11 +53,15 @@ def test_sixteen_views_match_oracle(torch_cuda, scene64):
         c, _ = ref.render_rows(y0, y0 + 4, k, 1, md, rr)
         c = c.reshape(4, W, 3)
         band = g[y0:y0 + 4]
-        ok = np.all(np.abs(band - c) <= 1e-3 * (1.0 + np.abs(c)), axis=2)
-        print(f"  agreement {ok.mean():.4f}", flush=True)
+        ok = conditioning.agree(band.reshape(-1, 3), c.reshape(-1, 3))
+        bad = np.nonzero(~ok)[0]
+        ill = conditioning.ill_conditioned(ref, y0 * W + bad, c.reshape(-1, 3)[bad], k, md, rr)
+        n_bad, n_expl = n_bad + bad.size, n_expl + int(ill.sum())
+        print(f"  agreement {ok.mean():.4f}, {bad.size} disagreeing, {int(ill.sum())} ill-conditioned", flush=True)
         assert ok.mean() >= 0.97
         agree_all.append(ok.ravel())
     assert np.concatenate(agree_all).mean() >= 0.98
+    assert n_expl >= 0.9 * n_bad   # rounding-driven decisions, not a systematic error
 
 
 def test_canvas_views_match_single_view_tracers(torch_cuda, scene64):

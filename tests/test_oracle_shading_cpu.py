@@ -25,6 +25,7 @@ import pytest
 from scipy import integrate
 
 import closed_form_scenes
+import conditioning
 import oracle
 from tfg_amd import _lib, scenes
 
@@ -221,3 +222,53 @@ def test_cache_capture_between_two_planes():
             continue
         assert tl.sum() > 0.5 * case.W * case.H and not (tl & ~sel).any()
         closed_form_scenes.check_tails(out["tail"][tl], out["tsuf"][tl], case.max_depth)
+
+
+def test_conditioning_probe():
+    """The oracle's conditioning probe (Renderer.pixels_perturbed), which the
+    per-path GPU tests use to show that their disagreeing paths are
+    rounding-driven:
+      * at delta 0 it reproduces the integrator bit for bit (it draws from
+        its own stream, never from the path's PCG32 stream);
+      * on the point-light scene every pixel whose camera ray and shadow ray
+        pass farther than 1% of the occluder's radius from its silhouette is
+        well-conditioned at the tests' delta;
+      * a planted systematic error (1% too bright on 3% of the pixels of a
+        Cornell box frame, as a wrong branch would give) is reported as
+        ill-conditioned at about the base rate, far below the 90% bar."""
+    case = closed_form_scenes.point_light()
+    host = _lib.CwbvhHost(case.scene.verts, _lib.TFG_TRI_WATERTIGHT)
+    W, H = case.W, case.H
+    ren = oracle.Renderer(case.scene, host.nodes, host.tris, W, H)
+    img, _ = ren.render(case.frame, 1, case.max_depth, case.rr_depth)
+    ys, xs = np.mgrid[0:H, 0:W]
+    xy = np.stack([xs.ravel(), ys.ravel()], 1)
+    p0 = ren.pixels_perturbed(xy, case.frame, case.max_depth, case.rr_depth, 0.0, 5)
+    assert np.array_equal(p0, img.astype(np.float32))
+    ill = conditioning.ill_conditioned(ren, np.arange(W * H), img, case.frame, case.max_depth, case.rr_depth)
+    o, d = closed_form_scenes.primary(case)
+    oc, orad, lp = np.array([0.6, 0.35, 0.2]), 0.15, np.array([0.3, 0.7, -0.2])
+    seen_d = np.linalg.norm(o + ((oc - o) @ d.T)[:, None] * d - oc, axis=1)
+    ps = o + d * (-o[1] / d[:, 1])[:, None] + np.array([0, 1e-3, 0])
+    dl = lp - ps
+    dist = np.linalg.norm(dl, axis=1)
+    dl /= dist[:, None]
+    tl = np.clip(np.einsum("ij,ij->i", oc - ps, dl), 0, dist)
+    shadow_d = np.linalg.norm(ps + tl[:, None] * dl - oc, axis=1)
+    far = (np.abs(seen_d - orad) > 0.01 * orad) & (np.abs(shadow_d - orad) > 0.01 * orad)
+    assert far.sum() > 0.9 * W * H
+    assert not ill[far].any()
+
+    sc = scenes.cornell(level=3)
+    host = _lib.CwbvhHost(sc.verts, _lib.TFG_TRI_WATERTIGHT)
+    W, H, md, rr = 64, 48, 5, 2
+    ren = oracle.Renderer(sc, host.nodes, host.tris, W, H)
+    ref, _ = ren.render(2, 1, md, rr)
+    got = ref.copy()
+    rng = np.random.default_rng(1)
+    lit = np.nonzero(ref.max(1) > 0.15)[0]   # where 1% exceeds the tolerance
+    planted = rng.choice(lit, size=int(0.03 * W * H), replace=False)
+    got[planted] *= 1.01
+    n_bad, n_expl, base = conditioning.explain(ren, got, ref, 2, md, rr)
+    assert n_bad == planted.size
+    assert n_expl < 0.3 * n_bad and base < 0.3

@@ -7,11 +7,17 @@ path: the per-path bar is that >= 98% of pixels agree within 1e-3 relative
 (the rest are paths that graze an edge and diverge).  The converged-image
 bar uses independent seeds: the per-block means of GPU and oracle images
 differ by less than 4.5 standard errors, and the image means by < 1%.
+The paths that disagree are checked one by one with the oracle's
+conditioning probe (tests/conditioning.py): >= 90% of them must be
+ill-conditioned, i.e. their own FP64 result moves beyond the tolerance when
+every ray is displaced by 1e-5 relative, which a systematic error (a wrong
+branch taken by some fraction of paths) would not satisfy.
 """
 import numpy as np
 import pytest
 
 import closed_form_scenes
+import conditioning
 import oracle
 from tfg_amd import pt, scenes
 
@@ -42,18 +48,24 @@ def test_pt_per_path_matches_oracle(torch_cuda, name):
     print(f"case={name} tris={sc.n_tris}", flush=True)
     ds, tracer, img = _setup(torch, sc, W, H, md, rr)
     ref = oracle.Renderer(sc, ds.host.nodes, ds.host.tris, W, H)
-    agree = []
+    agree, n_bad, n_expl, base = [], 0, 0, []
     for frame in range(3):
         tracer.render(frame, img)
         torch.cuda.synchronize()
         g = img.cpu().numpy().astype(np.float64)
         c, rays = ref.render(frame, 1, md, rr)
-        ok = np.all(np.abs(g - c) <= 1e-3 * (1.0 + np.abs(c)), axis=1)
+        ok = conditioning.agree(g, c)
         agree.append(ok.mean())
+        b, e, r = conditioning.explain(ref, g, c, frame, md, rr, seed=frame)
+        n_bad, n_expl = n_bad + b, n_expl + e
+        base.append(r)
         ext, shd = tracer.rays_last_frame()
         assert ext > W * H
-    print("agreement", agree, flush=True)
+    print(f"agreement {agree}; {n_bad} disagreeing paths, {n_expl} ill-conditioned in the oracle "
+          f"(base rate {np.mean(base):.4f})", flush=True)
     assert min(agree) >= 0.98
+    # the disagreements are rounding-driven decisions, not a systematic error
+    assert n_expl >= 0.9 * n_bad
     assert img.sum().item() > 0
 
 
