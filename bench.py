@@ -228,7 +228,8 @@ def bench_pt(args, torch):
     W, H, MD, RR = 1920, 1080, 8, 3
     img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
     accum = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
-    # direct launches: faster than graph replay on this workload
+    # direct launches: faster than per-frame graph replay in the overlapped
+    # schedule (the A/B below measures both)
     tracer = pt.PathTracer(W, H, MD, RR, mode=0, use_graph=False)
     tracer.set_scene(ds)
     tracer.set_camera(sc.camera.frame(W, H))
@@ -260,6 +261,10 @@ def bench_pt(args, torch):
     rays_frame = (ext + shd) / args.steps
     value = rays_frame / (ms_frame * 1e-3) / 1e6
     _log(f"timed {args.steps} frames: {ms_frame:.4f} ms per frame, {value:.1f} Mrays/s")
+    # the same frames replayed as per-frame hipGraphs (tfg_pt_render with the
+    # graph option), against direct launches, in the timed schedule and with
+    # the shadow stage on the main stream (no fork in the graph)
+    graph_ab = _graph_vs_direct(torch, tracer, ds, sc, W, H, MD, RR, img, accum, stream, args.steps)
     # one profiled frame in the timed schedule (stages overlapped as timed)
     prof = tracer.profile_frame(1000, img)
     # roofline frames: the same schedule serialised (overlap off)
@@ -357,11 +362,46 @@ def bench_pt(args, torch):
             "cwbvh_refit_ms": round(refit_ms, 3),
             "seeds": {"scene": sc.seed, "frames": [1000, 1000 + args.steps - 1]},
         },
+        "graph_replay": graph_ab,
         "wall_s": wall,
         "dtype": "fp32",
         "_cpu_inputs": (sc, ds.host, W, H, MD, RR),
     }
     return res
+
+
+def _graph_vs_direct(torch, direct, ds, sc, W, H, MD, RR, img, accum, stream, steps):
+    """ms per frame of per-frame hipGraph replay and of direct launches over
+    the same `steps` frames, in the direct tracer's schedule (overlap on) and
+    with the shadow stage on the main stream (overlap off)."""
+    from tfg_amd import pt
+
+    graph = pt.PathTracer(W, H, MD, RR, mode=0, use_graph=True)
+    graph.set_scene(ds)
+    graph.set_camera(sc.camera.frame(W, H))
+    shape = direct.launch_shape()
+    graph.set_schedule(**{k: v for k, v in shape.items() if k in graph.schedule() and k != "use_graph"})
+    out = {}
+    for overlap in (True, False):
+        ms = {}
+        for name, tr in (("direct", direct), ("graph", graph)):
+            tr.set_schedule(overlap=overlap)
+            for f in range(3):
+                tr.render(f, img, accum, stream=stream.cuda_stream)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for k in range(steps):
+                tr.render(1000 + k, img, accum, stream=stream.cuda_stream)
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms[name] = a.elapsed_time(b) / steps
+        out["overlap_on" if overlap else "overlap_off"] = {
+            "direct_ms_per_frame": round(ms["direct"], 4), "graph_ms_per_frame": round(ms["graph"], 4),
+            "graph_over_direct": round(ms["graph"] / ms["direct"], 4)}
+    direct.set_schedule(overlap=shape["overlap"])
+    graph.close()
+    _log(f"graph replay vs direct: {out}")
+    return out
 
 
 def bench_pt1080(args, torch):
@@ -907,6 +947,8 @@ def main():
         "roofline": res["roofline"],
         "cpu_baseline": cpu,
     }
+    if "graph_replay" in res:
+        out["graph_replay"] = res["graph_replay"]
     if "relmse" in res:
         out["relmse"] = res["relmse"]
     if "nrc_kernels" in res:
