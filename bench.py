@@ -542,6 +542,64 @@ def bench_copy(args, torch):
             "ms_per_step": ms, "bytes_read": 4 * n, "bytes_written": 4 * n, "wall_s": wall}
 
 
+def bench_c1(args, torch):
+    """C1: the 12-quad room with a box and the level-4 icosphere (5,144
+    triangles) and one point light, 256x256, 16 spp (16 frames, frame seed =
+    base seed + sample), max depth 5, Russian roulette after depth 10 (never
+    reached).  One step = one 16-spp image on the GPU; the CPU baseline runs
+    the oracle integrator over the whole 16-spp image on all host threads
+    and on one thread."""
+    import oracle
+    from tfg_amd import pt, scenes
+
+    sc = scenes.cornell(level=4)
+    ds = pt.DeviceScene(sc, torch)
+    W, H, MD, RR, SPP, BASE = 256, 256, 5, 10, 16, 500
+    tracer = pt.PathTracer(W, H, MD, RR, mode=0, use_graph=False)
+    tracer.set_scene(ds)
+    tracer.set_camera(sc.camera.frame(W, H))
+    img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
+    accum = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    for f in range(max(1, args.warmup)):
+        tracer.render(BASE + f, img, accum, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    tracer.reset_totals()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for k in range(args.steps):
+        for j in range(SPP):
+            tracer.render(BASE + j, img, accum, stream=stream.cuda_stream)
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms_image = a.elapsed_time(b) / args.steps
+    ext, shd = tracer.ray_totals()
+    rays_image = (ext + shd) / args.steps
+    tracer.close()
+    cpu = None
+    if not args.no_cpu:
+        cores = _host_cores()
+        R = oracle.Renderer(sc, ds.host.nodes, ds.host.tris, W, H)
+        t0 = time.perf_counter()
+        _, rays = R.render(BASE, SPP, MD, RR, nthreads=cores)
+        dt = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        _, rays1 = R.render(BASE, SPP, MD, RR, nthreads=1)
+        dt1 = time.perf_counter() - t1
+        cpu = {"value": float(rays.sum()) / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
+               "sample": f"the whole 16-spp image ({int(rays[0])} extension + {int(rays[1])} shadow rays) through "
+                         f"the oracle integrator on {cores} threads, {dt:.2f} s",
+               "single_thread": {"value": float(rays1.sum()) / dt1 / 1e6, "unit": "Mrays/s",
+                                 "sample": f"the same image on 1 thread, {dt1:.2f} s"}}
+    return {"metric": "C1 Mrays/s (Cornell room + L4 icosphere, 256x256, 16 spp, depth 5)",
+            "value": rays_image / (ms_image * 1e-3) / 1e6, "unit": "Mrays/s", "ms_per_step": ms_image,
+            "dtype": "fp32", "data": "synthetic (seeded procedural scene)", "cpu_baseline": cpu,
+            "config": {"workload": "C1: 12-quad room + box + level-4 icosphere (5,144 tris), 1 point light, "
+                                   "256x256, 16 spp per step (frame seeds 500..515), max depth 5, RR after 10; "
+                                   "plain FP32 wavefront PT, default schedule, direct launches",
+                       "rays_per_image": rays_image, "tris": sc.n_tris}}
+
+
 def _host_cores():
     """Threads for the CPU baseline: the cores this process may run on, capped
     by OMP_NUM_THREADS when that is set (the host's share for one GPU)."""
@@ -870,7 +928,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt1080", choices=["pt1080", "pt", "traversal", "copy", "nrc", "multiview", "sweep"])
+    ap.add_argument("--workload", default="pt1080", choices=["pt1080", "pt", "traversal", "copy", "nrc", "multiview", "sweep", "c1"])
     ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the relMSE comparison")
     ap.add_argument("--budget", type=float, default=2.0, help="equal-time budget in seconds per method")
     ap.add_argument("--repeats", type=int, default=5,
@@ -895,6 +953,12 @@ def main():
                  "xcd_group": 4096}
         res = bench_nrc_kernels(torch, scenes.instances(), sched, reps=args.steps)
         print(json.dumps({"metric": "cache kernel throughput (MFMA)", **res}), flush=True)
+        return
+    if args.workload == "c1":
+        res = bench_c1(args, torch)
+        res.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+                    "scaling": "weak", "vs_baseline": None})
+        print(json.dumps(res), flush=True)
         return
     if args.workload == "multiview":
         res = bench_multiview(args, torch)
