@@ -558,6 +558,8 @@ def bench_c1(args, torch):
     tracer = pt.PathTracer(W, H, MD, RR, mode=0, use_graph=False)
     tracer.set_scene(ds)
     tracer.set_camera(sc.camera.frame(W, H))
+    tracer.autotune()   # small frames: launch latency and the late depths' tail decide the schedule
+    shape = tracer.launch_shape()
     img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
     accum = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream()
@@ -596,8 +598,8 @@ def bench_c1(args, torch):
             "dtype": "fp32", "data": "synthetic (seeded procedural scene)", "cpu_baseline": cpu,
             "config": {"workload": "C1: 12-quad room + box + level-4 icosphere (5,144 tris), 1 point light, "
                                    "256x256, 16 spp per step (frame seeds 500..515), max depth 5, RR after 10; "
-                                   "plain FP32 wavefront PT, default schedule, direct launches",
-                       "rays_per_image": rays_image, "tris": sc.n_tris}}
+                                   "plain FP32 wavefront PT, schedule from tfg_pt_autotune in this run, direct launches",
+                       "rays_per_image": rays_image, "tris": sc.n_tris, "launch": shape}}
 
 
 def _host_cores():
