@@ -265,6 +265,8 @@ typedef struct tfg_pt_schedule {
     uint32_t top_nodes;
     uint32_t tail_from;
     uint32_t defer_training;
+    uint32_t shadow_sort_from;   /* 0: off; k: the shadow entries of depths >= k are traced in the order of a
+                                    12-bit Morton code of their point (a counting sort before each shadow stage) */
 } tfg_pt_schedule;
 int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out);
 int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* schedule);

@@ -178,6 +178,14 @@ void launch_connect(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t d_lo,
     }
 }
 
+// The counting sort of depth d's shadow entries (before its shadow stage).
+void launch_shadow_sort(hipStream_t s, const PtParams& T, uint32_t d) {
+    hipLaunchKernelGGL(k_sort_count, dim3(kSortBlocks), dim3(256), 0, s, T, d);
+    hipLaunchKernelGGL(k_sort_scan_bins, dim3(kSortBins / kSortTileBins), dim3(256), 0, s, T, d);
+    hipLaunchKernelGGL(k_sort_scan_totals, dim3(1), dim3(1024), 0, s, T, d);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(kSortBlocks), dim3(256), 0, s, T, d);
+}
+
 // Watertight triangle records (v0, v1, v2, primIdx in T3.w) for the affine
 // records of a tinybvh-layout tree: record k keeps its position (the nodes
 // address it as triBase + 4k) and its primIdx (T3.w), and takes that
@@ -225,6 +233,9 @@ void make_tiles(tfg_pt* pt) {
         T.qpath[1] = P.qpath[1] + b;
         T.hits = P.hits + b;
         T.shadow = P.shadow + 4 * (size_t)b;
+        T.skey = P.skey + b;
+        T.sperm = P.sperm + b;
+        T.sort_scr = P.sort_scr + kSortScr * P.max_depth * t;
         T.nee = P.nee + b;
         T.neemask = P.neemask + b;
         T.thr = P.thr + b;
@@ -340,6 +351,8 @@ int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, hipStream_t* side, tfg_color
             TFG_HIP_CHECK(hipStreamWaitEvent(cs, pt->ev_shaded[t][d], 0));
         }
         mark(pe ? pe->con[t][d_lo][0] : nullptr, cs);
+        if (T.sort_from)
+            for (uint32_t k = d_lo > T.sort_from ? d_lo : T.sort_from; k <= d; ++k) launch_shadow_sort(cs, T, k);
         if (stats) launch_connect<true>(pt, cs, T, d_lo, d);
         else launch_connect<false>(pt, cs, T, d_lo, d);
         mark(pe ? pe->con[t][d_lo][1] : nullptr, cs);
@@ -582,6 +595,9 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
     ALLOC(qpath[1], uint32_t, n)
     ALLOC(hits, float4, n)
     ALLOC(shadow, float4, 4 * (size_t)n * cfg->max_depth)
+    ALLOC(skey, uint16_t, (size_t)n * cfg->max_depth)
+    ALLOC(sperm, uint32_t, (size_t)n * cfg->max_depth)
+    ALLOC(sort_scr, uint32_t, kSortScr * cfg->max_depth * kMaxTiles)
     ALLOC(nee, float4, (size_t)n * cfg->max_depth)
     ALLOC(neemask, uint32_t, n)
     ALLOC(counters, uint32_t, (size_t)CNT_ALLOC * (1 + kMaxTiles))
@@ -957,6 +973,7 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     pt->overlap = true;
     pt->connect_batch = 0;
     pt->tail_from = 0;
+    pt->P.sort_from = 0;
     // 1. traversal launch shape and LDS-staged top nodes together (large
     //    multipliers give more workgroups than chunks: each then traces one
     //    chunk and retires, which pays only with cheap workgroup start-up)
@@ -983,6 +1000,7 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     const int cpcs[3] = {0, 2, 3};
     const uint32_t batches[6] = {0, 2, 3, 4, 5, 6};
     const uint32_t tails[6] = {0, 2, 3, 4, 5, 6};
+    const uint32_t sorts[3] = {0, 1, 2};
     for (int round = 0; round < 2 && rc == TFG_OK; ++round) {
         if (round > 0)
             sweep(4, [&](int i) {
@@ -999,6 +1017,7 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
         sweep(3, [&](int i) { pt->connect_per_cu = cpcs[i]; });
         sweep(6, [&](int i) { pt->connect_batch = batches[i]; });
         sweep(6, [&](int i) { pt->tail_from = tails[i]; });
+        sweep(3, [&](int i) { pt->P.sort_from = sorts[i]; });
     }
     drop_graph(pt);
     pt->P.mode = saved_mode;
@@ -1031,6 +1050,7 @@ int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out) {
     out->top_nodes = pt->top_nodes;
     out->tail_from = pt->tail_from;
     out->defer_training = pt->defer_train ? 1 : 0;
+    out->shadow_sort_from = pt->P.sort_from;
     return TFG_OK;
 }
 
@@ -1041,10 +1061,11 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
         sc->tiles > kMaxTiles || sc->side_streams == 0 || sc->side_streams > kMaxSide ||
         (sc->xcd_group != 0 && (sc->xcd_group % 256 != 0 || sc->xcd_group > (1u << 20))) ||
         sc->connect_per_cu > 64 || sc->connect_batch > kMaxDepth || sc->top_nodes > kTopNodes ||
-        sc->tail_from > kMaxDepth || sc->defer_training > 1) {
+        sc->tail_from > kMaxDepth || sc->defer_training > 1 || sc->shadow_sort_from > kMaxDepth) {
         set_error("tfg_pt_set_schedule: block 64/128/256, grid_mult 1..64, tiles 1..%u, side streams 1..%u, "
-                  "xcd_group 0 or a multiple of 256 up to 2^20, connect_per_cu 0..64, connect_batch and "
-                  "tail_from 0..%u, top_nodes 0..%u, defer_training 0/1", kMaxTiles, kMaxSide, kMaxDepth, kTopNodes);
+                  "xcd_group 0 or a multiple of 256 up to 2^20, connect_per_cu 0..64, connect_batch, tail_from and "
+                  "shadow_sort_from 0..%u, top_nodes 0..%u, defer_training 0/1", kMaxTiles, kMaxSide, kMaxDepth,
+                  kTopNodes);
         return TFG_E_INVALID;
     }
     if ((sc->use_graph != 0) != pt->run_graph) {
@@ -1072,6 +1093,7 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
     pt->connect_per_cu = (int)sc->connect_per_cu;
     pt->connect_batch = sc->connect_batch;
     pt->tail_from = sc->tail_from;
+    pt->P.sort_from = sc->shadow_sort_from;
     return TFG_OK;
 }
 

@@ -410,6 +410,29 @@ __device__ __forceinline__ void shade_entry(const PtParams& P, uint32_t depth, b
     }
 }
 
+// kSortBits-bit Morton code of p (bits 3b, 3b + 1, 3b + 2: bit b of the x, y,
+// z cell) in the root node's quantisation frame, where every scene point
+// lies in [origin, origin + 256 * 2^e) per axis; points outside clamp to the
+// border cells.
+__device__ __forceinline__ uint32_t shadow_sort_key(const PtParams& P, V3 p) {
+    constexpr int kAxisBits = (int)kSortBits / 3;
+    const float4 r = P.nodes[0];
+    const uint32_t w = __float_as_uint(r.w);
+    const float rel[3] = {p.x - r.x, p.y - r.y, p.z - r.z};
+    uint32_t c[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int e = (int)(int8_t)((w >> (8 * a)) & 0xFFu);
+        const float q = fminf(fmaxf(ldexpf(rel[a], -e), 0.0f), 255.0f);
+        c[a] = (uint32_t)q >> (8 - kAxisBits);
+    }
+    uint32_t key = 0;
+#pragma unroll
+    for (int b = 0; b < kAxisBits; ++b)
+        key |= (((c[0] >> b) & 1u) << (3 * b)) | (((c[1] >> b) & 1u) << (3 * b + 1)) | (((c[2] >> b) & 1u) << (3 * b + 2));
+    return key;
+}
+
 template <bool kNrc>
 __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t depth) {
     const uint32_t count = P.counters[CNT_QD + depth];
@@ -441,6 +464,8 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
             e[1] = e1;
             e[2] = e2;
             e[3] = e3;
+            if (P.sort_from && depth >= P.sort_from)
+                P.skey[(size_t)depth * P.depth_stride + ss] = (uint16_t)shadow_sort_key(P, xyz(e0));
             P.neemask[pid] |= 1u << depth;
         }
         if (want_ray) {
@@ -450,6 +475,126 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
         }
     }
     block_add<256>(facing, &P.gcounters[CNT_SHD + depth]);
+}
+
+// ------------------------------------------------------------ shadow sort
+// Counting sort of depth d's shadow entries by key (shade wrote the keys):
+// count (per workgroup share), exclusive scan over the workgroups per bin,
+// scan of the bin totals, scatter of the entry indices.  The permutation
+// changes only the order in which the shadow stage traces the entries:
+// every entry's result lands in nee[d][pid] as before.  Scratch of depth d
+// (kSortScr words): counts [kSortBlocks][kSortBins] (workgroup-major), bin
+// totals [kSortBins], bin bases [kSortBins].
+constexpr uint32_t kSortTileBins = 32;   // bins per workgroup of the per-bin scan
+
+__device__ __forceinline__ void sort_share(uint32_t count, uint32_t& lo, uint32_t& hi) {
+    const uint32_t per = (count + kSortBlocks - 1) / kSortBlocks;
+    lo = blockIdx.x * per;
+    hi = lo + per < count ? lo + per : count;   // empty when lo >= count
+}
+
+__global__ __launch_bounds__(256) void k_sort_count(PtParams P, uint32_t d) {
+    __shared__ uint32_t h[kSortBins];
+    uint32_t lo, hi;
+    sort_share(P.counters[CNT_SHQ + d], lo, hi);
+    for (uint32_t k = threadIdx.x; k < kSortBins; k += blockDim.x) h[k] = 0u;
+    __syncthreads();
+    const uint16_t* key = P.skey + (size_t)d * P.depth_stride;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&h[key[i]], 1u);
+    __syncthreads();
+    uint32_t* cnt = P.sort_scr + (size_t)d * kSortScr + (size_t)blockIdx.x * kSortBins;
+    for (uint32_t k = threadIdx.x; k < kSortBins; k += blockDim.x) cnt[k] = h[k];
+}
+
+// kSortTileBins bins per workgroup: their kSortBlocks x kSortTileBins counts
+// through LDS, each bin's column summed by 256 / kSortTileBins threads in
+// row ranges, then rewritten as exclusive prefixes over the workgroups; the
+// bin totals go to the totals array.
+__global__ __launch_bounds__(256) void k_sort_scan_bins(PtParams P, uint32_t d) {
+    constexpr uint32_t kParts = 256 / kSortTileBins, kRows = kSortBlocks / kParts;
+    __shared__ uint32_t tile[kSortBlocks][kSortTileBins + 1];
+    __shared__ uint32_t psum[kParts][kSortTileBins];
+    uint32_t* cnt = P.sort_scr + (size_t)d * kSortScr;
+    uint32_t* tot = cnt + (size_t)kSortBlocks * kSortBins;
+    const uint32_t bin0 = blockIdx.x * kSortTileBins;
+    for (uint32_t k = threadIdx.x; k < kSortBlocks * kSortTileBins; k += blockDim.x) {
+        const uint32_t r = k / kSortTileBins, c = k % kSortTileBins;
+        tile[r][c] = cnt[(size_t)r * kSortBins + bin0 + c];
+    }
+    __syncthreads();
+    const uint32_t c = threadIdx.x % kSortTileBins, part = threadIdx.x / kSortTileBins;
+    uint32_t s = 0;
+    for (uint32_t r = part * kRows; r < part * kRows + kRows; ++r) s += tile[r][c];
+    psum[part][c] = s;
+    __syncthreads();
+    uint32_t run = 0;
+    for (uint32_t k = 0; k < part; ++k) run += psum[k][c];
+    for (uint32_t r = part * kRows; r < part * kRows + kRows; ++r) {
+        const uint32_t v = tile[r][c];
+        tile[r][c] = run;
+        run += v;
+    }
+    if (part == kParts - 1) tot[bin0 + c] = run;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < kSortBlocks * kSortTileBins; k += blockDim.x) {
+        const uint32_t r = k / kSortTileBins, cc = k % kSortTileBins;
+        cnt[(size_t)r * kSortBins + bin0 + cc] = tile[r][cc];
+    }
+}
+
+// Exclusive scan of the kSortBins totals (one workgroup of 1024 threads,
+// kSortBins / 1024 bins per thread).
+__global__ __launch_bounds__(1024) void k_sort_scan_totals(PtParams P, uint32_t d) {
+    constexpr uint32_t kPer = kSortBins / 1024;
+    __shared__ uint32_t wsum[16];
+    const uint32_t* tot = P.sort_scr + (size_t)d * kSortScr + (size_t)kSortBlocks * kSortBins;
+    uint32_t* base = const_cast<uint32_t*>(tot) + kSortBins;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    uint32_t v[kPer], s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        v[k] = tot[kPer * t + k];
+        s += v[k];
+    }
+    uint32_t x = s;   // inclusive scan over the wave
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if ((int)lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        for (int w = 0; w < 16; ++w) {
+            const uint32_t c = wsum[w];
+            wsum[w] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    uint32_t run = wsum[wave] + x - s;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        base[kPer * t + k] = run;
+        run += v[k];
+    }
+}
+
+// Entry i of workgroup b's share goes to base[key] + (entries of that key in
+// the shares before b) + its rank among its share's entries of that key (LDS
+// atomics: the order within a key is arbitrary, which changes no result).
+__global__ __launch_bounds__(256) void k_sort_scatter(PtParams P, uint32_t d) {
+    __shared__ uint32_t cur[kSortBins];
+    uint32_t lo, hi;
+    sort_share(P.counters[CNT_SHQ + d], lo, hi);
+    const uint32_t* cnt = P.sort_scr + (size_t)d * kSortScr;
+    const uint32_t* base = cnt + (size_t)kSortBlocks * kSortBins + kSortBins;
+    for (uint32_t k = threadIdx.x; k < kSortBins; k += blockDim.x)
+        cur[k] = base[k] + cnt[(size_t)blockIdx.x * kSortBins + k];
+    __syncthreads();
+    const uint16_t* key = P.skey + (size_t)d * P.depth_stride;
+    uint32_t* perm = P.sperm + (size_t)d * P.depth_stride;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) perm[atomicAdd(&cur[key[i]], 1u)] = i;
 }
 
 // ---------------------------------------------------------------- connect
@@ -501,7 +646,9 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t d_lo, u
         V3 c = v3(0.f, 0.f, 0.f);
         float4 e0 = make_float4(0.f, 0.f, 0.f, 0.f), e3 = e0;
         if (w < work) {
-            const float4* e = entries + 4 * (size_t)i;
+            // the entry at position i of the key order (or of the queue order)
+            const uint32_t ei = P.sort_from && d >= P.sort_from ? P.sperm[(size_t)d * P.depth_stride + i] : i;
+            const float4* e = entries + 4 * (size_t)ei;
             e0 = e[0];
             e3 = e[3];
             if (l < P.n_lights) {
