@@ -266,7 +266,11 @@ typedef struct tfg_pt_schedule {
     uint32_t tail_from;
     uint32_t defer_training;
     uint32_t shadow_sort_from;   /* 0: off; k: the shadow entries of depths >= k are traced in the order of a
-                                    12-bit Morton code of their point (a counting sort before each shadow stage) */
+                                    12-bit Morton code of their point (a counting sort before each shadow stage),
+                                    at depths whose shadow queue holds at least 1/16 of the frame's (tile's) paths */
+    uint32_t ray_sort_from;      /* 0: off; k: the extension rays of depths >= k are traced in the order of their
+                                    direction's cube-map cell (6 x 4 x 4) and a coarse cell of their origin, at
+                                    depths whose queue holds at least 1/16 of the paths */
 } tfg_pt_schedule;
 int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out);
 int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* schedule);

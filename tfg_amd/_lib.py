@@ -67,7 +67,8 @@ class PtSchedule(C.Structure):
                 ("side_streams", C.c_uint32), ("overlap", C.c_int32), ("use_graph", C.c_int32),
                 ("xcd_group", C.c_uint32), ("octet_below", C.c_uint32), ("connect_per_cu", C.c_uint32),
                 ("connect_batch", C.c_uint32), ("top_nodes", C.c_uint32), ("tail_from", C.c_uint32),
-                ("defer_training", C.c_uint32), ("shadow_sort_from", C.c_uint32)]
+                ("defer_training", C.c_uint32), ("shadow_sort_from", C.c_uint32),
+                ("ray_sort_from", C.c_uint32)]
 
 
 class NrcConfig(C.Structure):

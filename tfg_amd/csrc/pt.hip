@@ -178,12 +178,23 @@ void launch_connect(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t d_lo,
     }
 }
 
+void launch_sort(hipStream_t s, const SortJob& J) {
+    hipLaunchKernelGGL(k_sort_count, dim3(kSortBlocks), dim3(256), 0, s, J);
+    hipLaunchKernelGGL(k_sort_scan_bins, dim3(kSortBins / kSortTileBins), dim3(256), 0, s, J);
+    hipLaunchKernelGGL(k_sort_scan_totals, dim3(1), dim3(1024), 0, s, J);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(kSortBlocks), dim3(256), 0, s, J);
+}
+
 // The counting sort of depth d's shadow entries (before its shadow stage).
 void launch_shadow_sort(hipStream_t s, const PtParams& T, uint32_t d) {
-    hipLaunchKernelGGL(k_sort_count, dim3(kSortBlocks), dim3(256), 0, s, T, d);
-    hipLaunchKernelGGL(k_sort_scan_bins, dim3(kSortBins / kSortTileBins), dim3(256), 0, s, T, d);
-    hipLaunchKernelGGL(k_sort_scan_totals, dim3(1), dim3(1024), 0, s, T, d);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(kSortBlocks), dim3(256), 0, s, T, d);
+    const size_t o = (size_t)d * T.depth_stride;
+    launch_sort(s, SortJob{T.skey + o, T.counters + CNT_SHQ + d, T.sperm + o, T.sort_scr + kSortScr * d,
+                           T.n_paths >> 4});
+}
+
+// The counting sort of depth d's extension rays (before its closest-hit stage).
+void launch_ray_sort(hipStream_t s, const PtParams& T, uint32_t d) {
+    launch_sort(s, SortJob{T.rkey, T.counters + CNT_QD + d, T.rperm, T.ray_scr, T.n_paths >> 4});
 }
 
 // Watertight triangle records (v0, v1, v2, primIdx in T3.w) for the affine
@@ -236,6 +247,9 @@ void make_tiles(tfg_pt* pt) {
         T.skey = P.skey + b;
         T.sperm = P.sperm + b;
         T.sort_scr = P.sort_scr + kSortScr * P.max_depth * t;
+        T.rkey = P.rkey + b;
+        T.rperm = P.rperm + b;
+        T.ray_scr = P.ray_scr + kSortScr * t;
         T.nee = P.nee + b;
         T.neemask = P.neemask + b;
         T.thr = P.thr + b;
@@ -333,6 +347,7 @@ int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, hipStream_t* side, tfg_color
     for (uint32_t d = 0; d < staged_d; ++d) {
         const uint32_t q = d & 1u;
         mark(pe ? pe->ext[t][d][0] : nullptr, ts);
+        if (T.ray_sort_from && d >= T.ray_sort_from) launch_ray_sort(ts, T, d);
         if (stats) launch_extend<true>(pt, ts, T, q, d);
         else launch_extend<false>(pt, ts, T, q, d);
         mark(pe ? pe->ext[t][d][1] : nullptr, ts);
@@ -598,6 +613,9 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
     ALLOC(skey, uint16_t, (size_t)n * cfg->max_depth)
     ALLOC(sperm, uint32_t, (size_t)n * cfg->max_depth)
     ALLOC(sort_scr, uint32_t, kSortScr * cfg->max_depth * kMaxTiles)
+    ALLOC(rkey, uint16_t, n)
+    ALLOC(rperm, uint32_t, n)
+    ALLOC(ray_scr, uint32_t, kSortScr * kMaxTiles)
     ALLOC(nee, float4, (size_t)n * cfg->max_depth)
     ALLOC(neemask, uint32_t, n)
     ALLOC(counters, uint32_t, (size_t)CNT_ALLOC * (1 + kMaxTiles))
@@ -974,6 +992,7 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     pt->connect_batch = 0;
     pt->tail_from = 0;
     pt->P.sort_from = 0;
+    pt->P.ray_sort_from = 0;
     // 1. traversal launch shape and LDS-staged top nodes together (large
     //    multipliers give more workgroups than chunks: each then traces one
     //    chunk and retires, which pays only with cheap workgroup start-up)
@@ -1018,6 +1037,7 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
         sweep(6, [&](int i) { pt->connect_batch = batches[i]; });
         sweep(6, [&](int i) { pt->tail_from = tails[i]; });
         sweep(3, [&](int i) { pt->P.sort_from = sorts[i]; });
+        sweep(3, [&](int i) { pt->P.ray_sort_from = sorts[i]; });
     }
     drop_graph(pt);
     pt->P.mode = saved_mode;
@@ -1051,6 +1071,7 @@ int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out) {
     out->tail_from = pt->tail_from;
     out->defer_training = pt->defer_train ? 1 : 0;
     out->shadow_sort_from = pt->P.sort_from;
+    out->ray_sort_from = pt->P.ray_sort_from;
     return TFG_OK;
 }
 
@@ -1061,10 +1082,11 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
         sc->tiles > kMaxTiles || sc->side_streams == 0 || sc->side_streams > kMaxSide ||
         (sc->xcd_group != 0 && (sc->xcd_group % 256 != 0 || sc->xcd_group > (1u << 20))) ||
         sc->connect_per_cu > 64 || sc->connect_batch > kMaxDepth || sc->top_nodes > kTopNodes ||
-        sc->tail_from > kMaxDepth || sc->defer_training > 1 || sc->shadow_sort_from > kMaxDepth) {
+        sc->tail_from > kMaxDepth || sc->defer_training > 1 || sc->shadow_sort_from > kMaxDepth ||
+        sc->ray_sort_from > kMaxDepth) {
         set_error("tfg_pt_set_schedule: block 64/128/256, grid_mult 1..64, tiles 1..%u, side streams 1..%u, "
-                  "xcd_group 0 or a multiple of 256 up to 2^20, connect_per_cu 0..64, connect_batch, tail_from and "
-                  "shadow_sort_from 0..%u, top_nodes 0..%u, defer_training 0/1", kMaxTiles, kMaxSide, kMaxDepth,
+                  "xcd_group 0 or a multiple of 256 up to 2^20, connect_per_cu 0..64, connect_batch, tail_from, "
+                  "shadow_sort_from and ray_sort_from 0..%u, top_nodes 0..%u, defer_training 0/1", kMaxTiles, kMaxSide, kMaxDepth,
                   kTopNodes);
         return TFG_E_INVALID;
     }
@@ -1094,6 +1116,7 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
     pt->connect_batch = sc->connect_batch;
     pt->tail_from = sc->tail_from;
     pt->P.sort_from = sc->shadow_sort_from;
+    pt->P.ray_sort_from = sc->ray_sort_from;
     return TFG_OK;
 }
 

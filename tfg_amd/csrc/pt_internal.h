@@ -112,18 +112,27 @@ struct PtParams {
     float4* queries;        // kQueryF4 float4 per query
     // spatial order of the shadow entries (depths >= sort_from; 0: queue order)
     uint32_t sort_from;
-    uint16_t* skey;         // per depth: depth_stride keys, kSortBits-bit Morton code of the shadow point
+    uint16_t* skey;         // per depth: depth_stride keys (kSortBits-bit Morton code of the shadow point)
     uint32_t* sperm;        // per depth: depth_stride entry indices in key order
     uint32_t* sort_scr;     // per depth: kSortScr words (per-workgroup bin counts, bin totals, bin bases)
+    // direction / origin order of the extension rays (depths >= ray_sort_from; 0: queue order)
+    uint32_t ray_sort_from;
+    uint16_t* rkey;         // key of each ray of the queue being built (written by shade)
+    uint32_t* rperm;        // queue slots in key order (for the next closest-hit launch)
+    uint32_t* ray_scr;      // kSortScr words
 };
 
 // Counting sort of a depth's shadow entries by the Morton code of their point
-// in the root node's quantisation frame (4 bits per axis), so that the rays a
-// wave traces start close together.
+// in the root node's quantisation frame (kSortBits / 3 bits per axis), so that
+// the rays a wave traces start close together.
 constexpr uint32_t kSortBits = 12;
 constexpr uint32_t kSortBins = 1u << kSortBits;
 constexpr uint32_t kSortBlocks = 256;   // workgroups of the count and scatter passes
 constexpr size_t kSortScr = (size_t)kSortBins * kSortBlocks + 2 * kSortBins;
+// Depths whose queue holds fewer entries than 1/16 of the tile's paths keep
+// the queue order (the sort's passes cost more there than the order saves);
+// the sort passes and the stage that reads the queue read the same device
+// count to decide (shadow_sorted, rays_sorted in pt_kernels.h).
 
 // Vertex data packing (4 float4): (pos, roughness), (dir mapped to [0,1], kd.r),
 // (normal mapped to [0,1], kd.g), (kd.b, ks.rgb).  Mirrors DatosGeometricos

@@ -134,6 +134,13 @@ __device__ __forceinline__ void stage_top_nodes(const PtParams& P, float4* top) 
     __syncthreads();
 }
 
+// Whether depth d's extension rays are traced in key order (count: the
+// depth's queue length).  The sort passes and the closest-hit stage decide
+// alike.
+__device__ __forceinline__ bool rays_sorted(const PtParams& P, uint32_t d, uint32_t count) {
+    return P.ray_sort_from != 0 && d >= P.ray_sort_from && count > 0 && count >= (P.n_paths >> 4);
+}
+
 template <int kBlock, bool kStats = false>
 __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint32_t depth) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -142,6 +149,9 @@ __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint3
     const uint32_t count = P.counters[CNT_QD + depth];
     if (blockIdx.x == 0 && threadIdx.x == 0 && count) atomicAdd(&P.gcounters[CNT_EXT + depth], count);
     const float4* rays = P.rays[q];
+    // with the ray order on, queue position i traces the ray at rperm[i] and
+    // writes its hit at that ray's own slot (shade reads hits in queue order)
+    const uint32_t* order = rays_sorted(P, depth, count) ? P.rperm : nullptr;
     // few rays: 8 lanes per ray (traverse_octet), kBlock / 8 rays per chunk
     const bool octet = count <= P.octet_below;
     const uint32_t per_chunk = octet ? kBlock / 8 : kBlock;
@@ -153,8 +163,9 @@ __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint3
         constexpr uint32_t kRays = kBlock / 8;
         const uint32_t j = threadIdx.x & 7u;
         for (; it.valid(); it.next()) {
-            const uint32_t i = it.chunk * kRays + (threadIdx.x >> 3);
-            if (i < count) {
+            const uint32_t iq = it.chunk * kRays + (threadIdx.x >> 3);
+            if (iq < count) {
+                const uint32_t i = order ? order[iq] : iq;
                 const float4 a = rays[2 * (size_t)i];
                 const float4 b = rays[2 * (size_t)i + 1];
                 const RayPre r = make_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
@@ -179,8 +190,9 @@ __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint3
     }
     for (; it.valid(); it.next()) {
         const uint32_t b0 = it.chunk * kBlock;
-        const uint32_t i = b0 + threadIdx.x;
-        if (i < count) {
+        const uint32_t iq = b0 + threadIdx.x;
+        if (iq < count) {
+            const uint32_t i = order ? order[iq] : iq;
             const float4 a = rays[2 * (size_t)i];
             const float4 b = rays[2 * (size_t)i + 1];
             const RayPre r = make_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
@@ -410,12 +422,11 @@ __device__ __forceinline__ void shade_entry(const PtParams& P, uint32_t depth, b
     }
 }
 
-// kSortBits-bit Morton code of p (bits 3b, 3b + 1, 3b + 2: bit b of the x, y,
-// z cell) in the root node's quantisation frame, where every scene point
-// lies in [origin, origin + 256 * 2^e) per axis; points outside clamp to the
-// border cells.
-__device__ __forceinline__ uint32_t shadow_sort_key(const PtParams& P, V3 p) {
-    constexpr int kAxisBits = (int)kSortBits / 3;
+// Cells of p in the root node's quantisation frame, where every scene point
+// lies in [origin, origin + 256 * 2^e) per axis: `bits` bits per axis,
+// interleaved as a Morton code (bits 3b, 3b + 1, 3b + 2: bit b of the x, y, z
+// cell); points outside clamp to the border cells.
+__device__ __forceinline__ uint32_t root_morton(const PtParams& P, V3 p, int bits) {
     const float4 r = P.nodes[0];
     const uint32_t w = __float_as_uint(r.w);
     const float rel[3] = {p.x - r.x, p.y - r.y, p.z - r.z};
@@ -424,13 +435,48 @@ __device__ __forceinline__ uint32_t shadow_sort_key(const PtParams& P, V3 p) {
     for (int a = 0; a < 3; ++a) {
         const int e = (int)(int8_t)((w >> (8 * a)) & 0xFFu);
         const float q = fminf(fmaxf(ldexpf(rel[a], -e), 0.0f), 255.0f);
-        c[a] = (uint32_t)q >> (8 - kAxisBits);
+        c[a] = (uint32_t)q >> (8 - bits);
     }
     uint32_t key = 0;
-#pragma unroll
-    for (int b = 0; b < kAxisBits; ++b)
+    for (int b = 0; b < bits; ++b)
         key |= (((c[0] >> b) & 1u) << (3 * b)) | (((c[1] >> b) & 1u) << (3 * b + 1)) | (((c[2] >> b) & 1u) << (3 * b + 2));
     return key;
+}
+
+// Shadow-entry key: the Morton code of the shadow point, kSortBits / 3 bits
+// per axis.
+__device__ __forceinline__ uint32_t shadow_sort_key(const PtParams& P, V3 p) {
+    return root_morton(P, p, (int)kSortBits / 3);
+}
+
+// Ray key: the direction's cube-map cell (6 faces x 4 x 4) times 32, plus the
+// origin's 2-bit-per-axis Morton code without its lowest bit (< 96 x 32 =
+// 3,072 <= kSortBins): rays of a wave then share direction first, origin
+// second.
+__device__ __forceinline__ uint32_t ray_sort_key(const PtParams& P, V3 o, V3 w) {
+    const float ax = fabsf(w.x), ay = fabsf(w.y), az = fabsf(w.z);
+    uint32_t face;
+    float m, u, v;
+    if (ax >= ay && ax >= az) {
+        face = w.x < 0.0f ? 1u : 0u;
+        m = ax;
+        u = w.y;
+        v = w.z;
+    } else if (ay >= az) {
+        face = w.y < 0.0f ? 3u : 2u;
+        m = ay;
+        u = w.x;
+        v = w.z;
+    } else {
+        face = w.z < 0.0f ? 5u : 4u;
+        m = az;
+        u = w.x;
+        v = w.y;
+    }
+    const float inv = m > 0.0f ? 1.0f / m : 0.0f;
+    const uint32_t iu = (uint32_t)fminf(fmaxf((u * inv + 1.0f) * 2.0f, 0.0f), 3.0f);
+    const uint32_t iv = (uint32_t)fminf(fmaxf((v * inv + 1.0f) * 2.0f, 0.0f), 3.0f);
+    return ((face * 4u + iu) * 4u + iv) * 32u + (root_morton(P, o, 2) >> 1);
 }
 
 template <bool kNrc>
@@ -472,37 +518,55 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
             P.rays[qo][2 * (size_t)rs] = n0;
             P.rays[qo][2 * (size_t)rs + 1] = n1;
             P.qpath[qo][rs] = pid;
+            if (P.ray_sort_from && depth + 1 >= P.ray_sort_from) P.rkey[rs] = (uint16_t)ray_sort_key(P, xyz(n0), xyz(n1));
         }
     }
     block_add<256>(facing, &P.gcounters[CNT_SHD + depth]);
 }
 
-// ------------------------------------------------------------ shadow sort
-// Counting sort of depth d's shadow entries by key (shade wrote the keys):
-// count (per workgroup share), exclusive scan over the workgroups per bin,
-// scan of the bin totals, scatter of the entry indices.  The permutation
-// changes only the order in which the shadow stage traces the entries:
-// every entry's result lands in nee[d][pid] as before.  Scratch of depth d
-// (kSortScr words): counts [kSortBlocks][kSortBins] (workgroup-major), bin
-// totals [kSortBins], bin bases [kSortBins].
+// ------------------------------------------------------------ key sorts
+// Counting sort of up to 2^24 keys (< kSortBins): count (per workgroup
+// share), exclusive scan over the workgroups per bin, scan of the bin
+// totals, scatter of the positions.  A stage that reads its queue through
+// the permutation traces the same entries in another order: every entry's
+// result lands where it did before.  Scratch (kSortScr words): counts
+// [kSortBlocks][kSortBins] (workgroup-major), bin totals [kSortBins], bin
+// bases [kSortBins].  Used for the shadow entries of a depth (keys from
+// shade) and for the extension rays of a depth (keys from the previous
+// depth's shade).
+struct SortJob {
+    const uint16_t* key;
+    const uint32_t* count;   // device count of keys
+    uint32_t* perm;          // positions in key order
+    uint32_t* scr;
+    uint32_t min_count;      // fewer keys: nothing is sorted (the consumer decides alike)
+};
 constexpr uint32_t kSortTileBins = 32;   // bins per workgroup of the per-bin scan
 
-__device__ __forceinline__ void sort_share(uint32_t count, uint32_t& lo, uint32_t& hi) {
+// Whether depth d's shadow entries are traced in key order (count: the
+// depth's shadow queue length).
+__device__ __forceinline__ bool shadow_sorted(const PtParams& P, uint32_t d, uint32_t count) {
+    return P.sort_from != 0 && d >= P.sort_from && count > 0 && count >= (P.n_paths >> 4);
+}
+
+__device__ __forceinline__ bool sort_share(const SortJob& J, uint32_t& lo, uint32_t& hi) {
+    const uint32_t count = *J.count;
+    if (count == 0 || count < J.min_count) return false;
     const uint32_t per = (count + kSortBlocks - 1) / kSortBlocks;
     lo = blockIdx.x * per;
     hi = lo + per < count ? lo + per : count;   // empty when lo >= count
+    return true;
 }
 
-__global__ __launch_bounds__(256) void k_sort_count(PtParams P, uint32_t d) {
+__global__ __launch_bounds__(256) void k_sort_count(SortJob J) {
     __shared__ uint32_t h[kSortBins];
     uint32_t lo, hi;
-    sort_share(P.counters[CNT_SHQ + d], lo, hi);
+    if (!sort_share(J, lo, hi)) return;
     for (uint32_t k = threadIdx.x; k < kSortBins; k += blockDim.x) h[k] = 0u;
     __syncthreads();
-    const uint16_t* key = P.skey + (size_t)d * P.depth_stride;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&h[key[i]], 1u);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&h[J.key[i]], 1u);
     __syncthreads();
-    uint32_t* cnt = P.sort_scr + (size_t)d * kSortScr + (size_t)blockIdx.x * kSortBins;
+    uint32_t* cnt = J.scr + (size_t)blockIdx.x * kSortBins;
     for (uint32_t k = threadIdx.x; k < kSortBins; k += blockDim.x) cnt[k] = h[k];
 }
 
@@ -510,11 +574,13 @@ __global__ __launch_bounds__(256) void k_sort_count(PtParams P, uint32_t d) {
 // through LDS, each bin's column summed by 256 / kSortTileBins threads in
 // row ranges, then rewritten as exclusive prefixes over the workgroups; the
 // bin totals go to the totals array.
-__global__ __launch_bounds__(256) void k_sort_scan_bins(PtParams P, uint32_t d) {
+__global__ __launch_bounds__(256) void k_sort_scan_bins(SortJob J) {
     constexpr uint32_t kParts = 256 / kSortTileBins, kRows = kSortBlocks / kParts;
     __shared__ uint32_t tile[kSortBlocks][kSortTileBins + 1];
     __shared__ uint32_t psum[kParts][kSortTileBins];
-    uint32_t* cnt = P.sort_scr + (size_t)d * kSortScr;
+    const uint32_t count = *J.count;
+    if (count == 0 || count < J.min_count) return;
+    uint32_t* cnt = J.scr;
     uint32_t* tot = cnt + (size_t)kSortBlocks * kSortBins;
     const uint32_t bin0 = blockIdx.x * kSortTileBins;
     for (uint32_t k = threadIdx.x; k < kSortBlocks * kSortTileBins; k += blockDim.x) {
@@ -544,11 +610,13 @@ __global__ __launch_bounds__(256) void k_sort_scan_bins(PtParams P, uint32_t d) 
 
 // Exclusive scan of the kSortBins totals (one workgroup of 1024 threads,
 // kSortBins / 1024 bins per thread).
-__global__ __launch_bounds__(1024) void k_sort_scan_totals(PtParams P, uint32_t d) {
+__global__ __launch_bounds__(1024) void k_sort_scan_totals(SortJob J) {
     constexpr uint32_t kPer = kSortBins / 1024;
     __shared__ uint32_t wsum[16];
-    const uint32_t* tot = P.sort_scr + (size_t)d * kSortScr + (size_t)kSortBlocks * kSortBins;
-    uint32_t* base = const_cast<uint32_t*>(tot) + kSortBins;
+    const uint32_t count = *J.count;
+    if (count == 0 || count < J.min_count) return;
+    const uint32_t* tot = J.scr + (size_t)kSortBlocks * kSortBins;
+    uint32_t* base = J.scr + (size_t)kSortBlocks * kSortBins + kSortBins;
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     uint32_t v[kPer], s = 0;
 #pragma unroll
@@ -580,21 +648,19 @@ __global__ __launch_bounds__(1024) void k_sort_scan_totals(PtParams P, uint32_t 
     }
 }
 
-// Entry i of workgroup b's share goes to base[key] + (entries of that key in
-// the shares before b) + its rank among its share's entries of that key (LDS
-// atomics: the order within a key is arbitrary, which changes no result).
-__global__ __launch_bounds__(256) void k_sort_scatter(PtParams P, uint32_t d) {
+// Key i of workgroup b's share goes to base[key] + (keys of that bin in the
+// shares before b) + its rank among its share's keys of that bin (LDS
+// atomics: the order within a bin is arbitrary, which changes no result).
+__global__ __launch_bounds__(256) void k_sort_scatter(SortJob J) {
     __shared__ uint32_t cur[kSortBins];
     uint32_t lo, hi;
-    sort_share(P.counters[CNT_SHQ + d], lo, hi);
-    const uint32_t* cnt = P.sort_scr + (size_t)d * kSortScr;
-    const uint32_t* base = cnt + (size_t)kSortBlocks * kSortBins + kSortBins;
+    if (!sort_share(J, lo, hi)) return;
+    const uint32_t* cnt = J.scr;
+    const uint32_t* base = J.scr + (size_t)kSortBlocks * kSortBins + kSortBins;
     for (uint32_t k = threadIdx.x; k < kSortBins; k += blockDim.x)
         cur[k] = base[k] + cnt[(size_t)blockIdx.x * kSortBins + k];
     __syncthreads();
-    const uint16_t* key = P.skey + (size_t)d * P.depth_stride;
-    uint32_t* perm = P.sperm + (size_t)d * P.depth_stride;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) perm[atomicAdd(&cur[key[i]], 1u)] = i;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) J.perm[atomicAdd(&cur[J.key[i]], 1u)] = i;
 }
 
 // ---------------------------------------------------------------- connect
@@ -615,7 +681,12 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t d_lo, u
     float4* top = reinterpret_cast<float4*>(lds);
     uint2* stack = reinterpret_cast<uint2*>(top + 5 * P.n_top);
     uint32_t count = 0;   // entries of all depths in the range
-    for (uint32_t d = d_lo; d <= d_hi; ++d) count += P.counters[CNT_SHQ + d];
+    uint32_t sorted_mask = 0;   // bit d - d_lo: depth d's entries are traced in key order
+    for (uint32_t d = d_lo; d <= d_hi; ++d) {
+        const uint32_t c = P.counters[CNT_SHQ + d];
+        count += c;
+        if (shadow_sorted(P, d, c)) sorted_mask |= 1u << (d - d_lo);
+    }
     const uint32_t shift = P.light_shift;
     const uint32_t group = 1u << shift;
     // few shadow rays: 8 lanes per ray (traverse_octet); needs <= 8 lights so
@@ -647,7 +718,7 @@ __global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t d_lo, u
         float4 e0 = make_float4(0.f, 0.f, 0.f, 0.f), e3 = e0;
         if (w < work) {
             // the entry at position i of the key order (or of the queue order)
-            const uint32_t ei = P.sort_from && d >= P.sort_from ? P.sperm[(size_t)d * P.depth_stride + i] : i;
+            const uint32_t ei = (sorted_mask >> (d - d_lo)) & 1u ? P.sperm[(size_t)d * P.depth_stride + i] : i;
             const float4* e = entries + 4 * (size_t)ei;
             e0 = e[0];
             e3 = e[3];
