@@ -52,8 +52,8 @@ HBM_PEAK_GBS = 8000.0
 # overlap off, saved schedule) for the path tracer's stages.
 MEASURED_TRAFFIC = {
     "k_trace_bounce": (366.2e6, "profiles/k_trace_bounce_displaced_sphere_1M.md"),
-    "k_extend": (1871.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
-    "k_connect": (3104.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
+    "k_extend": (2718.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
+    "k_connect": (3323.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
 }
 
 
@@ -180,7 +180,7 @@ def bench_traversal(args, torch):
 # varies from run to run); --tune re-tunes instead.
 TUNED_SCHEDULE_1080P = {"trace_block": 64, "grid_mult": 4, "tiles": 1, "side_streams": 2, "overlap": True,
                         "xcd_group": 0, "octet_below": 131072, "connect_per_cu": 0, "connect_batch": 0,
-                        "top_nodes": 1, "tail_from": 0}
+                        "top_nodes": 1, "tail_from": 0, "shadow_sort_from": 1, "ray_sort_from": 0}
 
 
 def _stage_roofline(prof, frames, stage, n_launch):

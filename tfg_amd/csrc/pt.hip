@@ -179,10 +179,9 @@ void launch_connect(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t d_lo,
 }
 
 void launch_sort(hipStream_t s, const SortJob& J) {
-    hipLaunchKernelGGL(k_sort_count, dim3(kSortBlocks), dim3(256), 0, s, J);
+    hipLaunchKernelGGL(k_sort_count, dim3(kSortBlocks), dim3(kSortThreads), 0, s, J);
     hipLaunchKernelGGL(k_sort_scan_bins, dim3(kSortBins / kSortTileBins), dim3(256), 0, s, J);
-    hipLaunchKernelGGL(k_sort_scan_totals, dim3(1), dim3(1024), 0, s, J);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(kSortBlocks), dim3(256), 0, s, J);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(kSortBlocks), dim3(kSortThreads), 0, s, J);
 }
 
 // The counting sort of depth d's shadow entries (before its shadow stage).
@@ -346,8 +345,8 @@ int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, hipStream_t* side, tfg_color
     const uint32_t staged_d = staged_depths(pt);
     for (uint32_t d = 0; d < staged_d; ++d) {
         const uint32_t q = d & 1u;
-        mark(pe ? pe->ext[t][d][0] : nullptr, ts);
         if (T.ray_sort_from && d >= T.ray_sort_from) launch_ray_sort(ts, T, d);
+        mark(pe ? pe->ext[t][d][0] : nullptr, ts);
         if (stats) launch_extend<true>(pt, ts, T, q, d);
         else launch_extend<false>(pt, ts, T, q, d);
         mark(pe ? pe->ext[t][d][1] : nullptr, ts);
@@ -365,9 +364,9 @@ int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, hipStream_t* side, tfg_color
             TFG_HIP_CHECK(hipEventRecord(pt->ev_shaded[t][d], ts));
             TFG_HIP_CHECK(hipStreamWaitEvent(cs, pt->ev_shaded[t][d], 0));
         }
-        mark(pe ? pe->con[t][d_lo][0] : nullptr, cs);
         if (T.sort_from)
             for (uint32_t k = d_lo > T.sort_from ? d_lo : T.sort_from; k <= d; ++k) launch_shadow_sort(cs, T, k);
+        mark(pe ? pe->con[t][d_lo][0] : nullptr, cs);
         if (stats) launch_connect<true>(pt, cs, T, d_lo, d);
         else launch_connect<false>(pt, cs, T, d_lo, d);
         mark(pe ? pe->con[t][d_lo][1] : nullptr, cs);
@@ -680,7 +679,10 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
         else pt->n_cu = prop.multiProcessorCount;
     }
     if (rc == TFG_OK) {
-        const hipError_t e = hipMemset(P.counters, 0, (size_t)CNT_ALLOC * (1 + kMaxTiles) * sizeof(uint32_t));
+        hipError_t e = hipMemset(P.counters, 0, (size_t)CNT_ALLOC * (1 + kMaxTiles) * sizeof(uint32_t));
+        // the sorts' done counters start at zero (each sort leaves its own at zero)
+        if (e == hipSuccess) e = hipMemset(P.sort_scr, 0, kSortScr * cfg->max_depth * kMaxTiles * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMemset(P.ray_scr, 0, kSortScr * kMaxTiles * sizeof(uint32_t));
         if (e != hipSuccess) rc = hip_fail(e, "hipMemset(counters)");
     }
     if (rc != TFG_OK) {

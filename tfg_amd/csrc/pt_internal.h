@@ -127,8 +127,9 @@ struct PtParams {
 // the rays a wave traces start close together.
 constexpr uint32_t kSortBits = 12;
 constexpr uint32_t kSortBins = 1u << kSortBits;
-constexpr uint32_t kSortBlocks = 256;   // workgroups of the count and scatter passes
-constexpr size_t kSortScr = (size_t)kSortBins * kSortBlocks + 2 * kSortBins;
+constexpr uint32_t kSortBlocks = 128;   // workgroups (of 512 threads) of the count and scatter passes
+// counts [kSortBlocks][kSortBins], bin totals, bin bases, a done counter (zero between sorts)
+constexpr size_t kSortScr = (size_t)kSortBins * kSortBlocks + 2 * kSortBins + 64;
 // Depths whose queue holds fewer entries than 1/16 of the tile's paths keep
 // the queue order (the sort's passes cost more there than the order saves);
 // the sort passes and the stage that reads the queue read the same device

@@ -525,15 +525,16 @@ __global__ __launch_bounds__(256) void k_shade(PtParams P, uint32_t q, uint32_t 
 }
 
 // ------------------------------------------------------------ key sorts
-// Counting sort of up to 2^24 keys (< kSortBins): count (per workgroup
-// share), exclusive scan over the workgroups per bin, scan of the bin
-// totals, scatter of the positions.  A stage that reads its queue through
-// the permutation traces the same entries in another order: every entry's
-// result lands where it did before.  Scratch (kSortScr words): counts
-// [kSortBlocks][kSortBins] (workgroup-major), bin totals [kSortBins], bin
-// bases [kSortBins].  Used for the shadow entries of a depth (keys from
-// shade) and for the extension rays of a depth (keys from the previous
-// depth's shade).
+// Counting sort of up to 2^24 keys (< kSortBins) in three launches: count
+// (per workgroup share), exclusive scan over the workgroups per bin (whose
+// last workgroup also scans the bin totals), scatter of the positions.  A
+// stage that reads its queue through the permutation traces the same
+// entries in another order: every entry's result lands where it did
+// before.  Scratch (kSortScr words): counts [kSortBlocks][kSortBins]
+// (workgroup-major), bin totals [kSortBins], bin bases [kSortBins], a done
+// counter that is zero between sorts (zeroed at creation, reset by the last
+// workgroup).  Used for the shadow entries of a depth (keys from shade) and
+// for the extension rays of a depth (keys from the previous depth's shade).
 struct SortJob {
     const uint16_t* key;
     const uint32_t* count;   // device count of keys
@@ -541,7 +542,8 @@ struct SortJob {
     uint32_t* scr;
     uint32_t min_count;      // fewer keys: nothing is sorted (the consumer decides alike)
 };
-constexpr uint32_t kSortTileBins = 32;   // bins per workgroup of the per-bin scan
+constexpr uint32_t kSortThreads = 512;   // threads of the count and scatter workgroups
+constexpr uint32_t kSortTileBins = 16;   // bins per workgroup of the per-bin scan
 
 // Whether depth d's shadow entries are traced in key order (count: the
 // depth's shadow queue length).
@@ -558,7 +560,7 @@ __device__ __forceinline__ bool sort_share(const SortJob& J, uint32_t& lo, uint3
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_sort_count(SortJob J) {
+__global__ __launch_bounds__(kSortThreads) void k_sort_count(SortJob J) {
     __shared__ uint32_t h[kSortBins];
     uint32_t lo, hi;
     if (!sort_share(J, lo, hi)) return;
@@ -573,15 +575,21 @@ __global__ __launch_bounds__(256) void k_sort_count(SortJob J) {
 // kSortTileBins bins per workgroup: their kSortBlocks x kSortTileBins counts
 // through LDS, each bin's column summed by 256 / kSortTileBins threads in
 // row ranges, then rewritten as exclusive prefixes over the workgroups; the
-// bin totals go to the totals array.
+// bin totals go to the totals array.  The last workgroup to finish (done
+// counter) turns the totals into the bins' bases by an exclusive scan.
 __global__ __launch_bounds__(256) void k_sort_scan_bins(SortJob J) {
     constexpr uint32_t kParts = 256 / kSortTileBins, kRows = kSortBlocks / kParts;
+    constexpr uint32_t kPer = kSortBins / 256;   // bins per thread of the totals scan
     __shared__ uint32_t tile[kSortBlocks][kSortTileBins + 1];
     __shared__ uint32_t psum[kParts][kSortTileBins];
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t is_last;
     const uint32_t count = *J.count;
     if (count == 0 || count < J.min_count) return;
     uint32_t* cnt = J.scr;
     uint32_t* tot = cnt + (size_t)kSortBlocks * kSortBins;
+    uint32_t* base = tot + kSortBins;
+    uint32_t* done = base + kSortBins;
     const uint32_t bin0 = blockIdx.x * kSortTileBins;
     for (uint32_t k = threadIdx.x; k < kSortBlocks * kSortTileBins; k += blockDim.x) {
         const uint32_t r = k / kSortTileBins, c = k % kSortTileBins;
@@ -606,52 +614,41 @@ __global__ __launch_bounds__(256) void k_sort_scan_bins(SortJob J) {
         const uint32_t r = k / kSortTileBins, cc = k % kSortTileBins;
         cnt[(size_t)r * kSortBins + bin0 + cc] = tile[r][cc];
     }
-}
-
-// Exclusive scan of the kSortBins totals (one workgroup of 1024 threads,
-// kSortBins / 1024 bins per thread).
-__global__ __launch_bounds__(1024) void k_sort_scan_totals(SortJob J) {
-    constexpr uint32_t kPer = kSortBins / 1024;
-    __shared__ uint32_t wsum[16];
-    const uint32_t count = *J.count;
-    if (count == 0 || count < J.min_count) return;
-    const uint32_t* tot = J.scr + (size_t)kSortBlocks * kSortBins;
-    uint32_t* base = J.scr + (size_t)kSortBlocks * kSortBins + kSortBins;
-    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-    uint32_t v[kPer], s = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < kPer; ++k) {
-        v[k] = tot[kPer * t + k];
-        s += v[k];
+    if (threadIdx.x == 0) {
+        __threadfence();   // this workgroup's totals before its count
+        is_last = atomicAdd(done, 1u) == gridDim.x - 1u ? 1u : 0u;
     }
-    uint32_t x = s;   // inclusive scan over the wave
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    uint32_t v[kPer], ts = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {   // other workgroups' totals: coherent loads
+        v[k] = __hip_atomic_load(&tot[kPer * t + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ts += v[k];
+    }
+    uint32_t x = ts;   // inclusive scan over the wave
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t y = __shfl_up(x, o);
         if ((int)lane >= o) x += y;
     }
     if (lane == 63) wsum[wave] = x;
     __syncthreads();
-    if (t == 0) {
-        uint32_t run = 0;
-        for (int w = 0; w < 16; ++w) {
-            const uint32_t c = wsum[w];
-            wsum[w] = run;
-            run += c;
-        }
-    }
-    __syncthreads();
-    uint32_t run = wsum[wave] + x - s;
+    uint32_t b = x - ts;
+    for (uint32_t w = 0; w < wave; ++w) b += wsum[w];
 #pragma unroll
     for (uint32_t k = 0; k < kPer; ++k) {
-        base[kPer * t + k] = run;
-        run += v[k];
+        base[kPer * t + k] = b;
+        b += v[k];
     }
+    if (t == 0) *done = 0u;   // for the next sort with this scratch
 }
 
 // Key i of workgroup b's share goes to base[key] + (keys of that bin in the
 // shares before b) + its rank among its share's keys of that bin (LDS
 // atomics: the order within a bin is arbitrary, which changes no result).
-__global__ __launch_bounds__(256) void k_sort_scatter(SortJob J) {
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(SortJob J) {
     __shared__ uint32_t cur[kSortBins];
     uint32_t lo, hi;
     if (!sort_share(J, lo, hi)) return;
