@@ -731,14 +731,14 @@ def _children():
 
 
 def _pick_cache_tail(torch, render, tr, frame0, frames=3):
-    """The faster of tail_from 0 and experiments.CACHE_TAIL_FROM for a
+    """The fastest of tail_from 0, 2 and experiments.CACHE_TAIL_FROM for a
     tracer's train+infer frames, timed over `frames` frames each (outside any
     timed region): the tail launch wins where late depths hold few paths and
     the per-depth stages win where they hold many (a large canvas)."""
     from tfg_amd import experiments
 
     best = None
-    for tail in (0, experiments.CACHE_TAIL_FROM):
+    for tail in sorted({0, 2, experiments.CACHE_TAIL_FROM}):
         tr.set_schedule(tail_from=tail)
         render(frame0)
         torch.cuda.synchronize()

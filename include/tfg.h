@@ -265,7 +265,7 @@ typedef struct tfg_pt_schedule {
     uint32_t top_nodes;
     uint32_t tail_from;
     uint32_t defer_training;
-    uint32_t shadow_sort_from;   /* 0: off; k: the shadow entries of depths >= k are traced in the order of a
+    uint32_t shadow_sort_from;   /* 0: off; k (default 1): the shadow entries of depths >= k are traced in the order of a
                                     12-bit Morton code of their point (a counting sort before each shadow stage),
                                     at depths whose shadow queue holds at least 1/16 of the frame's (tile's) paths */
     uint32_t ray_sort_from;      /* 0: off; k: the extension rays of depths >= k are traced in the order of their

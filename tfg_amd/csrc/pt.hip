@@ -594,6 +594,7 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
     P.depth_stride = n;
     P.xcd_group = 4096;
     P.octet_below = 1u << 18;
+    P.sort_from = 1;   // shadow entries of depths >= 1 in spatial order (no result changes)
     P.max_depth = cfg->max_depth;
     P.rr_depth = cfg->rr_depth;
     P.mode = cfg->mode;

@@ -125,12 +125,15 @@ def nrc_run(torch, dscene, W, H, max_depth, rr_depth, budget_s, warmup=48, preci
     return (acc / n).cpu().numpy(), n, secs + spent, cache.stats(), t_w - t0
 
 
-# Train+infer frames end most paths at their first cache query, so depths >= 2
-# hold little more than the 5% training paths and the per-depth launch chain
-# is bound by latency there: one tail launch for depths 2.. takes a 1080p
-# frame of the 64-instance scene from 2.24 to 2.07 ms.  Plain and
-# training-only frames, whose late depths are full, keep the tuned value (0).
-CACHE_TAIL_FROM = 2
+# Train+infer frames end most paths at their first cache query, so the late
+# depths hold little more than the 5% training paths and the per-depth launch
+# chain is bound by latency there.  On the 64-instance scene in its closed
+# room at 1080p the queues of depths 2..7 hold 126K, 63K, 37K, 3.5K, 1.7K and
+# 0.9K rays; one tail launch for depths 5.. takes a train+infer frame from
+# 2.81 ms (no tail launch; 3.10 ms with the tail from depth 2) to 2.76 ms.
+# Plain and training-only frames, whose late depths are full, keep the tuned
+# value (0).
+CACHE_TAIL_FROM = 5
 
 
 def pt_mode_train():
