@@ -928,28 +928,53 @@ __global__ __launch_bounds__(256) void k_finalize(PtParams P, tfg_color* image, 
             }
         }
         if (!kNrc) continue;
-        uint32_t rs, qs;
-        block_append2<256>(want_rec, &P.gcounters[CNT_RECORDS], rs, want_q, &P.gcounters[CNT_QUERIES], qs);
+        // The block's records and queries take contiguous slots (ranked in
+        // lane order), so they are written through LDS: each thread leaves
+        // its path id and the fields it computed at its rank, then the block
+        // copies the ranges float4 by float4, consecutive lanes writing
+        // consecutive float4 (and reading the head / tail vertices of
+        // consecutive paths).  The values and slots are those of writing
+        // each record and query from its own thread.
+        __shared__ uint32_t s_rpid[256], s_qpid[256], s_rtail[256];
+        __shared__ float4 s_rx[256][3], s_qx[256];
+        uint32_t rs, rb, nr, qs, qb, nq;
+        block_append2_range<256>(want_rec, &P.gcounters[CNT_RECORDS], rs, rb, nr, want_q, &P.gcounters[CNT_QUERIES],
+                                 qs, qb, nq);
         if (want_rec) {
-            float4* R = P.records + (size_t)kRecF4 * rs;
-            const float4* hd = P.head + (size_t)kVertF4 * pid;
-            for (int k = 0; k < kVertF4; ++k) R[k] = hd[k];
+            const uint32_t r = rs - rb;
             const bool tl = fl & PF_TAIL;
-            const float4* tv = P.tail + (size_t)kVertF4 * pid;
-            for (int k = 0; k < kVertF4; ++k) R[kVertF4 + k] = tl ? tv[k] : make_float4(0.f, 0.f, 0.f, 0.f);
-            R[8] = make_float4(tot.x - pre.x, tot.y - pre.y, tot.z - pre.z, 1.0f);
+            s_rpid[r] = pid;
+            s_rtail[r] = tl ? 1u : 0u;
+            s_rx[r][0] = make_float4(tot.x - pre.x, tot.y - pre.y, tot.z - pre.z, 1.0f);
             float4 ts = tl ? P.tsuf[pid] : make_float4(0.f, 0.f, 0.f, 0.f);
             ts.w = __uint_as_float(P.pix_base + pid);   // pixel index, for record-level checks
-            R[9] = ts;
-            R[10] = P.tthr[pid];
+            s_rx[r][1] = ts;
+            s_rx[r][2] = P.tthr[pid];
         }
         if (want_q) {
-            float4* Q = P.queries + (size_t)kQueryF4 * qs;
-            const float4* hd = P.head + (size_t)kVertF4 * pid;
-            for (int k = 0; k < kVertF4; ++k) Q[k] = hd[k];
+            const uint32_t r = qs - qb;
             const float4 tt = P.tthr[pid];
-            Q[4] = make_float4(tt.x, tt.y, tt.z, __uint_as_float(P.pix_base + pid));
+            s_qpid[r] = pid;
+            s_qx[r] = make_float4(tt.x, tt.y, tt.z, __uint_as_float(P.pix_base + pid));
         }
+        __syncthreads();
+        float4* R = P.records + (size_t)kRecF4 * rb;
+        for (uint32_t j = threadIdx.x; j < nr * (uint32_t)kRecF4; j += 256) {
+            const uint32_t r = j / (uint32_t)kRecF4, k = j - r * (uint32_t)kRecF4;
+            const uint32_t p = s_rpid[r];
+            float4 v;
+            if (k < (uint32_t)kVertF4) v = P.head[(size_t)kVertF4 * p + k];
+            else if (k < 2u * kVertF4)
+                v = s_rtail[r] ? P.tail[(size_t)kVertF4 * p + (k - kVertF4)] : make_float4(0.f, 0.f, 0.f, 0.f);
+            else v = s_rx[r][k - 2u * kVertF4];
+            R[j] = v;
+        }
+        float4* Q = P.queries + (size_t)kQueryF4 * qb;
+        for (uint32_t j = threadIdx.x; j < nq * (uint32_t)kQueryF4; j += 256) {
+            const uint32_t r = j / (uint32_t)kQueryF4, k = j - r * (uint32_t)kQueryF4;
+            Q[j] = k < (uint32_t)kVertF4 ? P.head[(size_t)kVertF4 * s_qpid[r] + k] : s_qx[r];
+        }
+        __syncthreads();   // the staging arrays are reused by the next iteration
     }
 }
 

@@ -83,6 +83,45 @@ __device__ __forceinline__ void block_append2(bool want_a, uint32_t* counter_a, 
     __syncthreads();   // the LDS words are reused by the next call
 }
 
+// block_append2, plus each counter's first slot for the block and the
+// block's count: the block's slots of a counter are base .. base + n - 1,
+// ranked in lane order.  Every thread of the block must call it.
+template <int kBlock>
+__device__ __forceinline__ void block_append2_range(bool want_a, uint32_t* counter_a, uint32_t& slot_a,
+                                                    uint32_t& base_a, uint32_t& n_a, bool want_b,
+                                                    uint32_t* counter_b, uint32_t& slot_b, uint32_t& base_b,
+                                                    uint32_t& n_b) {
+    constexpr int kWaves = kBlock / 64;
+    __shared__ uint32_t wave_off[2][kWaves];
+    __shared__ uint32_t block_base[2], block_n[2];
+    const uint64_t ma = __ballot(want_a), mb = __ballot(want_b);
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / 64u;
+    if (lane == 0) {
+        wave_off[0][wave] = (uint32_t)__popcll(ma);
+        wave_off[1][wave] = (uint32_t)__popcll(mb);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t tot = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = wave_off[threadIdx.x][w];
+            wave_off[threadIdx.x][w] = tot;
+            tot += c;
+        }
+        block_n[threadIdx.x] = tot;
+        block_base[threadIdx.x] = tot ? atomicAdd(threadIdx.x ? counter_b : counter_a, tot) : 0u;
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    base_a = block_base[0];
+    base_b = block_base[1];
+    n_a = block_n[0];
+    n_b = block_n[1];
+    slot_a = base_a + wave_off[0][wave] + (uint32_t)__popcll(ma & below);
+    slot_b = base_b + wave_off[1][wave] + (uint32_t)__popcll(mb & below);
+    __syncthreads();   // the LDS words are reused by the next call
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
     return x;
