@@ -560,16 +560,31 @@ __device__ __forceinline__ bool sort_share(const SortJob& J, uint32_t& lo, uint3
     return true;
 }
 
+// Loads (up to) kSortBatch keys of a thread's stride in flight at once.
+constexpr uint32_t kSortBatch = 8;
+
 __global__ __launch_bounds__(kSortThreads) void k_sort_count(SortJob J) {
     __shared__ uint32_t h[kSortBins];
     uint32_t lo, hi;
     if (!sort_share(J, lo, hi)) return;
-    for (uint32_t k = threadIdx.x; k < kSortBins; k += blockDim.x) h[k] = 0u;
+#pragma unroll
+    for (uint32_t m = 0; m < kSortBins / kSortThreads; ++m) h[threadIdx.x + m * kSortThreads] = 0u;
     __syncthreads();
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&h[J.key[i]], 1u);
+    for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kSortBatch * kSortThreads) {
+        uint32_t k[kSortBatch];
+#pragma unroll
+        for (uint32_t m = 0; m < kSortBatch; ++m) {
+            const uint32_t i = i0 + m * kSortThreads;
+            k[m] = i < hi ? (uint32_t)J.key[i] : kSortBins;
+        }
+#pragma unroll
+        for (uint32_t m = 0; m < kSortBatch; ++m)
+            if (k[m] < kSortBins) atomicAdd(&h[k[m]], 1u);
+    }
     __syncthreads();
     uint32_t* cnt = J.scr + (size_t)blockIdx.x * kSortBins;
-    for (uint32_t k = threadIdx.x; k < kSortBins; k += blockDim.x) cnt[k] = h[k];
+#pragma unroll
+    for (uint32_t m = 0; m < kSortBins / kSortThreads; ++m) cnt[threadIdx.x + m * kSortThreads] = h[threadIdx.x + m * kSortThreads];
 }
 
 // kSortTileBins bins per workgroup: their kSortBlocks x kSortTileBins counts
@@ -591,9 +606,17 @@ __global__ __launch_bounds__(256) void k_sort_scan_bins(SortJob J) {
     uint32_t* base = tot + kSortBins;
     uint32_t* done = base + kSortBins;
     const uint32_t bin0 = blockIdx.x * kSortTileBins;
-    for (uint32_t k = threadIdx.x; k < kSortBlocks * kSortTileBins; k += blockDim.x) {
-        const uint32_t r = k / kSortTileBins, c = k % kSortTileBins;
-        tile[r][c] = cnt[(size_t)r * kSortBins + bin0 + c];
+    constexpr uint32_t kLoads = kSortBlocks * kSortTileBins / 256;
+    uint32_t tv[kLoads];
+#pragma unroll
+    for (uint32_t m = 0; m < kLoads; ++m) {   // every load in flight before the LDS stores
+        const uint32_t k = threadIdx.x + 256u * m, r = k / kSortTileBins, c = k % kSortTileBins;
+        tv[m] = cnt[(size_t)r * kSortBins + bin0 + c];
+    }
+#pragma unroll
+    for (uint32_t m = 0; m < kLoads; ++m) {
+        const uint32_t k = threadIdx.x + 256u * m;
+        tile[k / kSortTileBins][k % kSortTileBins] = tv[m];
     }
     __syncthreads();
     const uint32_t c = threadIdx.x % kSortTileBins, part = threadIdx.x / kSortTileBins;
@@ -610,8 +633,9 @@ __global__ __launch_bounds__(256) void k_sort_scan_bins(SortJob J) {
     }
     if (part == kParts - 1) tot[bin0 + c] = run;
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < kSortBlocks * kSortTileBins; k += blockDim.x) {
-        const uint32_t r = k / kSortTileBins, cc = k % kSortTileBins;
+#pragma unroll
+    for (uint32_t m = 0; m < kLoads; ++m) {
+        const uint32_t k = threadIdx.x + 256u * m, r = k / kSortTileBins, cc = k % kSortTileBins;
         cnt[(size_t)r * kSortBins + bin0 + cc] = tile[r][cc];
     }
     if (threadIdx.x == 0) {
@@ -654,10 +678,29 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(SortJob J) {
     if (!sort_share(J, lo, hi)) return;
     const uint32_t* cnt = J.scr;
     const uint32_t* base = J.scr + (size_t)kSortBlocks * kSortBins + kSortBins;
-    for (uint32_t k = threadIdx.x; k < kSortBins; k += blockDim.x)
-        cur[k] = base[k] + cnt[(size_t)blockIdx.x * kSortBins + k];
+    {
+        constexpr uint32_t kPer = kSortBins / kSortThreads;
+        uint32_t a[kPer], b[kPer];
+#pragma unroll
+        for (uint32_t m = 0; m < kPer; ++m) {
+            a[m] = base[threadIdx.x + m * kSortThreads];
+            b[m] = cnt[(size_t)blockIdx.x * kSortBins + threadIdx.x + m * kSortThreads];
+        }
+#pragma unroll
+        for (uint32_t m = 0; m < kPer; ++m) cur[threadIdx.x + m * kSortThreads] = a[m] + b[m];
+    }
     __syncthreads();
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) J.perm[atomicAdd(&cur[J.key[i]], 1u)] = i;
+    for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kSortBatch * kSortThreads) {
+        uint32_t k[kSortBatch];
+#pragma unroll
+        for (uint32_t m = 0; m < kSortBatch; ++m) {
+            const uint32_t i = i0 + m * kSortThreads;
+            k[m] = i < hi ? (uint32_t)J.key[i] : kSortBins;
+        }
+#pragma unroll
+        for (uint32_t m = 0; m < kSortBatch; ++m)
+            if (k[m] < kSortBins) J.perm[atomicAdd(&cur[k[m]], 1u)] = i0 + m * kSortThreads;
+    }
 }
 
 // ---------------------------------------------------------------- connect
