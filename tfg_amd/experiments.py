@@ -256,7 +256,12 @@ def grid_search(torch, scene, W, H, max_depth, rr_depth, grid=None, frames=100, 
     lo = scene.verts.reshape(-1, 3).min(0) - 0.5
     hi = scene.verts.reshape(-1, 3).max(0) + 0.5
     keys = sorted(grid)
-    tr = pt.PathTracer(W, H, max_depth, rr_depth, mode=pt_mode_train(), use_graph=_graph(schedule))
+    # direct launches unless the schedule asks for graph replay: the sweep
+    # swaps the cache once per configuration, and every swap re-captures a
+    # replayed frame (a host segmentation fault inside the HIP runtime ended
+    # one such run of capture / destroy cycles; direct launches have none)
+    use_graph = bool(schedule.get("use_graph", False)) if schedule else False
+    tr = pt.PathTracer(W, H, max_depth, rr_depth, mode=pt_mode_train(), use_graph=use_graph)
     tr.set_scene(dscene)
     _apply(tr, schedule)
     tr.set_camera(scene.camera.frame(W, H))
