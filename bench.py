@@ -675,8 +675,8 @@ def cpu_baseline_traversal(host, prim, bounce, shadow):
 
 
 def run_record(torch):
-    """The machine the run measured (BASELINE.md asks for it with every
-    measurement): host CPU model, logical CPUs and the cores this process
+    """The machine the run measured, recorded with every measurement: host
+    CPU model, logical CPUs and the cores this process
     may use, GPU model, compute units and memory, and the device-to-device
     copy bandwidth measured now (1 GiB read + 1 GiB written per copy, HIP
     events, best of 5 after 2 warmups)."""
@@ -940,8 +940,8 @@ def main():
                     help="re-tune the path tracer's schedule (tfg_pt_autotune) instead of the saved one")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
-              file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path (frames would shard by pixels or views, "
+              "the cache by one gradient all-reduce per optimiser step)", file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -9,10 +9,10 @@ train+infer frames (main.cpp:958-1089).
     (positions and throughputs within 1e-3 relative), and the cache queried
     at >= 95% of the pixels;
   * the image finite, the optimiser steps exactly 4 per frame;
-  * bias: plain path tracing is the judge.  The cache learns from
+  * bias: plain path tracing is the yardstick.  The cache learns from
     bootstrapped targets (L_suf + T_suf x cache(tail)), which carry light
     past the tracer's maximum depth, so its radiance approaches the
-    untruncated one; the judge is therefore plain PT at depth 32 (plain PT
+    untruncated one; the yardstick is therefore plain PT at depth 32 (plain PT
     at depth 8 misses 2.0% of this scene's light, measured with the
     oracle).  After 200 further train+infer frames (the cache converges
     over the first ~150: the EMA of the weights has a time constant of 25
@@ -35,7 +35,7 @@ pytestmark = pytest.mark.gpu
 
 W, H, MD, RR = 1920, 1080, 8, 3
 WARMUP, CONVERGE, N = 48, 200, 64
-MD_JUDGE = 32
+MD_PLAIN = 32
 
 
 @pytest.fixture(scope="module")
@@ -107,7 +107,7 @@ def test_cache_frame_1080p_matches_oracle_and_plain(setup):
         tr.render(frame + 1 + k, img)
     c1, c2 = moments(lambda f: tr.render(f, img), frame + 1 + CONVERGE)
     assert bool(torch.isfinite(img).all()) and cache.stats()["steps"] == 4 * (WARMUP + 1 + CONVERGE + N)
-    plain = pt.PathTracer(W, H, MD_JUDGE, RR, mode=0, use_graph=False)
+    plain = pt.PathTracer(W, H, MD_PLAIN, RR, mode=0, use_graph=False)
     plain.set_scene(ds)
     plain.set_camera(sc.camera.frame(W, H))
     p1, p2 = moments(lambda f: plain.render(f, img), 700_000)
@@ -127,7 +127,7 @@ def test_cache_frame_1080p_matches_oracle_and_plain(setup):
     rel = dev / np.maximum(mp, 1e-6)
     Mc, Mp = mc.mean(), mp.mean()
     se_img = np.sqrt(vc.sum() + vp.sum()) / mc.size
-    print(f"bias vs plain PT (depth {MD_JUDGE}) over {mc.shape[0] * mc.shape[1]} blocks: median |z| "
+    print(f"bias vs plain PT (depth {MD_PLAIN}) over {mc.shape[0] * mc.shape[1]} blocks: median |z| "
           f"{np.median(z):.2f}, max |z| {z.max():.1f}, median rel {np.median(rel):.4f}, p95 rel "
           f"{np.percentile(rel, 95):.4f}, max rel {rel.max():.4f}; image mean cache {Mc:.5f} plain {Mp:.5f} "
           f"(rel {(Mc - Mp) / Mp:+.4f}, {abs(Mc - Mp) / se_img:.1f} s.e.)", flush=True)

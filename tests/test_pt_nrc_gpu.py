@@ -88,8 +88,8 @@ def test_cache_frame_trains_and_composites(torch_cuda):
     minimum is lowered to 64 records) until it has converged, then 64 more
     frames measured: the composited image is finite, the cache adds light
     over the prefix colours alone (an all-zero network), and plain path
-    tracing judges its bias.  The cache's bootstrapped targets carry light
-    past the tracer's maximum depth 5, so the judge is plain PT at depth 32
+    tracing measures its bias.  The cache's bootstrapped targets carry light
+    past the tracer's maximum depth 5, so the yardstick is plain PT at depth 32
     (depth 5 misses 6.8% of this room's light, measured with the oracle):
     with standard errors from the per-pixel sample variances of both images
     (oracle frames with independent seeds), the image means agree within 4
