@@ -1,8 +1,8 @@
 /*
  * oracle.h - CPU restatement of the reference path tracer's hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
- * library in tfg_amd/: only tests/, __graft_entry__.smoke() and the
+ * TEST INFRASTRUCTURE ONLY.  The HIP library in tfg_amd/ is tested against
+ * this library: only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py load it.  Product code never links it.
  *
  * Each function names the reference file:line it restates.  Pinning (what

@@ -9,7 +9,7 @@ entrenar_red = false) from a weight file, against the oracle.
     throughput x the FP64 oracle network (rounding its operands like the
     kernel) evaluated on the GPU's own query vertex, for >= 99% of the
     queries within 2e-2 relative (BF16 operands);
-  * the driver (experiments.reconstruct) loads the TFGNRC01 file, and its
+  * experiments.reconstruct loads the TFGNRC01 file, and its
     FP64 accumulation and 8-bit resolve equal those of the frames rendered
     one by one;
   * the launchKernelRenderTiny drop-in with flags (1, 0) writes queries and
@@ -107,7 +107,7 @@ def test_reconstruction_frames_match_oracle(torch_cuda, tmp_path):
     tr.close()
     cache.close()
 
-    # the driver: FP64 accumulation of spp frames and the 8-bit resolve
+    # experiments.reconstruct: FP64 accumulation of spp frames and the 8-bit resolve
     spp = 4
     png = tmp_path / "recon.png"
     mean, rgb8 = experiments.reconstruct(torch, ds, W, H, MD, RR, str(path), spp, frame0=frame, png_path=str(png),

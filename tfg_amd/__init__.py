@@ -2,7 +2,7 @@
 
 The compute path is libtfg_hip.so (hand-written HIP kernels behind the C ABI
 of include/tfg.h); this package holds its ctypes binding, the seeded scene
-and ray-batch generators, and the host-side frame drivers.
+and ray-batch generators, and the host-side frame routines.
 """
 from ._lib import TfgError, lib  # noqa: F401
 

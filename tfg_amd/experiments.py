@@ -196,7 +196,7 @@ def reconstruct(torch, dscene, W, H, max_depth, rr_depth, model_path, spp, frame
     PNG at png_path (optional).
 
     bounds: the cache's scene bounds (min, max); default: the scene's vertex
-    box widened by 0.5, as the training drivers here choose them.  Returns
+    box widened by 0.5, as the training routines here choose them.  Returns
     (mean image (H*W, 3) float64, rgb8 bytes (H*W, 3) uint8)."""
     from . import image
 

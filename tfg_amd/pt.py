@@ -1,4 +1,4 @@
-"""Host driver of the wavefront path tracer (tfg_pt_* in include/tfg.h).
+"""Host interface of the wavefront path tracer (tfg_pt_* in include/tfg.h).
 
 ``DeviceScene`` uploads a :class:`tfg_amd.scenes.Scene` (CWBVH, shading
 records, materials, lights) to device memory; ``PathTracer`` renders frames
