@@ -1,4 +1,4 @@
-"""Benchmark driver: prints ONE JSON line.
+"""Benchmark: prints ONE JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pt1080|pt|traversal]
 
