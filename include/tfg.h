@@ -200,9 +200,10 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
 /* Times candidate schedules (tfg_pt_schedule) on the current scene and keeps
  * the fastest: first the traversal launch shape together with top_nodes,
  * then tiles, side streams and overlap, then the XCD super-chunk size,
- * octet_below, connect_per_cu, connect_batch and tail_from, then a second
- * round over all of them but the launch shape (top_nodes included), all
- * timed on plain frames with direct launches (the tuner captures no graph).
+ * octet_below, connect_per_cu, connect_batch and tail_from, then the two
+ * queue orders and lane refill, then a second round over all of them but the
+ * launch shape (top_nodes included), all timed on plain frames with direct
+ * launches (the tuner captures no graph).
  * Drops any captured graph. */
 int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream);
 int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_grid, uint32_t* shade_grid);
@@ -271,6 +272,12 @@ typedef struct tfg_pt_schedule {
     uint32_t ray_sort_from;      /* 0: off; k: the extension rays of depths >= k are traced in the order of their
                                     direction's cube-map cell (6 x 4 x 4) and a coarse cell of their origin, at
                                     depths whose queue holds at least 1/16 of the paths */
+    uint32_t refill_from;        /* 0: off; k (default 1): the closest-hit stage of depths >= k keeps its lanes busy
+                                    (lane refill): a persistent wave takes new rays from the queue whenever at least
+                                    refill_idle of its lanes have finished theirs, instead of waiting for its slowest
+                                    ray; at depths with at least two rays per resident lane (fewer: lockstep, and at
+                                    most octet_below rays: 8 lanes per ray) */
+    uint32_t refill_idle;        /* 1..64 (default 32) */
 } tfg_pt_schedule;
 int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out);
 int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* schedule);

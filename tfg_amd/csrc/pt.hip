@@ -28,6 +28,7 @@ struct tfg_pt {
     int trace_block = 128;
     int trace_mult = 1;
     int trace_grid = 0;
+    int refill_grid = 0;   // resident grid of the lane-refill closest-hit kernel
     int connect_per_cu = 0;   // shadow-stage workgroups per CU (0: the closest-hit grid)
     uint32_t connect_batch = 0;   // depths >= this (if > 0) share one shadow-stage launch
     uint32_t top_nodes = 9;       // leading tree nodes each traversal workgroup stages in LDS
@@ -127,22 +128,41 @@ int occupancy_extend(const tfg_pt* pt, int* blocks) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_extend<B>, B, trace_lds(pt, B)) == hipSuccess ? 0 : -1;
 }
 
+template <int B>
+int occupancy_refill(const tfg_pt* pt, int* blocks) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_extend_refill<B>, B, trace_lds(pt, B)) == hipSuccess
+               ? 0
+               : -1;
+}
+
 int set_trace_shape(tfg_pt* pt, int block, int mult) {
-    int per_cu = 0;
-    if (block == 64) occupancy_extend<64>(pt, &per_cu);
-    else if (block == 128) occupancy_extend<128>(pt, &per_cu);
-    else occupancy_extend<256>(pt, &per_cu);
+    int per_cu = 0, refill_per_cu = 0;
+    if (block == 64) occupancy_extend<64>(pt, &per_cu), occupancy_refill<64>(pt, &refill_per_cu);
+    else if (block == 128) occupancy_extend<128>(pt, &per_cu), occupancy_refill<128>(pt, &refill_per_cu);
+    else occupancy_extend<256>(pt, &per_cu), occupancy_refill<256>(pt, &refill_per_cu);
     if (per_cu <= 0) per_cu = 1;
+    if (refill_per_cu <= 0) refill_per_cu = 1;
     pt->trace_block = block;
     pt->trace_mult = mult;
     pt->trace_grid = pt->n_cu * per_cu * mult;
+    // the refill kernel's waves stay for the whole launch: one resident grid
+    pt->refill_grid = pt->n_cu * refill_per_cu;
     return TFG_OK;
 }
 
 template <bool kStats>
 void launch_extend(tfg_pt* pt, hipStream_t s, const PtParams& T, uint32_t q, uint32_t d) {
-    const dim3 g(pt->trace_grid);
     const size_t lds = trace_lds(pt, pt->trace_block);
+    if (T.refill_from != 0 && d >= T.refill_from) {
+        const dim3 g(pt->refill_grid);
+        switch (pt->trace_block) {
+            case 64: hipLaunchKernelGGL((k_extend_refill<64, kStats>), g, dim3(64), lds, s, T, q, d); break;
+            case 256: hipLaunchKernelGGL((k_extend_refill<256, kStats>), g, dim3(256), lds, s, T, q, d); break;
+            default: hipLaunchKernelGGL((k_extend_refill<128, kStats>), g, dim3(128), lds, s, T, q, d); break;
+        }
+        return;
+    }
+    const dim3 g(pt->trace_grid);
     switch (pt->trace_block) {
         case 64: hipLaunchKernelGGL((k_extend<64, kStats>), g, dim3(64), lds, s, T, q, d); break;
         case 256: hipLaunchKernelGGL((k_extend<256, kStats>), g, dim3(256), lds, s, T, q, d); break;
@@ -249,6 +269,7 @@ void make_tiles(tfg_pt* pt) {
         T.rkey = P.rkey + b;
         T.rperm = P.rperm + b;
         T.ray_scr = P.ray_scr + kSortScr * t;
+        T.rcur = P.rcur + kRefillCurWords * t;
         T.nee = P.nee + b;
         T.neemask = P.neemask + b;
         T.thr = P.thr + b;
@@ -595,6 +616,8 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
     P.xcd_group = 4096;
     P.octet_below = 1u << 18;
     P.sort_from = 1;   // shadow entries of depths >= 1 in spatial order (no result changes)
+    P.refill_from = 1;   // lane refill in the closest-hit stage from depth 1 (no result changes)
+    P.refill_idle = 32;
     P.max_depth = cfg->max_depth;
     P.rr_depth = cfg->rr_depth;
     P.mode = cfg->mode;
@@ -616,6 +639,7 @@ int tfg_pt_create(const tfg_pt_config* cfg, tfg_pt** out) {
     ALLOC(rkey, uint16_t, n)
     ALLOC(rperm, uint32_t, n)
     ALLOC(ray_scr, uint32_t, kSortScr * kMaxTiles)
+    ALLOC(rcur, uint32_t, kRefillCurWords * kMaxTiles)
     ALLOC(nee, float4, (size_t)n * cfg->max_depth)
     ALLOC(neemask, uint32_t, n)
     ALLOC(counters, uint32_t, (size_t)CNT_ALLOC * (1 + kMaxTiles))
@@ -996,6 +1020,7 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     pt->tail_from = 0;
     pt->P.sort_from = 0;
     pt->P.ray_sort_from = 0;
+    pt->P.refill_from = 0;
     // 1. traversal launch shape and LDS-staged top nodes together (large
     //    multipliers give more workgroups than chunks: each then traces one
     //    chunk and retires, which pays only with cheap workgroup start-up)
@@ -1023,6 +1048,7 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     const uint32_t batches[6] = {0, 2, 3, 4, 5, 6};
     const uint32_t tails[6] = {0, 2, 3, 4, 5, 6};
     const uint32_t sorts[3] = {0, 1, 2};
+    const uint32_t idles[3] = {16, 32, 48};
     for (int round = 0; round < 2 && rc == TFG_OK; ++round) {
         if (round > 0)
             sweep(4, [&](int i) {
@@ -1041,6 +1067,8 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
         sweep(6, [&](int i) { pt->tail_from = tails[i]; });
         sweep(3, [&](int i) { pt->P.sort_from = sorts[i]; });
         sweep(3, [&](int i) { pt->P.ray_sort_from = sorts[i]; });
+        sweep(3, [&](int i) { pt->P.refill_from = sorts[i]; });
+        if (pt->P.refill_from) sweep(3, [&](int i) { pt->P.refill_idle = idles[i]; });
     }
     drop_graph(pt);
     pt->P.mode = saved_mode;
@@ -1075,6 +1103,8 @@ int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out) {
     out->defer_training = pt->defer_train ? 1 : 0;
     out->shadow_sort_from = pt->P.sort_from;
     out->ray_sort_from = pt->P.ray_sort_from;
+    out->refill_from = pt->P.refill_from;
+    out->refill_idle = pt->P.refill_idle;
     return TFG_OK;
 }
 
@@ -1086,11 +1116,12 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
         (sc->xcd_group != 0 && (sc->xcd_group % 256 != 0 || sc->xcd_group > (1u << 20))) ||
         sc->connect_per_cu > 64 || sc->connect_batch > kMaxDepth || sc->top_nodes > kTopNodes ||
         sc->tail_from > kMaxDepth || sc->defer_training > 1 || sc->shadow_sort_from > kMaxDepth ||
-        sc->ray_sort_from > kMaxDepth) {
+        sc->ray_sort_from > kMaxDepth || sc->refill_from > kMaxDepth || sc->refill_idle == 0 ||
+        sc->refill_idle > 64) {
         set_error("tfg_pt_set_schedule: block 64/128/256, grid_mult 1..64, tiles 1..%u, side streams 1..%u, "
                   "xcd_group 0 or a multiple of 256 up to 2^20, connect_per_cu 0..64, connect_batch, tail_from, "
-                  "shadow_sort_from and ray_sort_from 0..%u, top_nodes 0..%u, defer_training 0/1", kMaxTiles, kMaxSide, kMaxDepth,
-                  kTopNodes);
+                  "shadow_sort_from, ray_sort_from and refill_from 0..%u, refill_idle 1..64, top_nodes 0..%u, "
+                  "defer_training 0/1", kMaxTiles, kMaxSide, kMaxDepth, kTopNodes);
         return TFG_E_INVALID;
     }
     if ((sc->use_graph != 0) != pt->run_graph) {
@@ -1120,6 +1151,8 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
     pt->tail_from = sc->tail_from;
     pt->P.sort_from = sc->shadow_sort_from;
     pt->P.ray_sort_from = sc->ray_sort_from;
+    pt->P.refill_from = sc->refill_from;
+    pt->P.refill_idle = sc->refill_idle;
     return TFG_OK;
 }
 

@@ -120,7 +120,21 @@ struct PtParams {
     uint16_t* rkey;         // key of each ray of the queue being built (written by shade)
     uint32_t* rperm;        // queue slots in key order (for the next closest-hit launch)
     uint32_t* ray_scr;      // kSortScr words
+    // lane refill in the closest-hit stage (depths >= refill_from; 0: off)
+    uint32_t refill_from;
+    uint32_t refill_idle;   // a wave refills once at least this many of its lanes are idle (1..64)
+    uint32_t* rcur;         // per depth: kRefillCursors cursors, kRefillCurStride words apart (zeroed by generate)
 };
+
+// Lane refill: per depth, kRefillCursors cursors (kRefillCurStride words
+// apart, one 64-byte line each) deal out the queue in batches of
+// kRefillBatch positions, super-chunks of kRefillGroup batches interleaved
+// over the cursors.
+constexpr uint32_t kRefillCursors = 32;
+constexpr uint32_t kRefillCurStride = 16;
+constexpr uint32_t kRefillBatch = 64;
+constexpr uint32_t kRefillGroup = 16;
+constexpr size_t kRefillCurWords = (size_t)kMaxDepth * kRefillCursors * kRefillCurStride;
 
 // Counting sort of a depth's shadow entries by the Morton code of their point
 // in the root node's quantisation frame (kSortBits / 3 bits per axis), so that

@@ -62,11 +62,13 @@ __device__ __forceinline__ uint32_t primary_order(const PtParams& P, uint32_t k)
 }
 
 // Primary rays of the tile's pixels; block 0 also sets the tile's queue
-// counters (nothing else reads them while generate runs).
+// counters and zeroes its refill cursors (nothing else reads them while
+// generate runs).
 __global__ __launch_bounds__(256) void k_generate(PtParams P) {
     if (blockIdx.x == 0) {
         for (uint32_t k = threadIdx.x; k < kMaxDepth; k += blockDim.x) P.counters[CNT_SHQ + k] = 0u;
         for (uint32_t k = threadIdx.x; k <= kMaxDepth; k += blockDim.x) P.counters[CNT_QD + k] = k ? 0u : P.n_paths;
+        for (uint32_t k = threadIdx.x; k < kMaxDepth * kRefillCursors; k += blockDim.x) P.rcur[k * kRefillCurStride] = 0u;
     }
     const int32_t frame = *P.frame;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < P.n_paths; k += gridDim.x * blockDim.x) {
@@ -141,6 +143,81 @@ __device__ __forceinline__ bool rays_sorted(const PtParams& P, uint32_t d, uint3
     return P.ray_sort_from != 0 && d >= P.ray_sort_from && count > 0 && count >= (P.n_paths >> 4);
 }
 
+// The closest hit of the analytic primitives, after the mesh's
+// (escenario.h:323-334): a nearer analytic hit replaces (t, u, v, prim).
+__device__ __forceinline__ void analytic_hits(const PtParams& P, float4 a, float4 b, float& t, float& u, float& v,
+                                              uint32_t& prim) {
+    for (uint32_t k = 0; k < P.n_analytic; ++k) {
+        const float ta = analytic_t(P.analytic[k], v3(a.x, a.y, a.z), v3(b.x, b.y, b.z));
+        if (ta > a.w && ta < t) {
+            t = ta;
+            u = 0.0f;
+            v = 0.0f;
+            prim = kAnalyticBit | k;
+        }
+    }
+}
+
+// Closest hit of a launch with few rays: 8 lanes per ray (traverse_octet),
+// kBlock / 8 rays per chunk.  Workgroups without a chunk leave before
+// staging the top nodes.
+template <int kBlock, bool kStats>
+__device__ __forceinline__ void extend_octet(PtParams& P, const float4* __restrict__ rays,
+                                             const uint32_t* __restrict__ order, uint32_t count, float4* top,
+                                             uint2* stack) {
+    constexpr uint32_t kRays = kBlock / 8;
+    ChunkIter it((count + kRays - 1) / kRays, P.xcd_group / kRays, P.xcd_group != 0);
+    if (!it.valid()) return;
+    stage_top_nodes(P, top);
+    TraceStats st;   // accumulated over the chunks, flushed once per wave
+    const uint32_t j = threadIdx.x & 7u;
+    for (; it.valid(); it.next()) {
+        const uint32_t iq = it.chunk * kRays + (threadIdx.x >> 3);
+        if (iq < count) {
+            const uint32_t i = order ? order[iq] : iq;
+            const float4 a = rays[2 * (size_t)i];
+            const float4 b = rays[2 * (size_t)i + 1];
+            const RayPre r = make_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
+            float t, u, v;
+            uint32_t prim;
+            traverse_octet<false>(P.nodes, P.tris, top, P.n_top, r, stack + (threadIdx.x >> 3), kRays,
+                                  P.stack_depth, t, u, v, prim, kStats ? &st : nullptr);
+            analytic_hits(P, a, b, t, u, v, prim);
+            if (j == 0) P.hits[i] = make_float4(t, u, v, __uint_as_float(prim));
+        }
+    }
+    if (kStats) add_stats(P, 0, st);
+}
+
+// Closest hit in lockstep: one ray per lane, kBlock rays per chunk, chunks
+// in XCD super-chunk order.  Workgroups without a chunk leave before staging
+// the top nodes.
+template <int kBlock, bool kStats>
+__device__ __forceinline__ void extend_lockstep(PtParams& P, const float4* __restrict__ rays,
+                                                const uint32_t* __restrict__ order, uint32_t count, float4* top,
+                                                uint2* stack) {
+    ChunkIter it((count + kBlock - 1) / kBlock, P.xcd_group / kBlock, P.xcd_group != 0);
+    if (!it.valid()) return;
+    stage_top_nodes(P, top);
+    TraceStats st;   // accumulated over the chunks, flushed once per wave
+    for (; it.valid(); it.next()) {
+        const uint32_t iq = it.chunk * kBlock + threadIdx.x;
+        if (iq < count) {
+            const uint32_t i = order ? order[iq] : iq;
+            const float4 a = rays[2 * (size_t)i];
+            const float4 b = rays[2 * (size_t)i + 1];
+            const RayPre r = make_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
+            float t, u, v;
+            uint32_t prim;
+            traverse<0, false>(P.nodes, P.tris, top, P.n_top, r, stack + threadIdx.x, kBlock, P.stack_depth, t, u, v,
+                               prim, kStats ? &st : nullptr);
+            analytic_hits(P, a, b, t, u, v, prim);
+            P.hits[i] = make_float4(t, u, v, __uint_as_float(prim));
+        }
+    }
+    if (kStats) add_stats(P, 0, st);
+}
+
 template <int kBlock, bool kStats = false>
 __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint32_t depth) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -152,64 +229,165 @@ __global__ __launch_bounds__(kBlock) void k_extend(PtParams P, uint32_t q, uint3
     // with the ray order on, queue position i traces the ray at rperm[i] and
     // writes its hit at that ray's own slot (shade reads hits in queue order)
     const uint32_t* order = rays_sorted(P, depth, count) ? P.rperm : nullptr;
-    // few rays: 8 lanes per ray (traverse_octet), kBlock / 8 rays per chunk
-    const bool octet = count <= P.octet_below;
-    const uint32_t per_chunk = octet ? kBlock / 8 : kBlock;
-    ChunkIter it((count + per_chunk - 1) / per_chunk, P.xcd_group / per_chunk, P.xcd_group != 0);
-    if (!it.valid()) return;   // workgroups without a chunk leave before staging
-    stage_top_nodes(P, top);
-    TraceStats st;   // accumulated over the chunks, flushed once per wave
-    if (octet) {
-        constexpr uint32_t kRays = kBlock / 8;
-        const uint32_t j = threadIdx.x & 7u;
-        for (; it.valid(); it.next()) {
-            const uint32_t iq = it.chunk * kRays + (threadIdx.x >> 3);
-            if (iq < count) {
-                const uint32_t i = order ? order[iq] : iq;
-                const float4 a = rays[2 * (size_t)i];
-                const float4 b = rays[2 * (size_t)i + 1];
-                const RayPre r = make_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
-                float t, u, v;
-                uint32_t prim;
-                traverse_octet<false>(P.nodes, P.tris, top, P.n_top, r, stack + (threadIdx.x >> 3), kRays,
-                                      P.stack_depth, t, u, v, prim, kStats ? &st : nullptr);
-                for (uint32_t k = 0; k < P.n_analytic; ++k) {   // escenario.h:323-334
-                    const float ta = analytic_t(P.analytic[k], v3(a.x, a.y, a.z), v3(b.x, b.y, b.z));
-                    if (ta > a.w && ta < t) {
-                        t = ta;
-                        u = 0.0f;
-                        v = 0.0f;
-                        prim = kAnalyticBit | k;
-                    }
-                }
-                if (j == 0) P.hits[i] = make_float4(t, u, v, __uint_as_float(prim));
-            }
-        }
-        if (kStats) add_stats(P, 0, st);
+    if (count <= P.octet_below) extend_octet<kBlock, kStats>(P, rays, order, count, top, stack);
+    else extend_lockstep<kBlock, kStats>(P, rays, order, count, top, stack);
+}
+
+// Closest hit with lane refill (dynamic ray fetch, Aila & Laine 2009,
+// "Understanding the efficiency of ray traversal on GPUs").  Each lane runs
+// one ray's traversal loop (the steps of traverse(), so each ray decodes the
+// same nodes, tests the same triangles and gets the same hit); a wave no
+// longer waits for its slowest ray: whenever at least refill_idle of its
+// lanes have finished, they take the next rays of the queue.
+//
+// Hand-out: n = min(kRefillCursors, grid) cursors, workgroup b on cursor
+// b % n (with n a multiple of 8, each cursor's waves share one XCD:
+// workgroup b runs on XCD b % 8; every cursor has a workgroup).  Cursor c
+// deals out the queue's super-chunks c, c + n, ...
+// (kRefillGroup batches of kRefillBatch consecutive queue positions each),
+// one batch per atomic: every cursor gets an even share of the queue, and a
+// batch is a run of neighbouring rays.  Batch j of cursor c starts at a
+// position that grows with j, so once a wave draws a batch past the queue's
+// end every later draw on that cursor does too: the wave then stops drawing
+// and leaves when its lanes are done.  Each refill either starts a ray or
+// marks the wave drained, so every wave leaves.  A wave keeps the rest of its
+// batch (res_lo..res_hi) for its next refills.
+//
+// Launches with fewer than two rays per resident lane trace in lockstep (too
+// few batches per wave for refill to pay), launches of at most octet_below
+// rays with 8 lanes per ray.  The cursors (counters of this depth) are zeroed
+// by generate.
+template <int kBlock, bool kStats = false>
+__global__ __launch_bounds__(kBlock) void k_extend_refill(PtParams P, uint32_t q, uint32_t depth) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    float4* top = reinterpret_cast<float4*>(lds);
+    uint2* stack_base = reinterpret_cast<uint2*>(top + 5 * P.n_top);
+    const uint32_t count = P.counters[CNT_QD + depth];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && count) atomicAdd(&P.gcounters[CNT_EXT + depth], count);
+    if (count == 0) return;
+    const float4* rays = P.rays[q];
+    const uint32_t* order = rays_sorted(P, depth, count) ? P.rperm : nullptr;
+    if (count <= P.octet_below) {
+        extend_octet<kBlock, kStats>(P, rays, order, count, top, stack_base);
         return;
     }
-    for (; it.valid(); it.next()) {
-        const uint32_t b0 = it.chunk * kBlock;
-        const uint32_t iq = b0 + threadIdx.x;
-        if (iq < count) {
-            const uint32_t i = order ? order[iq] : iq;
-            const float4 a = rays[2 * (size_t)i];
-            const float4 b = rays[2 * (size_t)i + 1];
-            const RayPre r = make_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
-            float t, u, v;
-            uint32_t prim;
-            traverse<0, false>(P.nodes, P.tris, top, P.n_top, r, stack + threadIdx.x, kBlock, P.stack_depth, t, u, v,
-                               prim, kStats ? &st : nullptr);
-            for (uint32_t k = 0; k < P.n_analytic; ++k) {   // escenario.h:323-334
-                const float ta = analytic_t(P.analytic[k], v3(a.x, a.y, a.z), v3(b.x, b.y, b.z));
-                if (ta > a.w && ta < t) {
-                    t = ta;
-                    u = 0.0f;
-                    v = 0.0f;
-                    prim = kAnalyticBit | k;
+    if (count < 2u * gridDim.x * kBlock) {
+        extend_lockstep<kBlock, kStats>(P, rays, order, count, top, stack_base);
+        return;
+    }
+    stage_top_nodes(P, top);
+    uint2* stack = stack_base + threadIdx.x;
+    const uint32_t n_cur = min(kRefillCursors, gridDim.x);
+    const uint32_t c = blockIdx.x % n_cur;
+    uint32_t* cursor = P.rcur + ((size_t)depth * kRefillCursors + c) * kRefillCurStride;
+    const uint32_t lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;   // lanes below this one
+    uint32_t res_lo = 0, res_hi = 0;                // wave-uniform
+    bool drained = false;
+    // lane state: the ray and its traversal
+    bool busy = false;
+    uint32_t slot = 0;
+    RayPre r = make_ray(0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f);
+    int sp = 0;
+    uint32_t ng_x = 0, ng_y = 0, tg_x = 0, tg_y = 0, best_prim = kMiss;
+    float tcur = 0.f, bu = 0.f, bv = 0.f;
+    bool found = false;
+    TraceStats st;
+    for (;;) {
+        const uint64_t idle = __ballot(!busy);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (!drained && n_idle >= P.refill_idle) {
+            const uint32_t have = res_hi - res_lo;
+            uint32_t nb_lo = 0, nb_hi = 0;   // a new batch, drawn when the reservoir is short
+            if (have < n_idle) {
+                uint32_t j = 0;
+                if (lane == 0) j = atomicAdd(cursor, 1u);
+                j = __builtin_amdgcn_readfirstlane(j);   // every lane is active here
+                const uint32_t pos = ((j / kRefillGroup) * n_cur + c) * (kRefillGroup * kRefillBatch) +
+                                     (j % kRefillGroup) * kRefillBatch;
+                if (pos < count) {
+                    nb_lo = pos;
+                    nb_hi = min(pos + kRefillBatch, count);
+                } else {
+                    drained = true;
                 }
             }
-            P.hits[i] = make_float4(t, u, v, __uint_as_float(prim));
+            if (!busy) {   // the k-th idle lane takes the reservoir's k-th ray, then the batch's
+                const uint32_t k = (uint32_t)__popcll(idle & below);
+                uint32_t iq = kMiss;
+                if (k < have) iq = res_lo + k;
+                else if (k - have < nb_hi - nb_lo) iq = nb_lo + (k - have);
+                if (iq != kMiss) {
+                    slot = order ? order[iq] : iq;
+                    const float4 ra = rays[2 * (size_t)slot];
+                    const float4 rb = rays[2 * (size_t)slot + 1];
+                    r = make_ray(ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, ra.w, rb.w);
+                    sp = 0;
+                    ng_x = 0;
+                    ng_y = 0x80000000u;
+                    tg_x = 0;
+                    tg_y = 0;
+                    tcur = r.tmax;
+                    best_prim = kMiss;
+                    bu = 0.f;
+                    bv = 0.f;
+                    found = false;
+                    busy = true;
+                }
+            }
+            if (n_idle <= have) {
+                res_lo += n_idle;
+            } else {
+                res_lo = nb_lo + min(n_idle - have, nb_hi - nb_lo);
+                res_hi = nb_hi;
+            }
+        }
+        if (__ballot(busy) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (busy) {   // one step of traverse()'s loop
+            if (ng_y <= 0x00FFFFFFu && sp > 0) {
+                --sp;
+                const uint2 e = stack[sp * kBlock];
+                ng_x = e.x;
+                ng_y = e.y;
+            }
+            const bool have_node = ng_y > 0x00FFFFFFu;
+            float4 n0, n1, n2, n3, n4;
+            if (have_node) {
+                const uint32_t imask = ng_y;
+                const int bit = highest_bit(ng_y & 0xFF000000u);
+                ng_y &= ~(1u << bit);
+                if (ng_y > 0x00FFFFFFu) {
+                    if (sp < P.stack_depth) {
+                        stack[sp * kBlock] = make_uint2(ng_x, ng_y);
+                        ++sp;
+                    } else if (kStats) {
+                        st.overflow = 1;   // host refuses trees this deep
+                    }
+                }
+                const uint32_t cslot = ((uint32_t)bit - 24u) ^ r.octinv;
+                const uint32_t rel = (uint32_t)__popc(imask & ~(0xFFFFFFFFu << cslot));
+                load_node(P.nodes, top, P.n_top, ng_x + rel, n0, n1, n2, n3, n4);
+            }
+            test_triangles<0, false>(P.tris, tg_x, tg_y, r, tcur, bu, bv, best_prim, found,
+                                     kStats ? &st : nullptr);
+            if (have_node) {
+                if (kStats) st.nodes++;
+                const uint32_t hm = node_hitmask(n0, n1, n2, n3, n4, r, tcur);
+                ng_x = __float_as_uint(n1.x);
+                ng_y = (hm & 0xFF000000u) | (__float_as_uint(n0.w) >> 24);
+                tg_x = __float_as_uint(n1.y);
+                tg_y = hm & 0x00FFFFFFu;
+            } else {   // the ray is done: traverse()'s result, then the analytic primitives
+                float t = tcur, u = found ? bu : 0.f, v = found ? bv : 0.f;
+                uint32_t prim = found ? best_prim : kMiss;
+                analytic_hits(P, make_float4(r.O[0], r.O[1], r.O[2], r.tmin),
+                              make_float4(r.D[0], r.D[1], r.D[2], r.tmax), t, u, v, prim);
+                P.hits[slot] = make_float4(t, u, v, __uint_as_float(prim));
+                busy = false;
+            }
         }
     }
     if (kStats) add_stats(P, 0, st);
