@@ -206,7 +206,9 @@ def _stage_roofline(prof, frames, stage, n_launch):
 def bench_pt(args, torch):
     """Plain FP32 wavefront path tracer, 64-instance scene in its closed room,
     1920x1080, 1 spp per frame, max depth 8, Russian roulette after depth 3;
-    frames with the saved tuned schedule (or a fresh tune with --tune).  One
+    frames with the saved tuned schedule (or a fresh tune with --tune),
+    replayed as per-frame hipGraphs (the schedule is tuned, profiled and
+    compared on direct launches; the A/B of both is part of the line).  One
     step = one frame.
 
     Roofline: the traversal stages are then timed on `steps` further frames
@@ -229,8 +231,8 @@ def bench_pt(args, torch):
     W, H, MD, RR = 1920, 1080, 8, 3
     img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
     accum = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
-    # direct launches: faster than per-frame graph replay in the overlapped
-    # schedule (the A/B below measures both)
+    # direct launches for tuning and the profiled passes; the timed frames
+    # replay the same schedule as per-frame hipGraphs
     tracer = pt.PathTracer(W, H, MD, RR, mode=0, use_graph=False)
     tracer.set_scene(ds)
     tracer.set_camera(sc.camera.frame(W, H))
@@ -242,10 +244,15 @@ def bench_pt(args, torch):
         tuned = "saved (bench.TUNED_SCHEDULE_1080P, an earlier tfg_pt_autotune pick on MI355X)"
     shape = tracer.launch_shape()
     _log(f"schedule ({tuned}) {shape}")
+    timed = pt.PathTracer(W, H, MD, RR, mode=0, use_graph=True)
+    timed.set_scene(ds)
+    timed.set_camera(sc.camera.frame(W, H))
+    timed.set_schedule(**{k: v for k, v in shape.items() if k in timed.schedule() and k != "use_graph"})
+    timed.render(0, img, accum)   # captures the frame's graphs (set-up, whatever the warmup count)
     for f in range(args.warmup):
-        tracer.render(f, img, accum)
+        timed.render(f, img, accum)
     torch.cuda.synchronize()
-    tracer.reset_totals()
+    timed.reset_totals()
     stream = torch.cuda.current_stream()
     a = torch.cuda.Event(enable_timing=True)
     b = torch.cuda.Event(enable_timing=True)
@@ -253,15 +260,20 @@ def bench_pt(args, torch):
     t_start = time.perf_counter()
     a.record(stream)
     for k in range(args.steps):
-        tracer.render(1000 + k, img, accum, stream=stream.cuda_stream)
+        timed.render(1000 + k, img, accum, stream=stream.cuda_stream)
     b.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     ms_frame = a.elapsed_time(b) / args.steps
-    ext, shd = tracer.ray_totals()
+    ext, shd = timed.ray_totals()
     rays_frame = (ext + shd) / args.steps
     value = rays_frame / (ms_frame * 1e-3) / 1e6
-    _log(f"timed {args.steps} frames: {ms_frame:.4f} ms per frame, {value:.1f} Mrays/s")
+    timed_shape = timed.launch_shape()
+    timed.close()
+    _log(f"timed {args.steps} frames (graph replay): {ms_frame:.4f} ms per frame, {value:.1f} Mrays/s")
+    for f in range(args.warmup):
+        tracer.render(f, img, accum)
+    torch.cuda.synchronize()
     # the same frames replayed as per-frame hipGraphs (tfg_pt_render with the
     # graph option), against direct launches, in the timed schedule and with
     # the shadow stage on the main stream (no fork in the graph)
@@ -347,7 +359,7 @@ def bench_pt(args, torch):
         "config": {
             "workload": "plain FP32 wavefront PT: 64 seeded instances (32 displaced spheres + 32 boxes, "
                         f"{sc.n_tris} tris) in a closed 12-triangle room, 4 point lights, Lambertian+GGX; 1920x1080, "
-                        "1 spp/frame, max depth 8, RR after depth 3; direct launches, schedule in launch",
+                        "1 spp/frame, max depth 8, RR after depth 3; per-frame hipGraph replay, schedule in launch",
             "tris": sc.n_tris,
             "cwbvh_nodes": ds.host.num_nodes,
             "rays_per_frame": rays_frame,
@@ -357,7 +369,7 @@ def bench_pt(args, torch):
                 "extend": [round(float(np.mean([p["ms_extend"][i] for p in rprof])), 4) for i in range(MD)],
                 "shade": [round(float(np.mean([p["ms_shade"][i] for p in rprof])), 4) for i in range(MD)],
                 "connect": [round(float(np.mean([p["ms_connect"][i] for p in rprof])), 4) for i in range(MD)]},
-            "launch": shape,
+            "launch": timed_shape,
             "schedule_source": tuned,
             "cwbvh_build_s": round(t_build, 2),
             "cwbvh_refit_ms": round(refit_ms, 3),
