@@ -31,6 +31,7 @@ launch time is measured with HIP events on the stream it runs on.
 from __future__ import annotations
 
 import argparse
+import faulthandler
 import json
 import os
 import sys
@@ -40,6 +41,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# a fatal signal prints the Python stack of every thread before the process ends
+faulthandler.enable(all_threads=True)
 
 METRIC = "Mrays/s + equal-time relMSE, NRC vs plain PT, 1080p, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
@@ -269,15 +272,19 @@ def bench_pt(args, torch):
     rays_frame = (ext + shd) / args.steps
     value = rays_frame / (ms_frame * 1e-3) / 1e6
     timed_shape = timed.launch_shape()
-    timed.close()
     _log(f"timed {args.steps} frames (graph replay): {ms_frame:.4f} ms per frame, {value:.1f} Mrays/s")
     for f in range(args.warmup):
         tracer.render(f, img, accum)
     torch.cuda.synchronize()
     # the same frames replayed as per-frame hipGraphs (tfg_pt_render with the
     # graph option), against direct launches, in the timed schedule and with
-    # the shadow stage on the main stream (no fork in the graph)
-    graph_ab = _graph_vs_direct(torch, tracer, ds, sc, W, H, MD, RR, img, accum, stream, args.steps)
+    # the shadow stage on the main stream (no fork in the graph); each graph
+    # tracer captures its frame once and is closed only after the device is
+    # idle (host faults inside the HIP runtime were seen in runs that dropped
+    # and re-captured graphs many times)
+    graph_ab = _graph_vs_direct(torch, tracer, timed, ds, sc, W, H, MD, RR, img, accum, stream, args.steps)
+    torch.cuda.synchronize()
+    timed.close()
     # one profiled frame in the timed schedule (stages overlapped as timed)
     prof = tracer.profile_frame(1000, img)
     # roofline frames: the same schedule serialised (overlap off)
@@ -383,22 +390,26 @@ def bench_pt(args, torch):
     return res
 
 
-def _graph_vs_direct(torch, direct, ds, sc, W, H, MD, RR, img, accum, stream, steps):
+def _graph_vs_direct(torch, direct, graph_on, ds, sc, W, H, MD, RR, img, accum, stream, steps):
     """ms per frame of per-frame hipGraph replay and of direct launches over
-    the same `steps` frames, in the direct tracer's schedule (overlap on) and
-    with the shadow stage on the main stream (overlap off)."""
+    the same `steps` frames, in the direct tracer's schedule (overlap on:
+    graph_on, the timed tracer, already captured with the same image and
+    accumulation buffers) and with the shadow stage on the main stream
+    (overlap off: a second graph tracer, given its schedule before its first
+    frame, so that each graph tracer captures once)."""
     from tfg_amd import pt
 
-    graph = pt.PathTracer(W, H, MD, RR, mode=0, use_graph=True)
-    graph.set_scene(ds)
-    graph.set_camera(sc.camera.frame(W, H))
     shape = direct.launch_shape()
-    graph.set_schedule(**{k: v for k, v in shape.items() if k in graph.schedule() and k != "use_graph"})
+    graph_off = pt.PathTracer(W, H, MD, RR, mode=0, use_graph=True)
+    graph_off.set_scene(ds)
+    graph_off.set_camera(sc.camera.frame(W, H))
+    graph_off.set_schedule(**{k: v for k, v in shape.items() if k in graph_off.schedule() and k != "use_graph"})
+    graph_off.set_schedule(overlap=False)
     out = {}
     for overlap in (True, False):
         ms = {}
-        for name, tr in (("direct", direct), ("graph", graph)):
-            tr.set_schedule(overlap=overlap)
+        direct.set_schedule(overlap=overlap)
+        for name, tr in (("direct", direct), ("graph", graph_on if overlap else graph_off)):
             for f in range(3):
                 tr.render(f, img, accum, stream=stream.cuda_stream)
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -412,7 +423,8 @@ def _graph_vs_direct(torch, direct, ds, sc, W, H, MD, RR, img, accum, stream, st
             "direct_ms_per_frame": round(ms["direct"], 4), "graph_ms_per_frame": round(ms["graph"], 4),
             "graph_over_direct": round(ms["graph"] / ms["direct"], 4)}
     direct.set_schedule(overlap=shape["overlap"])
-    graph.close()
+    torch.cuda.synchronize()
+    graph_off.close()
     _log(f"graph replay vs direct: {out}")
     return out
 
@@ -430,8 +442,14 @@ def bench_pt1080(args, torch):
     sc = res["_cpu_inputs"][0]
     W, H, MD, RR = 1920, 1080, 8, 3
     t0 = time.time()
+    # the timed schedule on direct launches: per-frame graph replay and direct
+    # launches take the same time (graph_replay), the cache frames then train
+    # one frame behind beside the next frame's tracing (defer_training), and
+    # the runs create, re-mode and destroy a dozen tracers without capturing
+    # and dropping graphs each time
+    eq_sched = dict(res["config"]["launch"], use_graph=False)
     cmp = experiments.equal_time(torch, sc, W, H, MD, RR, ref_spp=args.ref_spp, budget_s=args.budget,
-                                 warmup=48, precision=0, schedule=res["config"]["launch"], repeats=args.repeats)
+                                 warmup=48, precision=0, schedule=eq_sched, repeats=args.repeats)
     _log(f"equal-time comparison: {json.dumps(cmp)} ({time.time() - t0:.1f}s)")
     res["relmse"] = cmp
     t0 = time.time()
@@ -444,6 +462,7 @@ def bench_pt1080(args, torch):
     res["config"]["equal_time_budget_s"] = args.budget
     res["config"]["equal_time_repeats"] = args.repeats
     res["config"]["cache_frame_tail_from"] = experiments.CACHE_TAIL_FROM
+    res["config"]["equal_time_launch"] = "direct launches of the timed schedule; cache frames train deferred"
     return res
 
 

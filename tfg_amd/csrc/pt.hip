@@ -103,7 +103,17 @@ int dalloc(tfg_pt* pt, T** p, size_t count) {
     return TFG_OK;
 }
 
+// Destroys the captured graphs.  Every entry point that calls this may follow
+// an asynchronous tfg_pt_render, so replays still in flight (tile graphs on
+// the tile streams, the cache graph on the tracer's stream) finish first.
 void drop_graph(tfg_pt* pt) {
+    bool any = pt->exec_post != nullptr;
+    for (uint32_t t = 0; t < kMaxTiles; ++t) any = any || pt->exec_tile[t] != nullptr;
+    if (any) {
+        for (uint32_t t = 0; t < kMaxTiles; ++t)
+            if (pt->tstream[t]) (void)hipStreamSynchronize(pt->tstream[t]);
+        if (pt->stream) (void)hipStreamSynchronize(pt->stream);
+    }
     for (uint32_t t = 0; t < kMaxTiles; ++t) {
         if (pt->exec_tile[t]) (void)hipGraphExecDestroy(pt->exec_tile[t]);
         if (pt->graph_tile[t]) (void)hipGraphDestroy(pt->graph_tile[t]);
