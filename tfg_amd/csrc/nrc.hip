@@ -44,20 +44,20 @@ struct tfg_nrc {
     float* adam_v = nullptr;
     float* ema = nullptr;
     float* grad = nullptr;
-    float* partial = nullptr;   // (batch / 64) x kNrcParams
+    float* partial = nullptr;      // one kNrcParams gradient per training workgroup
+    float* block_loss = nullptr;   // one loss per training workgroup
+    float* loss_sum = nullptr;     // their sum, for the step's closing
+    // prepared samples (head vertex + target) of one call's steps
+    float4* prep = nullptr;
+    size_t prep_cap = 0;           // samples
     // operand layouts, 16-bit (bf16 or fp16 by cfg.precision)
     void* fop_train = nullptr;
     void* bop_train = nullptr;
     void* fop_infer = nullptr;
-    // the inference (EMA) operands as they were when a call's optimiser
-    // steps were enqueued: every step of the call forms its targets with
-    // them (the reference computes a frame's targets once, before its step)
-    void* fop_tail = nullptr;
     // device scalars: [0] step count t, [1] seed counter, [2] flags of the
     // current step, [3] skipped (non-finite) steps, [4] no-data steps,
     // [5] loss of the last step (float bits), [6] steps taken
     uint32_t* state = nullptr;
-    float* loss_accum = nullptr;
     int n_cu = 0;
     // DatosMLP entry points (ColorMLP::train_step / inference): converted
     // inputs, outputs and the sample count, grown on demand
@@ -116,20 +116,23 @@ __host__ __device__ inline size_t bop_pos(int p) {   // layers 1..5
 // Copies `kTiles` operand tiles (64 x 16 B each) into LDS with every load of
 // a thread in flight before its first store: one memory latency per copy
 // instead of one per 16 B entry.
+// (The entries move as plain 16-byte words: copied as vectors of 16-bit
+// floats they were split into halves and re-packed.)
 template <typename T, int kTiles, int kThreads>
 __device__ __forceinline__ void load_ops(typename MfmaOps<T>::vec* dst, const typename MfmaOps<T>::vec* src) {
-    typedef typename MfmaOps<T>::vec V;
     constexpr int kN = kTiles * 64, kPer = (kN + kThreads - 1) / kThreads;
-    V tmp[kPer];
+    const uint4v* s = reinterpret_cast<const uint4v*>(src);
+    uint4v* d = reinterpret_cast<uint4v*>(dst);
+    uint4v tmp[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const int i = (int)threadIdx.x + k * kThreads;
-        if (i < kN) tmp[k] = src[i];
+        if (i < kN) tmp[k] = s[i];
     }
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const int i = (int)threadIdx.x + k * kThreads;
-        if (i < kN) dst[i] = tmp[k];
+        if (i < kN) d[i] = tmp[k];
     }
 }
 
@@ -224,95 +227,176 @@ __global__ __launch_bounds__(256) void k_nrc_eval(const float4* __restrict__ ver
 }
 
 // ------------------------------------------------------------- training
-struct TrainArgs {
-    const float4* records;   // all records, or (gathered) this step's batch in sample order
-    int gathered;
+// Prepared training sample: the head vertex (kVertF4 float4) and its target
+// (rgb, 0), what an optimiser step reads per sample.
+constexpr int kPrepF4 = kVertF4 + 1;
+
+// Training records -> the prepared samples of `steps` optimiser steps.  Step
+// k (seed = the seed counter + k: every step advances the counter by one,
+// mlp.cpp:102) draws records[pcg32(seed ^ i) % n] for its sample i
+// (generarIndicesAleatorios, NrcKernel.cu:10-16).  The sample's target
+// (L_suf + T_suf * cache(tail)) / max(normaliser, 1e-8), clamped to 1e5,
+// non-finite -> 0 (kernelPrepararInferenciaTail + ColorMLP::inference +
+// kernelCalcularTargets, RenderKernel.cu:92-166, main.cpp:1044-1052), uses
+// the inference (EMA) weights as they are when this kernel runs: once per
+// call, before its steps, as the reference computes a frame's targets once
+// before its step.  Workgroups of 8 waves share one LDS copy of the weights
+// and loop over the samples (the weights are staged once per workgroup, not
+// once per 128 samples); a wave prepares 16 samples at a time, lane group g
+// copying head float4 g.
+struct PrepArgs {
+    const float4* records;
     const uint32_t* n_rec;
     uint32_t min_records;
-    uint32_t* state;
+    const uint32_t* state;
+    uint32_t batch, steps;
+    NrcBounds b;
+    const void* fop;
+    float4* out;       // kPrepF4 float4 per sample, samples of step k at k * batch
+    uint32_t* out_n;   // nullable: receives the record count
+};
+
+constexpr int kPrepBlock = 512;
+template <typename T>
+__global__ __launch_bounds__(kPrepBlock) void k_nrc_prepare(PrepArgs A) {
+    typedef typename MfmaOps<T>::vec V;
+    __shared__ V W[kFopTiles * 64];
+    const uint32_t n = *A.n_rec;
+    if (A.out_n && blockIdx.x == 0 && threadIdx.x == 0) *A.out_n = n;
+    if (n <= A.min_records) return;   // the steps will skip for lack of data
+    load_ops<T, kFopTiles, kPrepBlock>(W, static_cast<const V*>(A.fop));
+    __syncthreads();
+    const uint32_t seed0 = A.state[kStSeed];
+    const uint32_t total = A.batch * A.steps;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, ml = lane & 15;
+    constexpr uint32_t kWaves = kPrepBlock / 64;
+    for (uint32_t base = (blockIdx.x * kWaves + w) * 16u; base < total; base += gridDim.x * kWaves * 16u) {
+        const uint32_t j = base + ml;
+        const bool valid = j < total;
+        const float4* R = A.records;
+        V x[1][2];
+        x[0][0] = V{};
+        x[0][1] = V{};
+        if (valid) {
+            const uint32_t k = j / A.batch, i = j - k * A.batch;
+            R += (size_t)kRecF4 * (pcg32_step((seed0 + k) ^ i) % n);
+            encode<T>(R[4], R[5], R[6], R[7], A.b, g, x[0][0], x[0][1]);
+        }
+        floatx4 out[1];
+        mlp_forward<T, 1>(W, lane, x, out);   // the whole wave: matrix-core operations
+        if (!valid) continue;
+        float4* o = A.out + (size_t)kPrepF4 * j;
+        o[g] = R[g];
+        if (g == 0) {
+            const float4 ls = R[8], ts = R[9], nm = R[10];
+            const float lsv[3] = {ls.x, ls.y, ls.z}, tsv[3] = {ts.x, ts.y, ts.z}, nmv[3] = {nm.x, nm.y, nm.z};
+            float tgt[3];
+            for (int c = 0; c < 3; ++c) {
+                float t = (lsv[c] + tsv[c] * sanitize_out(out[0][c])) / fmaxf(nmv[c], 1e-8f);
+                t = fminf(t, 100000.0f);
+                if (isnan(t) || isinf(t)) t = 0.0f;
+                tgt[c] = t;
+            }
+            o[kVertF4] = make_float4(tgt[0], tgt[1], tgt[2], 0.0f);
+        }
+    }
+}
+
+// One optimiser step's forward and backward passes.  A workgroup holds 128
+// samples (8 waves of 16), one LDS copy of the training weights (forward and
+// transposed operands, 78 KB) and the transposed dy / x tiles of its samples
+// (34 KB): one workgroup per CU, 2 waves per SIMD.  Per layer, from the
+// output back, each wave forms its samples' input gradient in registers,
+// then after a barrier two 16 x 16 tiles of the weight gradient summed over
+// the workgroup's 128 samples (4 matrix-core steps of 32 samples each),
+// written as this workgroup's partial gradient (batch / 128 partials per
+// step).  Measured at 2^14 samples: 4 steps in 0.131 ms; 16 waves of 256
+// samples per workgroup (64 workgroups, 4 waves per SIMD) took 0.146 ms.
+constexpr int kTrainWaves = 8;
+static_assert(kTrainWaves >= 4 && kTrainWaves <= 16 && 16 % kTrainWaves == 0, "training waves per workgroup");
+constexpr int kTrainThreads = 64 * kTrainWaves;
+constexpr int kTrainSamples = 16 * kTrainWaves;
+// Row stride (16-bit elements) of the transposed dy / x tiles in LDS: the
+// workgroup's samples plus 8, so that the 4 lane groups writing rows 4 apart
+// fall on different banks; rows stay 16-byte aligned for the operand reads.
+constexpr int kTS = kTrainSamples + 8;
+constexpr size_t kTrainLds =
+    (size_t)(kFopTiles + kBopTiles) * 64 * kOpBytes16 + 2 * 64 * kTS * 2 + kTrainWaves * sizeof(float);
+static_assert(kTrainLds <= 160 * 1024, "training workgroup exceeds the CU's LDS");
+
+inline uint32_t train_blocks(uint32_t batch) { return (batch + kTrainSamples - 1) / kTrainSamples; }
+
+struct TrainArgs {
+    const float4* prep;   // this step's prepared samples, in sample order
+    const uint32_t* n_rec;
+    uint32_t min_records;
     uint32_t batch;
     float grad_scale;   // loss_scale / (3 * batch)
     float loss_norm;    // 1 / (3 * batch)
     NrcBounds b;
     const void* fop_train;
     const void* bop_train;
-    const void* fop_tail;
-    float* partial;
-    float* loss_accum;
+    float* partial;      // kNrcParams per workgroup
+    float* block_loss;   // one per workgroup
 };
 
-// Row stride (16-bit elements) of the transposed dy / x tiles in LDS: 72,
-// not 64, so that the 4 lane groups writing rows 4 apart hit different banks
-// (a 128-byte stride put them on one bank); rows stay 16-byte aligned for the
-// operand reads.
-constexpr int kTS = 72;
-constexpr size_t kTrainLds = (size_t)(2 * kFopTiles + kBopTiles) * 64 * kOpBytes16 + 2 * 64 * kTS * 2;
-
-// One block = 64 training samples (4 waves x 16).  Tail inference with the
-// inference (EMA) weights of the call's start (kernelPrepararInferenciaTail
-// + ColorMLP::inference, RenderKernel.cu:139-166, main.cpp:1047-1050),
-// targets (RenderKernel.cu:92-137), head forward
-// with the training weights, RelativeL2Luminance gradient, backward through
-// all layers and this block's weight gradient (summed over its samples).
 template <typename T>
-__global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
+__device__ __forceinline__ floatx4 wgrad_tile(const T* s_dyT, const T* s_xT, int row0, int col0, int ml, int g) {
+    typedef MfmaOps<T> M;
+    typedef typename M::vec V;
+    floatx4 a0 = floatx4{0.f, 0.f, 0.f, 0.f}, a1 = floatx4{0.f, 0.f, 0.f, 0.f};
+    // two k-steps per iteration (the operands of all of them in flight at
+    // once cost registers the saved activations need)
+#pragma unroll 1
+    for (int c = 0; c < kTrainSamples / 32; c += 2) {
+        a0 = M::mfma(*reinterpret_cast<const V*>(&s_dyT[(row0 + ml) * kTS + 32 * c + 8 * g]),
+                     *reinterpret_cast<const V*>(&s_xT[(col0 + ml) * kTS + 32 * c + 8 * g]), a0);
+        a1 = M::mfma(*reinterpret_cast<const V*>(&s_dyT[(row0 + ml) * kTS + 32 * (c + 1) + 8 * g]),
+                     *reinterpret_cast<const V*>(&s_xT[(col0 + ml) * kTS + 32 * (c + 1) + 8 * g]), a1);
+    }
+    return a0 + a1;
+}
+
+// A packed pair of 16-bit values into column `at` of two consecutive rows
+// (`stride` elements apart) of a transposed tile: low half first.
+template <typename T>
+__device__ __forceinline__ void put_pair(T* tile, int at, int stride, uint32_t pair) {
+    uint16_t* p = reinterpret_cast<uint16_t*>(tile);
+    p[at] = (uint16_t)(pair & 0xFFFFu);
+    p[at + stride] = (uint16_t)(pair >> 16);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTrainThreads) void k_nrc_train(TrainArgs A) {
     typedef MfmaOps<T> M;
     typedef typename M::vec V;
     extern __shared__ __align__(16) unsigned char smem[];
     V* s_fop = reinterpret_cast<V*>(smem);
-    V* s_inf = s_fop + kFopTiles * 64;
-    V* s_bop = s_inf + kFopTiles * 64;
-    T* s_dyT = reinterpret_cast<T*>(s_bop + kBopTiles * 64);   // [64 neurons][kTS: 64 samples]
-    T* s_xT = s_dyT + 64 * kTS;                                // [64 inputs][kTS: 64 samples]
-    const uint32_t n_rec = *A.n_rec;
-    if (n_rec <= A.min_records) return;
+    V* s_bop = s_fop + kFopTiles * 64;
+    T* s_dyT = reinterpret_cast<T*>(s_bop + kBopTiles * 64);   // [64 neurons][kTS: samples]
+    T* s_xT = s_dyT + 64 * kTS;                                // [64 inputs][kTS: samples]
+    float* s_loss = reinterpret_cast<float*>(s_xT + 64 * kTS);
+    if (*A.n_rec <= A.min_records) return;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, ml = lane & 15;
-    const int m = 16 * w + ml;   // sample within the block
-    const uint32_t i = blockIdx.x * 64 + m;
-    // generarIndicesAleatorios (NrcKernel.cu:10-16): seed = step counter
-    const uint32_t idx = A.gathered ? i : pcg32_step(A.state[kStSeed] ^ i) % n_rec;
-    const float4* R = A.records + (size_t)kRecF4 * idx;
-    float4 rec[kRecF4];   // the sample's record, in flight while the weights are staged
+    const int m = 16 * w + ml;   // sample within the workgroup
+    const uint32_t i = blockIdx.x * kTrainSamples + m;
+    const bool valid = i < A.batch;
+    float4 smp[kPrepF4];   // in flight while the weights are staged
 #pragma unroll
-    for (int k = 0; k < kRecF4; ++k) rec[k] = R[k];
-    load_ops<T, kFopTiles, 256>(s_fop, static_cast<const V*>(A.fop_train));
-    load_ops<T, kFopTiles, 256>(s_inf, static_cast<const V*>(A.fop_tail));
-    load_ops<T, kBopTiles, 256>(s_bop, static_cast<const V*>(A.bop_train));
+    for (int k = 0; k < kPrepF4; ++k) smp[k] = valid ? A.prep[(size_t)kPrepF4 * i + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    load_ops<T, kFopTiles, kTrainThreads>(s_fop, static_cast<const V*>(A.fop_train));
+    load_ops<T, kBopTiles, kTrainThreads>(s_bop, static_cast<const V*>(A.bop_train));
     __syncthreads();
 
-    // tail inference (EMA weights) and head forward (training weights, layer
-    // inputs kept for the backward pass), one layer of each at a time: the
-    // two chains are independent and fill each other's latencies
-    V xt[1][2], x[1][2];
-    encode<T>(rec[4], rec[5], rec[6], rec[7], A.b, g, xt[0][0], xt[0][1]);
-    encode<T>(rec[0], rec[1], rec[2], rec[3], A.b, g, x[0][0], x[0][1]);
+    V x[1][2];
+    encode<T>(smp[0], smp[1], smp[2], smp[3], A.b, g, x[0][0], x[0][1]);
     V xs[kNrcLayers][2];
-#pragma unroll
-    for (int l = 0; l < kNrcHidden; ++l) {
-        xs[l][0] = x[0][0];
-        xs[l][1] = x[0][1];
-        mlp_layer<T, 1>(s_inf, lane, l, xt);
-        mlp_layer<T, 1>(s_fop, lane, l, x);
-    }
-    xs[5][0] = x[0][0];
-    xs[5][1] = x[0][1];
-    floatx4 out_t[1], out[1];
-    mlp_output<T, 1>(s_inf, lane, xt, out_t);
-    mlp_output<T, 1>(s_fop, lane, x, out);
-    float tgt[3] = {0.f, 0.f, 0.f};
-    if (g == 0) {
-        const float4 ls = rec[8], ts = rec[9], nm = rec[10];
-        const float lsv[3] = {ls.x, ls.y, ls.z}, tsv[3] = {ts.x, ts.y, ts.z}, nmv[3] = {nm.x, nm.y, nm.z};
-        for (int c = 0; c < 3; ++c) {
-            float t = (lsv[c] + tsv[c] * sanitize_out(out_t[0][c])) / fmaxf(nmv[c], 1e-8f);
-            t = fminf(t, 100000.0f);
-            if (isnan(t) || isinf(t)) t = 0.0f;
-            tgt[c] = t;
-        }
-    }
+    floatx4 out[1];
+    mlp_forward_keep<T>(s_fop, lane, x, out, xs);
     floatx4 dy5 = floatx4{0.f, 0.f, 0.f, 0.f};
     float loss = 0.0f;
-    if (g == 0) {   // RelativeL2Luminance: (p - t)^2 / (lum(p)^2 + 0.01), lum held constant
+    if (g == 0 && valid) {   // RelativeL2Luminance: (p - t)^2 / (lum(p)^2 + 0.01), lum held constant
+        const float tgt[3] = {smp[kVertF4].x, smp[kVertF4].y, smp[kVertF4].z};
         const float lum = kLumR * out[0][0] + kLumG * out[0][1] + kLumB * out[0][2];
         const float inv = 1.0f / (lum * lum + 0.01f);
         for (int c = 0; c < 3; ++c) {
@@ -322,17 +406,19 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
         }
     }
     for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o);
-    if (lane == 0) atomicAdd(A.loss_accum, loss * A.loss_norm);
+    if (lane == 0) s_loss[w] = loss;
 
     float* part = A.partial + (size_t)blockIdx.x * kNrcParams;
     // ---- layer 5 (16 x 64): every row of the 16 is written (rows 3..15 zero) ----
-    for (int ii = 0; ii < 4; ++ii) s_dyT[(4 * g + ii) * kTS + m] = (T)dy5[ii];
+    const uint32_t d5[2] = {cvt2<T>(dy5[0], dy5[1]), cvt2<T>(dy5[2], dy5[3])};
+    for (int h = 0; h < 2; ++h) {
+        put_pair(s_dyT, (4 * g + 2 * h) * kTS + m, kTS, d5[h]);
+    }
     for (int c = 0; c < 2; ++c)
         for (int j = 0; j < 8; ++j) s_xT[nrc_perm(c, g, j) * kTS + m] = xs[5][c][j];
     floatx4 dy[4];
     {   // input gradient of layer 5 (registers only), before the barrier
-        V bdy;
-        for (int j = 0; j < 4; ++j) { bdy[j] = (T)dy5[j]; bdy[4 + j] = (T)0.0f; }
+        const V bdy = __builtin_bit_cast(V, (uint4v){d5[0], d5[1], 0u, 0u});
         for (int u = 0; u < 4; ++u) {
             dy[u] = M::mfma(s_bop[(bop_tile_base(5) + u) * 64 + lane], bdy, floatx4{0.f, 0.f, 0.f, 0.f});
             for (int ii = 0; ii < 4; ++ii)
@@ -340,33 +426,33 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
         }
     }
     __syncthreads();
-    {
-        const int u = w;   // k-tile of this wave
-        floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
-        for (int c = 0; c < 2; ++c) {
-            const V a = *reinterpret_cast<const V*>(&s_dyT[ml * kTS + 32 * c + 8 * g]);
-            const V bb = *reinterpret_cast<const V*>(&s_xT[(16 * u + ml) * kTS + 32 * c + 8 * g]);
-            acc = M::mfma(a, bb, acc);
-        }
-        for (int ii = 0; ii < 4; ++ii) part[nrc_layer_offset(5) + (4 * g + ii) * 64 + 16 * u + ml] = acc[ii];
+    if (threadIdx.x == 0) {   // the workgroup's loss, waves summed in order
+        float L = 0.0f;
+        for (int k = 0; k < kTrainWaves; ++k) L += s_loss[k];
+        A.block_loss[blockIdx.x] = L * A.loss_norm;
+    }
+    if (w < 4) {   // 4 tiles of 16 inputs
+        const floatx4 acc = wgrad_tile<T>(s_dyT, s_xT, 0, 16 * w, ml, g);
+        for (int ii = 0; ii < 4; ++ii) part[nrc_layer_offset(5) + (4 * g + ii) * 64 + 16 * w + ml] = acc[ii];
     }
     __syncthreads();
     // ---- layers 4..0 (unrolled: the saved operands stay in registers) ----
 #pragma unroll
     for (int l = 4; l >= 0; --l) {
+        // dy in 16 bits, converted once: the transposed tile and the input
+        // gradient's operands (tiles 0-1 and 2-3) read the same pairs
+        uint32_t pd[4][2];
         for (int t = 0; t < 4; ++t)
-            for (int ii = 0; ii < 4; ++ii) s_dyT[(16 * t + 4 * g + ii) * kTS + m] = (T)dy[t][ii];
+            for (int h = 0; h < 2; ++h) {
+                pd[t][h] = cvt2<T>(dy[t][2 * h], dy[t][2 * h + 1]);
+                put_pair(s_dyT, (16 * t + 4 * g + 2 * h) * kTS + m, kTS, pd[t][h]);
+            }
         for (int c = 0; c < 2; ++c)
             for (int j = 0; j < 8; ++j) s_xT[nrc_perm(c, g, j) * kTS + m] = xs[l][c][j];
         floatx4 ndy[4];
         if (l > 0) {   // input gradient of layer l (registers only), before the barrier
-            V b0, b1;
-            for (int ii = 0; ii < 4; ++ii) {
-                b0[ii] = (T)dy[0][ii];
-                b0[4 + ii] = (T)dy[1][ii];
-                b1[ii] = (T)dy[2][ii];
-                b1[4 + ii] = (T)dy[3][ii];
-            }
+            const V b0 = __builtin_bit_cast(V, (uint4v){pd[0][0], pd[0][1], pd[1][0], pd[1][1]});
+            const V b1 = __builtin_bit_cast(V, (uint4v){pd[2][0], pd[2][1], pd[3][0], pd[3][1]});
             for (int u = 0; u < 4; ++u) {
                 floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
                 acc = M::mfma(s_bop[(bop_tile_base(l) + 2 * u) * 64 + lane], b0, acc);
@@ -377,14 +463,11 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
             }
         }
         __syncthreads();
-        for (int u = 0; u < 4; ++u) {
-            floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
-            for (int c = 0; c < 2; ++c) {
-                const V a = *reinterpret_cast<const V*>(&s_dyT[(16 * w + ml) * kTS + 32 * c + 8 * g]);
-                const V bb = *reinterpret_cast<const V*>(&s_xT[(16 * u + ml) * kTS + 32 * c + 8 * g]);
-                acc = M::mfma(a, bb, acc);
-            }
-            for (int ii = 0; ii < 4; ++ii) part[nrc_layer_offset(l) + (16 * w + 4 * g + ii) * 64 + 16 * u + ml] = acc[ii];
+        for (int tile = w; tile < 16; tile += kTrainWaves) {   // neurons 16 (tile / 4) .., inputs 16 (tile % 4) ..
+            const int rt = tile >> 2, ct = tile & 3;
+            const floatx4 acc = wgrad_tile<T>(s_dyT, s_xT, 16 * rt, 16 * ct, ml, g);
+            for (int ii = 0; ii < 4; ++ii)
+                part[nrc_layer_offset(l) + (16 * rt + 4 * g + ii) * 64 + 16 * ct + ml] = acc[ii];
         }
         if (l > 0)
             for (int u = 0; u < 4; ++u) dy[u] = ndy[u];
@@ -392,16 +475,19 @@ __global__ __launch_bounds__(256) void k_nrc_train(TrainArgs A) {
     }
 }
 
-// Sums the per-block gradients; flags non-finite values or a skipped step.
-// Sum of the per-block partial gradients.  A block covers 64 parameters
-// with 4 waves; wave k sums the partials of blocks b = k, k + 4, ... (lanes
-// read 64 consecutive parameters), and the 4 sums are added in wave order
-// through LDS, so the result is deterministic.
+// Sum of the per-workgroup partial gradients; flags non-finite values or a
+// skipped step.  A block covers 64 parameters with 4 waves; wave k sums the
+// partials of workgroups b = k, k + 4, ... (lanes read 64 consecutive
+// parameters), and the 4 sums are added in wave order through LDS, so the
+// result is deterministic.  One more block sums the workgroups' losses (each
+// thread a fixed strided subset, then a fixed tree) into *loss_sum.
 constexpr int kReduceParams = 64;
+constexpr int kReduceGrid = (kNrcParams + kReduceParams - 1) / kReduceParams + 1;
 __global__ __launch_bounds__(256) void k_nrc_grad_reduce(const float* __restrict__ partial, uint32_t nblk,
                                                          float* __restrict__ grad, uint32_t* state,
                                                          const uint32_t* __restrict__ n_rec, uint32_t min_records,
-                                                         float inv_scale) {
+                                                         float inv_scale, const float* __restrict__ block_loss,
+                                                         float* __restrict__ loss_sum) {
     __shared__ float sums[4][kReduceParams];
     const int lane = (int)(threadIdx.x & 63u), slice = (int)(threadIdx.x >> 6);
     const int p = blockIdx.x * kReduceParams + lane;
@@ -409,9 +495,22 @@ __global__ __launch_bounds__(256) void k_nrc_grad_reduce(const float* __restrict
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&state[kStFlags], 2u);
         return;
     }
+    if (blockIdx.x == gridDim.x - 1) {
+        float* tree = &sums[0][0];   // 256 floats
+        float L = 0.0f;
+        for (uint32_t b = threadIdx.x; b < nblk; b += 256) L += block_loss[b];
+        tree[threadIdx.x] = L;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) tree[threadIdx.x] += tree[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) *loss_sum = tree[0];
+        return;
+    }
     float s = 0.0f;
     if (p < kNrcParams) {
-#pragma unroll 8
+#pragma unroll 4
         for (uint32_t b = (uint32_t)slice; b < nblk; b += 4) s += partial[(size_t)b * kNrcParams + p];
     }
     sums[slice][lane] = s;
@@ -432,24 +531,28 @@ __device__ __forceinline__ void put16(void* base, size_t pos, float v) { static_
 // operand layouts; no parameter changes when the step was flagged.  The
 // last block to finish (a done counter) closes the step: it advances the
 // batch seed (mlp.cpp:102 step_counter++ on every call), counts the step as
-// taken or skipped, keeps its loss, and clears the flags and the loss sum
-// for the next step, so a step is three launches (train, reduce, Adam).
+// taken or skipped, keeps its loss (summed by the reduction) and clears the
+// flags for the next step, so a step is three launches (train, reduce, Adam).
 template <typename T>
 __global__ __launch_bounds__(256) void k_nrc_adam(float* __restrict__ master, float* __restrict__ am,
                                                   float* __restrict__ av, float* __restrict__ ema,
-                                                  const float* __restrict__ grad, uint32_t* state, float* loss_accum,
-                                                  Hyper h, void* fop_train, void* bop_train, void* fop_infer) {
+                                                  const float* __restrict__ grad, uint32_t* state,
+                                                  const float* __restrict__ loss_sum, Hyper h, void* fop_train,
+                                                  void* bop_train, void* fop_infer) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    // the parameter's operands are loaded together with the flags (one
+    // memory latency, not two); nothing is written unless the step is taken
+    const int q = p < kNrcParams ? p : 0;
+    const float g = grad[q], m0 = am[q], v0 = av[q], w0 = master[q], e0 = ema[q];
     const uint32_t flags = state[kStFlags];
+    const float t = (float)(state[kStLt] + 1u);
     if (p < kNrcParams && flags == 0u) {
-        const float t = (float)(state[kStLt] + 1u);
-        const float g = grad[p];
-        const float m = h.beta1 * am[p] + (1.0f - h.beta1) * g;
-        const float v = h.beta2 * av[p] + (1.0f - h.beta2) * g * g;
+        const float m = h.beta1 * m0 + (1.0f - h.beta1) * g;
+        const float v = h.beta2 * v0 + (1.0f - h.beta2) * g * g;
         const float mh = m / (1.0f - powf(h.beta1, t));
         const float vh = v / (1.0f - powf(h.beta2, t));
-        const float w = master[p] - h.lr * mh / (sqrtf(vh) + h.eps);
-        const float e = h.ema_decay * ema[p] + (1.0f - h.ema_decay) * w;
+        const float w = w0 - h.lr * mh / (sqrtf(vh) + h.eps);
+        const float e = h.ema_decay * e0 + (1.0f - h.ema_decay) * w;
         am[p] = m;
         av[p] = v;
         master[p] = w;
@@ -466,22 +569,16 @@ __global__ __launch_bounds__(256) void k_nrc_adam(float* __restrict__ master, fl
             if (flags == 0u) {
                 state[kStLt] += 1u;
                 state[kStSteps] += 1u;
-                state[kStLoss] = __float_as_uint(*loss_accum);
+                state[kStLoss] = __float_as_uint(*loss_sum);
             } else if (flags & 1u) {
                 state[kStSkipNan] += 1u;
             } else {
                 state[kStSkipData] += 1u;
             }
             state[kStFlags] = 0u;
-            *loss_accum = 0.0f;
             state[kStDone] = 0u;
         }
     }
-}
-
-// fop_tail = fop_infer (16-byte entries), once per call, before its steps
-__global__ __launch_bounds__(256) void k_nrc_snapshot(const uint4* __restrict__ src, uint4* __restrict__ dst, uint32_t n) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
 }
 
 template <typename T>
@@ -494,62 +591,67 @@ __global__ __launch_bounds__(256) void k_nrc_publish(const float* __restrict__ m
     put16<T>(fop_infer, fop_pos(p), ema[p]);
 }
 
-// The records the next `steps` optimiser steps will draw, copied in sample
-// order: step k (seed = the seed counter + k, as each step advances it by
-// one) takes records[pcg32(seed ^ i) % n] for sample i, the draw
-// k_nrc_train makes itself; plus the record count.  A copy lets the steps
-// run after the tracer has started appending the next frame's records.
-__global__ void k_nrc_gather(const float4* __restrict__ records, const uint32_t* __restrict__ n_rec, uint32_t min_records,
-                             const uint32_t* __restrict__ state, uint32_t batch, uint32_t steps, float4* __restrict__ out,
-                             uint32_t* __restrict__ out_n) {
-    const uint32_t n = *n_rec;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *out_n = n;
-    if (n <= min_records) return;   // the steps will skip for lack of data
-    const uint32_t seed0 = state[kStSeed];
-    const uint64_t total = (uint64_t)batch * steps * kRecF4;
-    for (uint64_t w = blockIdx.x * blockDim.x + threadIdx.x; w < total; w += gridDim.x * blockDim.x) {
-        const uint32_t j = (uint32_t)(w / kRecF4), c = (uint32_t)(w - (uint64_t)j * kRecF4);
-        const uint32_t k = j / batch, i = j - k * batch;
-        const uint32_t idx = pcg32_step((seed0 + k) ^ i) % n;
-        out[w] = records[(size_t)kRecF4 * idx + c];
-    }
+// Prepared samples of `steps` steps from records, on stream s (see k_nrc_prepare).
+template <typename T>
+int enqueue_prepare(tfg_nrc* nc, const float4* records, const uint32_t* n_rec, uint32_t steps, uint32_t min_records,
+                    float4* out, uint32_t* out_n, hipStream_t s) {
+    const PrepArgs P{records, n_rec, min_records, nc->state, nc->cfg.batch, steps, nc->bounds, nc->fop_infer, out,
+                     out_n};
+    const uint64_t total = (uint64_t)nc->cfg.batch * steps;
+    // one workgroup per CU: 65,536 samples are 2 groups of 16 per wave
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((total + 127) / 128, (uint64_t)nc->n_cu);
+    hipLaunchKernelGGL(k_nrc_prepare<T>, dim3(grid > 0 ? grid : 1), dim3(kPrepBlock), 0, s, P);
+    TFG_HIP_CHECK(hipGetLastError());
+    return TFG_OK;
 }
 
+// `steps` optimiser steps on prepared samples (steps x batch of them, step k's
+// at k * batch), three launches each.
 template <typename T>
-int enqueue_train_steps(tfg_nrc* nc, const float4* records, const uint32_t* n_rec, uint32_t steps, hipStream_t s,
-                        uint32_t min_records, bool gathered = false) {
-    const uint32_t nblk = nc->cfg.batch / 64;
+int enqueue_steps(tfg_nrc* nc, const float4* prep, const uint32_t* n_rec, uint32_t steps, uint32_t min_records,
+                  hipStream_t s) {
+    const uint32_t nblk = train_blocks(nc->cfg.batch);
     TrainArgs A;
-    A.records = records;
-    A.gathered = gathered ? 1 : 0;
+    A.prep = prep;
     A.n_rec = n_rec;
     A.min_records = min_records;
-    A.state = nc->state;
     A.batch = nc->cfg.batch;
     A.grad_scale = nc->cfg.loss_scale / (3.0f * (float)nc->cfg.batch);
     A.loss_norm = 1.0f / (3.0f * (float)nc->cfg.batch);
     A.b = nc->bounds;
     A.fop_train = nc->fop_train;
     A.bop_train = nc->bop_train;
-    A.fop_tail = nc->fop_tail;
     A.partial = nc->partial;
-    A.loss_accum = nc->loss_accum;
+    A.block_loss = nc->block_loss;
     const Hyper h{nc->cfg.lr, nc->cfg.beta1, nc->cfg.beta2, nc->cfg.eps, nc->cfg.ema_decay, 1.0f / nc->cfg.loss_scale};
     const int pblocks = (kNrcParams + 255) / 256;
-    const uint32_t op_entries = (uint32_t)kFopTiles * 64;
-    if (steps > 0)
-        hipLaunchKernelGGL(k_nrc_snapshot, dim3((op_entries + 255) / 256), dim3(256), 0, s,
-                           static_cast<const uint4*>(nc->fop_infer), static_cast<uint4*>(nc->fop_tail), op_entries);
     for (uint32_t k = 0; k < steps; ++k) {
-        if (gathered) A.records = records + (size_t)k * nc->cfg.batch * kRecF4;
-        hipLaunchKernelGGL(k_nrc_train<T>, dim3(nblk), dim3(256), kTrainLds, s, A);
-        hipLaunchKernelGGL(k_nrc_grad_reduce, dim3((kNrcParams + kReduceParams - 1) / kReduceParams), dim3(256), 0,
-                           s, nc->partial, nblk, nc->grad, nc->state, n_rec, min_records, h.inv_loss_scale);
+        A.prep = prep + (size_t)k * nc->cfg.batch * kPrepF4;
+        hipLaunchKernelGGL(k_nrc_train<T>, dim3(nblk), dim3(kTrainThreads), kTrainLds, s, A);
+        hipLaunchKernelGGL(k_nrc_grad_reduce, dim3(kReduceGrid), dim3(256), 0, s, nc->partial, nblk, nc->grad,
+                           nc->state, n_rec, min_records, h.inv_loss_scale, nc->block_loss, nc->loss_sum);
         hipLaunchKernelGGL(k_nrc_adam<T>, dim3(pblocks), dim3(256), 0, s, nc->master, nc->adam_m, nc->adam_v, nc->ema,
-                           nc->grad, nc->state, nc->loss_accum, h, nc->fop_train, nc->bop_train, nc->fop_infer);
+                           nc->grad, nc->state, nc->loss_sum, h, nc->fop_train, nc->bop_train, nc->fop_infer);
     }
     TFG_HIP_CHECK(hipGetLastError());
     return TFG_OK;
+}
+
+// One call's steps on training records: their samples and targets are
+// prepared first, with the inference weights of the call's start, then the
+// steps run on them.  The caller has reserved prep for steps x batch samples.
+template <typename T>
+int enqueue_train_steps(tfg_nrc* nc, const float4* records, const uint32_t* n_rec, uint32_t steps, hipStream_t s,
+                        uint32_t min_records) {
+    if (steps == 0) return TFG_OK;
+    if ((size_t)steps * nc->cfg.batch > nc->prep_cap) {
+        set_error("tfg_nrc: %u steps of %u samples exceed the prepared-sample buffer (%zu)", steps, nc->cfg.batch,
+                  nc->prep_cap);
+        return TFG_E_LIMIT;
+    }
+    const int rc = enqueue_prepare<T>(nc, records, n_rec, steps, min_records, nc->prep, nullptr, s);
+    if (rc != TFG_OK) return rc;
+    return enqueue_steps<T>(nc, nc->prep, n_rec, steps, min_records, s);
 }
 
 template <typename T>
@@ -611,25 +713,37 @@ int dm_reserve(tfg_nrc* nc, size_t n) {
     return TFG_OK;
 }
 
-size_t nrc_gather_records(const tfg_nrc* nc) {
-    return (size_t)nc->cfg.batch * nc->cfg.steps_per_frame;
+// Grows the prepared-sample buffer to `samples` (a tfg_nrc_train call with
+// more steps than steps_per_frame); work already enqueued may still read the
+// old buffer, so the device is idle first.
+int prep_reserve(tfg_nrc* nc, size_t samples) {
+    if (samples <= nc->prep_cap) return TFG_OK;
+    TFG_HIP_CHECK(hipDeviceSynchronize());
+    if (nc->prep) (void)hipFree(nc->prep);
+    nc->prep = nullptr;
+    nc->prep_cap = 0;
+    TFG_HIP_CHECK(hipMalloc((void**)&nc->prep, samples * kPrepF4 * sizeof(float4)));
+    nc->prep_cap = samples;
+    return TFG_OK;
+}
+
+size_t nrc_gather_f4(const tfg_nrc* nc) {
+    return (size_t)nc->cfg.batch * nc->cfg.steps_per_frame * kPrepF4;
 }
 
 int nrc_gather_enqueue(tfg_nrc* nc, const tfg_nrc_frame& f, float4* out, uint32_t* out_n, hipStream_t s) {
-    const size_t total = nrc_gather_records(nc) * kRecF4;
-    const uint32_t grid = (uint32_t)std::min<size_t>((total + 255) / 256, (size_t)nc->n_cu * 16);
-    hipLaunchKernelGGL(k_nrc_gather, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, f.records, f.counters + CNT_RECORDS,
-                       nc->cfg.min_records, nc->state, nc->cfg.batch, nc->cfg.steps_per_frame, out, out_n);
-    TFG_HIP_CHECK(hipGetLastError());
-    return TFG_OK;
+    return nc->cfg.precision == TFG_NRC_FP16
+               ? enqueue_prepare<_Float16>(nc, f.records, f.counters + CNT_RECORDS, nc->cfg.steps_per_frame,
+                                           nc->cfg.min_records, out, out_n, s)
+               : enqueue_prepare<__bf16>(nc, f.records, f.counters + CNT_RECORDS, nc->cfg.steps_per_frame,
+                                         nc->cfg.min_records, out, out_n, s);
 }
 
 int nrc_train_gathered_enqueue(tfg_nrc* nc, const float4* gathered, const uint32_t* n_rec, hipStream_t s) {
     if (nc->cfg.steps_per_frame == 0) return TFG_OK;
     return nc->cfg.precision == TFG_NRC_FP16
-               ? enqueue_train_steps<_Float16>(nc, gathered, n_rec, nc->cfg.steps_per_frame, s, nc->cfg.min_records,
-                                               true)
-               : enqueue_train_steps<__bf16>(nc, gathered, n_rec, nc->cfg.steps_per_frame, s, nc->cfg.min_records, true);
+               ? enqueue_steps<_Float16>(nc, gathered, n_rec, nc->cfg.steps_per_frame, nc->cfg.min_records, s)
+               : enqueue_steps<__bf16>(nc, gathered, n_rec, nc->cfg.steps_per_frame, nc->cfg.min_records, s);
 }
 
 int nrc_frame_enqueue(tfg_nrc* nc, const tfg_nrc_frame& f, tfg_color* image, hipStream_t s) {
@@ -724,13 +838,16 @@ int tfg_nrc_create(const tfg_nrc_config* cfg, tfg_nrc** out) {
     void** bufs[] = {(void**)&nc->master, (void**)&nc->adam_m, (void**)&nc->adam_v, (void**)&nc->ema, (void**)&nc->grad};
     for (void** b : bufs)
         if (e == hipSuccess) e = hipMalloc(b, pb);
-    if (e == hipSuccess) e = hipMalloc((void**)&nc->partial, (size_t)(cfg->batch / 64) * pb);
+    const size_t nblk = train_blocks(cfg->batch);
+    if (e == hipSuccess) e = hipMalloc((void**)&nc->partial, nblk * pb);
+    if (e == hipSuccess) e = hipMalloc((void**)&nc->block_loss, nblk * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&nc->loss_sum, sizeof(float));
+    nc->prep_cap = (size_t)cfg->batch * std::max<uint32_t>(cfg->steps_per_frame, 1u);
+    if (e == hipSuccess) e = hipMalloc((void**)&nc->prep, nc->prep_cap * kPrepF4 * sizeof(float4));
     if (e == hipSuccess) e = hipMalloc(&nc->fop_train, (size_t)kFopTiles * 64 * kOpBytes16);
     if (e == hipSuccess) e = hipMalloc(&nc->bop_train, (size_t)kBopTiles * 64 * kOpBytes16);
     if (e == hipSuccess) e = hipMalloc(&nc->fop_infer, (size_t)kFopTiles * 64 * kOpBytes16);
-    if (e == hipSuccess) e = hipMalloc(&nc->fop_tail, (size_t)kFopTiles * 64 * kOpBytes16);
     if (e == hipSuccess) e = hipMalloc((void**)&nc->state, 16 * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc((void**)&nc->loss_accum, sizeof(float));
     int dev = 0;
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDevice(&dev);
@@ -775,7 +892,6 @@ int tfg_nrc_set_weights(tfg_nrc* nc, const float* master, const float* ema) {
     TFG_HIP_CHECK(hipMemset(nc->bop_train, 0, (size_t)kBopTiles * 64 * kOpBytes16));
     TFG_HIP_CHECK(hipMemset(nc->fop_infer, 0, (size_t)kFopTiles * 64 * kOpBytes16));
     TFG_HIP_CHECK(hipMemset(nc->state, 0, 16 * sizeof(uint32_t)));
-    TFG_HIP_CHECK(hipMemset(nc->loss_accum, 0, sizeof(float)));   // the steps' loss sum starts cleared
     const int pblocks = (kNrcParams + 255) / 256;
     if (nc->cfg.precision == TFG_NRC_FP16)
         hipLaunchKernelGGL(k_nrc_publish<_Float16>, dim3(pblocks), dim3(256), 0, nullptr, nc->master, nc->ema,
@@ -836,6 +952,12 @@ int tfg_nrc_infer_queries(tfg_nrc* nc, const float* d_queries, const uint32_t* d
 
 int tfg_nrc_train(tfg_nrc* nc, const float* d_records, const uint32_t* d_n_records, uint32_t steps, tfg_stream stream) {
     if (!nc || !d_records || !d_n_records) { set_error("tfg_nrc_train: NULL argument"); return TFG_E_INVALID; }
+    if ((uint64_t)steps * nc->cfg.batch > (1ull << 27)) {
+        set_error("tfg_nrc_train: %u steps of %u samples: at most 2^27 samples per call", steps, nc->cfg.batch);
+        return TFG_E_LIMIT;
+    }
+    const int rc = prep_reserve(nc, (size_t)steps * nc->cfg.batch);
+    if (rc != TFG_OK) return rc;
     const float4* r = reinterpret_cast<const float4*>(d_records);
     return nc->cfg.precision == TFG_NRC_FP16
                ? enqueue_train_steps<_Float16>(nc, r, d_n_records, steps, stream, nc->cfg.min_records)
@@ -926,8 +1048,8 @@ int tfg_nrc_train_datos(tfg_nrc* nc, const tfg_datos_mlp* d_in, uint32_t n, floa
 int tfg_nrc_destroy(tfg_nrc* nc) {
     if (!nc) return TFG_OK;
     (void)hipDeviceSynchronize();
-    void* bufs[] = {nc->master, nc->adam_m, nc->adam_v, nc->ema, nc->grad, nc->partial, nc->fop_train,
-                    nc->bop_train, nc->fop_infer, nc->fop_tail, nc->state, nc->loss_accum, nc->dm_buf, nc->dm_out, nc->dm_count};
+    void* bufs[] = {nc->master, nc->adam_m, nc->adam_v, nc->ema, nc->grad, nc->partial, nc->block_loss, nc->loss_sum, nc->prep,
+                    nc->fop_train, nc->bop_train, nc->fop_infer, nc->state, nc->dm_buf, nc->dm_out, nc->dm_count};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete nc;

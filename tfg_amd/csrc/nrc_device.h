@@ -60,6 +60,33 @@ struct MfmaOps<_Float16> {
     }
 };
 
+typedef float float2v __attribute__((ext_vector_type(2)));
+typedef short short2v __attribute__((ext_vector_type(2)));
+typedef uint32_t uint4v __attribute__((ext_vector_type(4)));
+template <typename T>
+struct Pair16;
+template <>
+struct Pair16<__bf16> {
+    typedef __bf16 type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Pair16<_Float16> {
+    typedef _Float16 type __attribute__((ext_vector_type(2)));
+};
+// (a, b) -> one packed conversion to 16 bits (a in the low half): one
+// instruction, where element-wise inserts into an 8-vector cost a conversion
+// each plus a byte permute per pair
+template <typename T>
+__device__ __forceinline__ uint32_t cvt2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((float2v){a, b}, typename Pair16<T>::type));
+}
+// the same with ReLU (see pack_relu)
+template <typename T>
+__device__ __forceinline__ uint32_t cvt_relu2(float a, float b) {
+    const short2v s = __builtin_elementwise_max(__builtin_bit_cast(short2v, cvt2<T>(a, b)), (short2v)0);
+    return __builtin_bit_cast(uint32_t, s);
+}
+
 // ---------------------------------------------------------------- encoding
 // Vertex record (4 float4): (pos, roughness), (dir mapped to [0,1], kd.r),
 // (normal mapped to [0,1], kd.g), (kd.b, ks.rgb).
@@ -100,39 +127,44 @@ __device__ __forceinline__ void encode(const float4 v0, const float4 v1, const f
     const float px = fminf(fmaxf((v0.x - b.mn[0]) * b.inv_ext[0], 0.0f), 1.0f);
     const float py = fminf(fmaxf((v0.y - b.mn[1]) * b.inv_ext[1], 0.0f), 1.0f);
     const float pz = fminf(fmaxf((v0.z - b.mn[2]) * b.inv_ext[2], 0.0f), 1.0f);
-    x0[0] = (T)sin_pi_pow2(px, g);
-    x0[1] = (T)sin_pi_pow2(px, 4 + g);
-    x0[2] = (T)sin_pi_pow2(px, 8 + g);
-    x0[3] = (T)sin_pi_pow2(py, g);
-    x0[4] = (T)sin_pi_pow2(py, 4 + g);
-    x0[5] = (T)sin_pi_pow2(py, 8 + g);
-    x0[6] = (T)sin_pi_pow2(pz, g);
-    x0[7] = (T)sin_pi_pow2(pz, 4 + g);
-    float dth, dph, nth, nph;
-    sph01(v1.x, v1.y, v1.z, dth, dph);
-    sph01(v2.x, v2.y, v2.z, nth, nph);
+    typedef typename MfmaOps<T>::vec V;
+    const uint4v w0 = {cvt2<T>(sin_pi_pow2(px, g), sin_pi_pow2(px, 4 + g)),
+                       cvt2<T>(sin_pi_pow2(px, 8 + g), sin_pi_pow2(py, g)),
+                       cvt2<T>(sin_pi_pow2(py, 4 + g), sin_pi_pow2(py, 8 + g)),
+                       cvt2<T>(sin_pi_pow2(pz, g), sin_pi_pow2(pz, 4 + g))};
+    x0 = __builtin_bit_cast(V, w0);
+    // Each lane converts one of the two unit vectors to angles (lane groups 0
+    // and 1 the direction, 2 and 3 the normal) and takes the other vector's
+    // angles from lane l ^ 32, which holds the same sample: one acos and one
+    // atan2 per lane instead of two of each.  Callers that skip encode() for
+    // some samples skip both lanes of a pair (same sample), so every lane
+    // read here is active.
+    const bool is_dir = g < 2;
+    float th, ph;
+    sph01(is_dir ? v1.x : v2.x, is_dir ? v1.y : v2.y, is_dir ? v1.z : v2.z, th, ph);
+    const float th_o = __shfl_xor(th, 32), ph_o = __shfl_xor(ph, 32);
+    const float dth = is_dir ? th : th_o, dph = is_dir ? ph : ph_o;
+    const float nth = is_dir ? th_o : th, nph = is_dir ? ph_o : ph;
     const float kd[3] = {v1.w, v2.w, v3.x};
     const float ks[3] = {v3.y, v3.z, v3.w};
-    x1[0] = (T)sin_pi_pow2(pz, 8 + g);
-    x1[1] = (T)oneblob(dth, g);
-    x1[2] = (T)oneblob(dph, g);
-    x1[3] = (T)oneblob(nth, g);
-    x1[4] = (T)oneblob(nph, g);
-    x1[5] = (T)oneblob(v0.w, g);
-    x1[6] = (T)(g < 3 ? kd[g] : 1.0f);
-    x1[7] = (T)(g < 3 ? ks[g] : 0.0f);
+    const uint4v w1 = {cvt2<T>(sin_pi_pow2(pz, 8 + g), oneblob(dth, g)), cvt2<T>(oneblob(dph, g), oneblob(nth, g)),
+                       cvt2<T>(oneblob(nph, g), oneblob(v0.w, g)),
+                       cvt2<T>(g < 3 ? kd[g] : 1.0f, g < 3 ? ks[g] : 0.0f)};
+    x1 = __builtin_bit_cast(V, w1);
 }
 
 // D tiles (4 per 64 neurons) -> B operands of the next layer, with ReLU.
+// The ReLU is taken after the conversion, on the 16-bit patterns as signed
+// integers (two per packed integer max): a negative bf16 / fp16 value, -0
+// included, is a negative integer, a positive one keeps its order, so every
+// finite pre-activation gives the same operand as converting max(x, 0), in 16
+// VALU instead of 40 (a float max also canonicalises its input first).  A
+// NaN pre-activation stays NaN when its sign bit is clear.
 template <typename T>
 __device__ __forceinline__ typename MfmaOps<T>::vec pack_relu(floatx4 lo, floatx4 hi) {
-    typename MfmaOps<T>::vec v;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        v[i] = (T)fmaxf(lo[i], 0.0f);
-        v[4 + i] = (T)fmaxf(hi[i], 0.0f);
-    }
-    return v;
+    const uint4v w = {cvt_relu2<T>(lo[0], lo[1]), cvt_relu2<T>(lo[2], lo[3]), cvt_relu2<T>(hi[0], hi[1]),
+                      cvt_relu2<T>(hi[2], hi[3])};
+    return __builtin_bit_cast(typename MfmaOps<T>::vec, w);
 }
 
 // One hidden layer for G 16-sample groups: x <- relu(W_l x).

@@ -19,11 +19,12 @@ namespace tfg {
 // query composited into image, then (when training) tail inference,
 // targets and the optimiser steps.  Graph-capturable.
 int nrc_frame_enqueue(tfg_nrc* nrc, const tfg_nrc_frame& f, tfg_color* image, hipStream_t s);
-// Deferred training: the records the frame's optimiser steps will draw (in
-// sample order, nrc_gather_records of them) and the record count are copied
-// out on s; nrc_train_gathered_enqueue later runs those steps on the copy,
-// with the same results as training on the frame's records.
-size_t nrc_gather_records(const tfg_nrc* nrc);
+// Deferred training: the samples the frame's optimiser steps will draw, with
+// their targets (the head vertex and target of each, in sample order,
+// nrc_gather_f4 float4 in all), and the record count are prepared on s;
+// nrc_train_gathered_enqueue later runs those steps on them, with the same
+// results as training on the frame's records at the end of the frame.
+size_t nrc_gather_f4(const tfg_nrc* nrc);
 int nrc_gather_enqueue(tfg_nrc* nrc, const tfg_nrc_frame& f, float4* out, uint32_t* out_n, hipStream_t s);
 int nrc_train_gathered_enqueue(tfg_nrc* nrc, const float4* gathered, const uint32_t* n_rec, hipStream_t s);
 }  // namespace tfg

@@ -76,11 +76,12 @@ struct tfg_pt {
     bool allow_no_cache = false;   // drop-in export frames: queries leave the frame uncomposited
     // Deferred training (schedule defer_training, direct launches only): a
     // frame's optimiser steps run at the start of the next frame on cstream,
-    // beside its tracing, on a copy of the records they draw; the next cache
-    // inference waits for them, so every inference sees the weights it would
-    // see with the steps at the end of their own frame.
+    // beside its tracing, on the samples they draw, prepared with their
+    // targets at the end of the frame; the next cache inference waits for
+    // them, so every inference sees the weights it would see with the steps
+    // at the end of their own frame.
     bool defer_train = false;
-    bool train_pending = false;   // a copy waits for its steps
+    bool train_pending = false;   // prepared samples wait for their steps
     bool wait_trained = false;    // the next cache inference waits for ev_trained
     float4* train_buf = nullptr;
     uint32_t* train_nrec = nullptr;
@@ -421,8 +422,9 @@ int enqueue_tile(tfg_pt* pt, int t, hipStream_t ts, hipStream_t* side, tfg_color
 }
 
 int ensure_train_buf(tfg_pt* pt) {
-    const size_t need = nrc_gather_records(pt->nrc) * kRecF4;
+    const size_t need = nrc_gather_f4(pt->nrc);
     if (need <= pt->train_cap && pt->train_nrec) return TFG_OK;
+    TFG_HIP_CHECK(hipDeviceSynchronize());   // steps still pending on the old buffer
     if (pt->train_buf) (void)hipFree(pt->train_buf);
     pt->train_buf = nullptr;
     pt->train_cap = 0;
@@ -432,8 +434,8 @@ int ensure_train_buf(tfg_pt* pt) {
     return TFG_OK;
 }
 
-// Enqueues the pending optimiser steps on q, after their copy of the records
-// is complete; the next cache inference waits for them.
+// Enqueues the pending optimiser steps on q, after their prepared samples are
+// complete; the next cache inference waits for them.
 int run_pending_training(tfg_pt* pt, hipStream_t q) {
     if (!pt->train_pending || !pt->nrc) return TFG_OK;
     TFG_HIP_CHECK(hipStreamWaitEvent(q, pt->ev_gathered, 0));
@@ -456,7 +458,7 @@ int enqueue_post(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum, boo
             TFG_HIP_CHECK(hipStreamWaitEvent(s, pt->ev_trained, 0));
             pt->wait_trained = false;
         }
-        const bool defer = direct && pt->defer_train && (pt->P.mode & MODE_TRAIN) && nrc_gather_records(pt->nrc) > 0;
+        const bool defer = direct && pt->defer_train && (pt->P.mode & MODE_TRAIN) && nrc_gather_f4(pt->nrc) > 0;
         tfg_nrc_frame f = pt->nrc_frame;
         if (defer) f.mode &= ~MODE_TRAIN;
         int rc = nrc_frame_enqueue(pt->nrc, f, image, s);
