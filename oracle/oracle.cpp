@@ -94,9 +94,11 @@ RayPre make_ray(const tfg_ray& r) {
     p.kx = (p.kz + 1) % 3;
     p.ky = (p.kx + 1) % 3;
     if (p.D[p.kz] < 0.0f) std::swap(p.kx, p.ky);
-    p.Sx = p.D[p.kx] / p.D[p.kz];
-    p.Sy = p.D[p.ky] / p.D[p.kz];
-    p.Sz = 1.0f / p.D[p.kz];
+    // shear constants from the dominant axis's reciprocal (the same values
+    // for every triangle the ray meets, which is what watertightness needs)
+    p.Sz = p.rD[p.kz];
+    p.Sx = p.D[p.kx] * p.Sz;
+    p.Sy = p.D[p.ky] * p.Sz;
     p.octinv = 7u - ((p.rD[0] < 0.0f ? 4u : 0u) | (p.rD[1] < 0.0f ? 2u : 0u) | (p.rD[2] < 0.0f ? 1u : 0u));
     return p;
 }
