@@ -160,3 +160,19 @@ def test_empty_batch_and_bad_args(torch_cuda):
         _lib.check(_lib.lib().tfg_trace_closest(None, 0, r.data_ptr(), h.data_ptr(), None, 1, None))
     with pytest.raises(_lib.TfgError):
         _lib.check(_lib.lib().tfg_trace_closest(_lib.C.byref(dbvh.desc), 7, r.data_ptr(), h.data_ptr(), None, 1, None))
+
+
+def test_ray_setup_reciprocal_is_the_ieee_division(torch_cuda):
+    """The reciprocal of the ray set-up (v_rcp_f32 + one Newton step, the
+    division for |d| >= 2^126 and non-normal d) equals 1.0f / d bit for bit
+    on every normal float and on zeros, subnormals, infinities and NaN
+    (tests/cpp/rcp_check.hip, built by __graft_entry__.build())."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_build", "rcp_check")
+    assert os.path.exists(exe), "build tests/cpp first (__graft_entry__.build())"
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(p.stdout, p.stderr, flush=True)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "0 differ; " in p.stdout and p.stdout.rstrip().endswith("0 differ")
