@@ -184,7 +184,7 @@ def bench_traversal(args, torch):
 TUNED_SCHEDULE_1080P = {"trace_block": 64, "grid_mult": 4, "tiles": 1, "side_streams": 2, "overlap": True,
                         "xcd_group": 0, "octet_below": 131072, "connect_per_cu": 0, "connect_batch": 0,
                         "top_nodes": 1, "tail_from": 0, "shadow_sort_from": 1, "ray_sort_from": 0,
-                        "refill_from": 1, "refill_idle": 32}
+                        "refill_from": 1, "refill_idle": 32, "refill_per_cu": 14}
 
 
 def _stage_roofline(prof, frames, stage, n_launch):

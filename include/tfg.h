@@ -201,7 +201,7 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
  * the fastest: first the traversal launch shape together with top_nodes,
  * then tiles, side streams and overlap, then the XCD super-chunk size,
  * octet_below, connect_per_cu, connect_batch and tail_from, then the two
- * queue orders and lane refill, then a second round over all of them but the
+ * queue orders and lane refill (with its workgroups per CU), then a second round over all of them but the
  * launch shape (top_nodes included), all timed on plain frames with direct
  * launches (the tuner captures no graph).
  * Drops any captured graph. */
@@ -278,6 +278,9 @@ typedef struct tfg_pt_schedule {
                                     ray; at depths with at least two rays per resident lane (fewer: lockstep, and at
                                     most octet_below rays: 8 lanes per ray) */
     uint32_t refill_idle;        /* 1..64 (default 32) */
+    uint32_t refill_per_cu;      /* 0 (default): the lane-refill closest-hit kernel keeps as many workgroups resident
+                                    per CU as fit; k (1..64): at most k, leaving CU slots and LDS to the stages that
+                                    run beside it (shadow-entry sort, shadow stage) */
 } tfg_pt_schedule;
 int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out);
 int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* schedule);

@@ -189,7 +189,8 @@ def test_pt_schedules_match_serial(torch_cuda, use_graph):
     tail kernel, the shadow entries traced in spatial (Morton) order from
     depth 1, 2 or 3, the extension rays traced in direction / origin order
     from depth 1 or 2 and the closest-hit stage with lane refill (from depth 1
-    or 2, refilling at 1, 16 or 64 idle lanes, in queue or sorted order) give
+    or 2, refilling at 1, 16 or 64 idle lanes, in queue or sorted order, with
+    as many resident workgroups per CU as fit or at most 1, 6 or 12) give
     the same image bit for bit as one serial chain, for plain and
     cache-capture frames, with the same ray counts and the same captured
     records and queries."""
@@ -198,34 +199,34 @@ def test_pt_schedules_match_serial(torch_cuda, use_graph):
     W, H = 96, 64
     ds = pt.DeviceScene(sc, torch)
     big = 1 << 30
-    schedules = [(1, False, 1, 0, 0, 0, 73, 0, 0, 0, 0, 16), (1, True, 1, 4096, 0, 3, 9, 6, 0, 0, 0, 16),
-                 (1, True, 2, 256, big, 0, 0, 2, 0, 0, 0, 16), (2, True, 4, 1024, 2000, 1, 1, 4, 0, 0, 0, 16),
-                 (4, False, 1, 0, big, 5, 9, 0, 0, 0, 0, 16), (4, True, 3, 16384, 0, 7, 73, 1, 0, 0, 0, 16),
-                 (1, True, 2, 0, 0, 2, 9, 3, 0, 0, 0, 16), (1, False, 1, 0, 0, 0, 9, 8, 0, 0, 0, 16),
-                 (1, True, 2, 0, 0, 0, 1, 0, 1, 0, 0, 16), (1, False, 1, 0, 0, 0, 9, 0, 2, 1, 0, 16),
-                 (2, True, 2, 0, big, 4, 1, 6, 3, 2, 0, 16), (4, True, 4, 1024, 0, 2, 0, 0, 1, 1, 0, 16),
-                 (1, True, 2, 0, 0, 0, 1, 0, 0, 1, 0, 16), (1, True, 2, 4096, 0, 0, 1, 3, 1, 2, 0, 16),
-                 (1, True, 2, 0, 0, 0, 1, 0, 1, 0, 1, 16), (1, False, 1, 0, 0, 0, 9, 0, 0, 0, 1, 1),
-                 (1, True, 2, 0, 0, 0, 73, 0, 1, 1, 2, 64), (2, True, 2, 0, 0, 3, 1, 5, 1, 2, 1, 16),
-                 (4, True, 4, 0, 2000, 0, 0, 0, 1, 0, 1, 8)]
+    schedules = [(1, False, 1, 0, 0, 0, 73, 0, 0, 0, 0, 16, 0), (1, True, 1, 4096, 0, 3, 9, 6, 0, 0, 0, 16, 6),
+                 (1, True, 2, 256, big, 0, 0, 2, 0, 0, 0, 16, 0), (2, True, 4, 1024, 2000, 1, 1, 4, 0, 0, 0, 16, 12),
+                 (4, False, 1, 0, big, 5, 9, 0, 0, 0, 0, 16, 1), (4, True, 3, 16384, 0, 7, 73, 1, 0, 0, 0, 16, 0),
+                 (1, True, 2, 0, 0, 2, 9, 3, 0, 0, 0, 16, 6), (1, False, 1, 0, 0, 0, 9, 8, 0, 0, 0, 16, 0),
+                 (1, True, 2, 0, 0, 0, 1, 0, 1, 0, 0, 16, 12), (1, False, 1, 0, 0, 0, 9, 0, 2, 1, 0, 16, 1),
+                 (2, True, 2, 0, big, 4, 1, 6, 3, 2, 0, 16, 0), (4, True, 4, 1024, 0, 2, 0, 0, 1, 1, 0, 16, 6),
+                 (1, True, 2, 0, 0, 0, 1, 0, 0, 1, 0, 16, 0), (1, True, 2, 4096, 0, 0, 1, 3, 1, 2, 0, 16, 12),
+                 (1, True, 2, 0, 0, 0, 1, 0, 1, 0, 1, 16, 1), (1, False, 1, 0, 0, 0, 9, 0, 0, 0, 1, 1, 0),
+                 (1, True, 2, 0, 0, 0, 73, 0, 1, 1, 2, 64, 6), (2, True, 2, 0, 0, 3, 1, 5, 1, 2, 1, 16, 0),
+                 (4, True, 4, 0, 2000, 0, 0, 0, 1, 0, 1, 8, 12)]
     for mode in (0, 2):
         imgs, counts, caps = [], [], []
-        for tiles, overlap, sides, xcd, octet, batch, top, tail, sort_from, ray_from, refill, idle in schedules:
+        for tiles, overlap, sides, xcd, octet, batch, top, tail, sort_from, ray_from, refill, idle, cap in schedules:
             print(f"mode={mode} tiles={tiles} overlap={overlap} sides={sides} xcd_group={xcd} octet_below={octet} "
                   f"connect_batch={batch} top_nodes={top} tail_from={tail} shadow_sort_from={sort_from} "
-                  f"ray_sort_from={ray_from} refill_from={refill} refill_idle={idle}", flush=True)
+                  f"ray_sort_from={ray_from} refill_from={refill} refill_idle={idle} refill_per_cu={cap}", flush=True)
             tr = pt.PathTracer(W, H, 8, 3, mode=mode, use_graph=use_graph)
             tr.set_scene(ds)
             tr.set_camera(sc.camera.frame(W, H))
             tr.set_schedule(overlap=overlap, tiles=tiles, side_streams=sides, xcd_group=xcd, octet_below=octet,
                             connect_batch=batch, top_nodes=top, tail_from=tail, shadow_sort_from=sort_from,
-                            ray_sort_from=ray_from, refill_from=refill, refill_idle=idle)
+                            ray_sort_from=ray_from, refill_from=refill, refill_idle=idle, refill_per_cu=cap)
             shape = tr.launch_shape()
             assert shape["overlap"] == overlap and shape["tiles"] == tiles and shape["side_streams"] == sides
             assert shape["xcd_group"] == xcd and shape["octet_below"] == octet and shape["connect_batch"] == batch
             assert shape["top_nodes"] == top and shape["tail_from"] == tail
             assert shape["shadow_sort_from"] == sort_from and shape["ray_sort_from"] == ray_from
-            assert shape["refill_from"] == refill and shape["refill_idle"] == idle
+            assert shape["refill_from"] == refill and shape["refill_idle"] == idle and shape["refill_per_cu"] == cap
             assert shape["use_graph"] == use_graph
             img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
             out = []
@@ -272,7 +273,7 @@ def test_autotune_schedule_in_range_and_image_unchanged(torch_cuda):
     assert shape["connect_per_cu"] <= 64 and shape["connect_batch"] <= 32 and shape["tail_from"] <= 32
     assert shape["top_nodes"] <= 73 and not shape["defer_training"] and not shape["use_graph"]
     assert shape["shadow_sort_from"] <= 32 and shape["ray_sort_from"] <= 32
-    assert shape["refill_from"] <= 32 and 1 <= shape["refill_idle"] <= 64
+    assert shape["refill_from"] <= 32 and 1 <= shape["refill_idle"] <= 64 and shape["refill_per_cu"] <= 64
     assert shape["trace_grid"] > 0 and shape["shade_grid"] > 0
     a = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
     b = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")

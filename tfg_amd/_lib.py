@@ -68,7 +68,8 @@ class PtSchedule(C.Structure):
                 ("xcd_group", C.c_uint32), ("octet_below", C.c_uint32), ("connect_per_cu", C.c_uint32),
                 ("connect_batch", C.c_uint32), ("top_nodes", C.c_uint32), ("tail_from", C.c_uint32),
                 ("defer_training", C.c_uint32), ("shadow_sort_from", C.c_uint32),
-                ("ray_sort_from", C.c_uint32), ("refill_from", C.c_uint32), ("refill_idle", C.c_uint32)]
+                ("ray_sort_from", C.c_uint32), ("refill_from", C.c_uint32), ("refill_idle", C.c_uint32),
+                ("refill_per_cu", C.c_uint32)]
 
 
 class NrcConfig(C.Structure):

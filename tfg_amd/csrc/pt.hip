@@ -29,6 +29,7 @@ struct tfg_pt {
     int trace_mult = 1;
     int trace_grid = 0;
     int refill_grid = 0;   // resident grid of the lane-refill closest-hit kernel
+    int refill_cap = 0;    // at most this many of its workgroups per CU (0: as many as fit)
     int connect_per_cu = 0;   // shadow-stage workgroups per CU (0: the closest-hit grid)
     uint32_t connect_batch = 0;   // depths >= this (if > 0) share one shadow-stage launch
     uint32_t top_nodes = 9;       // leading tree nodes each traversal workgroup stages in LDS
@@ -153,6 +154,7 @@ int set_trace_shape(tfg_pt* pt, int block, int mult) {
     else occupancy_extend<256>(pt, &per_cu), occupancy_refill<256>(pt, &refill_per_cu);
     if (per_cu <= 0) per_cu = 1;
     if (refill_per_cu <= 0) refill_per_cu = 1;
+    if (pt->refill_cap > 0 && refill_per_cu > pt->refill_cap) refill_per_cu = pt->refill_cap;
     pt->trace_block = block;
     pt->trace_mult = mult;
     pt->trace_grid = pt->n_cu * per_cu * mult;
@@ -1033,6 +1035,7 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     pt->P.sort_from = 0;
     pt->P.ray_sort_from = 0;
     pt->P.refill_from = 0;
+    pt->refill_cap = 0;
     // 1. traversal launch shape and LDS-staged top nodes together (large
     //    multipliers give more workgroups than chunks: each then traces one
     //    chunk and retires, which pays only with cheap workgroup start-up)
@@ -1061,6 +1064,7 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
     const uint32_t tails[6] = {0, 2, 3, 4, 5, 6};
     const uint32_t sorts[3] = {0, 1, 2};
     const uint32_t idles[3] = {16, 32, 48};
+    const int refill_caps[4] = {0, 16, 14, 12};
     for (int round = 0; round < 2 && rc == TFG_OK; ++round) {
         if (round > 0)
             sweep(4, [&](int i) {
@@ -1081,6 +1085,10 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
         sweep(3, [&](int i) { pt->P.ray_sort_from = sorts[i]; });
         sweep(3, [&](int i) { pt->P.refill_from = sorts[i]; });
         if (pt->P.refill_from) sweep(3, [&](int i) { pt->P.refill_idle = idles[i]; });
+        if (pt->P.refill_from) sweep(4, [&](int i) {
+            pt->refill_cap = refill_caps[i];
+            set_trace_shape(pt, pt->trace_block, pt->trace_mult);
+        });
     }
     drop_graph(pt);
     pt->P.mode = saved_mode;
@@ -1117,6 +1125,7 @@ int tfg_pt_get_schedule(const tfg_pt* pt, tfg_pt_schedule* out) {
     out->ray_sort_from = pt->P.ray_sort_from;
     out->refill_from = pt->P.refill_from;
     out->refill_idle = pt->P.refill_idle;
+    out->refill_per_cu = (uint32_t)pt->refill_cap;
     return TFG_OK;
 }
 
@@ -1129,10 +1138,10 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
         sc->connect_per_cu > 64 || sc->connect_batch > kMaxDepth || sc->top_nodes > kTopNodes ||
         sc->tail_from > kMaxDepth || sc->defer_training > 1 || sc->shadow_sort_from > kMaxDepth ||
         sc->ray_sort_from > kMaxDepth || sc->refill_from > kMaxDepth || sc->refill_idle == 0 ||
-        sc->refill_idle > 64) {
+        sc->refill_idle > 64 || sc->refill_per_cu > 64) {
         set_error("tfg_pt_set_schedule: block 64/128/256, grid_mult 1..64, tiles 1..%u, side streams 1..%u, "
                   "xcd_group 0 or a multiple of 256 up to 2^20, connect_per_cu 0..64, connect_batch, tail_from, "
-                  "shadow_sort_from, ray_sort_from and refill_from 0..%u, refill_idle 1..64, top_nodes 0..%u, "
+                  "shadow_sort_from, ray_sort_from and refill_from 0..%u, refill_idle 1..64, refill_per_cu 0..64, top_nodes 0..%u, "
                   "defer_training 0/1", kMaxTiles, kMaxSide, kMaxDepth, kTopNodes);
         return TFG_E_INVALID;
     }
@@ -1152,6 +1161,7 @@ int tfg_pt_set_schedule(tfg_pt* pt, const tfg_pt_schedule* sc) {
     drop_graph(pt);
     pt->top_nodes = sc->top_nodes;
     pt->P.n_top = pt->scene_nodes < pt->top_nodes ? pt->scene_nodes : pt->top_nodes;
+    pt->refill_cap = (int)sc->refill_per_cu;
     set_trace_shape(pt, (int)b, (int)sc->grid_mult);
     pt->tiles = (int)sc->tiles;
     pt->sides = (int)sc->side_streams;

@@ -149,12 +149,14 @@ class PathTracer:
                 "xcd_group": sc.xcd_group, "octet_below": sc.octet_below, "connect_per_cu": sc.connect_per_cu,
                 "connect_batch": sc.connect_batch, "top_nodes": sc.top_nodes, "tail_from": sc.tail_from,
                 "defer_training": bool(sc.defer_training), "shadow_sort_from": sc.shadow_sort_from,
-                "ray_sort_from": sc.ray_sort_from, "refill_from": sc.refill_from, "refill_idle": sc.refill_idle}
+                "ray_sort_from": sc.ray_sort_from, "refill_from": sc.refill_from, "refill_idle": sc.refill_idle,
+                "refill_per_cu": sc.refill_per_cu}
 
     def set_schedule(self, **kw):
         """Change any of trace_block, grid_mult, tiles, side_streams, overlap, xcd_group,
         octet_below, connect_per_cu, connect_batch, top_nodes, tail_from, defer_training, shadow_sort_from,
-        ray_sort_from, refill_from, refill_idle (use_graph is fixed when the tracer is created)."""
+        ray_sort_from, refill_from, refill_idle, refill_per_cu (use_graph is fixed when the tracer is
+        created)."""
         cur = self.schedule()
         unknown = set(kw) - set(cur)
         if unknown:
@@ -164,7 +166,7 @@ class PathTracer:
                              1 if cur["overlap"] else 0, 1 if cur["use_graph"] else 0, cur["xcd_group"],
                              cur["octet_below"], cur["connect_per_cu"], cur["connect_batch"], cur["top_nodes"],
                              cur["tail_from"], 1 if cur["defer_training"] else 0, cur["shadow_sort_from"],
-                             cur["ray_sort_from"], cur["refill_from"], cur["refill_idle"])
+                             cur["ray_sort_from"], cur["refill_from"], cur["refill_idle"], cur["refill_per_cu"])
         check(lib().tfg_pt_set_schedule(self._h, C.byref(sc)), "tfg_pt_set_schedule")
 
     def launch_shape(self):
