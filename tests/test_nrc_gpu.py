@@ -186,6 +186,32 @@ def test_steps_of_one_call_share_frame_start_targets(torch_cuda, prec):
     assert cos_wrong < 0.99 and rel_wrong > 0.5
 
 
+@pytest.mark.parametrize("batch", [64, 192])
+def test_partial_training_workgroup_matches_oracle(torch_cuda, batch):
+    """A batch that is not a multiple of a training workgroup's 128 samples:
+    the last workgroup's missing samples add nothing to the gradient or the
+    loss.  Gradient and loss of one step against the oracle on the same
+    PCG32 batch, as in test_gradient_and_adam_match_oracle."""
+    torch = torch_cuda
+    c = nrc.RadianceCache(precision=0, seed=5, batch=batch)
+    w0, e0 = c.weights()
+    n_rec = 3000
+    rec = _records(n_rec, 13, tail=True)
+    d_rec = torch.from_numpy(rec).cuda()
+    d_n = torch.tensor([n_rec], dtype=torch.int32, device="cuda")
+    c.train(d_rec, d_n, steps=1)
+    g = c.last_gradient().astype(np.float64)
+    st = c.stats()
+    assert st["steps"] == 1
+    loss, g_ref = _oracle_grad(w0, e0, rec, 0, batch, "bf16")
+    cos = _cosine(g, g_ref)
+    rel = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    print(f"batch {batch}: loss gpu {st['last_loss']:.6f} oracle {loss:.6f}; grad cos {cos:.6f} rel {rel:.3e}",
+          flush=True)
+    assert cos >= 0.999 and rel <= 5e-2
+    assert abs(st["last_loss"] - loss) <= 1e-2 * loss
+
+
 def test_training_fits_a_smooth_function(torch_cuda):
     torch = torch_cuda
     c = nrc.RadianceCache(batch=4096, seed=2)
