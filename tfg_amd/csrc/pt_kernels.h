@@ -892,9 +892,12 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(SortJob J) {
 // stage reads only the shadow entries of its depths and writes only
 // nee[depth][pid], so it can run on a side stream while later depths are
 // extended and shaded; finalize adds the contributions in depth order.  The
-// shadow rays are counted by the shading stage (light_facing).
+// shadow rays are counted by the shading stage (light_facing).  At most 96
+// registers (5 waves per SIMD): left to itself the compiler takes 98 and 4
+// waves, and the launch runs about 5% longer.
 template <int kBlock, bool kStats = false>
-__global__ __launch_bounds__(kBlock) void k_connect(PtParams P, uint32_t d_lo, uint32_t d_hi) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_connect(PtParams P, uint32_t d_lo,
+                                                                                            uint32_t d_hi) {
     extern __shared__ __align__(16) unsigned char lds[];
     float4* top = reinterpret_cast<float4*>(lds);
     uint2* stack = reinterpret_cast<uint2*>(top + 5 * P.n_top);

@@ -163,6 +163,23 @@ __device__ __forceinline__ bool affine(const float4* __restrict__ tri, const Ray
 
 __device__ __forceinline__ float ubyte_f(uint32_t w, int j) { return (float)((w >> (8 * j)) & 0xFFu); }
 
+// Quantised bounds as FP16 subnormals: the half with bit pattern b is
+// b * 2^-24, exactly, so v_perm_b32 turns a word of 4 bounds into two
+// registers of two halves each, and v_fma_mix_f32 reads either half as an
+// f32 operand: 2 + 4 instructions per 4 bounds instead of 4 v_cvt_f32_ubyte
+// and 4 v_fma_f32.
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+struct SubBytes {
+    uint32_t p01, p23;   // halves {b0, b1}, {b2, b3}
+    __device__ __forceinline__ explicit SubBytes(uint32_t w)
+        : p01(__builtin_amdgcn_perm(0u, w, 0x0c010c00u)), p23(__builtin_amdgcn_perm(0u, w, 0x0c030c02u)) {}
+    __device__ __forceinline__ float operator[](int j) const {   // byte j * 2^-24
+        const half2v h = __builtin_bit_cast(half2v, j < 2 ? p01 : p23);
+        return (float)((j & 1) ? h.y : h.x);
+    }
+};
+constexpr float kSlabScale = 0x1p-24f;
+
 // One child box test, the oracle's node_hitmask loop body: lo* / hi* are the
 // child's near and far quantised bounds per axis (the low or high bound by
 // the sign of the ray direction on that axis).
@@ -188,9 +205,15 @@ __device__ __forceinline__ uint32_t node_hitmask(const float4 n0, const float4 n
     const float idirx = __uint_as_float((uint32_t)(ex + 127) << 23) * r.rD[0];
     const float idiry = __uint_as_float((uint32_t)(ey + 127) << 23) * r.rD[1];
     const float idirz = __uint_as_float((uint32_t)(ez + 127) << 23) * r.rD[2];
-    const float origx = (n0.x - r.O[0]) * r.rD[0];
-    const float origy = (n0.y - r.O[1]) * r.rD[1];
-    const float origz = (n0.z - r.O[2]) * r.rD[2];
+    // Slab distances scaled by 2^-24: fma(b * 2^-24, idir, orig * 2^-24) is
+    // 2^-24 * fma(b, idir, orig) (the same rounding), and the test compares
+    // them only with each other and with tmin and tcur scaled alike, so every
+    // decision is the unscaled test's while the scaled values stay normal
+    // (|t| >= 2^-102).
+    const float origx = (n0.x - r.O[0]) * r.rD[0] * kSlabScale;
+    const float origy = (n0.y - r.O[1]) * r.rD[1] * kSlabScale;
+    const float origz = (n0.z - r.O[2]) * r.rD[2] * kSlabScale;
+    const float tmin_s = r.tmin * kSlabScale, tcur_s = tcur * kSlabScale;
     const uint32_t m[2] = {__float_as_uint(n1.z), __float_as_uint(n1.w)};
     // near / far bound words per axis, chosen once per node by the direction signs
     const bool nx = r.rD[0] < 0.0f, ny = r.rD[1] < 0.0f, nz = r.rD[2] < 0.0f;
@@ -212,13 +235,14 @@ __device__ __forceinline__ uint32_t node_hitmask(const float4 n0, const float4 n
         const uint32_t inner = (mw & (mw << 1)) & 0x10101010u;
         const uint32_t idx4 = (mw ^ ((r.octinv * 0x01010101u) & ((inner >> 4) * 0x1Fu))) & 0x1F1F1F1Fu;
         const uint32_t cnt4 = (mw >> 5) & 0x07070707u;
+        const SubBytes blx(lx), bhx(hx), bly(ly), bhy(hy), blz(lz), bhz(hz);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float tminx = fmaf(ubyte_f(lx, j), idirx, origx), tmaxx = fmaf(ubyte_f(hx, j), idirx, origx);
-            const float tminy = fmaf(ubyte_f(ly, j), idiry, origy), tmaxy = fmaf(ubyte_f(hy, j), idiry, origy);
-            const float tminz = fmaf(ubyte_f(lz, j), idirz, origz), tmaxz = fmaf(ubyte_f(hz, j), idirz, origz);
-            const float tnear = fmaxf(fmaxf(fmaxf(tminx, tminy), tminz), r.tmin);
-            const float tfar = fminf(fminf(fminf(tmaxx, tmaxy), tmaxz), tcur);
+            const float tminx = fmaf(blx[j], idirx, origx), tmaxx = fmaf(bhx[j], idirx, origx);
+            const float tminy = fmaf(bly[j], idiry, origy), tmaxy = fmaf(bhy[j], idiry, origy);
+            const float tminz = fmaf(blz[j], idirz, origz), tmaxz = fmaf(bhz[j], idirz, origz);
+            const float tnear = fmaxf(fmaxf(fmaxf(tminx, tminy), tminz), tmin_s);
+            const float tfar = fminf(fminf(fminf(tmaxx, tmaxy), tmaxz), tcur_s);
             const uint32_t cnt = tnear <= tfar * kBoxSlack ? (cnt4 >> (8 * j)) & 7u : 0u;
             hitmask |= cnt << ((idx4 >> (8 * j)) & 0x1Fu);
         }
