@@ -55,8 +55,8 @@ HBM_PEAK_GBS = 8000.0
 # overlap off, saved schedule) for the path tracer's stages.
 MEASURED_TRAFFIC = {
     "k_trace_bounce": (366.2e6, "profiles/k_trace_bounce_displaced_sphere_1M.md"),
-    "k_extend": (2023.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
-    "k_connect": (3325.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
+    "k_extend": (2010.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
+    "k_connect": (3327.0e6, "profiles/k_extend_k_connect_pt_1080p_instances.md"),
 }
 
 
