@@ -167,7 +167,9 @@ __device__ __forceinline__ float ubyte_f(uint32_t w, int j) { return (float)((w 
 // b * 2^-24, exactly, so v_perm_b32 turns a word of 4 bounds into two
 // registers of two halves each, and v_fma_mix_f32 reads either half as an
 // f32 operand: 2 + 4 instructions per 4 bounds instead of 4 v_cvt_f32_ubyte
-// and 4 v_fma_f32.
+// and 4 v_fma_f32.  Needs FP16 subnormals kept, not flushed (the kernels'
+// default FP16/FP64 denormal mode; the traversal parity tests fail at once
+// otherwise).
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 struct SubBytes {
     uint32_t p01, p23;   // halves {b0, b1}, {b2, b3}
