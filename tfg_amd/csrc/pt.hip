@@ -489,16 +489,24 @@ int enqueue_post(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum, boo
     return TFG_OK;
 }
 
-// Frame prologue on s (reset of the frame counters) and the fork of the tile
-// streams: each waits for the prologue.
-int fork_tiles(tfg_pt* pt, hipStream_t s) {
-    hipLaunchKernelGGL(k_frame_reset, dim3(1), dim3(128), 0, s, pt->P.counters);
+// The stream of tile t's chain for a frame on s: s itself when the frame is
+// one tile (no fork and join, two cross-stream waits fewer per frame), else
+// the tile's own stream.
+hipStream_t tile_stream(const tfg_pt* pt, hipStream_t s, int t) { return pt->tiles == 1 ? s : pt->tstream[t]; }
+
+// Frame prologue on s (reset of the frame counters, the frame index) and the
+// fork of the tile streams: each waits for the prologue.
+int fork_tiles(tfg_pt* pt, hipStream_t s, int32_t frame) {
+    hipLaunchKernelGGL(k_frame_reset, dim3(1), dim3(128), 0, s, pt->P.counters, const_cast<int32_t*>(pt->P.frame),
+                       frame);
+    if (pt->tiles == 1) return TFG_OK;
     TFG_HIP_CHECK(hipEventRecord(pt->ev_start, s));
     for (int t = 0; t < pt->tiles; ++t) TFG_HIP_CHECK(hipStreamWaitEvent(pt->tstream[t], pt->ev_start, 0));
     return TFG_OK;
 }
 
 int join_tiles(tfg_pt* pt, hipStream_t s) {
+    if (pt->tiles == 1) return TFG_OK;
     for (int t = 0; t < pt->tiles; ++t) {
         TFG_HIP_CHECK(hipEventRecord(pt->ev_tdone[t], pt->tstream[t]));
         TFG_HIP_CHECK(hipStreamWaitEvent(s, pt->ev_tdone[t], 0));
@@ -510,14 +518,14 @@ int join_tiles(tfg_pt* pt, hipStream_t s) {
 // their streams, the cache work on s after every tile.  Tiles are
 // independent, so the latency tails of their small late-depth launches
 // overlap.
-int enqueue_frame(tfg_pt* pt, hipStream_t s, tfg_color* image, double* accum, StageEvents* pe = nullptr,
-                  bool stats = false) {
+int enqueue_frame(tfg_pt* pt, hipStream_t s, int32_t frame, tfg_color* image, double* accum,
+                  StageEvents* pe = nullptr, bool stats = false) {
     make_tiles(pt);
     mark(pe ? pe->frame[0] : nullptr, s);
     int rc = run_pending_training(pt, pt->cstream);   // the previous frame's steps, beside this frame's tracing
-    if (rc == TFG_OK) rc = fork_tiles(pt, s);
+    if (rc == TFG_OK) rc = fork_tiles(pt, s, frame);
     for (int t = 0; t < pt->tiles && rc == TFG_OK; ++t)
-        rc = enqueue_tile(pt, t, pt->tstream[t], pt->tside[t], image, accum, pe, stats);
+        rc = enqueue_tile(pt, t, tile_stream(pt, s, t), pt->tside[t], image, accum, pe, stats);
     if (rc == TFG_OK) rc = join_tiles(pt, s);
     if (rc == TFG_OK) rc = enqueue_post(pt, s, image, accum, true);
     mark(pe ? pe->frame[1] : nullptr, s);
@@ -561,9 +569,10 @@ int capture_graphs(tfg_pt* pt, tfg_color* image, double* accum) {
 
 // The frame from the captured graphs: prologue on s, each tile graph on its
 // tile stream, the cache graph on s after every tile.
-int launch_graphs(tfg_pt* pt, hipStream_t s) {
-    int rc = fork_tiles(pt, s);
-    for (int t = 0; t < pt->tiles && rc == TFG_OK; ++t) TFG_HIP_CHECK(hipGraphLaunch(pt->exec_tile[t], pt->tstream[t]));
+int launch_graphs(tfg_pt* pt, hipStream_t s, int32_t frame) {
+    int rc = fork_tiles(pt, s, frame);
+    for (int t = 0; t < pt->tiles && rc == TFG_OK; ++t)
+        TFG_HIP_CHECK(hipGraphLaunch(pt->exec_tile[t], tile_stream(pt, s, t)));
     if (rc == TFG_OK) rc = join_tiles(pt, s);
     if (rc == TFG_OK && pt->exec_post) TFG_HIP_CHECK(hipGraphLaunch(pt->exec_post, s));
     return rc;
@@ -880,7 +889,6 @@ int tfg_pt_render(tfg_pt* pt, int32_t frame, tfg_color* d_image, double* d_accum
     hipStream_t s = pt->stream;
     TFG_HIP_CHECK(hipEventRecord(pt->ev_in, stream));
     TFG_HIP_CHECK(hipStreamWaitEvent(s, pt->ev_in, 0));
-    hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(1), 0, s, const_cast<int32_t*>(pt->P.frame), frame);
     int rc = TFG_OK;
     if (pt->run_graph) {
         if (pt->graphs_ready && (pt->key_image != d_image || pt->key_accum != d_accum || pt->key_nrc != pt->nrc ||
@@ -898,10 +906,10 @@ int tfg_pt_render(tfg_pt* pt, int32_t frame, tfg_color* d_image, double* d_accum
             pt->key_nrc = pt->nrc;
             pt->key_mode = pt->P.mode;
         }
-        rc = launch_graphs(pt, s);
+        rc = launch_graphs(pt, s, frame);
         if (rc != TFG_OK) return rc;
     } else {
-        rc = enqueue_frame(pt, s, d_image, d_accum);
+        rc = enqueue_frame(pt, s, frame, d_image, d_accum);
         if (rc != TFG_OK) return rc;
     }
     TFG_HIP_CHECK(hipEventRecord(pt->ev_out, s));
@@ -952,13 +960,12 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
     if (rc != TFG_OK) return rc;
     // the frame twice: once with the counting kernels (node / triangle
     // visits), then with the plain kernels, timed
-    hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(1), 0, s, const_cast<int32_t*>(pt->P.frame), frame);
-    rc = enqueue_frame(pt, s, d_image, nullptr, nullptr, true);
+    rc = enqueue_frame(pt, s, frame, d_image, nullptr, nullptr, true);
     uint64_t st[4] = {0, 0, 0, 0};
     if (rc == TFG_OK) {
         TFG_HIP_CHECK(hipStreamSynchronize(s));
         TFG_HIP_CHECK(hipMemcpy(st, pt->P.counters + CNT_STAT, sizeof(st), hipMemcpyDeviceToHost));
-        rc = enqueue_frame(pt, s, d_image, nullptr, &pe, false);
+        rc = enqueue_frame(pt, s, frame, d_image, nullptr, &pe, false);
     }
     if (rc == TFG_OK) {
         TFG_HIP_CHECK(hipStreamSynchronize(s));
@@ -1022,9 +1029,10 @@ int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream) {
         for (int pass = 0; pass < 2 && rc == TFG_OK; ++pass) {
             for (int i = 0; i < n && rc == TFG_OK; ++i) {
                 apply(i);
-                for (int w = 0; w < 2 && rc == TFG_OK; ++w) rc = enqueue_frame(pt, pt->stream, img, nullptr);
+                // the same 8 timed frame seeds for every candidate
+                for (int w = 0; w < 2 && rc == TFG_OK; ++w) rc = enqueue_frame(pt, pt->stream, 1000 + w, img, nullptr);
                 (void)hipEventRecord(a, pt->stream);
-                for (int f = 0; f < 8 && rc == TFG_OK; ++f) rc = enqueue_frame(pt, pt->stream, img, nullptr);
+                for (int f = 0; f < 8 && rc == TFG_OK; ++f) rc = enqueue_frame(pt, pt->stream, f, img, nullptr);
                 (void)hipEventRecord(b, pt->stream);
                 (void)hipEventSynchronize(b);
                 float ms = 0.f;

@@ -26,8 +26,10 @@ __device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
 // ------------------------------------------------------------------ reset
 // One block of 128 threads on the frame counter block: folds the previous
 // frame's ray counts into the 64-bit running totals, then clears the
-// per-frame counts.  (Each tile's queue counters are set by its generate.)
-__global__ void k_frame_reset(uint32_t* gc) {
+// per-frame counts, and sets the frame index (the seed generate reads).
+// (Each tile's queue counters are set by its generate.)
+__global__ void k_frame_reset(uint32_t* gc, int32_t* frame, int32_t value) {
+    if (threadIdx.x == 0) *frame = value;
     const uint32_t i = threadIdx.x;
     if (i < 64) {
         const uint32_t e = wave_sum(i < kMaxDepth ? gc[CNT_EXT + i] : 0u);
@@ -41,8 +43,6 @@ __global__ void k_frame_reset(uint32_t* gc) {
     __syncthreads();
     if (i < CNT_TOTAL) gc[i] = 0u;
 }
-
-__global__ void k_set_frame(int32_t* frame, int32_t value) { *frame = value; }
 
 // --------------------------------------------------------------- generate
 // Depth-0 queue slot k -> path.  Paths are pixels; the primary rays of one
