@@ -176,3 +176,22 @@ def test_ray_setup_reciprocal_is_the_ieee_division(torch_cuda):
     print(p.stdout, p.stderr, flush=True)
     assert p.returncode == 0, p.stdout + p.stderr
     assert "0 differ; " in p.stdout and p.stdout.rstrip().endswith("0 differ")
+
+
+def test_node_test_scaled_slabs_match_unscaled(torch_cuda):
+    """The node test reads the quantised bounds as FP16 subnormals (b * 2^-24)
+    through v_fma_mix_f32 and compares slab distances scaled by 2^-24; on
+    134M pseudo-random nodes and rays (about 10M of them hitting some child;
+    origins on the node's planes, zero direction components and nonzero tmin
+    among them) its hit masks equal those of the unscaled test with
+    v_cvt_f32_ubyte bounds (tests/cpp/node_check.hip, built by
+    __graft_entry__.build())."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_build", "node_check")
+    assert os.path.exists(exe), "build tests/cpp first (__graft_entry__.build())"
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(p.stdout, p.stderr, flush=True)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert p.stdout.rstrip().endswith("; 0 differ")
