@@ -515,7 +515,8 @@ int join_tiles(tfg_pt* pt, hipStream_t s) {
 }
 
 // The whole frame, issued eagerly (no graph): prologue on s, the tiles on
-// their streams, the cache work on s after every tile.  Tiles are
+// their streams (s itself for a single tile), the cache work on s after
+// every tile.  Tiles are
 // independent, so the latency tails of their small late-depth launches
 // overlap.
 int enqueue_frame(tfg_pt* pt, hipStream_t s, int32_t frame, tfg_color* image, double* accum,
@@ -568,7 +569,8 @@ int capture_graphs(tfg_pt* pt, tfg_color* image, double* accum) {
 }
 
 // The frame from the captured graphs: prologue on s, each tile graph on its
-// tile stream, the cache graph on s after every tile.
+// tile stream (s itself for a single tile), the cache graph on s after every
+// tile.
 int launch_graphs(tfg_pt* pt, hipStream_t s, int32_t frame) {
     int rc = fork_tiles(pt, s, frame);
     for (int t = 0; t < pt->tiles && rc == TFG_OK; ++t)
