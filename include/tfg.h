@@ -206,6 +206,9 @@ int tfg_pt_profile_frame(tfg_pt* pt, int32_t frame, tfg_color* d_image, tfg_pt_p
  * launches (the tuner captures no graph).
  * Drops any captured graph. */
 int tfg_pt_autotune(tfg_pt* pt, tfg_stream stream);
+/* The traversal stages' workgroup size and grid, and the grid of the
+ * per-path kernels (generate, shade, finalize: 256-thread workgroups, one
+ * path per thread within 8-64 workgroups per CU) for tile 0. */
 int tfg_pt_get_launch(const tfg_pt* pt, uint32_t* trace_block, uint32_t* trace_grid, uint32_t* shade_grid);
 /* How a frame is launched.  None of these change the results: every
  * schedule gives the same image bit for bit.  No reference counterpart (the
