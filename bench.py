@@ -595,21 +595,39 @@ def bench_c1(args, torch):
     img = torch.zeros((H * W, 3), dtype=torch.float32, device="cuda")
     accum = torch.zeros((H * W, 3), dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream()
-    for f in range(max(1, args.warmup)):
-        tracer.render(BASE + f, img, accum, stream=stream.cuda_stream)
-    torch.cuda.synchronize()
-    tracer.reset_totals()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record(stream)
-    for k in range(args.steps):
-        for j in range(SPP):
-            tracer.render(BASE + j, img, accum, stream=stream.cuda_stream)
-    b.record(stream)
-    torch.cuda.synchronize()
-    ms_image = a.elapsed_time(b) / args.steps
-    ext, shd = tracer.ray_totals()
-    rays_image = (ext + shd) / args.steps
+
+    def time_images(tr):
+        for f in range(max(1, args.warmup)):
+            tr.render(BASE + f, img, accum, stream=stream.cuda_stream)
+        torch.cuda.synchronize()
+        tr.reset_totals()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for k in range(args.steps):
+            for j in range(SPP):
+                tr.render(BASE + j, img, accum, stream=stream.cuda_stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ext, shd = tr.ray_totals()
+        return a.elapsed_time(b) / args.steps, (ext + shd) / args.steps
+
+    # the tuned schedule, timed with direct launches and with per-frame graph
+    # replay (at 65,536 paths a frame is ~50 short launches, so launch cost
+    # matters); the faster is the step
+    ms_direct, rays_image = time_images(tracer)
+    tuned = {k: v for k, v in tracer.schedule().items() if k != "use_graph"}
     tracer.close()
+    gtracer = pt.PathTracer(W, H, MD, RR, mode=0, use_graph=True)
+    gtracer.set_scene(ds)
+    gtracer.set_camera(sc.camera.frame(W, H))
+    gtracer.set_schedule(**tuned)
+    ms_graph, rays_graph = time_images(gtracer)
+    torch.cuda.synchronize()
+    gtracer.close()
+    assert rays_graph == rays_image, (rays_graph, rays_image)
+    use_graph = ms_graph < ms_direct
+    ms_image = min(ms_direct, ms_graph)
+    shape = dict(shape, use_graph=use_graph)
     cpu = None
     if not args.no_cpu:
         cores = _host_cores()
@@ -630,8 +648,10 @@ def bench_c1(args, torch):
             "dtype": "fp32", "data": "synthetic (seeded procedural scene)", "cpu_baseline": cpu,
             "config": {"workload": "C1: 12-quad room + box + level-4 icosphere (5,144 tris), 1 point light, "
                                    "256x256, 16 spp per step (frame seeds 500..515), max depth 5, RR after 10; "
-                                   "plain FP32 wavefront PT, schedule from tfg_pt_autotune in this run, direct launches",
-                       "rays_per_image": rays_image, "tris": sc.n_tris, "launch": shape}}
+                                   "plain FP32 wavefront PT, schedule from tfg_pt_autotune in this run, the faster of "
+                                   "direct launches and per-frame graph replay",
+                       "rays_per_image": rays_image, "tris": sc.n_tris, "launch": shape,
+                       "ms_per_image": {"direct": ms_direct, "graph_replay": ms_graph}}}
 
 
 def _host_cores():
