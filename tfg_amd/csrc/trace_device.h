@@ -199,7 +199,12 @@ __device__ __forceinline__ uint32_t child_bits(uint32_t meta, float lox, float l
     return ((meta >> 5) & 7u) << bit_index;
 }
 
-// Decodes node (n0..n4) and returns the 32-bit hit mask.
+// Decodes node (n0..n4) and returns the 32-bit hit mask.  kHalf: the bounds
+// as FP16 subnormals through v_fma_mix_f32 with slab distances scaled by
+// 2^-24 (fewer instructions, 6 more registers), else v_cvt_f32_ubyte and
+// unscaled distances; both give the same mask (tests/cpp/node_check.hip), so
+// each kernel takes the form that leaves it the better occupancy.
+template <bool kHalf = true>
 __device__ __forceinline__ uint32_t node_hitmask(const float4 n0, const float4 n1, const float4 n2, const float4 n3,
                                                  const float4 n4, const RayPre& r, float tcur) {
     const uint32_t w = __float_as_uint(n0.w);
@@ -212,10 +217,11 @@ __device__ __forceinline__ uint32_t node_hitmask(const float4 n0, const float4 n
     // them only with each other and with tmin and tcur scaled alike, so every
     // decision is the unscaled test's while the scaled values stay normal
     // (|t| >= 2^-102).
-    const float origx = (n0.x - r.O[0]) * r.rD[0] * kSlabScale;
-    const float origy = (n0.y - r.O[1]) * r.rD[1] * kSlabScale;
-    const float origz = (n0.z - r.O[2]) * r.rD[2] * kSlabScale;
-    const float tmin_s = r.tmin * kSlabScale, tcur_s = tcur * kSlabScale;
+    constexpr float kS = kHalf ? kSlabScale : 1.0f;
+    const float origx = (n0.x - r.O[0]) * r.rD[0] * kS;
+    const float origy = (n0.y - r.O[1]) * r.rD[1] * kS;
+    const float origz = (n0.z - r.O[2]) * r.rD[2] * kS;
+    const float tmin_s = r.tmin * kS, tcur_s = tcur * kS;
     const uint32_t m[2] = {__float_as_uint(n1.z), __float_as_uint(n1.w)};
     // near / far bound words per axis, chosen once per node by the direction signs
     const bool nx = r.rD[0] < 0.0f, ny = r.rD[1] < 0.0f, nz = r.rD[2] < 0.0f;
@@ -238,11 +244,13 @@ __device__ __forceinline__ uint32_t node_hitmask(const float4 n0, const float4 n
         const uint32_t idx4 = (mw ^ ((r.octinv * 0x01010101u) & ((inner >> 4) * 0x1Fu))) & 0x1F1F1F1Fu;
         const uint32_t cnt4 = (mw >> 5) & 0x07070707u;
         const SubBytes blx(lx), bhx(hx), bly(ly), bhy(hy), blz(lz), bhz(hz);
+        // bound j of word w: b * 2^-24 (kHalf) or b
+        auto bnd = [&](const SubBytes& sb, uint32_t w, int j) { return kHalf ? sb[j] : ubyte_f(w, j); };
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float tminx = fmaf(blx[j], idirx, origx), tmaxx = fmaf(bhx[j], idirx, origx);
-            const float tminy = fmaf(bly[j], idiry, origy), tmaxy = fmaf(bhy[j], idiry, origy);
-            const float tminz = fmaf(blz[j], idirz, origz), tmaxz = fmaf(bhz[j], idirz, origz);
+            const float tminx = fmaf(bnd(blx, lx, j), idirx, origx), tmaxx = fmaf(bnd(bhx, hx, j), idirx, origx);
+            const float tminy = fmaf(bnd(bly, ly, j), idiry, origy), tmaxy = fmaf(bnd(bhy, hy, j), idiry, origy);
+            const float tminz = fmaf(bnd(blz, lz, j), idirz, origz), tmaxz = fmaf(bnd(bhz, hz, j), idirz, origz);
             const float tnear = fmaxf(fmaxf(fmaxf(tminx, tminy), tminz), tmin_s);
             const float tfar = fminf(fminf(fminf(tmaxx, tmaxy), tmaxz), tcur_s);
             const uint32_t cnt = tnear <= tfar * kBoxSlack ? (cnt4 >> (8 * j)) & 7u : 0u;
@@ -364,7 +372,7 @@ __device__ __forceinline__ void load_node(const float4* __restrict__ nodes, cons
 // triangle results, and each node's child test sees every triangle hit found
 // before it, so visits and results are those of testing a node's triangles
 // before selecting the next node.
-template <int kFormat, bool kAnyHit>
+template <int kFormat, bool kAnyHit, bool kHalf = true>
 __device__ __forceinline__ bool traverse(const float4* __restrict__ nodes, const float4* __restrict__ tris,
                                          const float4* __restrict__ top_nodes, uint32_t n_top, const RayPre& r,
                                          uint2* stack, int stride, int stack_depth, float& t_out, float& u_out,
@@ -406,7 +414,7 @@ __device__ __forceinline__ bool traverse(const float4* __restrict__ nodes, const
             return true;
         if (!have) break;
         if (stats) stats->nodes++;
-        const uint32_t hm = node_hitmask(n0, n1, n2, n3, n4, r, tcur);
+        const uint32_t hm = node_hitmask<kHalf>(n0, n1, n2, n3, n4, r, tcur);
         ng_x = __float_as_uint(n1.x);
         ng_y = (hm & 0xFF000000u) | (__float_as_uint(n0.w) >> 24);
         tg_x = __float_as_uint(n1.y);
